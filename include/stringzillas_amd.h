@@ -1,0 +1,334 @@
+/**
+ *  @file include/stringzillas_amd.h
+ *  @brief C ABI of the MI355X-native similarity engine: a drop-in for the `szs_*` similarity symbols of
+ *         StringZillas (include/stringzillas/stringzillas.h) plus batch and serving extensions.
+ *
+ *  Every reference symbol keeps its name, signature, argument meaning and status codes. The type layouts below
+ *  are ABI-identical to include/stringzilla/types.h so a caller compiled against the reference header can link
+ *  against `libstringzillas_amd.so` unchanged.
+ *
+ *  Execution model: every engine runs on an AMD GPU (gfx950). A GPU scope or the default scope (device 0) runs
+ *  it; a CPU-cores scope is rejected with `sz_device_code_mismatch_k`, as the reference does for a GPU engine on
+ *  a CPU scope (c/stringzillas/levenshtein.cuh:76-79). Inputs and outputs may live in host memory (they are
+ *  staged through HBM) or in device memory (used in place).
+ */
+#ifndef STRINGZILLAS_AMD_H_
+#define STRINGZILLAS_AMD_H_
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#pragma region Types shared with include/stringzilla/types.h
+
+#ifndef STRINGZILLA_TYPES_H_
+typedef uint8_t sz_u8_t;
+typedef uint32_t sz_u32_t;
+typedef uint64_t sz_u64_t;
+typedef int8_t sz_i8_t;
+typedef size_t sz_size_t;                 /* include/stringzilla/types.h:538 */
+typedef ptrdiff_t sz_ssize_t;             /* include/stringzilla/types.h:539 */
+typedef char const *sz_cptr_t;            /* include/stringzilla/types.h:633 */
+typedef sz_i8_t sz_error_cost_t;          /* include/stringzilla/types.h:634 */
+typedef sz_u64_t sz_sorted_idx_t;
+
+/** Status codes, include/stringzilla/types.h:699-722. */
+typedef enum sz_status_t {
+    sz_success_k = 0,
+    sz_bad_alloc_k = -10,
+    sz_invalid_utf8_k = -12,
+    sz_contains_duplicates_k = -13,
+    sz_overflow_risk_k = -14,
+    sz_unexpected_dimensions_k = -15,
+    sz_missing_gpu_k = -16,
+    sz_device_code_mismatch_k = -17,
+    sz_device_memory_mismatch_k = -18,
+    sz_authentication_failed_k = -19,
+    sz_status_unknown_k = -1,
+} sz_status_t;
+
+/** Capability bits, include/stringzilla/types.h:727-767, plus the ROCm bit this library adds (bit 25). */
+typedef enum sz_capability_t {
+    sz_cap_serial_k = 1,
+    sz_cap_parallel_k = 1 << 2,
+    sz_cap_any_k = 0x7FFFFFFF,
+    sz_cap_cuda_k = 1 << 21,
+    sz_cap_kepler_k = 1 << 22,
+    sz_cap_hopper_k = 1 << 23,
+    sz_cap_rocm_k = 1 << 25, /**< new: AMD GPU (HIP) capability, no reference counterpart */
+    sz_caps_none_k = 0,
+} sz_capability_t;
+
+/** Sequence of strings behind two callbacks, include/stringzilla/types.h:1168-1182. */
+typedef sz_cptr_t (*sz_sequence_member_start_t)(void const *, sz_sorted_idx_t);
+typedef sz_size_t (*sz_sequence_member_length_t)(void const *, sz_sorted_idx_t);
+typedef struct sz_sequence_t {
+    void const *handle;
+    sz_size_t count;
+    sz_sequence_member_start_t get_start;
+    sz_sequence_member_length_t get_length;
+} sz_sequence_t;
+
+/** Memory allocator, include/stringzilla/types.h:859-870. */
+typedef void *(*sz_memory_allocate_t)(sz_size_t, void *);
+typedef void (*sz_memory_free_t)(void *, sz_size_t, void *);
+typedef struct sz_memory_allocator_t {
+    sz_memory_allocate_t allocate;
+    sz_memory_free_t free;
+    void *handle;
+} sz_memory_allocator_t;
+#endif
+
+/** Arrow-like tapes of `count` strings with `count + 1` offsets; include/stringzillas/stringzillas.h:68-82. */
+typedef struct sz_sequence_u32tape_t {
+    sz_cptr_t data;
+    sz_u32_t const *offsets;
+    sz_size_t count;
+} sz_sequence_u32tape_t;
+typedef struct sz_sequence_u64tape_t {
+    sz_cptr_t data;
+    sz_u64_t const *offsets;
+    sz_size_t count;
+} sz_sequence_u64tape_t;
+
+#pragma endregion
+
+#pragma region Version and capabilities (stringzillas.h:33-62)
+
+int szs_version_major(void);                          /* replaces stringzillas.h:33 */
+int szs_version_minor(void);                          /* replaces stringzillas.h:38 */
+int szs_version_patch(void);                          /* replaces stringzillas.h:43 */
+sz_capability_t szs_capabilities_comptime(void);      /* replaces stringzillas.h:49 */
+sz_capability_t szs_capabilities_runtime(void);       /* replaces stringzillas.h:56 */
+sz_capability_t szs_capabilities(void);               /* replaces stringzillas.h:62 */
+
+/** Replaces stringzillas.h:88. Fills @p alloc with a `hipMallocManaged`-backed allocator (malloc without a GPU). */
+sz_status_t sz_memory_allocator_init_unified(sz_memory_allocator_t *alloc, char const **error_message);
+
+#pragma endregion
+
+#pragma region Device scopes (stringzillas.h:101-152)
+
+typedef void *szs_device_scope_t;
+sz_status_t szs_device_scope_init_default(szs_device_scope_t *scope, char const **error_message);      /* :107 */
+sz_status_t szs_device_scope_init_cpu_cores(sz_size_t cpu_cores, szs_device_scope_t *scope,            /* :114 */
+                                            char const **error_message);
+sz_status_t szs_device_scope_init_gpu_device(sz_size_t gpu_device, szs_device_scope_t *scope,          /* :122 */
+                                             char const **error_message);
+sz_status_t szs_device_scope_get_cpu_cores(szs_device_scope_t scope, sz_size_t *cpu_cores,             /* :130 */
+                                           char const **error_message);
+sz_status_t szs_device_scope_get_gpu_device(szs_device_scope_t scope, sz_size_t *gpu_device,           /* :138 */
+                                            char const **error_message);
+sz_status_t szs_device_scope_get_capabilities(szs_device_scope_t scope, sz_capability_t *capabilities, /* :146 */
+                                              char const **error_message);
+void szs_device_scope_free(szs_device_scope_t scope);                                                  /* :152 */
+
+#pragma endregion
+
+#pragma region Levenshtein distances (stringzillas.h:153-292)
+
+typedef void *szs_levenshtein_distances_t;
+typedef void *szs_levenshtein_distances_utf8_t;
+
+/** Replaces stringzillas.h:170. Costs are non-negative here (the reference stores distances in unsigned cells). */
+sz_status_t szs_levenshtein_distances_init(sz_error_cost_t match, sz_error_cost_t mismatch, sz_error_cost_t open,
+                                           sz_error_cost_t extend, sz_memory_allocator_t const *alloc,
+                                           sz_capability_t capabilities, szs_levenshtein_distances_t *engine,
+                                           char const **error_message);
+/** Replaces stringzillas.h:184: `results[q * results_row_stride + c]`; `candidates == NULL` = symmetric. */
+sz_status_t szs_levenshtein_distances(szs_levenshtein_distances_t engine, szs_device_scope_t device,
+                                      sz_sequence_t const *queries, sz_sequence_t const *candidates,
+                                      sz_size_t *results, sz_size_t results_row_stride, char const **error_message);
+/** Replaces stringzillas.h:199. */
+sz_status_t szs_levenshtein_distances_u32tape(szs_levenshtein_distances_t engine, szs_device_scope_t device,
+                                              sz_sequence_u32tape_t const *queries,
+                                              sz_sequence_u32tape_t const *candidates, sz_size_t *results,
+                                              sz_size_t results_row_stride, char const **error_message);
+/** Replaces stringzillas.h:214. */
+sz_status_t szs_levenshtein_distances_u64tape(szs_levenshtein_distances_t engine, szs_device_scope_t device,
+                                              sz_sequence_u64tape_t const *queries,
+                                              sz_sequence_u64tape_t const *candidates, sz_size_t *results,
+                                              sz_size_t results_row_stride, char const **error_message);
+/** Replaces stringzillas.h:223. */
+void szs_levenshtein_distances_free(szs_levenshtein_distances_t engine);
+
+/** Replaces stringzillas.h:239. Unlike the reference's CUDA path, affine and non-unit costs run on the GPU too. */
+sz_status_t szs_levenshtein_distances_utf8_init(sz_error_cost_t match, sz_error_cost_t mismatch,
+                                                sz_error_cost_t open, sz_error_cost_t extend,
+                                                sz_memory_allocator_t const *alloc, sz_capability_t capabilities,
+                                                szs_levenshtein_distances_utf8_t *engine,
+                                                char const **error_message);
+/** Replaces stringzillas.h:253. */
+sz_status_t szs_levenshtein_distances_utf8(szs_levenshtein_distances_utf8_t engine, szs_device_scope_t device,
+                                           sz_sequence_t const *queries, sz_sequence_t const *candidates,
+                                           sz_size_t *results, sz_size_t results_row_stride,
+                                           char const **error_message);
+/** Replaces stringzillas.h:268. */
+sz_status_t szs_levenshtein_distances_utf8_u32tape(szs_levenshtein_distances_utf8_t engine, szs_device_scope_t device,
+                                                   sz_sequence_u32tape_t const *queries,
+                                                   sz_sequence_u32tape_t const *candidates, sz_size_t *results,
+                                                   sz_size_t results_row_stride, char const **error_message);
+/** Replaces stringzillas.h:283. */
+sz_status_t szs_levenshtein_distances_utf8_u64tape(szs_levenshtein_distances_utf8_t engine, szs_device_scope_t device,
+                                                   sz_sequence_u64tape_t const *queries,
+                                                   sz_sequence_u64tape_t const *candidates, sz_size_t *results,
+                                                   sz_size_t results_row_stride, char const **error_message);
+/** Replaces stringzillas.h:292. */
+void szs_levenshtein_distances_utf8_free(szs_levenshtein_distances_utf8_t engine);
+
+#pragma endregion
+
+#pragma region Needleman-Wunsch and Smith-Waterman scores (stringzillas.h:293-434)
+
+typedef void *szs_needleman_wunsch_scores_t;
+typedef void *szs_smith_waterman_scores_t;
+
+/** Replaces stringzillas.h:310: `byte_to_class[256]` + row-major `class_substitution_costs[32 * 32]`. */
+sz_status_t szs_needleman_wunsch_scores_init(sz_u8_t const *byte_to_class,
+                                             sz_error_cost_t const *class_substitution_costs, sz_error_cost_t open,
+                                             sz_error_cost_t extend, sz_memory_allocator_t const *alloc,
+                                             sz_capability_t capabilities, szs_needleman_wunsch_scores_t *engine,
+                                             char const **error_message);
+/** Replaces stringzillas.h:325. */
+sz_status_t szs_needleman_wunsch_scores(szs_needleman_wunsch_scores_t engine, szs_device_scope_t device,
+                                        sz_sequence_t const *queries, sz_sequence_t const *candidates,
+                                        sz_ssize_t *results, sz_size_t results_row_stride,
+                                        char const **error_message);
+/** Replaces stringzillas.h:340. */
+sz_status_t szs_needleman_wunsch_scores_u32tape(szs_needleman_wunsch_scores_t engine, szs_device_scope_t device,
+                                                sz_sequence_u32tape_t const *queries,
+                                                sz_sequence_u32tape_t const *candidates, sz_ssize_t *results,
+                                                sz_size_t results_row_stride, char const **error_message);
+/** Replaces stringzillas.h:355. */
+sz_status_t szs_needleman_wunsch_scores_u64tape(szs_needleman_wunsch_scores_t engine, szs_device_scope_t device,
+                                                sz_sequence_u64tape_t const *queries,
+                                                sz_sequence_u64tape_t const *candidates, sz_ssize_t *results,
+                                                sz_size_t results_row_stride, char const **error_message);
+/** Replaces stringzillas.h:364. */
+void szs_needleman_wunsch_scores_free(szs_needleman_wunsch_scores_t engine);
+
+/** Replaces stringzillas.h:380. */
+sz_status_t szs_smith_waterman_scores_init(sz_u8_t const *byte_to_class,
+                                           sz_error_cost_t const *class_substitution_costs, sz_error_cost_t open,
+                                           sz_error_cost_t extend, sz_memory_allocator_t const *alloc,
+                                           sz_capability_t capabilities, szs_smith_waterman_scores_t *engine,
+                                           char const **error_message);
+/** Replaces stringzillas.h:395. */
+sz_status_t szs_smith_waterman_scores(szs_smith_waterman_scores_t engine, szs_device_scope_t device,
+                                      sz_sequence_t const *queries, sz_sequence_t const *candidates,
+                                      sz_ssize_t *results, sz_size_t results_row_stride, char const **error_message);
+/** Replaces stringzillas.h:410. */
+sz_status_t szs_smith_waterman_scores_u32tape(szs_smith_waterman_scores_t engine, szs_device_scope_t device,
+                                              sz_sequence_u32tape_t const *queries,
+                                              sz_sequence_u32tape_t const *candidates, sz_ssize_t *results,
+                                              sz_size_t results_row_stride, char const **error_message);
+/** Replaces stringzillas.h:425. */
+sz_status_t szs_smith_waterman_scores_u64tape(szs_smith_waterman_scores_t engine, szs_device_scope_t device,
+                                              sz_sequence_u64tape_t const *queries,
+                                              sz_sequence_u64tape_t const *candidates, sz_ssize_t *results,
+                                              sz_size_t results_row_stride, char const **error_message);
+/** Replaces stringzillas.h:434. */
+void szs_smith_waterman_scores_free(szs_smith_waterman_scores_t engine);
+
+#pragma endregion
+
+#pragma region Unified memory (stringzillas.h:545-551)
+
+void *szs_unified_alloc(sz_size_t size_bytes);           /* replaces stringzillas.h:545 */
+void szs_unified_free(void *ptr, sz_size_t size_bytes);  /* replaces stringzillas.h:551 */
+
+#pragma endregion
+
+#pragma region Extensions: independent pairs, mixed batches, serving (no reference counterpart)
+
+/**
+ *  Pair-list mode: `results[i] = score(firsts[i], seconds[i])` for `i < firsts->count` (both tapes must hold the
+ *  same count). The reference offers only the cross-product shape; its callers score independent pairs as N
+ *  1x1 calls. `results` is `sz_size_t*` for the Levenshtein engines and `sz_ssize_t*` for NW/SW, as above.
+ */
+sz_status_t szs_levenshtein_distances_pairs_u64tape(szs_levenshtein_distances_t engine, szs_device_scope_t device,
+                                                    sz_sequence_u64tape_t const *firsts,
+                                                    sz_sequence_u64tape_t const *seconds, sz_size_t *results,
+                                                    char const **error_message);
+sz_status_t szs_levenshtein_distances_utf8_pairs_u64tape(szs_levenshtein_distances_utf8_t engine,
+                                                         szs_device_scope_t device,
+                                                         sz_sequence_u64tape_t const *firsts,
+                                                         sz_sequence_u64tape_t const *seconds, sz_size_t *results,
+                                                         char const **error_message);
+sz_status_t szs_needleman_wunsch_scores_pairs_u64tape(szs_needleman_wunsch_scores_t engine, szs_device_scope_t device,
+                                                      sz_sequence_u64tape_t const *firsts,
+                                                      sz_sequence_u64tape_t const *seconds, sz_ssize_t *results,
+                                                      char const **error_message);
+sz_status_t szs_smith_waterman_scores_pairs_u64tape(szs_smith_waterman_scores_t engine, szs_device_scope_t device,
+                                                    sz_sequence_u64tape_t const *firsts,
+                                                    sz_sequence_u64tape_t const *seconds, sz_ssize_t *results,
+                                                    char const **error_message);
+
+/** One homogeneous slice of a mixed batch: an engine handle of the given kind plus its pair list. */
+typedef enum szs_engine_kind_t {
+    szs_engine_levenshtein_k = 0,
+    szs_engine_levenshtein_utf8_k = 1,
+    szs_engine_needleman_wunsch_k = 2,
+    szs_engine_smith_waterman_k = 3,
+} szs_engine_kind_t;
+
+typedef struct szs_pairs_segment_t {
+    szs_engine_kind_t kind;
+    void *engine;                          /**< handle returned by the matching `szs_*_init` */
+    sz_sequence_u64tape_t const *firsts;   /**< host or device tapes */
+    sz_sequence_u64tape_t const *seconds;
+    void *results;                         /**< `count` x 8-byte results (u64 or i64 by kind) */
+} szs_pairs_segment_t;
+
+/**
+ *  Mixed batch: all segments are scored in ONE persistent, length-binned launch. A prepared batch can be replayed
+ *  (as a captured HIP graph) when its inputs are device-resident; the "plain" mode instead issues one launch per
+ *  pair, the one-launch-per-call baseline.
+ */
+typedef void *szs_batch_t;
+typedef enum szs_batch_mode_t {
+    szs_batch_persistent_k = 0, /**< device-side binning + one persistent launch (default) */
+    szs_batch_graph_k = 1,      /**< the same, captured once into a HIP graph and replayed */
+    szs_batch_per_pair_k = 2,   /**< one kernel launch per pair, no binning: the per-call baseline */
+} szs_batch_mode_t;
+
+sz_status_t szs_batch_init(szs_device_scope_t device, szs_pairs_segment_t const *segments, sz_size_t segments_count,
+                           szs_batch_mode_t mode, szs_batch_t *batch, char const **error_message);
+/** Runs the prepared batch once and waits for it; results land where the segments point. */
+sz_status_t szs_batch_run(szs_batch_t batch, char const **error_message);
+/** Queues the batch on its stream without waiting (`szs_batch_wait` joins it). */
+sz_status_t szs_batch_launch(szs_batch_t batch, char const **error_message);
+sz_status_t szs_batch_wait(szs_batch_t batch, char const **error_message);
+/** Total DP cells (sum of n*m, runes for UTF-8) the batch scores per run. */
+sz_size_t szs_batch_cells(szs_batch_t batch);
+/** Device time of the last run's dominant (scoring) kernel in milliseconds, from HIP events on its stream. */
+double szs_batch_last_kernel_ms(szs_batch_t batch);
+void szs_batch_free(szs_batch_t batch);
+
+/**
+ *  Serving: a continuous-batching request loop. Requests (one segment each) are admitted while their tapes fit the
+ *  HBM budget; a dispatcher thread coalesces everything admitted into the next device round while earlier rounds
+ *  are still running, so requests that arrive during a launch overlap with it.
+ */
+typedef void *szs_server_t;
+typedef sz_u64_t szs_ticket_t;
+sz_status_t szs_server_init(szs_device_scope_t device, sz_size_t hbm_budget_bytes, sz_size_t max_inflight_rounds,
+                            szs_server_t *server, char const **error_message);
+/** Copies the request's tapes into the server's device pool; fails with `sz_bad_alloc_k` if over budget. */
+sz_status_t szs_server_submit(szs_server_t server, szs_pairs_segment_t const *request, szs_ticket_t *ticket,
+                              char const **error_message);
+/** Blocks until @p ticket completes; its results are then in the request's host `results` buffer. */
+sz_status_t szs_server_wait(szs_server_t server, szs_ticket_t ticket, char const **error_message);
+/** Non-blocking completion check: 1 if done, 0 if pending. */
+int szs_server_poll(szs_server_t server, szs_ticket_t ticket);
+sz_size_t szs_server_bytes_in_use(szs_server_t server);
+void szs_server_free(szs_server_t server);
+
+#pragma endregion
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,470 @@
+"""MI355X-native drop-in for the ``stringzillas`` similarity classes.
+
+Mirrors the reference's Python surface (python/stringzillas/similarities.c, device_scope.c, stringzillas.c):
+``LevenshteinDistances``, ``LevenshteinDistancesUTF8``, ``NeedlemanWunschScores``, ``SmithWatermanScores``,
+``DeviceScope``, ``Strs``, ``__capabilities__``, ``__version__``, ``reset_capabilities`` and ``to_device``, with
+the same argument names, defaults, dtypes and exception types. Every score is computed by the HIP kernels in
+``libstringzillas_amd.so``; importing this package fails loudly when that library is missing.
+
+Extensions beyond the reference: ``engine.pairs(firsts, seconds)`` scores independent pairs, ``Batch`` scores a
+mixed list of (engine, firsts, seconds) segments in one persistent launch, ``Server`` is the continuous-batching
+request loop, and ``DeviceStrs`` keeps a tape resident in HBM.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Iterable, List, Optional, Sequence, Tuple, Union
+
+import numpy as np
+
+from . import _native
+from ._native import SzsError, check as _check, lib as _lib, U64Tape as _U64Tape
+
+__version__ = "%d.%d.%d" % (_lib.szs_version_major(), _lib.szs_version_minor(), _lib.szs_version_patch())
+
+_CAPABILITY_NAMES = {1: "serial", 1 << 2: "parallel", 1 << 21: "cuda", 1 << 25: "rocm"}
+
+
+def _caps_to_tuple(mask: int) -> Tuple[str, ...]:
+    return tuple(name for bit, name in sorted(_CAPABILITY_NAMES.items()) if mask & bit)
+
+
+__capabilities__ = _caps_to_tuple(_lib.szs_capabilities())
+_active_capabilities = _lib.szs_capabilities()
+
+
+def reset_capabilities(capabilities: Sequence[str]) -> None:
+    """Restricts the capabilities later engines are created with (python/stringzillas/stringzillas.c:111-147)."""
+    global _active_capabilities, __capabilities__
+    mask = 0
+    names = {v: k for k, v in _CAPABILITY_NAMES.items()}
+    for name in capabilities:
+        if name not in names:
+            raise ValueError(f"Unknown capability: {name}")
+        mask |= names[name]
+    _active_capabilities = mask & _lib.szs_capabilities()
+    __capabilities__ = _caps_to_tuple(_active_capabilities)
+
+
+#  Strings -----------------------------------------------------------------------------------------------------
+
+
+class Strs:
+    """An Arrow-style tape of byte strings (``data`` + ``count + 1`` u64 offsets), the input type the engines take.
+
+    Plays the role of ``stringzilla.Strs`` (python/stringzilla/strs.c), which the reference's engines require.
+    """
+
+    __slots__ = ("data", "offsets", "_keep")
+
+    def __init__(self, strings: Union[Iterable[Union[str, bytes]], "Strs", None] = None):
+        if isinstance(strings, Strs):
+            self.data, self.offsets, self._keep = strings.data, strings.offsets, strings._keep
+            return
+        encoded = [s.encode("utf-8") if isinstance(s, str) else bytes(s) for s in (strings or [])]
+        offsets = np.zeros(len(encoded) + 1, dtype=np.uint64)
+        if encoded:
+            offsets[1:] = np.cumsum([len(s) for s in encoded], dtype=np.uint64)
+        self.data = np.frombuffer(b"".join(encoded) + b"\0", dtype=np.uint8)
+        self.offsets = offsets
+        self._keep = None
+
+    @classmethod
+    def from_tape(cls, data: np.ndarray, offsets: np.ndarray) -> "Strs":
+        strs = cls.__new__(cls)
+        strs.data = np.ascontiguousarray(data, dtype=np.uint8)
+        strs.offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        strs._keep = None
+        return strs
+
+    @classmethod
+    def from_arrow(cls, array) -> "Strs":
+        """Zero-copy view of a pyarrow ``string``/``large_string``/``binary`` array without nulls."""
+        import pyarrow as pa
+
+        if array.null_count:
+            raise ValueError("Strs cannot hold null strings")
+        if pa.types.is_string(array.type) or pa.types.is_binary(array.type):
+            array = array.cast(pa.large_binary())
+        buffers = array.buffers()
+        offsets = np.frombuffer(buffers[1], dtype=np.int64)[array.offset: array.offset + len(array) + 1]
+        data = np.frombuffer(buffers[2], dtype=np.uint8) if buffers[2] is not None else np.zeros(1, np.uint8)
+        strs = cls.from_tape(data, offsets.astype(np.uint64))
+        strs._keep = array
+        return strs
+
+    def __len__(self) -> int:
+        return len(self.offsets) - 1
+
+    def __getitem__(self, index: int) -> bytes:
+        begin, end = int(self.offsets[index]), int(self.offsets[index + 1])
+        return self.data[begin:end].tobytes()
+
+    def tape(self) -> _U64Tape:
+        return _U64Tape(self.data.ctypes.data, self.offsets.ctypes.data, len(self))
+
+
+class DeviceStrs:
+    """A tape resident in HBM (torch device tensors), used in place by every call.
+
+    ``to_device`` (python/stringzillas/stringzillas.c:148-162) returns one of these.
+    """
+
+    def __init__(self, strs: Strs, device: int = 0):
+        import torch
+
+        self.device = torch.device("cuda", device)
+        self._data = torch.from_numpy(np.ascontiguousarray(strs.data)).to(self.device)
+        self._offsets = torch.from_numpy(strs.offsets.astype(np.int64)).to(self.device)
+        self.count = len(strs)
+        self.host = strs
+
+    def __len__(self) -> int:
+        return self.count
+
+    def tape(self) -> _U64Tape:
+        return _U64Tape(self._data.data_ptr(), self._offsets.data_ptr(), self.count)
+
+
+def to_device(strs: Strs, device: int = 0) -> DeviceStrs:
+    return DeviceStrs(strs if isinstance(strs, Strs) else Strs(strs), device)
+
+
+def _as_tape(obj, name: str):
+    if isinstance(obj, (Strs, DeviceStrs)):
+        return obj
+    try:
+        import pyarrow as pa
+
+        if isinstance(obj, (pa.Array, pa.ChunkedArray)):
+            if isinstance(obj, pa.ChunkedArray):
+                obj = obj.combine_chunks()
+            return Strs.from_arrow(obj)
+    except ImportError:  # pragma: no cover - pyarrow is optional
+        pass
+    raise TypeError(
+        f"Expected Strs objects, got {type(obj).__name__} for '{name}'. Convert using: Strs(your_string_list)"
+    )
+
+
+#  Device scopes -----------------------------------------------------------------------------------------------
+
+
+class DeviceScope:
+    """``DeviceScope(cpu_cores=None, gpu_device=None)`` (python/stringzillas/device_scope.c:17-61)."""
+
+    def __init__(self, cpu_cores: Optional[int] = None, gpu_device: Optional[int] = None):
+        if cpu_cores is not None and gpu_device is not None:
+            raise ValueError("Specify either cpu_cores or gpu_device, not both")
+        handle = ctypes.c_void_p()
+        error = ctypes.c_char_p()
+        if gpu_device is not None:
+            status = _lib.szs_device_scope_init_gpu_device(ctypes.c_size_t(gpu_device), ctypes.byref(handle), ctypes.byref(error))
+            self.description = f"gpu_device={gpu_device}"
+        elif cpu_cores is not None:
+            status = _lib.szs_device_scope_init_cpu_cores(ctypes.c_size_t(cpu_cores), ctypes.byref(handle), ctypes.byref(error))
+            self.description = f"cpu_cores={cpu_cores}"
+        else:
+            status = _lib.szs_device_scope_init_default(ctypes.byref(handle), ctypes.byref(error))
+            self.description = "default"
+        _check(status, error, "Device scope initialization")
+        self.handle = handle
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            _lib.szs_device_scope_free(self.handle)
+            self.handle = None
+
+    def __repr__(self) -> str:
+        return f"DeviceScope({self.description})"
+
+
+_default_scope: Optional[DeviceScope] = None
+
+
+def _scope(device: Optional[DeviceScope]) -> DeviceScope:
+    global _default_scope
+    if device is None:
+        if _default_scope is None:
+            _default_scope = DeviceScope()
+        return _default_scope
+    if not isinstance(device, DeviceScope):
+        raise TypeError("device must be a DeviceScope instance")
+    return device
+
+
+#  Engines -----------------------------------------------------------------------------------------------------
+
+
+def _int8(value, name: str) -> int:
+    value = int(value)
+    if value < -128 or value > 127:
+        raise ValueError(f"{name} cost must fit in 8-bit signed integer")
+    return value
+
+
+class _Engine:
+    _kind = 0
+    _dtype = np.uint64
+    _prefix = ""
+
+    def _finish_init(self, status, handle, error, description):
+        _check(status, error, f"{type(self).__name__} initialization")
+        self.handle = handle
+        self.description = description
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            getattr(_lib, f"{self._prefix}_free")(self.handle)
+            self.handle = None
+
+    def __repr__(self) -> str:
+        return f"{type(self).__name__}({self.description})"
+
+    def __call__(self, queries, candidates=None, device: Optional[DeviceScope] = None, out=None) -> np.ndarray:
+        """Cross-product ``out[q, c] = score(queries[q], candidates[c])``; symmetric when ``candidates`` is None."""
+        queries = _as_tape(queries, "queries")
+        if candidates is not None:
+            candidates = _as_tape(candidates, "candidates")
+        scope = _scope(device)
+        rows = len(queries)
+        columns = len(candidates) if candidates is not None else rows
+        if out is None:
+            results = np.empty((rows, columns), dtype=self._dtype)
+        else:
+            if not isinstance(out, np.ndarray):
+                raise TypeError("out argument must be a NumPy array")
+            if out.ndim != 2:
+                raise ValueError("out array must be 2-dimensional")
+            if out.shape[0] < rows or out.shape[1] < columns:
+                raise ValueError("out array is too small for results")
+            if out.dtype != self._dtype:
+                raise TypeError(f"out array must have {np.dtype(self._dtype).name} dtype")
+            if out.strides[1] != 8:
+                raise ValueError("out array rows must be contiguous (unit stride along columns)")
+            results = out
+        stride = results.strides[0] // 8 if rows else columns
+        q_tape = queries.tape()
+        c_tape = candidates.tape() if candidates is not None else None
+        error = ctypes.c_char_p()
+        status = getattr(_lib, f"{self._prefix}_u64tape")(
+            self.handle, scope.handle, ctypes.byref(q_tape), ctypes.byref(c_tape) if c_tape is not None else None,
+            ctypes.c_void_p(results.ctypes.data), ctypes.c_size_t(stride), ctypes.byref(error))
+        _check(status, error, type(self).__name__)
+        return results
+
+    def pairs(self, firsts, seconds, device: Optional[DeviceScope] = None, out=None):
+        """Independent pairs: ``out[i] = score(firsts[i], seconds[i])`` (an extension of the reference API)."""
+        firsts = _as_tape(firsts, "firsts")
+        seconds = _as_tape(seconds, "seconds")
+        if len(firsts) != len(seconds):
+            raise ValueError("firsts and seconds must hold the same number of strings")
+        scope = _scope(device)
+        if out is None:
+            out = np.empty(len(firsts), dtype=self._dtype)
+        pointer = out.data_ptr() if hasattr(out, "data_ptr") else out.ctypes.data
+        a, b = firsts.tape(), seconds.tape()
+        error = ctypes.c_char_p()
+        status = getattr(_lib, f"{self._prefix}_pairs_u64tape")(
+            self.handle, scope.handle, ctypes.byref(a), ctypes.byref(b), ctypes.c_void_p(pointer), ctypes.byref(error))
+        _check(status, error, type(self).__name__)
+        return out
+
+
+class LevenshteinDistances(_Engine):
+    """Byte-level Levenshtein distances (python/stringzillas/similarities.c:36-140); ``uint64`` results."""
+
+    _kind = 0
+    _dtype = np.uint64
+    _prefix = "szs_levenshtein_distances"
+
+    def __init__(self, match: int = 0, mismatch: int = 1, open: int = 1, extend: int = 1, capabilities=None):
+        match, mismatch = _int8(match, "match"), _int8(mismatch, "mismatch")
+        open, extend = _int8(open, "open"), _int8(extend, "extend")
+        handle, error = ctypes.c_void_p(), ctypes.c_char_p()
+        status = getattr(_lib, f"{self._prefix}_init")(
+            ctypes.c_int8(match), ctypes.c_int8(mismatch), ctypes.c_int8(open), ctypes.c_int8(extend), None,
+            ctypes.c_int(_active_capabilities), ctypes.byref(handle), ctypes.byref(error))
+        self._finish_init(status, handle, error, f"{match},{mismatch},{open},{extend}")
+
+
+class LevenshteinDistancesUTF8(LevenshteinDistances):
+    """Codepoint-level Levenshtein distances over UTF-8 (python/stringzillas/similarities.c:379-560)."""
+
+    _kind = 1
+    _prefix = "szs_levenshtein_distances_utf8"
+
+
+class NeedlemanWunschScores(_Engine):
+    """Global alignment scores (python/stringzillas/similarities.c:793-926); ``int64`` results."""
+
+    _kind = 2
+    _dtype = np.int64
+    _prefix = "szs_needleman_wunsch_scores"
+
+    def __init__(self, byte_to_class=None, class_substitution_costs=None, open: int = -1, extend: int = -1,
+                 capabilities=None):
+        if byte_to_class is None or class_substitution_costs is None:
+            raise TypeError(f"{type(self).__name__} requires 'byte_to_class' and 'class_substitution_costs'")
+        if not isinstance(byte_to_class, np.ndarray):
+            raise TypeError("byte_to_class must be a NumPy array")
+        if byte_to_class.ndim != 1 or byte_to_class.shape[0] != 256:
+            raise ValueError("byte_to_class must be a 256-element array")
+        if byte_to_class.dtype != np.uint8:
+            raise TypeError("byte_to_class must have uint8 dtype")
+        if not byte_to_class.flags.c_contiguous:
+            raise ValueError("byte_to_class must be a C-contiguous array. Use np.ascontiguousarray() to convert.")
+        if not isinstance(class_substitution_costs, np.ndarray):
+            raise TypeError("class_substitution_costs must be a NumPy array")
+        if class_substitution_costs.shape != (32, 32):
+            raise ValueError("class_substitution_costs must be a 32x32 array")
+        if class_substitution_costs.dtype != np.int8:
+            raise TypeError("class_substitution_costs must have int8 dtype")
+        if not class_substitution_costs.flags.c_contiguous:
+            raise ValueError("class_substitution_costs must be a C-contiguous array. Use np.ascontiguousarray() to convert.")
+        # The reference truncates open/extend to int8 silently (python/stringzillas/similarities.c:856-865).
+        open8 = int(np.int64(open).astype(np.int8))
+        extend8 = int(np.int64(extend).astype(np.int8))
+        self._tables = (byte_to_class, class_substitution_costs)
+        handle, error = ctypes.c_void_p(), ctypes.c_char_p()
+        status = getattr(_lib, f"{self._prefix}_init")(
+            ctypes.c_void_p(byte_to_class.ctypes.data), ctypes.c_void_p(class_substitution_costs.ctypes.data),
+            ctypes.c_int8(open8), ctypes.c_int8(extend8), None, ctypes.c_int(_active_capabilities),
+            ctypes.byref(handle), ctypes.byref(error))
+        self._finish_init(status, handle, error, f"open={open8},extend={extend8}")
+
+
+class SmithWatermanScores(NeedlemanWunschScores):
+    """Local alignment scores (python/stringzillas/similarities.c:1090-1520); ``int64`` results."""
+
+    _kind = 3
+    _prefix = "szs_smith_waterman_scores"
+
+
+#  Batches and serving (extensions) ----------------------------------------------------------------------------
+
+BATCH_PERSISTENT, BATCH_GRAPH, BATCH_PER_PAIR = 0, 1, 2
+
+
+class Batch:
+    """Several (engine, firsts, seconds) pair lists scored together in one persistent, length-binned launch.
+
+    ``mode``: ``BATCH_PERSISTENT`` (default), ``BATCH_GRAPH`` (captured once into a HIP graph, replayed by
+    ``run``) or ``BATCH_PER_PAIR`` (one kernel launch per pair: the per-call baseline). ``outputs`` may be NumPy
+    arrays or device tensors with 8-byte elements.
+    """
+
+    def __init__(self, segments, mode: int = BATCH_PERSISTENT, device: Optional[DeviceScope] = None, outputs=None):
+        self.scope = _scope(device)
+        self._keep = []
+        array = (_native.Segment * max(len(segments), 1))()
+        self.outputs = []
+        for i, (engine, firsts, seconds) in enumerate(segments):
+            firsts, seconds = _as_tape(firsts, "firsts"), _as_tape(seconds, "seconds")
+            if outputs is not None:
+                out = outputs[i]
+            else:
+                out = np.zeros(len(firsts), dtype=engine._dtype)
+            pointer = out.data_ptr() if hasattr(out, "data_ptr") else out.ctypes.data
+            a, b = firsts.tape(), seconds.tape()
+            self._keep += [firsts, seconds, a, b, out]
+            array[i].kind = engine._kind
+            array[i].engine = engine.handle
+            array[i].firsts = ctypes.pointer(a)
+            array[i].seconds = ctypes.pointer(b)
+            array[i].results = ctypes.c_void_p(pointer)
+            self.outputs.append(out)
+        self._engines = [s[0] for s in segments]
+        handle, error = ctypes.c_void_p(), ctypes.c_char_p()
+        status = _lib.szs_batch_init(self.scope.handle, array, ctypes.c_size_t(len(segments)), ctypes.c_int(mode),
+                                     ctypes.byref(handle), ctypes.byref(error))
+        _check(status, error, "Batch preparation")
+        self.handle = handle
+
+    @property
+    def cells(self) -> int:
+        return int(_lib.szs_batch_cells(self.handle))
+
+    @property
+    def last_kernel_ms(self) -> float:
+        return float(_lib.szs_batch_last_kernel_ms(self.handle))
+
+    def launch(self) -> None:
+        error = ctypes.c_char_p()
+        _check(_lib.szs_batch_launch(self.handle, ctypes.byref(error)), error, "Batch launch")
+
+    def wait(self) -> None:
+        error = ctypes.c_char_p()
+        _check(_lib.szs_batch_wait(self.handle, ctypes.byref(error)), error, "Batch")
+
+    def run(self):
+        self.launch()
+        self.wait()
+        return self.outputs
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            _lib.szs_batch_free(self.handle)
+            self.handle = None
+
+
+class Server:
+    """Continuous-batching request loop (native, in the library): ``submit`` returns a ticket, ``wait`` joins it."""
+
+    def __init__(self, hbm_budget_bytes: int = 64 << 30, max_inflight_rounds: int = 2,
+                 device: Optional[DeviceScope] = None):
+        self.scope = _scope(device)
+        handle, error = ctypes.c_void_p(), ctypes.c_char_p()
+        status = _lib.szs_server_init(self.scope.handle, ctypes.c_size_t(hbm_budget_bytes),
+                                      ctypes.c_size_t(max_inflight_rounds), ctypes.byref(handle), ctypes.byref(error))
+        _check(status, error, "Server initialization")
+        self.handle = handle
+        self._requests = {}
+
+    def submit(self, engine: _Engine, firsts, seconds) -> int:
+        firsts, seconds = _as_tape(firsts, "firsts"), _as_tape(seconds, "seconds")
+        out = np.zeros(len(firsts), dtype=engine._dtype)
+        a, b = firsts.tape(), seconds.tape()
+        segment = _native.Segment()
+        segment.kind = engine._kind
+        segment.engine = engine.handle
+        segment.firsts = ctypes.pointer(a)
+        segment.seconds = ctypes.pointer(b)
+        segment.results = ctypes.c_void_p(out.ctypes.data)
+        ticket = ctypes.c_uint64()
+        error = ctypes.c_char_p()
+        _check(_lib.szs_server_submit(self.handle, ctypes.byref(segment), ctypes.byref(ticket), ctypes.byref(error)),
+               error, "Server submission")
+        self._requests[ticket.value] = (firsts, seconds, a, b, out, engine)
+        return ticket.value
+
+    def wait(self, ticket: int) -> np.ndarray:
+        error = ctypes.c_char_p()
+        status = _lib.szs_server_wait(self.handle, ctypes.c_uint64(ticket), ctypes.byref(error))
+        kept = self._requests.pop(ticket)
+        _check(status, error, "Server request")
+        return kept[4]
+
+    def poll(self, ticket: int) -> bool:
+        return bool(_lib.szs_server_poll(self.handle, ctypes.c_uint64(ticket)))
+
+    @property
+    def bytes_in_use(self) -> int:
+        return int(_lib.szs_server_bytes_in_use(self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            _lib.szs_server_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+__all__ = [
+    "Strs", "DeviceStrs", "to_device", "DeviceScope", "LevenshteinDistances", "LevenshteinDistancesUTF8",
+    "NeedlemanWunschScores", "SmithWatermanScores", "Batch", "Server", "reset_capabilities", "SzsError",
+    "BATCH_PERSISTENT", "BATCH_GRAPH", "BATCH_PER_PAIR", "__capabilities__", "__version__",
+]

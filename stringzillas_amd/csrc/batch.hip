@@ -1,0 +1,502 @@
+/**
+ *  @file batch.hip
+ *  @brief Batch preparation and launch: tape staging into HBM, device-side materialization and length binning,
+ *         the persistent scoring launch, its HIP-graph replay, and the one-launch-per-pair baseline.
+ *
+ *  `batch_prepare` may synchronize (it sizes buffers from the materialized tasks); `batch_launch` only enqueues
+ *  device work on the scope's stream and is what a graph captures.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "szs_runtime.hpp"
+
+
+extern "C" {
+hipError_t szs_amd_launch_fill(uint32_t *data, uint32_t count, uint32_t value, hipStream_t stream);
+hipError_t szs_amd_launch_utf8_decode(uint8_t const *bytes, uint64_t const *offsets, uint64_t count, uint32_t *runes,
+                                      uint32_t *rune_counts, hipStream_t stream);
+hipError_t szs_amd_launch_widen_offsets(uint32_t const *narrow, uint64_t *wide, uint64_t count, hipStream_t stream);
+hipError_t szs_amd_launch_materialize(szs_amd::materialize_args_t const *args, szs_amd::shape_table_t const *table,
+                                      szs_amd::task_t *tasks, uint32_t *bin_of_task, uint32_t *histogram,
+                                      szs_amd::segment_view_t const *segments, hipStream_t stream);
+hipError_t szs_amd_launch_scan_bins(szs_amd::bins_view_t const *bins, hipStream_t stream);
+hipError_t szs_amd_launch_scatter_bins(szs_amd::bins_view_t const *bins, uint32_t const *bin_of_task,
+                                       uint32_t const *position_of_bin, uint32_t tasks_count, hipStream_t stream);
+hipError_t szs_amd_launch_persistent(szs_amd::task_t const *tasks, szs_amd::segment_view_t const *segments,
+                                     szs_amd::bins_view_t const *bins, int32_t *scratch, uint32_t scratch_capacity,
+                                     uint32_t waves, uint32_t lds_bytes, hipStream_t stream);
+hipError_t szs_amd_launch_single(szs_amd::task_t const *tasks, szs_amd::segment_view_t const *segments,
+                                 uint32_t const *identity, uint32_t task_index, int variant, int32_t *scratch,
+                                 uint32_t scratch_capacity, uint32_t lds_bytes, hipStream_t stream);
+hipError_t szs_amd_persistent_occupancy(int *blocks_per_cu, uint32_t lds_bytes);
+}
+
+namespace szs_amd {
+
+#define SZS_TRY_HIP(expr, context)                                                                                   \
+    do {                                                                                                             \
+        hipError_t const error_ = (expr);                                                                            \
+        if (error_ != hipSuccess) return from_hip(error_, error_message, context);                                   \
+    } while (0)
+
+#pragma region Static tables
+
+/** Default shape choice: the smallest single-pass shape that holds all rows, else 1024-row stripes. */
+static shape_table_t default_shape_table() {
+    shape_table_t table;
+    std::memset(&table, 0, sizeof(table));
+    std::vector<int> by_capacity(shapes_count_k);
+    std::iota(by_capacity.begin(), by_capacity.end(), 0);
+    std::stable_sort(by_capacity.begin(), by_capacity.end(), [](int a, int b) {
+        return shapes_k[a].group * shapes_k[a].rows_per_lane < shapes_k[b].group * shapes_k[b].rows_per_lane;
+    });
+    for (int f = 0; f < families_count_k; ++f)
+        for (int s = 0; s < 2; ++s) {
+            for (int i = 0; i < shapes_count_k; ++i) {
+                int const shape = by_capacity[i];
+                table.limits[f][s][shape] = shapes_k[shape].group * shapes_k[shape].rows_per_lane;
+                table.order[f][s][i] = shape;
+            }
+            table.used[f][s] = shapes_count_k;
+            table.multi_pass_shape[f][s] = shapes_count_k - 1;
+        }
+    return table;
+}
+
+static shape_table_t &active_shape_table() {
+    static shape_table_t table = default_shape_table();
+    return table;
+}
+
+/** Rough cost of one work item of a bin, for heaviest-first ordering of the persistent queue. */
+static double bin_item_cost(uint32_t bin) {
+    int const variant = (int)(bin / column_buckets_k);
+    int const bucket = (int)(bin % column_buckets_k);
+    int const shape = variant % shapes_count_k;
+    int const family = variant / shapes_count_k / 2;
+    double const rows = (double)shapes_k[shape].group * shapes_k[shape].rows_per_lane;
+    double const per_cell = (family == family_lev_affine_k || family == family_nw_affine_k ||
+                             family == family_sw_affine_k) ? 1.8 : 1.0;
+    return rows / 64.0 * std::pow(2.0, bucket / 2.0) * per_cell;
+}
+
+static std::vector<uint32_t> const &bin_priority() {
+    static std::vector<uint32_t> priority = [] {
+        std::vector<uint32_t> order(bins_count_k);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(),
+                         [](uint32_t a, uint32_t b) { return bin_item_cost(a) > bin_item_cost(b); });
+        return order;
+    }();
+    return priority;
+}
+
+#pragma endregion
+
+#pragma region Tape staging
+
+template <typename offset_type_>
+static sz_status_t stage_tape(sz_cptr_t data, offset_type_ const *offsets, uint64_t count, bool runes,
+                              staged_tape_t &tape, hipStream_t stream, char const **error_message) {
+    tape.count = count;
+    bool const device_offsets = is_device_pointer(offsets);
+    // Offsets as u64 on the device.
+    if (count == 0 || !offsets) {
+        SZS_TRY_HIP(tape.offsets_copy.reserve(16), "Failed to allocate tape offsets");
+        SZS_TRY_HIP(hipMemsetAsync(tape.offsets_copy.data, 0, 16, stream), "hipMemsetAsync failed");
+        tape.offsets = tape.offsets_copy.as<uint64_t>();
+        tape.bytes = 0;
+    }
+    else if (sizeof(offset_type_) == 8 && device_offsets) {
+        tape.offsets = reinterpret_cast<uint64_t const *>(offsets);
+        uint64_t last = 0;
+        SZS_TRY_HIP(hipMemcpyAsync(&last, offsets + count, 8, hipMemcpyDeviceToHost, stream), "offset readback");
+        SZS_TRY_HIP(hipStreamSynchronize(stream), "offset readback");
+        tape.bytes = last;
+    }
+    else if (device_offsets) { // u32 offsets on the device
+        SZS_TRY_HIP(tape.offsets_copy.reserve((count + 1) * 8), "Failed to allocate tape offsets");
+        SZS_TRY_HIP(szs_amd_launch_widen_offsets(reinterpret_cast<uint32_t const *>(offsets),
+                                                 tape.offsets_copy.as<uint64_t>(), count + 1, stream),
+                    "offset widening");
+        tape.offsets = tape.offsets_copy.as<uint64_t>();
+        uint32_t last = 0;
+        SZS_TRY_HIP(hipMemcpyAsync(&last, offsets + count, 4, hipMemcpyDeviceToHost, stream), "offset readback");
+        SZS_TRY_HIP(hipStreamSynchronize(stream), "offset readback");
+        tape.bytes = last;
+    }
+    else {
+        std::vector<uint64_t> wide(count + 1);
+        for (uint64_t i = 0; i <= count; ++i) wide[i] = (uint64_t)offsets[i];
+        SZS_TRY_HIP(tape.offsets_copy.reserve((count + 1) * 8), "Failed to allocate tape offsets");
+        SZS_TRY_HIP(hipMemcpy(tape.offsets_copy.data, wide.data(), (count + 1) * 8, hipMemcpyHostToDevice),
+                    "tape offsets upload");
+        tape.offsets = tape.offsets_copy.as<uint64_t>();
+        tape.bytes = wide[count];
+    }
+    // Data.
+    if (tape.bytes == 0 || is_device_pointer(data)) tape.data = reinterpret_cast<uint8_t const *>(data);
+    else {
+        SZS_TRY_HIP(tape.data_copy.reserve(tape.bytes + 16), "Failed to allocate the tape in HBM");
+        SZS_TRY_HIP(hipMemcpy(tape.data_copy.data, data, tape.bytes, hipMemcpyHostToDevice), "tape upload");
+        tape.data = tape.data_copy.as<uint8_t>();
+    }
+    if (!tape.data) { // empty strings only: any valid device address works, nothing is read
+        SZS_TRY_HIP(tape.data_copy.reserve(16), "Failed to allocate the tape in HBM");
+        tape.data = tape.data_copy.as<uint8_t>();
+    }
+    if (runes) {
+        SZS_TRY_HIP(tape.runes_buffer.reserve(std::max<uint64_t>(tape.bytes, 1) * 4), "Failed to allocate runes");
+        SZS_TRY_HIP(tape.rune_counts_buffer.reserve(std::max<uint64_t>(count, 1) * 4), "Failed to allocate runes");
+        tape.runes = tape.runes_buffer.as<uint32_t>();
+        tape.rune_counts = tape.rune_counts_buffer.as<uint32_t>();
+    }
+    return ok(error_message);
+}
+
+static sz_status_t stage_input(segment_input_t const &input, bool queries, bool runes, staged_tape_t &tape,
+                               hipStream_t stream, char const **error_message) {
+    if (queries) {
+        if (input.queries)
+            return stage_tape(input.queries->data, input.queries->offsets, input.queries->count, runes, tape, stream,
+                              error_message);
+        return stage_tape(input.queries32->data, input.queries32->offsets, input.queries32->count, runes, tape,
+                          stream, error_message);
+    }
+    if (input.candidates)
+        return stage_tape(input.candidates->data, input.candidates->offsets, input.candidates->count, runes, tape,
+                          stream, error_message);
+    return stage_tape(input.candidates32->data, input.candidates32->offsets, input.candidates32->count, runes, tape,
+                      stream, error_message);
+}
+
+#pragma endregion
+
+batch_t::~batch_t() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (kernel_begin) (void)hipEventDestroy(kernel_begin);
+    if (kernel_end) (void)hipEventDestroy(kernel_end);
+    for (auto *tape : tapes) delete tape;
+    for (auto *buffer : result_buffers) delete buffer;
+}
+
+static bins_view_t bins_view(batch_t &b) {
+    bins_view_t bins;
+    bins.histogram = b.histogram.as<uint32_t>();
+    bins.cursor = b.cursor.as<uint32_t>();
+    bins.task_start = b.task_start.as<uint32_t>();
+    bins.item_start = b.item_start.as<uint32_t>();
+    bins.priority = b.priority.as<uint32_t>();
+    bins.order = b.order.as<uint32_t>();
+    bins.queue_head = b.queue_head.as<uint32_t>();
+    bins.shape_of_task = b.shape_of_task.as<uint32_t>();
+    return bins;
+}
+
+/** Device work up to (excluding) scoring: clear counters, transcode UTF-8, materialize, bin. */
+static sz_status_t enqueue_front(batch_t &b, bool bin, char const **error_message) {
+    hipStream_t const stream = b.stream;
+    SZS_TRY_HIP(szs_amd_launch_fill(b.histogram.as<uint32_t>(), bins_count_k, 0, stream), "fill");
+    SZS_TRY_HIP(szs_amd_launch_fill(b.cursor.as<uint32_t>(), bins_count_k, 0, stream), "fill");
+    SZS_TRY_HIP(szs_amd_launch_fill(b.queue_head.as<uint32_t>(), 1, 0, stream), "fill");
+    for (auto *tape : b.tapes)
+        if (tape && tape->runes)
+            SZS_TRY_HIP(szs_amd_launch_utf8_decode(tape->data, tape->offsets, tape->count, tape->runes,
+                                                   tape->rune_counts, stream),
+                        "UTF-8 decoding");
+    auto const *args = reinterpret_cast<materialize_args_t const *>(b.materialize_args.data());
+    for (size_t s = 0; s < b.inputs.size(); ++s)
+        SZS_TRY_HIP(szs_amd_launch_materialize(&args[s], b.shape_table.as<shape_table_t>(), b.tasks.as<task_t>(),
+                                               b.bin_of_task.as<uint32_t>(), b.histogram.as<uint32_t>(),
+                                               b.segments.as<segment_view_t>(), stream),
+                    "materialization");
+    if (bin) {
+        bins_view_t const bins = bins_view(b);
+        SZS_TRY_HIP(szs_amd_launch_scan_bins(&bins, stream), "bin scan");
+        SZS_TRY_HIP(szs_amd_launch_scatter_bins(&bins, b.bin_of_task.as<uint32_t>(),
+                                                b.position_of_bin.as<uint32_t>(), (uint32_t)b.tasks_count, stream),
+                    "bin scatter");
+    }
+    return ok(error_message);
+}
+
+static sz_status_t enqueue_scoring(batch_t &b, char const **error_message) {
+    hipStream_t const stream = b.stream;
+    SZS_TRY_HIP(hipEventRecord(b.kernel_begin, stream), "event");
+    if (b.mode == szs_batch_per_pair_k) {
+        for (size_t i = 0; i < b.per_pair_tasks.size(); ++i)
+            SZS_TRY_HIP(szs_amd_launch_single(b.tasks.as<task_t>(), b.segments.as<segment_view_t>(),
+                                              b.identity.as<uint32_t>(), b.per_pair_tasks[i], b.per_pair_variants[i],
+                                              b.scratch.as<int32_t>(), b.scratch_capacity, b.lds_bytes, stream),
+                        "per-pair launch");
+    }
+    else {
+        bins_view_t const bins = bins_view(b);
+        SZS_TRY_HIP(szs_amd_launch_persistent(b.tasks.as<task_t>(), b.segments.as<segment_view_t>(), &bins,
+                                              b.scratch.as<int32_t>(), b.scratch_capacity, b.waves, b.lds_bytes,
+                                              stream),
+                    "persistent scoring launch");
+    }
+    SZS_TRY_HIP(hipEventRecord(b.kernel_end, stream), "event");
+    return ok(error_message);
+}
+
+static sz_status_t enqueue_results(batch_t &b, char const **error_message) {
+    for (size_t s = 0; s < b.inputs.size(); ++s) {
+        if (!b.host_results[s] || !b.results_elements[s]) continue;
+        if (b.inputs[s].mode == segment_pairs_k)
+            SZS_TRY_HIP(hipMemcpyAsync(b.host_results[s], b.device_results[s], b.results_elements[s] * 8,
+                                       hipMemcpyDeviceToHost, b.stream),
+                        "results download");
+        else
+            SZS_TRY_HIP(hipMemcpy2DAsync(b.host_results[s], b.inputs[s].results_row_stride * 8, b.device_results[s],
+                                         b.inputs[s].results_row_stride * 8, b.result_columns[s] * 8,
+                                         b.result_rows[s], hipMemcpyDeviceToHost, b.stream),
+                        "results download");
+    }
+    return ok(error_message);
+}
+
+sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> const &inputs, szs_batch_mode_t mode,
+                          batch_t **out, char const **error_message, hipStream_t stream_override) {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    sz_status_t status = resolve_gpu(scope, &device, &stream, error_message);
+    if (status != sz_success_k) return status;
+    if (stream_override) stream = stream_override;
+    batch_t *batch = new (std::nothrow) batch_t();
+    if (!batch) return fail(sz_bad_alloc_k, error_message, "Failed to allocate a batch");
+    struct guard_t {
+        batch_t *&b;
+        ~guard_t() { delete b; }
+    } guard {batch};
+    batch_t &b = *batch;
+    b.device = device;
+    b.stream = stream;
+    b.mode = mode;
+    b.inputs = inputs;
+    size_t const segments_count = inputs.size();
+    b.host_results.assign(segments_count, nullptr);
+    b.results_elements.assign(segments_count, 0);
+    b.result_rows.assign(segments_count, 0);
+    b.result_columns.assign(segments_count, 0);
+    b.device_results.assign(segments_count, nullptr);
+    b.materialize_args.resize(sizeof(materialize_args_t) * std::max<size_t>(segments_count, 1));
+    auto *args = reinterpret_cast<materialize_args_t *>(b.materialize_args.data());
+
+    std::vector<engine_params_t> params(segments_count);
+    std::vector<segment_view_t> views(segments_count);
+    uint64_t tasks_total = 0;
+    uint32_t lds_bytes = 0;
+    for (size_t s = 0; s < segments_count; ++s) {
+        segment_input_t const &input = inputs[s];
+        engine_t const *engine = input.engine;
+        bool const runes = engine->kind == szs_engine_levenshtein_utf8_k;
+        auto *queries = new (std::nothrow) staged_tape_t();
+        if (!queries) return fail(sz_bad_alloc_k, error_message, "Failed to allocate a tape");
+        b.tapes.push_back(queries);
+        status = stage_input(input, true, runes, *queries, stream, error_message);
+        if (status != sz_success_k) return status;
+        staged_tape_t *candidates = queries;
+        if (input.mode != segment_symmetric_k) {
+            candidates = new (std::nothrow) staged_tape_t();
+            if (!candidates) return fail(sz_bad_alloc_k, error_message, "Failed to allocate a tape");
+            b.tapes.push_back(candidates);
+            status = stage_input(input, false, runes, *candidates, stream, error_message);
+            if (status != sz_success_k) return status;
+        }
+        uint64_t const q = queries->count, c = candidates->count;
+        uint64_t tasks = 0, elements = 0;
+        if (input.mode == segment_pairs_k) {
+            if (q != c) return fail(sz_unexpected_dimensions_k, error_message, "Pair tapes must hold equal counts");
+            tasks = q, elements = q;
+            b.result_rows[s] = q, b.result_columns[s] = 1;
+        }
+        else {
+            if (input.results_row_stride < c)
+                return fail(sz_unexpected_dimensions_k, error_message, "Result row stride is below the column count");
+            tasks = input.mode == segment_cross_k ? q * c : q * (q + 1) / 2;
+            elements = q && c ? (q - 1) * input.results_row_stride + c : 0;
+            b.result_rows[s] = q, b.result_columns[s] = c;
+        }
+        b.results_elements[s] = elements;
+        if (!input.results && elements) return fail(sz_unexpected_dimensions_k, error_message, "Results are null");
+        if (elements && !is_device_pointer(input.results)) {
+            auto *buffer = new (std::nothrow) device_buffer_t();
+            if (!buffer) return fail(sz_bad_alloc_k, error_message, "Failed to allocate results");
+            b.result_buffers.push_back(buffer);
+            SZS_TRY_HIP(buffer->reserve(elements * 8), "Failed to allocate results in HBM");
+            b.host_results[s] = input.results;
+            b.device_results[s] = buffer->as<int64_t>();
+            if (input.mode != segment_pairs_k) // cells outside the matrix stay as the caller left them
+                SZS_TRY_HIP(hipMemcpy2D(buffer->data, input.results_row_stride * 8, input.results,
+                                        input.results_row_stride * 8, c * 8, q, hipMemcpyHostToDevice),
+                            "results upload");
+        }
+        else b.device_results[s] = reinterpret_cast<int64_t *>(input.results);
+
+        params[s] = engine->params;
+        views[s].params = nullptr; // patched after upload
+        views[s].query_symbols = runes ? (void const *)queries->runes : (void const *)queries->data;
+        views[s].candidate_symbols = runes ? (void const *)candidates->runes : (void const *)candidates->data;
+        views[s].results = b.device_results[s];
+        views[s].symbols = runes ? symbols_u32_k : symbols_u8_k;
+        views[s].family = engine->params.family;
+
+        materialize_args_t &a = args[s];
+        std::memset(&a, 0, sizeof(a));
+        a.query_offsets = queries->offsets;
+        a.candidate_offsets = input.mode == segment_symmetric_k ? nullptr : candidates->offsets;
+        a.query_rune_counts = queries->rune_counts;
+        a.candidate_rune_counts = input.mode == segment_symmetric_k ? nullptr : candidates->rune_counts;
+        a.queries_count = q;
+        a.candidates_count = c;
+        a.results_row_stride = input.results_row_stride;
+        a.tasks_count = tasks;
+        a.task_base = (uint32_t)tasks_total;
+        a.segment = (uint32_t)s;
+        a.family = engine->params.family;
+        a.symbols = views[s].symbols;
+        a.mode = input.mode;
+        a.open = engine->params.open;
+        a.extend = engine->params.extend;
+        tasks_total += tasks;
+        if (tasks_total >= 0xFFFFFFF0ull)
+            return fail(sz_overflow_risk_k, error_message, "A batch holds at most 2^32 - 16 scored cells");
+        if (engine->params.family >= family_nw_linear_k)
+            lds_bytes = std::max<uint32_t>(lds_bytes, (uint32_t)engine->params.dense_count * 64u * 16u);
+    }
+    b.tasks_count = tasks_total;
+    b.lds_bytes = lds_bytes;
+
+    // Device-side tables.
+    SZS_TRY_HIP(b.params.reserve(sizeof(engine_params_t) * std::max<size_t>(segments_count, 1)), "allocation");
+    SZS_TRY_HIP(hipMemcpy(b.params.data, params.data(), sizeof(engine_params_t) * segments_count,
+                          hipMemcpyHostToDevice), "upload");
+    for (size_t s = 0; s < segments_count; ++s) views[s].params = b.params.as<engine_params_t>() + s;
+    SZS_TRY_HIP(b.segments.reserve(sizeof(segment_view_t) * std::max<size_t>(segments_count, 1)), "allocation");
+    SZS_TRY_HIP(hipMemcpy(b.segments.data, views.data(), sizeof(segment_view_t) * segments_count,
+                          hipMemcpyHostToDevice), "upload");
+    SZS_TRY_HIP(b.shape_table.reserve(sizeof(shape_table_t)), "allocation");
+    SZS_TRY_HIP(hipMemcpy(b.shape_table.data, &active_shape_table(), sizeof(shape_table_t), hipMemcpyHostToDevice),
+                "upload");
+    std::vector<uint32_t> const &priority = bin_priority();
+    std::vector<uint32_t> position(bins_count_k);
+    for (uint32_t p = 0; p < bins_count_k; ++p) position[priority[p]] = p;
+    SZS_TRY_HIP(b.priority.reserve(bins_count_k * 4), "allocation");
+    SZS_TRY_HIP(hipMemcpy(b.priority.data, priority.data(), bins_count_k * 4, hipMemcpyHostToDevice), "upload");
+    SZS_TRY_HIP(b.position_of_bin.reserve(bins_count_k * 4), "allocation");
+    SZS_TRY_HIP(hipMemcpy(b.position_of_bin.data, position.data(), bins_count_k * 4, hipMemcpyHostToDevice), "upload");
+    uint64_t const slots = std::max<uint64_t>(tasks_total, 1);
+    SZS_TRY_HIP(b.tasks.reserve(slots * sizeof(task_t)), "Failed to allocate tasks");
+    SZS_TRY_HIP(b.bin_of_task.reserve(slots * 4), "Failed to allocate tasks");
+    SZS_TRY_HIP(b.order.reserve(slots * 4), "Failed to allocate tasks");
+    SZS_TRY_HIP(b.shape_of_task.reserve(slots * 4), "Failed to allocate tasks");
+    SZS_TRY_HIP(b.histogram.reserve(bins_count_k * 4), "allocation");
+    SZS_TRY_HIP(b.cursor.reserve(bins_count_k * 4), "allocation");
+    SZS_TRY_HIP(b.task_start.reserve(bins_count_k * 4), "allocation");
+    SZS_TRY_HIP(b.item_start.reserve((bins_count_k + 1) * 4), "allocation");
+    SZS_TRY_HIP(b.queue_head.reserve(64), "allocation");
+    SZS_TRY_HIP(hipEventCreate(&b.kernel_begin), "event");
+    SZS_TRY_HIP(hipEventCreate(&b.kernel_end), "event");
+
+    // Materialize once to size the scratch rows, check the reach bound and count the cells.
+    status = enqueue_front(b, false, error_message);
+    if (status != sz_success_k) return status;
+    std::vector<task_t> tasks(tasks_total);
+    std::vector<uint32_t> bins(tasks_total);
+    if (tasks_total) {
+        SZS_TRY_HIP(hipMemcpyAsync(tasks.data(), b.tasks.data, tasks_total * sizeof(task_t), hipMemcpyDeviceToHost,
+                                   stream), "task readback");
+        SZS_TRY_HIP(hipMemcpyAsync(bins.data(), b.bin_of_task.data, tasks_total * 4, hipMemcpyDeviceToHost, stream),
+                    "task readback");
+    }
+    SZS_TRY_HIP(hipStreamSynchronize(stream), "materialization");
+    uint64_t widest_multi_pass = 0;
+    for (uint64_t t = 0; t < tasks_total; ++t) {
+        if (bins[t] == ~0u) continue;
+        task_t const &task = tasks[t];
+        engine_t const *engine = inputs[task.segment].engine;
+        int const variant = (int)(bins[t] / column_buckets_k);
+        shape_t const shape = shapes_k[variant % shapes_count_k];
+        uint64_t const n = task.rows_length, m = task.columns_length;
+        b.cells += n * m;
+        if (n > (uint64_t)shape.group * shape.rows_per_lane) widest_multi_pass = std::max(widest_multi_pass, m);
+        // Reach bound of serial.hpp:117-140, applied to int32 cells.
+        int const family = engine->params.family;
+        bool const affine = family == family_lev_affine_k || family == family_nw_affine_k ||
+                            family == family_sw_affine_k;
+        uint64_t const magnitude = (uint64_t)std::max(1, std::max(engine->substitution_magnitude, engine->gap_magnitude));
+        uint64_t const span = family <= family_lev_affine_k ? std::max(n, m) : n + m;
+        if ((span + (affine ? 3 : 1)) * magnitude >= 0x7FFFFFFFull)
+            return fail(sz_overflow_risk_k, error_message, "Scores of this pair would overflow 32-bit cells");
+        if (mode == szs_batch_per_pair_k) {
+            b.per_pair_tasks.push_back((uint32_t)t);
+            b.per_pair_variants.push_back(variant);
+        }
+    }
+    b.scratch_capacity = (uint32_t)(widest_multi_pass + 128);
+    int device_count = 0, compute_units = 0, blocks_per_cu = 0;
+    (void)device_count;
+    SZS_TRY_HIP(hipDeviceGetAttribute(&compute_units, hipDeviceAttributeMultiprocessorCount, device), "attributes");
+    SZS_TRY_HIP(szs_amd_persistent_occupancy(&blocks_per_cu, lds_bytes), "occupancy");
+    blocks_per_cu = std::max(1, std::min(blocks_per_cu, 32));
+    b.waves = mode == szs_batch_per_pair_k ? 1u : (uint32_t)(compute_units * blocks_per_cu);
+    uint64_t const scratch_waves = widest_multi_pass ? b.waves : 1;
+    SZS_TRY_HIP(b.scratch.reserve(scratch_waves * 2ull * b.scratch_capacity * 4 + 256), "Failed to allocate scratch");
+    if (mode == szs_batch_per_pair_k) {
+        std::vector<uint32_t> identity(slots);
+        std::iota(identity.begin(), identity.end(), 0u);
+        SZS_TRY_HIP(b.identity.reserve(slots * 4), "allocation");
+        SZS_TRY_HIP(hipMemcpy(b.identity.data, identity.data(), slots * 4, hipMemcpyHostToDevice), "upload");
+    }
+    if (mode == szs_batch_graph_k) {
+        SZS_TRY_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "graph capture");
+        sz_status_t front = enqueue_front(b, true, error_message);
+        sz_status_t scoring = front == sz_success_k ? enqueue_scoring(b, error_message) : front;
+        hipGraph_t graph = nullptr;
+        hipError_t const ended = hipStreamEndCapture(stream, &graph);
+        if (scoring != sz_success_k) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return scoring;
+        }
+        SZS_TRY_HIP(ended, "graph capture");
+        b.graph = graph;
+        SZS_TRY_HIP(hipGraphInstantiate(&b.graph_exec, graph, nullptr, nullptr, 0), "graph instantiation");
+    }
+    *out = batch;
+    batch = nullptr; // ownership moves to the caller
+    return ok(error_message);
+}
+
+sz_status_t batch_launch(batch_t *batch, char const **error_message) {
+    batch_t &b = *batch;
+    SZS_TRY_HIP(hipSetDevice(b.device), "hipSetDevice failed");
+    if (b.mode == szs_batch_graph_k) SZS_TRY_HIP(hipGraphLaunch(b.graph_exec, b.stream), "graph launch");
+    else {
+        sz_status_t status = enqueue_front(b, b.mode != szs_batch_per_pair_k, error_message);
+        if (status != sz_success_k) return status;
+        status = enqueue_scoring(b, error_message);
+        if (status != sz_success_k) return status;
+    }
+    return enqueue_results(b, error_message);
+}
+
+sz_status_t batch_wait(batch_t *batch, char const **error_message) {
+    batch_t &b = *batch;
+    SZS_TRY_HIP(hipSetDevice(b.device), "hipSetDevice failed");
+    SZS_TRY_HIP(hipStreamSynchronize(b.stream), "scoring failed on the device");
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, b.kernel_begin, b.kernel_end) == hipSuccess) b.last_kernel_ms = ms;
+    else (void)hipGetLastError();
+    return ok(error_message);
+}
+
+} // namespace szs_amd

@@ -1,0 +1,190 @@
+/**
+ *  @file runtime.hip
+ *  @brief Host runtime of the MI355X similarity engine: device scopes, engines, tape staging, batch preparation
+ *         (materialize, bin, persistent launch, graph capture, per-pair baseline) and the `szs_*` C ABI.
+ *
+ *  The C ABI mirrors include/stringzillas/stringzillas.h of the reference (names, signatures, statuses); the
+ *  engine selection mirrors c/stringzillas/levenshtein.cuh:96-179 and needleman_wunsch.cuh:86-164 (`open == extend`
+ *  picks the linear recurrence). Everything scores on the GPU; there is no CPU scoring path in this library.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/stringzillas_amd.h"
+#include "szs_device.hpp"
+#include "szs_runtime.hpp"
+
+using namespace szs_amd;
+
+extern "C" {
+hipError_t szs_amd_launch_fill(uint32_t *data, uint32_t count, uint32_t value, hipStream_t stream);
+hipError_t szs_amd_launch_utf8_decode(uint8_t const *bytes, uint64_t const *offsets, uint64_t count, uint32_t *runes,
+                                      uint32_t *rune_counts, hipStream_t stream);
+hipError_t szs_amd_launch_widen_offsets(uint32_t const *narrow, uint64_t *wide, uint64_t count, hipStream_t stream);
+}
+
+namespace szs_amd {
+
+#pragma region Status helpers
+
+sz_status_t fail(sz_status_t status, char const **error_message, char const *text) {
+    if (error_message) *error_message = text;
+    return status;
+}
+
+sz_status_t ok(char const **error_message) {
+    if (error_message) *error_message = nullptr;
+    return sz_success_k;
+}
+
+sz_status_t from_hip(hipError_t error, char const **error_message, char const *context) {
+    if (error == hipSuccess) return ok(error_message);
+    (void)hipGetLastError();
+    if (error_message) *error_message = context;
+    if (error == hipErrorOutOfMemory || error == hipErrorMemoryAllocation) return sz_bad_alloc_k;
+    if (error == hipErrorNoDevice || error == hipErrorInvalidDevice) return sz_missing_gpu_k;
+    return sz_status_unknown_k;
+}
+
+#pragma endregion
+
+#pragma region Device buffers
+
+void device_buffer_t::release() {
+    if (data) (void)hipFree(data);
+    data = nullptr;
+    capacity = 0;
+}
+
+hipError_t device_buffer_t::reserve(size_t bytes) {
+    if (bytes <= capacity) return hipSuccess;
+    release();
+    size_t const rounded = (bytes + 4095) & ~size_t(4095);
+    hipError_t const error = hipMalloc(&data, rounded);
+    if (error != hipSuccess) {
+        data = nullptr;
+        (void)hipGetLastError();
+        return error;
+    }
+    capacity = rounded;
+    return hipSuccess;
+}
+
+bool is_device_pointer(void const *pointer) {
+    if (!pointer) return false;
+    hipPointerAttribute_t attributes;
+    hipError_t const error = hipPointerGetAttributes(&attributes, pointer);
+    if (error != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attributes.type == hipMemoryTypeDevice || attributes.type == hipMemoryTypeManaged ||
+           attributes.type == hipMemoryTypeUnified;
+}
+
+#pragma endregion
+
+#pragma region Scopes
+
+sz_status_t resolve_gpu(device_scope_t *scope, int *device, hipStream_t *stream, char const **error_message) {
+    if (!scope) return fail(sz_device_code_mismatch_k, error_message, "Device scope must be initialized");
+    if (scope->kind == scope_cpu_k)
+        return fail(sz_device_code_mismatch_k, error_message, "GPU engines cannot run on a CPU-cores device scope");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return fail(sz_missing_gpu_k, error_message, "No AMD GPU is visible to this process");
+    }
+    if (scope->gpu_device >= count) return fail(sz_missing_gpu_k, error_message, "GPU device index is out of range");
+    std::lock_guard<std::mutex> lock(scope->mutex);
+    hipError_t error = hipSetDevice(scope->gpu_device);
+    if (error != hipSuccess) return from_hip(error, error_message, "hipSetDevice failed");
+    if (!scope->stream) {
+        error = hipStreamCreateWithFlags(&scope->stream, hipStreamNonBlocking);
+        if (error != hipSuccess) return from_hip(error, error_message, "hipStreamCreate failed");
+    }
+    *device = scope->gpu_device;
+    *stream = scope->stream;
+    return ok(error_message);
+}
+
+#pragma endregion
+
+#pragma region Engines
+
+/** Maps `byte_to_class` onto the dense range of classes it actually uses, so LDS profiles stay small. */
+static sz_status_t densify(engine_t *engine, uint8_t const *byte_to_class, int8_t const *class_costs,
+                           char const **error_message) {
+    int dense_of_class[32];
+    int class_of_dense[32];
+    for (int c = 0; c < 32; ++c) dense_of_class[c] = -1;
+    int dense = 0;
+    for (int b = 0; b < 256; ++b) {
+        uint8_t const c = byte_to_class[b];
+        if (c >= 32)
+            return fail(sz_unexpected_dimensions_k, error_message, "byte_to_class values must be below 32");
+        if (dense_of_class[c] < 0) class_of_dense[dense] = c, dense_of_class[c] = dense++;
+        engine->params.byte_to_dense[b] = (uint8_t)dense_of_class[c];
+    }
+    engine->params.dense_count = dense;
+    std::memset(engine->params.dense_costs, 0, sizeof(engine->params.dense_costs));
+    for (int x = 0; x < dense; ++x)
+        for (int y = 0; y < dense; ++y)
+            engine->params.dense_costs[x * max_dense_classes_k + y] =
+                class_costs[class_of_dense[x] * 32 + class_of_dense[y]];
+    int magnitude = 0;
+    for (int i = 0; i < 1024; ++i) magnitude = std::max(magnitude, std::abs((int)class_costs[i]));
+    engine->substitution_magnitude = magnitude; // over all 1024 entries, as serial.hpp:180-186
+    return ok(error_message);
+}
+
+sz_status_t make_engine(int kind, int match, int mismatch, int open, int extend, uint8_t const *byte_to_class,
+                        int8_t const *class_costs, void **handle, char const **error_message) {
+    if (!handle) return fail(sz_unexpected_dimensions_k, error_message, "Engine output pointer is null");
+    engine_t *engine = new (std::nothrow) engine_t();
+    if (!engine) return fail(sz_bad_alloc_k, error_message, "Failed to allocate the engine");
+    std::memset(&engine->params, 0, sizeof(engine->params));
+    engine->kind = kind;
+    bool const linear = open == extend;
+    engine->params.open = open;
+    engine->params.extend = extend;
+    engine->params.match = match;
+    engine->params.mismatch = mismatch;
+    engine->gap_magnitude = std::max(std::abs(open), std::abs(extend));
+    if (kind == szs_engine_levenshtein_k || kind == szs_engine_levenshtein_utf8_k) {
+        if (match < 0 || mismatch < 0 || open < 0 || extend < 0) {
+            delete engine;
+            return fail(sz_overflow_risk_k, error_message,
+                        "Levenshtein costs must be non-negative: distances are kept in unsigned cells");
+        }
+        engine->params.family = linear ? family_lev_linear_k : family_lev_affine_k;
+        engine->params.dense_count = 0;
+        engine->substitution_magnitude = std::max(std::abs(match), std::abs(mismatch));
+    }
+    else {
+        if (!byte_to_class || !class_costs) {
+            delete engine;
+            return fail(sz_unexpected_dimensions_k, error_message, "Substitution tables must not be null");
+        }
+        sz_status_t const status = densify(engine, byte_to_class, class_costs, error_message);
+        if (status != sz_success_k) {
+            delete engine;
+            return status;
+        }
+        if (kind == szs_engine_needleman_wunsch_k) engine->params.family = linear ? family_nw_linear_k : family_nw_affine_k;
+        else engine->params.family = linear ? family_sw_linear_k : family_sw_affine_k;
+    }
+    *handle = engine;
+    return ok(error_message);
+}
+
+#pragma endregion
+
+} // namespace szs_amd

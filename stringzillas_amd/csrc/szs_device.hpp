@@ -1,0 +1,133 @@
+/**
+ *  @file szs_device.hpp
+ *  @brief Layouts shared by the host runtime and the gfx950 kernels: engine parameters, segments, tasks, bins.
+ *
+ *  Data layout in HBM for one batch:
+ *  - symbol tapes: the caller's byte tapes are used in place (or staged once); a UTF-8 segment additionally owns a
+ *    rune tape `u32[bytes]` where the runes of string `s` start at the same index as its bytes;
+ *  - `task_t[tasks]`: one 48-byte record per scored pair, rows = the shorter side;
+ *  - `u32 order[tasks]`: task indices grouped by bin, heaviest bins first (device-side counting sort);
+ *  - per-bin counters and prefix sums; a per-wave scratch row for pairs taller than one stripe.
+ */
+#pragma once
+#include <stdint.h>
+
+namespace szs_amd {
+
+/** Recurrence family of one engine instance, derived from its kind and gap costs. */
+enum family_t : int32_t {
+    family_lev_linear_k = 0, /**< minimize, uniform costs, linear gaps */
+    family_lev_affine_k = 1, /**< minimize, uniform costs, affine gaps */
+    family_nw_linear_k = 2,  /**< maximize, class costs, global, linear */
+    family_nw_affine_k = 3,  /**< maximize, class costs, global, affine */
+    family_sw_linear_k = 4,  /**< maximize, class costs, local, linear */
+    family_sw_affine_k = 5,  /**< maximize, class costs, local, affine */
+    families_count_k = 6,
+};
+
+/** Symbol width of a segment: bytes, or runes decoded from UTF-8. */
+enum symbols_t : int32_t { symbols_u8_k = 0, symbols_u32_k = 1 };
+
+constexpr int max_dense_classes_k = 32;
+
+/** Device copy of one engine's parameters. */
+struct engine_params_t {
+    int32_t family;
+    int32_t open, extend;     /**< gap costs (sign as given: positive for distances, negative for scores) */
+    int32_t match, mismatch;  /**< uniform substitution costs */
+    int32_t dense_count;      /**< number of distinct classes actually used by `byte_to_class` */
+    uint8_t byte_to_dense[256];
+    int8_t dense_costs[max_dense_classes_k * max_dense_classes_k]; /**< [dense(query)][dense(candidate)] */
+};
+
+/** One homogeneous slice of a batch, as the kernels see it. */
+struct segment_view_t {
+    engine_params_t const *params;
+    void const *query_symbols;     /**< u8 tape data, or u32 rune tape for UTF-8 */
+    void const *candidate_symbols; /**< same for candidates */
+    int64_t *results;
+    int32_t symbols;               /**< `symbols_t` */
+    int32_t family;
+};
+
+/** One scored pair. `rows` is the shorter side; `transposed` says the rows are the candidate. */
+struct task_t {
+    uint64_t rows_offset;
+    uint64_t columns_offset;
+    uint32_t rows_length;
+    uint32_t columns_length;
+    uint32_t segment;
+    uint32_t transposed;
+    uint64_t output;
+    uint64_t mirror; /**< second output slot of a symmetric cross-product, or ~0 */
+};
+static_assert(sizeof(task_t) == 48, "task layout");
+
+constexpr uint64_t no_mirror_k = ~0ull;
+
+/**
+ *  A kernel shape: `group` lanes share one pair (64 / group pairs per wave), each lane holds `rows_per_lane`
+ *  consecutive rows, so one pass covers `group * rows_per_lane` rows. Only 64-lane groups run more than one pass.
+ */
+struct shape_t {
+    int32_t group;
+    int32_t rows_per_lane;
+};
+
+/** The shapes compiled for every family; index = shape id. */
+constexpr int shapes_count_k = 9;
+constexpr shape_t shapes_k[shapes_count_k] = {
+    {16, 1}, {16, 2}, {16, 4},            // up to 16 / 32 / 64 rows, four pairs per wave
+    {64, 2}, {64, 3}, {64, 4}, {64, 6},   // up to 128 / 192 / 256 / 384 rows
+    {64, 8}, {64, 16},                    // up to 512 rows, then 1024-row stripes
+};
+
+/** Columns-length buckets per shape for binning: half-octaves of `columns_length`. */
+constexpr int column_buckets_k = 64;
+/** Variant = (family, symbols, shape); bins = variants x column buckets. */
+constexpr int variants_count_k = families_count_k * 2 * shapes_count_k;
+constexpr int bins_count_k = variants_count_k * column_buckets_k;
+
+__host__ __device__ inline int variant_of(int family, int symbols, int shape) {
+    return (family * 2 + symbols) * shapes_count_k + shape;
+}
+
+/** Per-batch counters and prefix sums the binning kernels fill and the persistent kernel reads. */
+struct bins_view_t {
+    uint32_t *histogram;     /**< [bins] tasks per bin */
+    uint32_t *cursor;        /**< [bins] scatter cursor */
+    uint32_t *task_start;    /**< [bins] first slot of the bin in `order`, in priority order */
+    uint32_t *item_start;    /**< [bins + 1] first work item of the bin (prefix over priority order) */
+    uint32_t *priority;      /**< [bins] bin ids sorted by estimated cost per item, heaviest first */
+    uint32_t *order;         /**< [tasks] task ids grouped by bin */
+    uint32_t *queue_head;    /**< persistent-kernel work counter */
+    uint32_t *shape_of_task; /**< [tasks] chosen shape id (kept for the per-pair mode) */
+};
+
+/** Shape choice per (family, symbols): single-pass shapes by ascending capacity, then the multi-pass shape. */
+struct shape_table_t {
+    int32_t limits[families_count_k][2][shapes_count_k]; /**< rows capacity per shape, -1 = unused */
+    int32_t order[families_count_k][2][shapes_count_k];  /**< shapes to try, ascending capacity */
+    int32_t used[families_count_k][2];                   /**< entries in `order` */
+    int32_t multi_pass_shape[families_count_k][2];       /**< shape for rows beyond every single-pass capacity */
+};
+
+/** Inputs of one segment for materialization. */
+struct materialize_args_t {
+    uint64_t const *query_offsets;
+    uint64_t const *candidate_offsets;   /**< null for the symmetric cross-product */
+    uint32_t const *query_rune_counts;   /**< UTF-8 only */
+    uint32_t const *candidate_rune_counts;
+    uint64_t queries_count;
+    uint64_t candidates_count;
+    uint64_t results_row_stride;
+    uint64_t tasks_count;                /**< pairs, or cells of the (triangular) cross-product */
+    uint32_t task_base;
+    uint32_t segment;
+    int32_t family;
+    int32_t symbols;
+    int32_t mode;                        /**< 0 pairs, 1 cross-product, 2 symmetric */
+    int32_t open, extend;
+};
+
+} // namespace szs_amd

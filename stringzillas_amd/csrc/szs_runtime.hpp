@@ -1,0 +1,126 @@
+/**
+ *  @file szs_runtime.hpp
+ *  @brief Host-side objects of the runtime: device buffers, scopes, engines and prepared batches.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+#include "../../include/stringzillas_amd.h"
+#include "szs_device.hpp"
+
+namespace szs_amd {
+
+sz_status_t fail(sz_status_t status, char const **error_message, char const *text);
+sz_status_t ok(char const **error_message);
+sz_status_t from_hip(hipError_t error, char const **error_message, char const *context);
+bool is_device_pointer(void const *pointer);
+
+/** Grow-only device allocation. */
+struct device_buffer_t {
+    void *data = nullptr;
+    size_t capacity = 0;
+    hipError_t reserve(size_t bytes);
+    void release();
+    template <typename type_>
+    type_ *as() const noexcept {
+        return reinterpret_cast<type_ *>(data);
+    }
+    ~device_buffer_t() { release(); }
+    device_buffer_t() = default;
+    device_buffer_t(device_buffer_t const &) = delete;
+    device_buffer_t &operator=(device_buffer_t const &) = delete;
+};
+
+enum scope_kind_t { scope_default_k = 0, scope_cpu_k = 1, scope_gpu_k = 2 };
+
+/** `szs_device_scope_t` target (c/stringzillas/stringzillas.cuh:241-321 in the reference). */
+struct device_scope_t {
+    scope_kind_t kind = scope_default_k;
+    int gpu_device = 0;
+    size_t cpu_cores = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mutex;
+};
+
+sz_status_t resolve_gpu(device_scope_t *scope, int *device, hipStream_t *stream, char const **error_message);
+
+/** Engine handle: host parameters plus the reach-bound inputs. */
+struct engine_t {
+    int kind = 0; /**< `szs_engine_kind_t` */
+    engine_params_t params;
+    int substitution_magnitude = 0;
+    int gap_magnitude = 0;
+    std::mutex mutex; /**< one caller at a time, as the reference's engines are not re-entrant */
+};
+
+sz_status_t make_engine(int kind, int match, int mismatch, int open, int extend, uint8_t const *byte_to_class,
+                        int8_t const *class_costs, void **handle, char const **error_message);
+
+/** A tape normalized for the device: u64 offsets, data and offsets both device-resident. */
+struct staged_tape_t {
+    uint8_t const *data = nullptr;
+    uint64_t const *offsets = nullptr;
+    uint64_t count = 0;
+    uint64_t bytes = 0;             /**< offsets[count] */
+    uint32_t *runes = nullptr;      /**< UTF-8 only: u32[bytes] */
+    uint32_t *rune_counts = nullptr;
+    device_buffer_t data_copy, offsets_copy, runes_buffer, rune_counts_buffer;
+};
+
+/** How a segment's cells are enumerated. */
+enum segment_mode_t { segment_pairs_k = 0, segment_cross_k = 1, segment_symmetric_k = 2 };
+
+/** Host description of one segment before staging. */
+struct segment_input_t {
+    engine_t *engine = nullptr;
+    segment_mode_t mode = segment_pairs_k;
+    sz_sequence_u64tape_t const *queries = nullptr;
+    sz_sequence_u32tape_t const *queries32 = nullptr;
+    sz_sequence_u64tape_t const *candidates = nullptr;
+    sz_sequence_u32tape_t const *candidates32 = nullptr;
+    void *results = nullptr;
+    uint64_t results_row_stride = 0;
+};
+
+/** A batch staged on one device: everything `run` needs, ready to be launched (or replayed from a graph). */
+struct batch_t {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    szs_batch_mode_t mode = szs_batch_persistent_k;
+    std::vector<segment_input_t> inputs;
+    std::vector<staged_tape_t *> tapes;        /**< owned; queries then candidates per segment */
+    std::vector<void *> host_results;          /**< per segment: the caller's host pointer, or null if device */
+    std::vector<uint64_t> results_elements;    /**< per segment: elements spanned by the result matrix */
+    std::vector<uint64_t> result_rows, result_columns;
+    std::vector<int64_t *> device_results;     /**< per segment: where the kernels write */
+    std::vector<device_buffer_t *> result_buffers;
+
+    device_buffer_t params, segments, shape_table, tasks, bin_of_task, order, shape_of_task, identity;
+    device_buffer_t histogram, cursor, task_start, item_start, priority, position_of_bin, queue_head, scratch, stats;
+    std::vector<char> materialize_args;        /**< one `materialize_args_t` per segment (opaque here) */
+    uint64_t tasks_count = 0;
+    uint64_t cells = 0;                        /**< sum of n*m over non-empty tasks */
+    uint32_t scratch_capacity = 0;
+    uint32_t waves = 0;
+    uint32_t lds_bytes = 0;
+    std::vector<uint32_t> per_pair_tasks;      /**< per-pair mode: non-empty task ids */
+    std::vector<int32_t> per_pair_variants;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipEvent_t kernel_begin = nullptr, kernel_end = nullptr;
+    double last_kernel_ms = 0;
+
+    ~batch_t();
+};
+
+/** Stages and bins a batch on the scope's device; `stream` overrides the scope's stream when given. */
+sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> const &inputs, szs_batch_mode_t mode,
+                          batch_t **out, char const **error_message, hipStream_t stream = nullptr);
+sz_status_t batch_launch(batch_t *batch, char const **error_message);
+sz_status_t batch_wait(batch_t *batch, char const **error_message);
+
+} // namespace szs_amd

@@ -1,0 +1,146 @@
+"""GPU parity: every engine, shape and mode scored by the HIP kernels (through the C ABI) against the CPU oracle.
+
+Bit-exact integer equality is required. Lengths are chosen so every compiled kernel shape runs, including
+multi-pass stripes (rows > 1024) and 16-lane groups holding four pairs per wave.
+"""
+
+import zlib
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import DNA, PROTEIN, dna_tables, protein_tables, random_bytes, random_utf8
+
+pytestmark = pytest.mark.gpu
+
+szs = pytest.importorskip("stringzillas_amd")
+
+RANGES = [(0, 20), (10, 70), (60, 300), (200, 600), (500, 1100), (1000, 2600)]
+
+
+def _engines(rng):
+    b2c, costs = protein_tables(rng)
+    b2c_a, costs_a = protein_tables(rng, asymmetric=True)
+    dna_b2c, dna_costs = dna_tables()
+    return {
+        "lev_unit": (szs.LevenshteinDistances(), O.Params("levenshtein"), DNA, False),
+        "lev_affine": (szs.LevenshteinDistances(0, 4, 3, 2), O.Params("levenshtein", 0, 4, 3, 2), DNA, False),
+        "lev_linear": (szs.LevenshteinDistances(1, 3, 2, 2), O.Params("levenshtein", 1, 3, 2, 2), b"ACGTN", False),
+        "utf8_unit": (szs.LevenshteinDistancesUTF8(), O.Params("levenshtein_utf8"), None, True),
+        "utf8_affine": (szs.LevenshteinDistancesUTF8(0, 4, 3, 2), O.Params("levenshtein_utf8", 0, 4, 3, 2), None, True),
+        "nw_affine": (szs.NeedlemanWunschScores(b2c, costs, -11, -1),
+                      O.Params("needleman_wunsch", open=-11, extend=-1, byte_to_class=b2c, class_costs=costs), PROTEIN, False),
+        "nw_linear": (szs.NeedlemanWunschScores(b2c, costs, -2, -2),
+                      O.Params("needleman_wunsch", open=-2, extend=-2, byte_to_class=b2c, class_costs=costs), PROTEIN, False),
+        "nw_asymmetric": (szs.NeedlemanWunschScores(b2c_a, costs_a, -5, -1),
+                          O.Params("needleman_wunsch", open=-5, extend=-1, byte_to_class=b2c_a, class_costs=costs_a), PROTEIN, False),
+        "nw_odd_signs": (szs.NeedlemanWunschScores(b2c, costs, 3, 1),
+                         O.Params("needleman_wunsch", open=3, extend=1, byte_to_class=b2c, class_costs=costs), PROTEIN, False),
+        "sw_linear": (szs.SmithWatermanScores(dna_b2c, dna_costs, -2, -2),
+                      O.Params("smith_waterman", open=-2, extend=-2, byte_to_class=dna_b2c, class_costs=dna_costs), DNA, False),
+        "sw_affine": (szs.SmithWatermanScores(b2c, costs, -5, -1),
+                      O.Params("smith_waterman", open=-5, extend=-1, byte_to_class=b2c, class_costs=costs), PROTEIN, False),
+        "sw_odd_signs": (szs.SmithWatermanScores(b2c_a, costs_a, 3, 1),
+                         O.Params("smith_waterman", open=3, extend=1, byte_to_class=b2c_a, class_costs=costs_a), PROTEIN, False),
+    }
+
+
+ENGINE_NAMES = list(_engines(np.random.default_rng(0)).keys())
+
+
+def _strings(rng, alphabet, utf8, count, low, high):
+    if utf8:
+        return [random_utf8(rng, low, high) for _ in range(count)]
+    return [random_bytes(rng, alphabet, low, high) for _ in range(count)]
+
+
+@pytest.mark.parametrize("low,high", RANGES)
+@pytest.mark.parametrize("name", ENGINE_NAMES)
+def test_pairs_match_oracle(name, low, high):
+    rng = np.random.default_rng(zlib.crc32(f"{name}:{low}:{high}".encode()))
+    engine, params, alphabet, utf8 = _engines(rng)[name]
+    count = 96 if high <= 600 else 24 if high <= 1100 else 8
+    a = _strings(rng, alphabet, utf8, count, low, high)
+    b = _strings(rng, alphabet, utf8, count, low, high)
+    expected = O.pairs(params, O.tape(a), O.tape(b), threads=16)
+    got = engine.pairs(szs.Strs(a), szs.Strs(b))
+    np.testing.assert_array_equal(got.astype(np.int64), expected, err_msg=f"{name} lengths {low}-{high}")
+
+
+@pytest.mark.parametrize("name", ENGINE_NAMES)
+def test_cross_product_and_symmetric(name):
+    rng = np.random.default_rng(7)
+    engine, params, alphabet, utf8 = _engines(rng)[name]
+    queries = _strings(rng, alphabet, utf8, 9, 0, 90)
+    candidates = _strings(rng, alphabet, utf8, 13, 0, 300)
+    expected = O.cross(params, O.tape(queries), O.tape(candidates), threads=16)
+    got = engine(szs.Strs(queries), szs.Strs(candidates))
+    assert got.shape == (9, 13)
+    np.testing.assert_array_equal(got.astype(np.int64), expected)
+    expected_sym = O.cross(params, O.tape(queries), None, threads=16)
+    got_sym = engine(szs.Strs(queries))
+    np.testing.assert_array_equal(got_sym.astype(np.int64), expected_sym)
+
+
+def test_strided_out_buffer_and_untouched_padding():
+    engine = szs.LevenshteinDistances()
+    queries, candidates = szs.Strs(["abc", "hello", ""]), szs.Strs(["abd", "", "help", "xyz"])
+    out = np.full((3, 6), 777, dtype=np.uint64)
+    result = engine(queries, candidates, out=out)
+    assert result is out
+    expected = O.cross(O.Params("levenshtein"), O.tape(["abc", "hello", ""]), O.tape(["abd", "", "help", "xyz"]))
+    np.testing.assert_array_equal(out[:, :4].astype(np.int64), expected)
+    assert (out[:, 4:] == 777).all()
+
+
+def test_known_answers_reference_cases():
+    """Known answers from the reference's test/similarities.py:134-143 and 160-168."""
+    lev = szs.LevenshteinDistances()
+    cases = [("hello", "hello", 0), ("hello", "hell", 1), ("", "", 0), ("", "abc", 3), ("abc", "", 3),
+             ("abc", "ac", 1), ("abc", "a_bc", 1), ("abc", "adc", 1), ("ggbuzgjux{}l", "gbuzgjux{}l", 1),
+             ("abcdefgABCDEFG", "ABCDEFGabcdefg", 14)]
+    got = lev.pairs(szs.Strs([c[0] for c in cases]), szs.Strs([c[1] for c in cases]))
+    assert got.tolist() == [c[2] for c in cases]
+    utf8 = szs.LevenshteinDistancesUTF8()
+    cases = [("hello", "hell", 1), ("𠜎 𠜱 𠝹 𠱓", "𠜎𠜱𠝹𠱓", 3), ("💖", "💗", 1), ("αβγδ", "αγδ", 1),
+             ("\u00e9cole", "e\u0301cole", 2), ("façade", "facade", 1), ("Sch\u00f6n", "Scho\u0308n", 2),
+             ("München", "Muenchen", 2), ("こんにちは世界", "こんばんは世界", 2)]
+    got = utf8.pairs(szs.Strs([c[0] for c in cases]), szs.Strs([c[1] for c in cases]))
+    assert got.tolist() == [c[2] for c in cases]
+
+
+def test_cpu_scope_is_rejected():
+    """A GPU engine on a CPU scope fails with device_code_mismatch (reference test/stringzillas.py:65-73)."""
+    engine = szs.LevenshteinDistances()
+    with pytest.raises(RuntimeError):
+        engine(szs.Strs(["a"]), szs.Strs(["b"]), device=szs.DeviceScope(cpu_cores=2))
+
+
+def test_batch_modes_agree_and_graph_replays():
+    rng = np.random.default_rng(11)
+    b2c, costs = protein_tables(rng)
+    lev = szs.LevenshteinDistances()
+    utf8 = szs.LevenshteinDistancesUTF8()
+    nw = szs.NeedlemanWunschScores(b2c, costs, -11, -1)
+    segments = [
+        (lev, szs.Strs([random_bytes(rng, DNA, 1, 1500) for _ in range(150)]),
+         szs.Strs([random_bytes(rng, DNA, 1, 1500) for _ in range(150)])),
+        (utf8, szs.Strs([random_utf8(rng, 0, 400) for _ in range(100)]),
+         szs.Strs([random_utf8(rng, 0, 400) for _ in range(100)])),
+        (nw, szs.Strs([random_bytes(rng, PROTEIN, 50, 700) for _ in range(80)]),
+         szs.Strs([random_bytes(rng, PROTEIN, 50, 700) for _ in range(80)])),
+    ]
+    params = [O.Params("levenshtein"), O.Params("levenshtein_utf8"),
+              O.Params("needleman_wunsch", open=-11, extend=-1, byte_to_class=b2c, class_costs=costs)]
+    expected = [O.pairs(p, O.tape([s[1][i] for i in range(len(s[1]))]), O.tape([s[2][i] for i in range(len(s[2]))]),
+                        threads=16) for p, s in zip(params, segments)]
+    for mode in (szs.BATCH_PERSISTENT, szs.BATCH_PER_PAIR, szs.BATCH_GRAPH):
+        batch = szs.Batch(segments, mode=mode)
+        for replay in range(3 if mode == szs.BATCH_GRAPH else 1):
+            for out in batch.outputs:
+                out.fill(12345)
+            outputs = batch.run()
+            for got, want in zip(outputs, expected):
+                np.testing.assert_array_equal(got.astype(np.int64), want, err_msg=f"mode {mode} replay {replay}")
+        assert batch.cells > 0
