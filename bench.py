@@ -1,0 +1,296 @@
+"""Benchmark: GCUPS of the batched persistent similarity engine on one MI355X, against the one-launch-per-pair
+form and the reference's CPU path, on the uneven mixed batch (1,000,000 pairs with lengths log-uniform in
+[16, 16384] plus 32 pairs of 65,536: 50% byte Levenshtein, 25% UTF-8 Levenshtein, 25% affine Needleman-Wunsch).
+
+One step = one replay of the HIP graph that transcodes, materializes, bins and scores the whole batch, with the
+tapes already resident in HBM. Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GCUPS on 1 MI355X (batched vs per-call) + p50/p99 latency in serving loop"
+INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # CUs x lanes x clock: 39.3 T lane-ops/s
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    parser = argparse.ArgumentParser(description=__doc__)
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=3)
+    parser.add_argument("--warmup", type=int, default=1)
+    parser.add_argument("--pairs", type=int, default=1_000_000, help="pairs of the mixed batch (1,000,000)")
+    parser.add_argument("--long-pairs", type=int, default=32)
+    parser.add_argument("--per-call-sample", type=int, default=2000, help="pairs timed with one launch each")
+    parser.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
+    parser.add_argument("--check-pairs", type=int, default=48, help="pairs per segment checked against the oracle")
+    parser.add_argument("--mode", choices=["graph", "persistent"], default="graph")
+    parser.add_argument("--skip-cpu", action="store_true")
+    parser.add_argument("--skip-per-call", action="store_true")
+    return parser.parse_args()
+
+
+def log(message: str) -> None:
+    print(f"[bench] {message}", file=sys.stderr, flush=True)
+
+
+class TensorTape:
+    """A tape held by torch device tensors, handed to the C ABI in place."""
+
+    def __init__(self, data, offsets):
+        self.data, self.offsets = data.contiguous(), offsets.contiguous()
+
+    def __len__(self):
+        return self.offsets.numel() - 1
+
+    def tape(self):
+        from stringzillas_amd import _native
+
+        return _native.U64Tape(self.data.data_ptr(), self.offsets.data_ptr(), len(self))
+
+
+def engines_for(segments, szs):
+    engines = []
+    for segment in segments:
+        if segment.kind == "levenshtein":
+            engines.append(szs.LevenshteinDistances(segment.match, segment.mismatch, segment.open, segment.extend))
+        elif segment.kind == "levenshtein_utf8":
+            engines.append(szs.LevenshteinDistancesUTF8(segment.match, segment.mismatch, segment.open, segment.extend))
+        elif segment.kind == "needleman_wunsch":
+            engines.append(szs.NeedlemanWunschScores(segment.byte_to_class, segment.class_costs, segment.open,
+                                                     segment.extend))
+        else:
+            engines.append(szs.SmithWatermanScores(segment.byte_to_class, segment.class_costs, segment.open,
+                                                   segment.extend))
+    return engines
+
+
+def oracle_params(segment):
+    from oracle import oracle as O
+
+    kwargs = dict(match=segment.match, mismatch=segment.mismatch, open=segment.open, extend=segment.extend)
+    if segment.byte_to_class is not None:
+        kwargs.update(byte_to_class=segment.byte_to_class, class_costs=segment.class_costs)
+    return O.Params(segment.kind, **kwargs)
+
+
+def sample_indices(rng, count: int, size: int) -> np.ndarray:
+    return np.sort(rng.choice(count, size=min(size, count), replace=False))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}): the pairs are independent, so a "
+              "multi-GPU run is a sharded copy of this one; see DESIGN.md for the sharding plan.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import stringzillas_amd as szs
+    from stringzillas_amd import workloads as W
+
+    device = torch.device("cuda", 0)
+    t0 = time.time()
+    workload = W.mixed_c3(args.pairs, args.long_pairs, device="cuda")
+    torch.cuda.synchronize()
+    log(f"generated {workload.name}: {workload.pairs} pairs in {time.time() - t0:.1f}s")
+    cells = workload.cells()
+    ops_per_cell = workload.weighted_ops_per_cell()
+    log(f"{cells / 1e12:.3f} T cells, {ops_per_cell:.2f} VALU ops per cell (cell-weighted)")
+
+    engines = engines_for(workload.segments, szs)
+    tapes, outputs = [], []
+    for segment in workload.segments:
+        a = TensorTape(segment.firsts_data, segment.firsts_offsets)
+        b = TensorTape(segment.seconds_data, segment.seconds_offsets)
+        tapes.append((a, b))
+        outputs.append(torch.zeros(segment.count, dtype=torch.int64, device=device))
+    mode = szs.BATCH_GRAPH if args.mode == "graph" else szs.BATCH_PERSISTENT
+    t0 = time.time()
+    batch = szs.Batch([(e, a, b) for e, (a, b) in zip(engines, tapes)], mode=mode, outputs=outputs)
+    log(f"prepared the batch in {time.time() - t0:.1f}s ({batch.cells / 1e12:.3f} T cells)")
+    assert batch.cells == cells, (batch.cells, cells)
+
+    for _ in range(args.warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        batch.launch()
+        batch.wait()
+        kernel_ms.append(batch.last_kernel_ms)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    ms_per_step = elapsed / args.steps * 1e3
+    gcups = cells / (elapsed / args.steps) / 1e9
+    replay_ms_mean = float(np.mean(kernel_ms))
+    # The scoring kernel alone, from HIP events on its stream, in the non-graph form of the same batch.
+    scoring = szs.Batch([(e, a, b) for e, (a, b) in zip(engines, tapes)], mode=szs.BATCH_PERSISTENT, outputs=outputs)
+    scoring.run()
+    scoring_ms = []
+    for _ in range(max(1, min(args.steps, 3))):
+        scoring.run()
+        scoring_ms.append(scoring.last_kernel_ms)
+    kernel_ms_mean = float(np.mean(scoring_ms))
+    del scoring
+    log(f"batched: {ms_per_step:.1f} ms/step ({replay_ms_mean:.1f} ms replay), {gcups:.1f} GCUPS; "
+        f"scoring kernel {kernel_ms_mean:.1f} ms")
+
+    # Parity spot check against the oracle, including one of the longest pairs of each kind.
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(2024)
+    checked = mismatches = 0
+    host_segments = []
+    for segment, output in zip(workload.segments, outputs):
+        host = segment.host()
+        host_segments.append(host)
+        got = output.cpu().numpy()
+        picks = sample_indices(rng, host.count, args.check_pairs)
+        longest = int(np.argmax(host.lengths("firsts")))
+        picks = np.unique(np.concatenate([picks, [longest]]))
+        sub = host.subset(picks)
+        expected = O.pairs(oracle_params(host), (sub.firsts_data, sub.firsts_offsets),
+                           (sub.seconds_data, sub.seconds_offsets), threads=min(16, os.cpu_count() or 1))
+        mismatches += int((got[picks] != expected).sum())
+        checked += len(picks)
+    log(f"parity: {checked} pairs checked against the oracle, {mismatches} mismatches")
+    if mismatches:
+        raise SystemExit("batched scores differ from the oracle")
+
+    # One launch per pair on a uniform sample, and the persistent form on the same sample.
+    per_call = None
+    if not args.skip_per_call:
+        rng = np.random.default_rng(7)
+        total = workload.pairs
+        sample_segments, sample_engines = [], []
+        for segment, host, engine in zip(workload.segments, host_segments, engines):
+            share = max(1, int(round(args.per_call_sample * segment.count / total)))
+            sub = host.subset(sample_indices(rng, host.count, share))
+            sample_segments.append(sub)
+            sample_engines.append(engine)
+        sample_cells = sum(s.cells() for s in sample_segments)
+        results = {}
+        for name, sample_mode in (("per_call", szs.BATCH_PER_PAIR), ("batched", szs.BATCH_PERSISTENT)):
+            strs = [(e, szs.Strs.from_tape(s.firsts_data, s.firsts_offsets),
+                     szs.Strs.from_tape(s.seconds_data, s.seconds_offsets))
+                    for e, s in zip(sample_engines, sample_segments)]
+            sample_batch = szs.Batch(strs, mode=sample_mode)
+            sample_batch.run()
+            t = time.perf_counter()
+            sample_batch.run()
+            results[name] = sample_cells / (time.perf_counter() - t) / 1e9
+            results[name + "_outputs"] = [o.copy() for o in sample_batch.outputs]
+        for a, b in zip(results["per_call_outputs"], results["batched_outputs"]):
+            if not np.array_equal(a, b):
+                raise SystemExit("per-call and batched scores differ")
+        per_call = {
+            "gcups": round(results["per_call"], 2),
+            "batched_same_sample_gcups": round(results["batched"], 2),
+            "pairs": int(sum(s.count for s in sample_segments)),
+            "cells": int(sample_cells),
+            "launch": "one 64-lane workgroup per pair, one launch per pair, same scorer",
+        }
+        log(f"per-call: {per_call['gcups']} GCUPS; batched on the same sample: {per_call['batched_same_sample_gcups']}")
+
+    cpu = None
+    if not args.skip_cpu:
+        cpu = cpu_baseline(host_segments, args.cpu_seconds)
+        log(f"cpu baseline: {cpu}")
+
+    achieved = cells * ops_per_cell / (kernel_ms_mean / 1e3) / 1e12  # T lane-ops/s over the scoring kernel
+    line = {
+        "metric": METRIC,
+        "value": round(gcups, 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded, generated on the device)",
+        "config": {
+            "workload": workload.name,
+            "pairs": workload.pairs,
+            "cells": cells,
+            "mix": "50% byte Levenshtein ACGT, 25% UTF-8 Levenshtein, 25% NW affine -11/-1 (21-class matrix)",
+            "launch": args.mode,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": round(INT32_VALU_PEAK_TOPS, 2),
+            "unit": "TOP/s (INT32 VALU lane-ops)",
+            "frac": round(achieved / INT32_VALU_PEAK_TOPS, 4),
+            "traffic": None,
+            "ops_per_cell": round(ops_per_cell, 3),
+            "kernel_ms": round(kernel_ms_mean, 3),
+        },
+        "cpu_baseline": cpu,
+        "per_call": per_call,
+        "batched_over_per_call": round(gcups / per_call["gcups"], 1) if per_call else None,
+        "parity": {"checked_pairs": checked, "mismatches": mismatches},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(host_segments, target_seconds: float):
+    """The reference's own CPU scorers (Ice Lake AVX-512 when the host has it) on a bounded uniform sample."""
+    from oracle import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    variant = "icelake" if O.reference_available("icelake") else "serial" if O.reference_available("serial") else None
+    rng = np.random.default_rng(99)
+    total = sum(s.count for s in host_segments)
+
+    def run(sample_pairs):
+        cells = 0
+        start = time.perf_counter()
+        for segment in host_segments:
+            share = max(1, int(round(sample_pairs * segment.count / total)))
+            sub = segment.subset(sample_indices(rng, segment.count, share))
+            tapes = ((sub.firsts_data, sub.firsts_offsets), (sub.seconds_data, sub.seconds_offsets))
+            if variant:
+                O.reference_pairs(oracle_params(sub), *tapes, threads=threads, variant=variant)
+            else:
+                O.pairs(oracle_params(sub), *tapes, threads=threads)
+            cells += sub.cells()
+        return cells, time.perf_counter() - start
+
+    sample_pairs = 64
+    cells, seconds = run(sample_pairs)
+    while seconds < target_seconds / 3 and sample_pairs < 200_000:
+        sample_pairs = int(min(sample_pairs * max(2.0, 0.9 * target_seconds / max(seconds, 1e-3)), 200_000))
+        cells, seconds = run(sample_pairs)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((line.split(":", 1)[1].strip() for line in f if line.startswith("model name")), "")
+    except OSError:
+        pass
+    return {
+        "value": round(cells / seconds / 1e9, 3),
+        "unit": "GCUPS",
+        "cores": threads,
+        "kind": "reference" if variant else "port",
+        "sample": f"{sample_pairs} pairs drawn uniformly from the same batch ({cells / 1e9:.1f} G cells, "
+                  f"{seconds:.1f}s), scorer={'reference ' + variant if variant else 'oracle'}, cpu={model}",
+    }
+
+
+if __name__ == "__main__":
+    main()

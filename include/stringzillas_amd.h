@@ -303,7 +303,8 @@ sz_status_t szs_batch_launch(szs_batch_t batch, char const **error_message);
 sz_status_t szs_batch_wait(szs_batch_t batch, char const **error_message);
 /** Total DP cells (sum of n*m, runes for UTF-8) the batch scores per run. */
 sz_size_t szs_batch_cells(szs_batch_t batch);
-/** Device time of the last run's dominant (scoring) kernel in milliseconds, from HIP events on its stream. */
+/** Device time of the last run in milliseconds from HIP events on the batch stream: the scoring kernel alone for the
+ *  persistent and per-pair modes, the whole replay for the graph mode. */
 double szs_batch_last_kernel_ms(szs_batch_t batch);
 void szs_batch_free(szs_batch_t batch);
 

@@ -133,7 +133,7 @@ def to_device(strs: Strs, device: int = 0) -> DeviceStrs:
 
 
 def _as_tape(obj, name: str):
-    if isinstance(obj, (Strs, DeviceStrs)):
+    if isinstance(obj, (Strs, DeviceStrs)) or (hasattr(obj, "tape") and hasattr(obj, "__len__")):
         return obj
     try:
         import pyarrow as pa

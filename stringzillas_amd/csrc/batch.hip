@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <utility>
 #include <vector>
 
 #include "szs_runtime.hpp"
@@ -47,25 +48,44 @@ namespace szs_amd {
 
 #pragma region Static tables
 
-/** Default shape choice: the smallest single-pass shape that holds all rows, else 1024-row stripes. */
+/**
+ *  Default shape choice. Uniform-cost (Levenshtein) families: the smallest single-stripe shape that holds all
+ *  rows, 1024-row stripes on one wave up to 4096 rows, then the cooperative 1024-row shape. Class-cost families
+ *  keep LDS profiles small: at most 8 rows per lane, 512-row stripes on one wave up to 2048 rows, then the
+ *  cooperative 512-row shape. Unit-cost bytes: DP up to 64 rows, then bit-parallel words: 4 lanes up to 256 rows,
+ *  16 lanes up to 1024, 4096-row stripes on one wave up to 16384, then cooperative.
+ */
 static shape_table_t default_shape_table() {
     shape_table_t table;
     std::memset(&table, 0, sizeof(table));
-    std::vector<int> by_capacity(shapes_count_k);
-    std::iota(by_capacity.begin(), by_capacity.end(), 0);
-    std::stable_sort(by_capacity.begin(), by_capacity.end(), [](int a, int b) {
-        return shapes_k[a].group * shapes_k[a].rows_per_lane < shapes_k[b].group * shapes_k[b].rows_per_lane;
-    });
-    for (int f = 0; f < families_count_k; ++f)
-        for (int s = 0; s < 2; ++s) {
-            for (int i = 0; i < shapes_count_k; ++i) {
-                int const shape = by_capacity[i];
-                table.limits[f][s][shape] = shapes_k[shape].group * shapes_k[shape].rows_per_lane;
-                table.order[f][s][i] = shape;
-            }
-            table.used[f][s] = shapes_count_k;
-            table.multi_pass_shape[f][s] = shapes_count_k - 1;
+    for (int slot = 0; slot < shape_slots_k; ++slot) {
+        for (int shape = 0; shape < shapes_count_k; ++shape) table.limits[slot][shape] = -1;
+        std::vector<std::pair<int, int>> ladder; // (shape, most rows)
+        int multi = 10;
+        int const family = slot / 2;
+        if (slot == unit_bytes_slot_k) {
+            ladder = {{0, 16}, {1, 32}, {2, 64}, {11, 256}, {12, 1024}, {13, 16384}};
+            multi = 14;
         }
+        else if (family == family_lev_linear_k || family == family_lev_affine_k) {
+            for (int shape = 0; shape <= 8; ++shape)
+                ladder.push_back({shape, shapes_k[shape].group * shapes_k[shape].rows_per_lane});
+            ladder.back().second = 4096; // up to 4 stripes on one wave
+            multi = 10;
+        }
+        else {
+            for (int shape = 0; shape <= 7; ++shape)
+                ladder.push_back({shape, shapes_k[shape].group * shapes_k[shape].rows_per_lane});
+            ladder.back().second = 2048;
+            multi = 9;
+        }
+        for (size_t i = 0; i < ladder.size(); ++i) {
+            table.order[slot][i] = ladder[i].first;
+            table.limits[slot][ladder[i].first] = ladder[i].second;
+        }
+        table.used[slot] = (int32_t)ladder.size();
+        table.multi_pass_shape[slot] = multi;
+    }
     return table;
 }
 
@@ -74,16 +94,14 @@ static shape_table_t &active_shape_table() {
     return table;
 }
 
-/** Rough cost of one work item of a bin, for heaviest-first ordering of the persistent queue. */
+/** Rough latency of one work item of a bin (its bucket is the half-octave of lane-row-steps), heaviest first. */
 static double bin_item_cost(uint32_t bin) {
     int const variant = (int)(bin / column_buckets_k);
     int const bucket = (int)(bin % column_buckets_k);
-    int const shape = variant % shapes_count_k;
     int const family = variant / shapes_count_k / 2;
-    double const rows = (double)shapes_k[shape].group * shapes_k[shape].rows_per_lane;
     double const per_cell = (family == family_lev_affine_k || family == family_nw_affine_k ||
                              family == family_sw_affine_k) ? 1.8 : 1.0;
-    return rows / 64.0 * std::pow(2.0, bucket / 2.0) * per_cell;
+    return std::pow(2.0, bucket / 2.0) * per_cell;
 }
 
 static std::vector<uint32_t> const &bin_priority() {
@@ -227,9 +245,10 @@ static sz_status_t enqueue_front(batch_t &b, bool bin, char const **error_messag
     return ok(error_message);
 }
 
-static sz_status_t enqueue_scoring(batch_t &b, char const **error_message) {
+/** Scoring launches; HIP events bracket them unless they are being captured into a graph. */
+static sz_status_t enqueue_scoring(batch_t &b, bool timed, char const **error_message) {
     hipStream_t const stream = b.stream;
-    SZS_TRY_HIP(hipEventRecord(b.kernel_begin, stream), "event");
+    if (timed) SZS_TRY_HIP(hipEventRecord(b.kernel_begin, stream), "event");
     if (b.mode == szs_batch_per_pair_k) {
         for (size_t i = 0; i < b.per_pair_tasks.size(); ++i)
             SZS_TRY_HIP(szs_amd_launch_single(b.tasks.as<task_t>(), b.segments.as<segment_view_t>(),
@@ -244,7 +263,7 @@ static sz_status_t enqueue_scoring(batch_t &b, char const **error_message) {
                                               stream),
                     "persistent scoring launch");
     }
-    SZS_TRY_HIP(hipEventRecord(b.kernel_end, stream), "event");
+    if (timed) SZS_TRY_HIP(hipEventRecord(b.kernel_end, stream), "event");
     return ok(error_message);
 }
 
@@ -367,11 +386,13 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
         a.mode = input.mode;
         a.open = engine->params.open;
         a.extend = engine->params.extend;
+        a.unit_costs = engine->params.match == 0 && engine->params.mismatch == 1 && engine->params.open == 1 &&
+                       engine->params.extend == 1;
         tasks_total += tasks;
         if (tasks_total >= 0xFFFFFFF0ull)
             return fail(sz_overflow_risk_k, error_message, "A batch holds at most 2^32 - 16 scored cells");
-        if (engine->params.family >= family_nw_linear_k)
-            lds_bytes = std::max<uint32_t>(lds_bytes, (uint32_t)engine->params.dense_count * 64u * 16u);
+        if (engine->params.family >= family_nw_linear_k) // per-wave profile: classes x 64 lanes x 8 rows
+            lds_bytes = std::max<uint32_t>(lds_bytes, (uint32_t)engine->params.dense_count * 64u * 8u);
     }
     b.tasks_count = tasks_total;
     b.lds_bytes = lds_bytes;
@@ -429,6 +450,7 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
         uint64_t const n = task.rows_length, m = task.columns_length;
         b.cells += n * m;
         if (n > (uint64_t)shape.group * shape.rows_per_lane) widest_multi_pass = std::max(widest_multi_pass, m);
+        (void)shape;
         // Reach bound of serial.hpp:117-140, applied to int32 cells.
         int const family = engine->params.family;
         bool const affine = family == family_lev_affine_k || family == family_nw_affine_k ||
@@ -447,10 +469,11 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
     (void)device_count;
     SZS_TRY_HIP(hipDeviceGetAttribute(&compute_units, hipDeviceAttributeMultiprocessorCount, device), "attributes");
     SZS_TRY_HIP(szs_amd_persistent_occupancy(&blocks_per_cu, lds_bytes), "occupancy");
-    blocks_per_cu = std::max(1, std::min(blocks_per_cu, 32));
-    b.waves = mode == szs_batch_per_pair_k ? 1u : (uint32_t)(compute_units * blocks_per_cu);
-    uint64_t const scratch_waves = widest_multi_pass ? b.waves : 1;
-    SZS_TRY_HIP(b.scratch.reserve(scratch_waves * 2ull * b.scratch_capacity * 4 + 256), "Failed to allocate scratch");
+    blocks_per_cu = std::max(1, std::min(blocks_per_cu, 8));
+    b.waves = mode == szs_batch_per_pair_k ? 1u : (uint32_t)(compute_units * blocks_per_cu); // workgroups
+    uint64_t const scratch_groups = widest_multi_pass ? b.waves : 1;
+    SZS_TRY_HIP(b.scratch.reserve(scratch_groups * waves_per_group_k * 2ull * b.scratch_capacity * 4 + 256),
+                "Failed to allocate scratch");
     if (mode == szs_batch_per_pair_k) {
         std::vector<uint32_t> identity(slots);
         std::iota(identity.begin(), identity.end(), 0u);
@@ -460,7 +483,7 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
     if (mode == szs_batch_graph_k) {
         SZS_TRY_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "graph capture");
         sz_status_t front = enqueue_front(b, true, error_message);
-        sz_status_t scoring = front == sz_success_k ? enqueue_scoring(b, error_message) : front;
+        sz_status_t scoring = front == sz_success_k ? enqueue_scoring(b, false, error_message) : front;
         hipGraph_t graph = nullptr;
         hipError_t const ended = hipStreamEndCapture(stream, &graph);
         if (scoring != sz_success_k) {
@@ -479,11 +502,15 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
 sz_status_t batch_launch(batch_t *batch, char const **error_message) {
     batch_t &b = *batch;
     SZS_TRY_HIP(hipSetDevice(b.device), "hipSetDevice failed");
-    if (b.mode == szs_batch_graph_k) SZS_TRY_HIP(hipGraphLaunch(b.graph_exec, b.stream), "graph launch");
+    if (b.mode == szs_batch_graph_k) { // the events then time the whole replay
+        SZS_TRY_HIP(hipEventRecord(b.kernel_begin, b.stream), "event");
+        SZS_TRY_HIP(hipGraphLaunch(b.graph_exec, b.stream), "graph launch");
+        SZS_TRY_HIP(hipEventRecord(b.kernel_end, b.stream), "event");
+    }
     else {
         sz_status_t status = enqueue_front(b, b.mode != szs_batch_per_pair_k, error_message);
         if (status != sz_success_k) return status;
-        status = enqueue_scoring(b, error_message);
+        status = enqueue_scoring(b, true, error_message);
         if (status != sz_success_k) return status;
     }
     return enqueue_results(b, error_message);

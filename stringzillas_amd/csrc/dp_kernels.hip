@@ -1,299 +1,103 @@
 /**
  *  @file dp_kernels.hip
- *  @brief gfx950 kernels of the similarity engine: UTF-8 transcoding, pair materialization, length binning and
- *         the persistent dynamic-programming kernel.
+ *  @brief gfx950 kernels of the similarity engine: the persistent scoring kernel (independent and cooperative
+ *         work items over the sweep of dp_sweep.hpp), the one-pair baseline kernel, UTF-8 transcoding, pair
+ *         materialization and the device-side length binning.
  *
- *  Recurrences restated from include/stringzillas/similarities/serial.hpp:669-1079 (tile scorers) with the border
- *  conventions of its walkers (serial.hpp:1153-1158, 1310-1320, 1627-1633) and the empty-string rules of
- *  serial.hpp:142-151. The traversal is this file's own: a wave-synchronous systolic sweep.
- *
- *  Sweep: a group of G lanes scores one pair. The rows side (the shorter string) is split into stripes of G*R
- *  rows; lane k keeps rows [k*R, k*R+R) of the stripe in VGPRs, together with the score of each row in the last
- *  column it has seen. At step t lane k scores column j = t + 1 - k for all its R rows, then hands the score and
- *  vertical-gap value of its bottom row, plus the column symbol, to lane k+1 through one DPP shift each
- *  (`wave_shr:1` for G = 64, `row_shr:1` for G = 16). Lane 0 takes those values from the stripe above: the
- *  matrix border for the first stripe, the previous stripe's bottom row (spilled to per-wave HBM scratch)
- *  otherwise. Per step a lane does R cell updates and a fixed ~10 VALU ops of exchange and bookkeeping.
- *  Cells are int32; the host proves the reach bound of serial.hpp:117-140 fits before launching.
+ *  Workgroups have 4 waves. A work item of a non-cooperative shape gives each wave 64/G pairs of the same bin;
+ *  a cooperative item gives the whole workgroup one tall pair, whose stripes the waves sweep in a lock-step
+ *  pipeline: stripe s runs on wave s mod 4 during phases [s*D, s*D + C) (C = 64-column chunks per stripe,
+ *  D = max(2, ceil(C / 4))), reading stripe s-1's bottom row from an HBM ring of 4 rows that stripe s-1 wrote at
+ *  least one phase (one workgroup barrier) earlier.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dp_sweep.hpp"
+#include "myers_sweep.hpp"
 #include "szs_device.hpp"
 
 namespace szs_amd {
 
-#pragma region DPP helpers
+#pragma region Work items
 
-/** DPP controls (GFX9 encoding). */
-enum : int {
-    dpp_row_shr1_k = 0x111,  /**< lane i <- lane i-1 within each row of 16; lane 0 of a row keeps `old` */
-    dpp_row_ror15_k = 0x12F, /**< lane i <- lane (i+1) mod 16 within each row */
-    dpp_wave_shl1_k = 0x130, /**< lane i <- lane i+1; lane 63 keeps `old` */
-    dpp_wave_rol1_k = 0x134, /**< lane i <- lane (i+1) mod 64 */
-    dpp_wave_shr1_k = 0x138, /**< lane i <- lane i-1; lane 0 keeps `old` */
-};
-
-template <int G>
-__device__ __forceinline__ int32_t shift_down(int32_t old, int32_t src) {
-    if constexpr (G == 64) return __builtin_amdgcn_update_dpp(old, src, dpp_wave_shr1_k, 0xF, 0xF, false);
-    else return __builtin_amdgcn_update_dpp(old, src, dpp_row_shr1_k, 0xF, 0xF, false);
-}
-
-template <int G>
-__device__ __forceinline__ int32_t rotate_up(int32_t src) {
-    if constexpr (G == 64) return __builtin_amdgcn_update_dpp(src, src, dpp_wave_rol1_k, 0xF, 0xF, false);
-    else return __builtin_amdgcn_update_dpp(src, src, dpp_row_ror15_k, 0xF, 0xF, false);
-}
-
-/** Shift register toward lane 0: lane 63 appends its own `fresh` value. Used for G = 64 only. */
-__device__ __forceinline__ int32_t collect_from_last(int32_t fresh, int32_t stage) {
-    return __builtin_amdgcn_update_dpp(fresh, stage, dpp_wave_shl1_k, 0xF, 0xF, false);
-}
-
-#pragma endregion
-
-#pragma region Family traits
-
-template <int family_>
-struct family_traits {
-    static constexpr bool minimize_k = family_ == family_lev_linear_k || family_ == family_lev_affine_k;
-    static constexpr bool affine_k =
-        family_ == family_lev_affine_k || family_ == family_nw_affine_k || family_ == family_sw_affine_k;
-    static constexpr bool local_k = family_ == family_sw_linear_k || family_ == family_sw_affine_k;
-    static constexpr bool uniform_k = minimize_k;
-};
-
-template <bool minimize_>
-__device__ __forceinline__ int32_t opt(int32_t a, int32_t b) {
-    if constexpr (minimize_) return a < b ? a : b;
-    else return a > b ? a : b;
-}
-
-/** Score of the matrix border at distance `k` from the origin (serial.hpp:142-151, 716-727, 889-895). */
-template <int family_>
-__device__ __forceinline__ int32_t border_score(int32_t k, int32_t open, int32_t extend) {
-    using traits = family_traits<family_>;
-    if constexpr (traits::local_k) return 0;
-    else if constexpr (traits::affine_k) return k ? open + extend * (k - 1) : 0;
-    else return open * k;
-}
-
-#pragma endregion
-
-#pragma region Scoring one work item
-
-/** Everything the persistent kernel hands to one work item. */
-struct item_args_t {
-    task_t const *tasks;
-    segment_view_t const *segments;
-    uint32_t const *order; /**< task ids, `first .. first + count` belong to this item */
-    uint32_t first;
-    uint32_t count;
-    int32_t *scratch;      /**< this wave's 2 x `scratch_capacity` int32: stripe-bottom scores and vertical gaps */
-    uint32_t scratch_capacity;
-    int8_t *profile;       /**< this wave's LDS substitution profile: [dense class][lane][R] int8 */
-};
-
-/**
- *  Scores up to 64 / G pairs with one wave. All lanes execute every step (the DPP exchange needs them); lanes
- *  whose column is outside [1, m] skip the cell updates under the exec mask.
- */
-template <int family_, typename symbol_type_, int G, int R>
-__device__ void score_item(item_args_t const &args) {
-    using traits = family_traits<family_>;
-    constexpr bool minimize_k = traits::minimize_k;
-    constexpr bool affine_k = traits::affine_k;
-    constexpr bool local_k = traits::local_k;
-    constexpr bool uniform_k = traits::uniform_k;
-    constexpr int RP = (R + 3) / 4 * 4; // profile bytes per lane and class, padded to whole dwords
-
-    int const lane = (int)__lane_id();
-    int const group = lane / G;
-    int const k = lane % G;
-    bool const active = (uint32_t)group < args.count;
-    task_t const task = args.tasks[args.order[args.first + (active ? group : 0)]];
-    segment_view_t const segment = args.segments[task.segment];
-    engine_params_t const *params = segment.params;
-    int32_t const n = active ? (int32_t)task.rows_length : 0;
-    int32_t const m = active ? (int32_t)task.columns_length : 0;
-    int32_t const open = params->open, extend = params->extend;
-    int32_t const match = params->match, mismatch = params->mismatch;
-    int32_t const dense_count = params->dense_count;
-    bool const transposed = task.transposed != 0;
-    symbol_type_ const *query = (symbol_type_ const *)segment.query_symbols;
-    symbol_type_ const *candidate = (symbol_type_ const *)segment.candidate_symbols;
-    symbol_type_ const *rows = (transposed ? candidate : query) + task.rows_offset;
-    symbol_type_ const *columns = (transposed ? query : candidate) + task.columns_offset;
-
-    // Wave-wide loop bounds: one pair per wave for G = 64, otherwise the longest of the wave's pairs.
-    int32_t longest = m;
+/** Each wave scores 64/G pairs of the item on its own, stripe after stripe for tall pairs (G = 64). */
+template <typename sweep_type_>
+__device__ void run_independent(item_args_t const &a) {
+    constexpr int G = sweep_type_::group_k;
+    constexpr int R = sweep_type_::rows_per_lane;
+    constexpr uint32_t per_wave = 64 / G;
+    uint32_t const wave = threadIdx.x / 64;
+    if (wave * per_wave >= a.count) return; // wave-uniform: no pairs left for this wave
+    uint32_t const slot = wave * per_wave + (uint32_t)__lane_id() / G;
+    bool const active = slot < a.count;
+    sweep_type_ sweep;
+    sweep.load(a, a.order[a.first + (active ? slot : wave * per_wave)], active, wave);
+    int32_t longest = sweep.m;
     if constexpr (G < 64) {
         for (int offset = G; offset < 64; offset <<= 1) longest = max(longest, __shfl_xor(longest, offset, 64));
     }
     int32_t const stripe_rows = G * R;
-    int32_t const passes = G == 64 ? (n + stripe_rows - 1) / stripe_rows : 1;
-    int32_t const steps = longest + G - 1;
-    int32_t *scratch_scores = args.scratch;
-    int32_t *scratch_gaps = args.scratch + args.scratch_capacity;
-    int32_t running_best = 0; // local alignment: best cell over all passes (serial.hpp:781 starts it at 0)
-
+    int32_t const passes = G == 64 ? (sweep.n + stripe_rows - 1) / stripe_rows : 1;
+    int32_t const chunks = (longest + 2 * G - 2) / G; // steps = longest + G - 1
+    int32_t *scores_row = a.scratch + (size_t)wave * 2 * a.scratch_capacity;
+    int32_t *gaps_row = scores_row + a.scratch_capacity;
     for (int32_t pass = 0; pass < passes; ++pass) {
-        int32_t const base = pass * stripe_rows; // rows above this stripe
-        int32_t const lane_top = base + k * R;   // rows above this lane's first row
-        bool const from_scratch = pass > 0;
-
-        // Per-row state: symbol (or dense class), score in the last column seen, horizontal gap track, best.
-        uint32_t row_symbols[R];
-        int32_t scores[R];
-        int32_t horizontal[R];
-        int32_t best[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            int32_t const i = lane_top + r + 1; // 1-based matrix row
-            uint32_t symbol = 0xFFFFFFFFu;
-            if (i <= n) symbol = (uint32_t)rows[i - 1];
-            row_symbols[r] = symbol;
-            scores[r] = border_score<family_>(i, open, extend);
-            horizontal[r] = (open + extend) + scores[r]; // discard value of the gap track on column 0
-            best[r] = 0;
-        }
-        if constexpr (!uniform_k) {
-            // Profile: this lane's R row costs against every dense column class, packed 4 per dword.
-            int8_t *mine = args.profile + lane * RP;
-            for (int32_t c = 0; c < dense_count; ++c) {
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    uint32_t const row_class = row_symbols[r] == 0xFFFFFFFFu ? 0u : params->byte_to_dense[row_symbols[r] & 0xFF];
-                    int8_t const cost = transposed ? params->dense_costs[c * max_dense_classes_k + row_class]
-                                                   : params->dense_costs[row_class * max_dense_classes_k + c];
-                    mine[c * 64 * RP + r] = cost;
-                }
-            }
-        }
-
-        int32_t diagonal_top = border_score<family_>(lane_top, open, extend); // H[top - 1][0]
-        int32_t bottom_score = scores[R - 1];
-        int32_t bottom_gap = 0;
-        uint32_t column_symbol = 0;
-
-        for (int32_t chunk = 0; chunk < steps; chunk += G) {
-            // Next G column symbols for this group and the next G border values for its lane 0.
-            int32_t const j_lane = chunk + k + 1; // 1-based column this lane loads
-            uint32_t chunk_symbols = 0;
-            if (j_lane <= m) {
-                uint32_t const raw = (uint32_t)columns[j_lane - 1];
-                chunk_symbols = uniform_k ? raw : (uint32_t)params->byte_to_dense[raw & 0xFF];
-            }
-            int32_t chunk_scores, chunk_gaps;
-            if (from_scratch) {
-                chunk_scores = j_lane <= m ? scratch_scores[j_lane] : 0;
-                chunk_gaps = j_lane <= m ? scratch_gaps[j_lane] : 0;
-            }
-            else {
-                chunk_scores = border_score<family_>(j_lane, open, extend);
-                chunk_gaps = (open + extend) + chunk_scores; // discard value of the vertical track on row 0
-            }
-            int32_t stage_scores = 0, stage_gaps = 0;
-
-#pragma unroll 2
-            for (int32_t step = 0; step < G; ++step) {
-                int32_t const t = chunk + step;
-                int32_t const up_score = shift_down<G>(chunk_scores, bottom_score);
-                int32_t up_gap = 0;
-                if constexpr (affine_k) up_gap = shift_down<G>(chunk_gaps, bottom_gap);
-                column_symbol = (uint32_t)shift_down<G>((int32_t)chunk_symbols, (int32_t)column_symbol);
-                chunk_scores = rotate_up<G>(chunk_scores);
-                if constexpr (affine_k) chunk_gaps = rotate_up<G>(chunk_gaps);
-                chunk_symbols = (uint32_t)rotate_up<G>((int32_t)chunk_symbols);
-
-                int32_t const j = t + 1 - k;
-                if ((uint32_t)(j - 1) < (uint32_t)m) {
-                    int32_t diagonal = diagonal_top;
-                    int32_t up = up_score;
-                    int32_t vertical = up_gap;
-                    uint32_t costs_packed[RP / 4];
-                    if constexpr (!uniform_k) {
-                        uint32_t const *row_costs =
-                            (uint32_t const *)(args.profile + (column_symbol * 64 + lane) * RP);
-#pragma unroll
-                        for (int w = 0; w < RP / 4; ++w) costs_packed[w] = row_costs[w];
-                    }
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        int32_t cost;
-                        if constexpr (uniform_k) cost = row_symbols[r] == column_symbol ? match : mismatch;
-                        else cost = (int32_t)(int8_t)(costs_packed[r / 4] >> (8 * (r % 4)));
-                        int32_t const left = scores[r];
-                        int32_t const if_substitution = diagonal + cost;
-                        int32_t cell;
-                        if constexpr (affine_k) {
-                            int32_t const h = opt<minimize_k>(left + open, horizontal[r] + extend);
-                            int32_t const v = opt<minimize_k>(up + open, vertical + extend);
-                            horizontal[r] = h;
-                            vertical = v;
-                            int32_t const gap = opt<minimize_k>(h, v);
-                            if constexpr (local_k) cell = max(gap, max(if_substitution, 0));
-                            else cell = opt<minimize_k>(gap, if_substitution);
-                        }
-                        else {
-                            int32_t const gap = opt<minimize_k>(up, left) + open;
-                            if constexpr (local_k) cell = max(gap, max(if_substitution, 0));
-                            else cell = opt<minimize_k>(gap, if_substitution);
-                        }
-                        if constexpr (local_k) best[r] = max(best[r], cell);
-                        diagonal = left;
-                        up = cell;
-                        scores[r] = cell;
-                    }
-                    bottom_score = up;
-                    bottom_gap = vertical;
-                }
-                diagonal_top = up_score;
-                if constexpr (G == 64) {
-                    stage_scores = collect_from_last(bottom_score, stage_scores);
-                    if constexpr (affine_k) stage_gaps = collect_from_last(bottom_gap, stage_gaps);
-                }
-            }
-            // Lane 63's bottom row of the last G steps, oldest in lane 0: columns chunk - 62 .. chunk + 1.
-            if constexpr (G == 64) {
-                if (pass + 1 < passes) {
-                    int32_t const j_out = chunk + lane - 62;
-                    if (j_out >= 1 && j_out <= m) {
-                        scratch_scores[j_out] = stage_scores;
-                        if constexpr (affine_k) scratch_gaps[j_out] = stage_gaps;
-                    }
-                }
-            }
-        }
-        if constexpr (G == 64) {
-            if (pass + 1 < passes) __threadfence_block(); // lane 63's stores before lane 0's loads next pass
-        }
-
-        // Results.
-        if constexpr (local_k) {
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                if (lane_top + r + 1 <= n) running_best = max(running_best, best[r]);
-        }
-        else if (pass + 1 == passes) {
-            int32_t const last_row = n - 1 - base; // 0-based row within this stripe
-            if (active && last_row / R == k) {
-                int32_t value = scores[0];
-#pragma unroll
-                for (int r = 1; r < R; ++r)
-                    if (last_row % R == r) value = scores[r];
-                segment.results[task.output] = (int64_t)value;
-                if (task.mirror != no_mirror_k) segment.results[task.mirror] = (int64_t)value;
-            }
-        }
+        int32_t const base = pass * stripe_rows;
+        bool const more = pass + 1 < passes;
+        sweep.begin_stripe(base);
+        for (int32_t chunk = 0; chunk < chunks; ++chunk)
+            sweep.run_chunk(chunk, pass ? scores_row : nullptr, gaps_row, more ? scores_row : nullptr, gaps_row);
+        if (more) __threadfence_block(); // lane 63's stores before lane 0's loads in the next pass
+        sweep.end_stripe(base, !more);
     }
-    if constexpr (local_k) {
-        for (int offset = 1; offset < G; offset <<= 1) running_best = max(running_best, __shfl_xor(running_best, offset, 64));
-        if (active && k == 0) {
-            segment.results[task.output] = (int64_t)running_best;
-            if (task.mirror != no_mirror_k) segment.results[task.mirror] = (int64_t)running_best;
+    if constexpr (sweep_type_::reduce_k != reduce_none_k) sweep.write_result(sweep.partial());
+}
+
+/** All waves of the workgroup pipeline the stripes of one pair (see the file comment for the schedule). */
+template <typename sweep_type_>
+__device__ void run_cooperative(item_args_t const &a) {
+    constexpr int32_t W = waves_per_group_k;
+    static_assert(sweep_type_::group_k == 64, "cooperative items sweep 64-lane stripes");
+    int32_t const wave = (int32_t)(threadIdx.x / 64);
+    sweep_type_ sweep;
+    sweep.load(a, a.order[a.first], true, (uint32_t)wave);
+    int32_t const stripe_rows = 64 * sweep_type_::rows_per_lane;
+    int32_t const stripes = (sweep.n + stripe_rows - 1) / stripe_rows;
+    int32_t const chunks = (sweep.m + 126) / 64;
+    int32_t const delay = max(2, (chunks + W - 1) / W);
+    int32_t const phases = (stripes - 1) * delay + chunks;
+    size_t const row_pitch = 2 * (size_t)a.scratch_capacity;
+    if constexpr (sweep_type_::reduce_k != reduce_none_k) {
+        if (threadIdx.x == 0) *a.shared_result = 0;
+    }
+    for (int32_t phase = 0; phase < phases; ++phase) {
+        int32_t stripe = -1, chunk = 0;
+        if (phase >= wave * delay) {
+            int32_t const since = phase - wave * delay;
+            int32_t const candidate = wave + (since / (W * delay)) * W;
+            int32_t const at = since % (W * delay);
+            if (candidate < stripes && at < chunks) stripe = candidate, chunk = at;
         }
+        if (stripe >= 0) { // wave-uniform
+            int32_t const base = stripe * stripe_rows;
+            if (chunk == 0) sweep.begin_stripe(base);
+            int32_t *in_row = stripe ? a.scratch + ((stripe - 1) % W) * row_pitch : nullptr;
+            int32_t *out_row = stripe + 1 < stripes ? a.scratch + (stripe % W) * row_pitch : nullptr;
+            sweep.run_chunk(chunk, in_row, in_row ? in_row + a.scratch_capacity : nullptr, out_row,
+                            out_row ? out_row + a.scratch_capacity : nullptr);
+            if (chunk == chunks - 1) sweep.end_stripe(base, stripe == stripes - 1);
+        }
+        __syncthreads(); // publishes this phase's boundary rows to the next stripe's wave
+    }
+    if constexpr (sweep_type_::reduce_k != reduce_none_k) {
+        int32_t const mine = sweep.partial();
+        if (__lane_id() == 0) {
+            if constexpr (sweep_type_::reduce_k == reduce_max_k) atomicMax(a.shared_result, mine);
+            else atomicAdd(a.shared_result, mine);
+        }
+        __syncthreads();
+        if (wave == 0) sweep.write_result(*a.shared_result);
     }
 }
 
@@ -302,40 +106,58 @@ __device__ void score_item(item_args_t const &args) {
 #pragma region Dispatch
 
 template <int family_, typename symbol_type_, int shape_>
-__device__ __forceinline__ void run_shape(item_args_t const &args) {
+__device__ __forceinline__ void run_shape(item_args_t const &a) {
     constexpr shape_t shape = shapes_k[shape_];
-    score_item<family_, symbol_type_, shape.group, shape.rows_per_lane>(args);
+    constexpr bool uniform = family_traits<family_>::uniform_k;
+    if constexpr (shape.bit_parallel) {
+        // Myers shapes exist for the unit-cost byte Levenshtein only (the binning never picks them otherwise).
+        if constexpr (family_ == family_lev_linear_k && sizeof(symbol_type_) == 1) {
+            if constexpr (shape.cooperative) run_cooperative<myers_sweep_t<64>>(a);
+            else run_independent<myers_sweep_t<shape.group>>(a);
+        }
+    }
+    // Class-cost families keep their LDS profile at <= 8 rows per lane; larger shapes are never binned for them.
+    else if constexpr (!uniform && shape.rows_per_lane > 8) return;
+    else if constexpr (shape.cooperative)
+        run_cooperative<sweep_t<family_, symbol_type_, 64, shape.rows_per_lane>>(a);
+    else run_independent<sweep_t<family_, symbol_type_, shape.group, shape.rows_per_lane>>(a);
 }
 
 template <int family_, typename symbol_type_>
-__device__ void run_family(int shape, item_args_t const &args) {
+__device__ void run_family(int shape, item_args_t const &a) {
     switch (shape) {
-    case 0: run_shape<family_, symbol_type_, 0>(args); break;
-    case 1: run_shape<family_, symbol_type_, 1>(args); break;
-    case 2: run_shape<family_, symbol_type_, 2>(args); break;
-    case 3: run_shape<family_, symbol_type_, 3>(args); break;
-    case 4: run_shape<family_, symbol_type_, 4>(args); break;
-    case 5: run_shape<family_, symbol_type_, 5>(args); break;
-    case 6: run_shape<family_, symbol_type_, 6>(args); break;
-    case 7: run_shape<family_, symbol_type_, 7>(args); break;
-    case 8: run_shape<family_, symbol_type_, 8>(args); break;
+    case 0: run_shape<family_, symbol_type_, 0>(a); break;
+    case 1: run_shape<family_, symbol_type_, 1>(a); break;
+    case 2: run_shape<family_, symbol_type_, 2>(a); break;
+    case 3: run_shape<family_, symbol_type_, 3>(a); break;
+    case 4: run_shape<family_, symbol_type_, 4>(a); break;
+    case 5: run_shape<family_, symbol_type_, 5>(a); break;
+    case 6: run_shape<family_, symbol_type_, 6>(a); break;
+    case 7: run_shape<family_, symbol_type_, 7>(a); break;
+    case 8: run_shape<family_, symbol_type_, 8>(a); break;
+    case 9: run_shape<family_, symbol_type_, 9>(a); break;
+    case 10: run_shape<family_, symbol_type_, 10>(a); break;
+    case 11: run_shape<family_, symbol_type_, 11>(a); break;
+    case 12: run_shape<family_, symbol_type_, 12>(a); break;
+    case 13: run_shape<family_, symbol_type_, 13>(a); break;
+    case 14: run_shape<family_, symbol_type_, 14>(a); break;
     default: break;
     }
 }
-static_assert(shapes_count_k == 9, "dispatch switch lists every shape");
+static_assert(shapes_count_k == 15, "dispatch switch lists every shape");
 
-__device__ void run_variant(int variant, item_args_t const &args) {
+__device__ void run_variant(int variant, item_args_t const &a) {
     int const shape = variant % shapes_count_k;
     int const combo = variant / shapes_count_k; // family * 2 + symbols
     switch (combo) {
-    case family_lev_linear_k * 2 + symbols_u8_k: run_family<family_lev_linear_k, uint8_t>(shape, args); break;
-    case family_lev_linear_k * 2 + symbols_u32_k: run_family<family_lev_linear_k, uint32_t>(shape, args); break;
-    case family_lev_affine_k * 2 + symbols_u8_k: run_family<family_lev_affine_k, uint8_t>(shape, args); break;
-    case family_lev_affine_k * 2 + symbols_u32_k: run_family<family_lev_affine_k, uint32_t>(shape, args); break;
-    case family_nw_linear_k * 2 + symbols_u8_k: run_family<family_nw_linear_k, uint8_t>(shape, args); break;
-    case family_nw_affine_k * 2 + symbols_u8_k: run_family<family_nw_affine_k, uint8_t>(shape, args); break;
-    case family_sw_linear_k * 2 + symbols_u8_k: run_family<family_sw_linear_k, uint8_t>(shape, args); break;
-    case family_sw_affine_k * 2 + symbols_u8_k: run_family<family_sw_affine_k, uint8_t>(shape, args); break;
+    case family_lev_linear_k * 2 + symbols_u8_k: run_family<family_lev_linear_k, uint8_t>(shape, a); break;
+    case family_lev_linear_k * 2 + symbols_u32_k: run_family<family_lev_linear_k, uint32_t>(shape, a); break;
+    case family_lev_affine_k * 2 + symbols_u8_k: run_family<family_lev_affine_k, uint8_t>(shape, a); break;
+    case family_lev_affine_k * 2 + symbols_u32_k: run_family<family_lev_affine_k, uint32_t>(shape, a); break;
+    case family_nw_linear_k * 2 + symbols_u8_k: run_family<family_nw_linear_k, uint8_t>(shape, a); break;
+    case family_nw_affine_k * 2 + symbols_u8_k: run_family<family_nw_affine_k, uint8_t>(shape, a); break;
+    case family_sw_linear_k * 2 + symbols_u8_k: run_family<family_sw_linear_k, uint8_t>(shape, a); break;
+    case family_sw_affine_k * 2 + symbols_u8_k: run_family<family_sw_affine_k, uint8_t>(shape, a); break;
     default: break; // class-cost families never run over runes
     }
 }
@@ -351,27 +173,34 @@ struct persistent_args_t {
     bins_view_t bins;
     int32_t *scratch;
     uint32_t scratch_capacity;
+    uint32_t profile_bytes;
 };
 
 /**
- *  Persistent scoring kernel: one 64-lane workgroup per resident wave slot; each wave pops work items (64/G pairs
- *  of one bin) from a device counter, heaviest bins first, until the queue is drained. The queue holds
- *  `item_start[bins]` items; every wave exits once the counter passes it.
+ *  Persistent scoring kernel: each 4-wave workgroup pops work items from a device counter, heaviest bins first,
+ *  until the queue (`item_start[bins]` items) is drained; every workgroup exits once the counter passes it.
  */
-__global__ __launch_bounds__(64) void persistent_score_kernel(persistent_args_t args) {
-    extern __shared__ int8_t profile_lds[];
+__global__ __launch_bounds__(256) void persistent_score_kernel(persistent_args_t args) {
+    extern __shared__ int8_t profiles_lds[];
+    __shared__ myers_tables_t myers_lds[waves_per_group_k];
+    __shared__ uint32_t taken_shared;
+    __shared__ int32_t result_shared;
     uint32_t const total_items = args.bins.item_start[bins_count_k];
     item_args_t item;
     item.tasks = args.tasks;
     item.segments = args.segments;
     item.order = args.bins.order;
-    item.scratch = args.scratch + (size_t)blockIdx.x * 2 * args.scratch_capacity;
+    item.scratch = args.scratch + (size_t)blockIdx.x * waves_per_group_k * 2 * args.scratch_capacity;
     item.scratch_capacity = args.scratch_capacity;
-    item.profile = profile_lds;
+    item.profiles = profiles_lds;
+    item.profile_bytes = args.profile_bytes;
+    item.myers = myers_lds;
+    item.shared_result = &result_shared;
     for (;;) {
-        uint32_t taken = 0;
-        if (__lane_id() == 0) taken = atomicAdd(args.bins.queue_head, 1u);
-        taken = __shfl(taken, 0, 64);
+        if (threadIdx.x == 0) taken_shared = atomicAdd(args.bins.queue_head, 1u);
+        __syncthreads();
+        uint32_t const taken = taken_shared;
+        __syncthreads();
         if (taken >= total_items) break;
         // Priority position p with item_start[p] <= taken < item_start[p + 1].
         uint32_t low = 0, high = bins_count_k;
@@ -382,8 +211,7 @@ __global__ __launch_bounds__(64) void persistent_score_kernel(persistent_args_t 
         }
         uint32_t const bin = args.bins.priority[low];
         int const variant = (int)(bin / column_buckets_k);
-        shape_t const shape = shapes_k[variant % shapes_count_k];
-        uint32_t const per_item = 64u / (uint32_t)shape.group;
+        uint32_t const per_item = pairs_per_item(shapes_k[variant % shapes_count_k]);
         uint32_t const item_in_bin = taken - args.bins.item_start[low];
         uint32_t const in_bin = args.bins.histogram[bin];
         item.first = args.bins.task_start[low] + item_in_bin * per_item;
@@ -392,11 +220,14 @@ __global__ __launch_bounds__(64) void persistent_score_kernel(persistent_args_t 
     }
 }
 
-/** One-pair launch for the per-call baseline: the same scorer, one wave, no binning. */
-__global__ __launch_bounds__(64) void single_task_kernel(task_t const *tasks, segment_view_t const *segments,
-                                                         uint32_t const *identity, uint32_t task_index, int variant,
-                                                         int32_t *scratch, uint32_t scratch_capacity) {
-    extern __shared__ int8_t profile_lds[];
+/** One-pair launch for the per-call baseline: the same scorers, one workgroup, no binning. */
+__global__ __launch_bounds__(256) void single_task_kernel(task_t const *tasks, segment_view_t const *segments,
+                                                          uint32_t const *identity, uint32_t task_index, int variant,
+                                                          int32_t *scratch, uint32_t scratch_capacity,
+                                                          uint32_t profile_bytes) {
+    extern __shared__ int8_t profiles_lds[];
+    __shared__ myers_tables_t myers_lds[waves_per_group_k];
+    __shared__ int32_t result_shared;
     item_args_t item;
     item.tasks = tasks;
     item.segments = segments;
@@ -405,7 +236,10 @@ __global__ __launch_bounds__(64) void single_task_kernel(task_t const *tasks, se
     item.count = 1;
     item.scratch = scratch;
     item.scratch_capacity = scratch_capacity;
-    item.profile = profile_lds;
+    item.profiles = profiles_lds;
+    item.profile_bytes = profile_bytes;
+    item.myers = myers_lds;
+    item.shared_result = &result_shared;
     run_variant(variant, item);
 }
 
@@ -510,18 +344,23 @@ __global__ void materialize_kernel(materialize_args_t a, shape_table_t const *ta
         else {
             uint32_t const rows = task.rows_length;
             int const family = a.family, symbols = a.symbols;
-            int shape = table->multi_pass_shape[family][symbols];
-            for (int s = 0; s < table->used[family][symbols]; ++s) {
-                int const candidate_shape = table->order[family][symbols][s];
-                if ((int64_t)rows <= table->limits[family][symbols][candidate_shape]) {
+            int const slot = (a.unit_costs && family == family_lev_linear_k && symbols == symbols_u8_k)
+                                 ? unit_bytes_slot_k : family * 2 + symbols;
+            int shape = table->multi_pass_shape[slot];
+            for (int s = 0; s < table->used[slot]; ++s) {
+                int const candidate_shape = table->order[slot][s];
+                if ((int64_t)rows <= table->limits[slot][candidate_shape]) {
                     shape = candidate_shape;
                     break;
                 }
             }
             shape_t const sh = shapes_k[shape];
+            // Latency of the item in lane-row-steps, the key for heaviest-first ordering.
             uint64_t const stripe = (uint64_t)sh.group * sh.rows_per_lane;
-            uint64_t const passes = (rows + stripe - 1) / stripe;
-            uint64_t const work = passes * ((uint64_t)task.columns_length + sh.group);
+            uint64_t passes = (rows + stripe - 1) / stripe;
+            if (sh.cooperative) passes = (passes + waves_per_group_k - 1) / waves_per_group_k + 1;
+            uint64_t const per_step = sh.bit_parallel ? 8u : (uint64_t)sh.rows_per_lane; // ~DP rows of VALU work
+            uint64_t const work = passes * ((uint64_t)task.columns_length + sh.group) * per_step;
             bin = (uint32_t)variant_of(family, symbols, shape) * column_buckets_k + half_octave(work);
             atomicAdd(&histogram[bin], 1u);
         }
@@ -544,8 +383,7 @@ __global__ __launch_bounds__(1024) void scan_bins_kernel(bins_view_t bins) {
         uint32_t tasks_here = 0, items_here = 0;
         if (p < bins_count_k) {
             uint32_t const bin = bins.priority[p];
-            shape_t const shape = shapes_k[(bin / column_buckets_k) % shapes_count_k];
-            uint32_t const per_item = 64u / (uint32_t)shape.group;
+            uint32_t const per_item = pairs_per_item(shapes_k[(bin / column_buckets_k) % shapes_count_k]);
             tasks_here = bins.histogram[bin];
             items_here = (tasks_here + per_item - 1) / per_item;
         }
@@ -643,28 +481,32 @@ hipError_t szs_amd_launch_scatter_bins(szs_amd::bins_view_t const *bins, uint32_
 
 hipError_t szs_amd_launch_persistent(szs_amd::task_t const *tasks, szs_amd::segment_view_t const *segments,
                                      szs_amd::bins_view_t const *bins, int32_t *scratch, uint32_t scratch_capacity,
-                                     uint32_t waves, uint32_t lds_bytes, hipStream_t stream) {
+                                     uint32_t workgroups, uint32_t profile_bytes, hipStream_t stream) {
     szs_amd::persistent_args_t args;
     args.tasks = tasks;
     args.segments = segments;
     args.bins = *bins;
     args.scratch = scratch;
     args.scratch_capacity = scratch_capacity;
-    hipLaunchKernelGGL(szs_amd::persistent_score_kernel, dim3(waves), dim3(64), lds_bytes, stream, args);
+    args.profile_bytes = profile_bytes;
+    hipLaunchKernelGGL(szs_amd::persistent_score_kernel, dim3(workgroups), dim3(64 * szs_amd::waves_per_group_k),
+                       profile_bytes * szs_amd::waves_per_group_k, stream, args);
     return hipGetLastError();
 }
 
 hipError_t szs_amd_launch_single(szs_amd::task_t const *tasks, szs_amd::segment_view_t const *segments,
                                  uint32_t const *identity, uint32_t task_index, int variant, int32_t *scratch,
-                                 uint32_t scratch_capacity, uint32_t lds_bytes, hipStream_t stream) {
-    hipLaunchKernelGGL(szs_amd::single_task_kernel, dim3(1), dim3(64), lds_bytes, stream, tasks, segments, identity,
-                       task_index, variant, scratch, scratch_capacity);
+                                 uint32_t scratch_capacity, uint32_t profile_bytes, hipStream_t stream) {
+    hipLaunchKernelGGL(szs_amd::single_task_kernel, dim3(1), dim3(64 * szs_amd::waves_per_group_k),
+                       profile_bytes * szs_amd::waves_per_group_k, stream, tasks, segments, identity, task_index,
+                       variant, scratch, scratch_capacity, profile_bytes);
     return hipGetLastError();
 }
 
-hipError_t szs_amd_persistent_occupancy(int *blocks_per_cu, uint32_t lds_bytes) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, szs_amd::persistent_score_kernel, 64,
-                                                        lds_bytes);
+hipError_t szs_amd_persistent_occupancy(int *workgroups_per_cu, uint32_t profile_bytes) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(workgroups_per_cu, szs_amd::persistent_score_kernel,
+                                                        64 * szs_amd::waves_per_group_k,
+                                                        profile_bytes * szs_amd::waves_per_group_k);
 }
 
 } // extern "C"

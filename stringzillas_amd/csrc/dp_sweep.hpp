@@ -1,0 +1,330 @@
+/**
+ *  @file dp_sweep.hpp
+ *  @brief The wave-systolic dynamic-programming sweep: one stripe of rows of one pair, column after column.
+ *
+ *  Recurrences restated from include/stringzillas/similarities/serial.hpp:669-1079 (the four tile scorers) with
+ *  the border conventions of its walkers (serial.hpp:1153-1158, 1310-1320, 1627-1633):
+ *  - linear global:  H = opt(opt(up, left) + g, diag + s)
+ *  - linear local:   H = max(max(up, left) + g, max(diag + s, 0)), best over all cells, starting at 0
+ *  - affine global:  E = opt(H_left + o, E_left + e), V = opt(H_up + o, V_up + e), H = opt(opt(E, V), diag + s)
+ *  - affine local:   H = max(max(E, V), max(diag + s, 0))
+ *  Border scores: H[i][0] = g*i (linear) or o + e*(i-1) (affine, 0 at i = 0), 0 for local; the gap tracks start
+ *  from the discard value (o + e) + H_border.
+ *
+ *  Layout: a group of G lanes holds one stripe of G*R rows of the shorter string, R consecutive rows per lane,
+ *  in VGPRs. At step t lane k updates its R rows at column j = t + 1 - k, then passes its bottom score, bottom
+ *  vertical-gap value and the column symbol to lane k+1 with one DPP shift each. Lane 0 receives the row above
+ *  the stripe: the matrix border, or the previous stripe's bottom row read back from HBM scratch. Lane G-1's
+ *  bottom row is collected 64 columns at a time (G = 64) and written to scratch for the next stripe.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "szs_device.hpp"
+
+namespace szs_amd {
+
+#pragma region DPP helpers
+
+enum : int {
+    dpp_row_shr1_k = 0x111,  /**< lane i <- lane i-1 within each row of 16; lane 0 of a row keeps `old` */
+    dpp_row_ror15_k = 0x12F, /**< lane i <- lane (i+1) mod 16 within each row */
+    dpp_wave_shl1_k = 0x130, /**< lane i <- lane i+1; lane 63 keeps `old` */
+    dpp_wave_rol1_k = 0x134, /**< lane i <- lane (i+1) mod 64 */
+    dpp_wave_shr1_k = 0x138, /**< lane i <- lane i-1; lane 0 keeps `old` */
+};
+
+template <int G>
+__device__ __forceinline__ int32_t shift_down(int32_t old, int32_t src) {
+    if constexpr (G == 64) return __builtin_amdgcn_update_dpp(old, src, dpp_wave_shr1_k, 0xF, 0xF, false);
+    else return __builtin_amdgcn_update_dpp(old, src, dpp_row_shr1_k, 0xF, 0xF, false);
+}
+
+template <int G>
+__device__ __forceinline__ int32_t rotate_up(int32_t src) {
+    if constexpr (G == 64) return __builtin_amdgcn_update_dpp(src, src, dpp_wave_rol1_k, 0xF, 0xF, false);
+    else return __builtin_amdgcn_update_dpp(src, src, dpp_row_ror15_k, 0xF, 0xF, false);
+}
+
+/** Shift register toward lane 0 in which lane 63 appends its own `fresh` value (G = 64 only). */
+__device__ __forceinline__ int32_t collect_from_last(int32_t fresh, int32_t stage) {
+    return __builtin_amdgcn_update_dpp(fresh, stage, dpp_wave_shl1_k, 0xF, 0xF, false);
+}
+
+#pragma endregion
+
+#pragma region Family traits
+
+template <int family_>
+struct family_traits {
+    static constexpr bool minimize_k = family_ == family_lev_linear_k || family_ == family_lev_affine_k;
+    static constexpr bool affine_k =
+        family_ == family_lev_affine_k || family_ == family_nw_affine_k || family_ == family_sw_affine_k;
+    static constexpr bool local_k = family_ == family_sw_linear_k || family_ == family_sw_affine_k;
+    static constexpr bool uniform_k = minimize_k;
+};
+
+template <bool minimize_>
+__device__ __forceinline__ int32_t opt(int32_t a, int32_t b) {
+    if constexpr (minimize_) return a < b ? a : b;
+    else return a > b ? a : b;
+}
+
+/** Score of the matrix border at distance `k` from the origin (serial.hpp:142-151, 716-727, 889-895). */
+template <int family_>
+__device__ __forceinline__ int32_t border_score(int32_t k, int32_t open, int32_t extend) {
+    using traits = family_traits<family_>;
+    if constexpr (traits::local_k) return 0;
+    else if constexpr (traits::affine_k) return k ? open + extend * (k - 1) : 0;
+    else return open * k;
+}
+
+#pragma endregion
+
+struct myers_tables_t;
+
+/** Everything a workgroup needs to score one work item. */
+struct item_args_t {
+    task_t const *tasks;
+    segment_view_t const *segments;
+    uint32_t const *order; /**< task ids; `first .. first + count` belong to this item */
+    uint32_t first;
+    uint32_t count;
+    int32_t *scratch;      /**< this workgroup's 4 x 2 x `scratch_capacity` int32 rows */
+    uint32_t scratch_capacity;
+    int8_t *profiles;      /**< this workgroup's LDS profiles, `profile_bytes` per wave */
+    uint32_t profile_bytes;
+    myers_tables_t *myers; /**< this workgroup's LDS symbol tables, one per wave */
+    int32_t *shared_result; /**< LDS word for cooperative reductions */
+};
+
+/** How a sweep's per-lane partial results combine into the pair's score. */
+enum reduce_kind_t { reduce_none_k = 0, reduce_max_k = 1, reduce_sum_k = 2 };
+
+/** Per-lane state of one group sweeping one stripe of one pair. */
+template <int family_, typename symbol_type_, int G, int R>
+struct sweep_t {
+    using traits = family_traits<family_>;
+    static constexpr bool minimize_k = traits::minimize_k;
+    static constexpr bool affine_k = traits::affine_k;
+    static constexpr bool local_k = traits::local_k;
+    static constexpr bool uniform_k = traits::uniform_k;
+    static constexpr int RP = (R + 3) / 4 * 4; // profile bytes per lane and class, padded to whole dwords
+    static constexpr int group_k = G;
+    static constexpr int rows_per_lane = R;
+    static constexpr reduce_kind_t reduce_k = local_k ? reduce_max_k : reduce_none_k;
+
+    // The pair.
+    int lane, k;
+    bool active;
+    int32_t n, m;
+    int32_t open, extend, match, mismatch, dense_count;
+    bool transposed;
+    engine_params_t const *params;
+    symbol_type_ const *rows;
+    symbol_type_ const *columns;
+    int64_t *results;
+    uint64_t output, mirror;
+    int8_t *profile; /**< this wave's LDS profile: [dense class][lane][RP] int8 */
+
+    // The stripe.
+    int32_t lane_top;
+    uint32_t row_symbols[R];
+    int32_t scores[R];
+    int32_t horizontal[R];
+    int32_t best[R];
+    int32_t diagonal_top, bottom_score, bottom_gap;
+    uint32_t column_symbol;
+    int32_t running_best;
+
+    __device__ __forceinline__ void load(item_args_t const &a, uint32_t task_index, bool is_active, uint32_t wave) {
+        int8_t *wave_profile = a.profiles + wave * a.profile_bytes;
+        lane = (int)__lane_id();
+        k = lane % G;
+        active = is_active;
+        task_t const task = a.tasks[task_index];
+        segment_view_t const segment = a.segments[task.segment];
+        params = segment.params;
+        n = active ? (int32_t)task.rows_length : 0;
+        m = active ? (int32_t)task.columns_length : 0;
+        open = params->open, extend = params->extend;
+        match = params->match, mismatch = params->mismatch;
+        dense_count = params->dense_count;
+        transposed = task.transposed != 0;
+        symbol_type_ const *query = (symbol_type_ const *)segment.query_symbols;
+        symbol_type_ const *candidate = (symbol_type_ const *)segment.candidate_symbols;
+        rows = (transposed ? candidate : query) + task.rows_offset;
+        columns = (transposed ? query : candidate) + task.columns_offset;
+        results = segment.results;
+        output = task.output;
+        mirror = task.mirror;
+        profile = wave_profile;
+        running_best = 0; // serial.hpp:781 starts the local best at 0
+    }
+
+    /** Loads the rows `base + k*R .. + R` of this lane and resets the stripe state to the column-0 border. */
+    __device__ __forceinline__ void begin_stripe(int32_t base) {
+        lane_top = base + k * R;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            int32_t const i = lane_top + r + 1; // 1-based matrix row
+            uint32_t symbol = 0xFFFFFFFFu;
+            if (i <= n) symbol = (uint32_t)rows[i - 1];
+            row_symbols[r] = symbol;
+            scores[r] = border_score<family_>(i, open, extend);
+            horizontal[r] = (open + extend) + scores[r]; // discard value of the horizontal track on column 0
+            best[r] = 0;
+        }
+        if constexpr (!uniform_k) {
+            int8_t *mine = profile + lane * RP;
+            for (int32_t c = 0; c < dense_count; ++c) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    uint32_t const row_class =
+                        row_symbols[r] == 0xFFFFFFFFu ? 0u : params->byte_to_dense[row_symbols[r] & 0xFF];
+                    int8_t const cost = transposed ? params->dense_costs[c * max_dense_classes_k + row_class]
+                                                   : params->dense_costs[row_class * max_dense_classes_k + c];
+                    mine[c * 64 * RP + r] = cost;
+                }
+            }
+        }
+        diagonal_top = border_score<family_>(lane_top, open, extend); // H[top - 1][0]
+        bottom_score = scores[R - 1];
+        bottom_gap = 0;
+        column_symbol = 0;
+    }
+
+    /**
+     *  Steps `chunk * G .. + G`. The row above the stripe comes from `in_scores`/`in_gaps` (indexed by 1-based
+     *  column) or, when null, from the matrix border. When `out_scores` is set (G = 64 only), lane 63's bottom
+     *  row is stored there for the next stripe.
+     */
+    __device__ __forceinline__ void run_chunk(int32_t chunk, int32_t const *in_scores, int32_t const *in_gaps,
+                                              int32_t *out_scores, int32_t *out_gaps) {
+        int32_t const first_step = chunk * G;
+        int32_t const j_lane = first_step + k + 1; // 1-based column this lane loads for its group
+        uint32_t chunk_symbols = 0;
+        if (j_lane <= m) {
+            uint32_t const raw = (uint32_t)columns[j_lane - 1];
+            chunk_symbols = uniform_k ? raw : (uint32_t)params->byte_to_dense[raw & 0xFF];
+        }
+        int32_t chunk_scores, chunk_gaps;
+        if (in_scores) {
+            chunk_scores = j_lane <= m ? in_scores[j_lane] : 0;
+            chunk_gaps = (affine_k && j_lane <= m) ? in_gaps[j_lane] : 0;
+        }
+        else {
+            chunk_scores = border_score<family_>(j_lane, open, extend);
+            chunk_gaps = (open + extend) + chunk_scores; // discard value of the vertical track on row 0
+        }
+        int32_t stage_scores = 0, stage_gaps = 0;
+
+#pragma unroll 2
+        for (int32_t step = 0; step < G; ++step) {
+            int32_t const t = first_step + step;
+            int32_t const up_score = shift_down<G>(chunk_scores, bottom_score);
+            int32_t up_gap = 0;
+            if constexpr (affine_k) up_gap = shift_down<G>(chunk_gaps, bottom_gap);
+            column_symbol = (uint32_t)shift_down<G>((int32_t)chunk_symbols, (int32_t)column_symbol);
+            chunk_scores = rotate_up<G>(chunk_scores);
+            if constexpr (affine_k) chunk_gaps = rotate_up<G>(chunk_gaps);
+            chunk_symbols = (uint32_t)rotate_up<G>((int32_t)chunk_symbols);
+
+            int32_t const j = t + 1 - k;
+            if ((uint32_t)(j - 1) < (uint32_t)m) {
+                int32_t diagonal = diagonal_top;
+                int32_t up = up_score;
+                int32_t vertical = up_gap;
+                uint32_t costs_packed[RP / 4];
+                if constexpr (!uniform_k) {
+                    uint32_t const *row_costs = (uint32_t const *)(profile + (column_symbol * 64 + lane) * RP);
+#pragma unroll
+                    for (int w = 0; w < RP / 4; ++w) costs_packed[w] = row_costs[w];
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    int32_t cost;
+                    if constexpr (uniform_k) cost = row_symbols[r] == column_symbol ? match : mismatch;
+                    else cost = (int32_t)(int8_t)(costs_packed[r / 4] >> (8 * (r % 4)));
+                    int32_t const left = scores[r];
+                    int32_t const if_substitution = diagonal + cost;
+                    int32_t cell;
+                    if constexpr (affine_k) {
+                        int32_t const h = opt<minimize_k>(left + open, horizontal[r] + extend);
+                        int32_t const v = opt<minimize_k>(up + open, vertical + extend);
+                        horizontal[r] = h;
+                        vertical = v;
+                        int32_t const gap = opt<minimize_k>(h, v);
+                        if constexpr (local_k) cell = max(gap, max(if_substitution, 0));
+                        else cell = opt<minimize_k>(gap, if_substitution);
+                    }
+                    else {
+                        int32_t const gap = opt<minimize_k>(up, left) + open;
+                        if constexpr (local_k) cell = max(gap, max(if_substitution, 0));
+                        else cell = opt<minimize_k>(gap, if_substitution);
+                    }
+                    if constexpr (local_k) best[r] = max(best[r], cell);
+                    diagonal = left;
+                    up = cell;
+                    scores[r] = cell;
+                }
+                bottom_score = up;
+                bottom_gap = vertical;
+            }
+            diagonal_top = up_score;
+            if constexpr (G == 64) {
+                if (out_scores) {
+                    stage_scores = collect_from_last(bottom_score, stage_scores);
+                    if constexpr (affine_k) stage_gaps = collect_from_last(bottom_gap, stage_gaps);
+                }
+            }
+        }
+        // Lane 63's bottom row of these 64 steps, oldest in lane 0: columns first_step - 62 .. first_step + 1.
+        if constexpr (G == 64) {
+            if (out_scores) {
+                int32_t const j_out = first_step + lane - 62;
+                if (j_out >= 1 && j_out <= m) {
+                    out_scores[j_out] = stage_scores;
+                    if constexpr (affine_k) out_gaps[j_out] = stage_gaps;
+                }
+            }
+        }
+    }
+
+    /** Folds the stripe into the result: local best of its valid rows, or the final score of a global pair. */
+    __device__ __forceinline__ void end_stripe(int32_t base, bool last_stripe) {
+        if constexpr (local_k) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (lane_top + r + 1 <= n) running_best = max(running_best, best[r]);
+        }
+        else if (last_stripe) {
+            int32_t const last_row = n - 1 - base; // 0-based row within this stripe
+            if (active && last_row / R == k) {
+                int32_t value = scores[0];
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    if (last_row % R == r) value = scores[r];
+                results[output] = (int64_t)value;
+                if (mirror != no_mirror_k) results[mirror] = (int64_t)value;
+            }
+        }
+    }
+
+    /** Local alignment: maximum of `running_best` over the group's lanes. */
+    __device__ __forceinline__ int32_t partial() const {
+        int32_t value = running_best;
+        for (int offset = 1; offset < G; offset <<= 1) value = max(value, __shfl_xor(value, offset, 64));
+        return value;
+    }
+
+    __device__ __forceinline__ void write_result(int32_t value) const {
+        if (active && k == 0) {
+            results[output] = (int64_t)value;
+            if (mirror != no_mirror_k) results[mirror] = (int64_t)value;
+        }
+    }
+};
+
+} // namespace szs_amd

@@ -1,0 +1,231 @@
+/**
+ *  @file myers_sweep.hpp
+ *  @brief Bit-parallel unit-cost Levenshtein (match 0, mismatch 1, gap 1) over bytes: Myers' algorithm in
+ *         Hyyrö's block form, laid out on the same wave-systolic sweep as the DP kernels.
+ *
+ *  The reference routes unit costs to Myers as well (include/stringzillas/similarities/serial.hpp:1657-1865,
+ *  2126-2140); results equal the unit-cost DP, which is what the tests pin against the oracle.
+ *
+ *  Layout: lane k of a group owns one 64-row word of the stripe (rows 64k .. 64k+63, bit 0 = top row) as the
+ *  vertical-delta vectors Pv/Mv. At step t it processes column j = t + 1 - k and passes the horizontal delta of
+ *  its bottom row (+1, 0 or -1, coded in 2 bits) to lane k+1 by DPP; lane 0 receives +1 from the matrix border
+ *  or the previous stripe's bottom deltas from HBM scratch. The match vector of a column symbol is built from
+ *  bit-planes: plane b holds bit b of every row's symbol code, and Eq = AND_b (plane_b XNOR bit b of the column
+ *  code), one `v_bitop3` per plane and half-word. For 64-lane groups the symbols are first renumbered to a dense
+ *  code (0 = absent from the rows) so that only ceil(log2(distinct + 1)) planes are needed, e.g. 3 for DNA.
+ *  The distance is m + the sum of the vertical deltas of the last column over all rows.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dp_sweep.hpp"
+#include "szs_device.hpp"
+
+namespace szs_amd {
+
+enum : int {
+    dpp_quad_shr1_k = 0x90,  /**< quad_perm [0,0,1,2]: lane i <- lane i-1 within each quad (lane 0 <- itself) */
+    dpp_quad_rol1_k = 0x39,  /**< quad_perm [1,2,3,0]: lane i <- lane (i+1) mod 4 within each quad */
+};
+
+/** Shift toward higher lanes within groups of G; the group's first lane takes `old`. */
+template <int G>
+__device__ __forceinline__ uint32_t group_shift_down(uint32_t old, uint32_t src, bool first_in_group) {
+    if constexpr (G == 4) {
+        uint32_t const shifted = (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, dpp_quad_shr1_k, 0xF, 0xF, false);
+        return first_in_group ? old : shifted;
+    }
+    else return (uint32_t)shift_down<G>((int32_t)old, (int32_t)src);
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t group_rotate_up(uint32_t src) {
+    if constexpr (G == 4) return (uint32_t)__builtin_amdgcn_update_dpp((int)src, (int)src, dpp_quad_rol1_k, 0xF, 0xF, false);
+    else return (uint32_t)rotate_up<G>((int32_t)src);
+}
+
+/** LDS a 64-lane Myers group uses to renumber its pair's symbols. */
+struct myers_tables_t {
+    uint32_t presence[8]; /**< 256-bit set of the byte values in the rows string */
+    uint8_t codes[256];   /**< byte -> dense code, 0 for bytes absent from the rows */
+};
+
+template <int G>
+struct myers_sweep_t {
+    static constexpr int group_k = G;
+    static constexpr int rows_per_lane = 64;
+    static constexpr bool local_k = false;
+    static constexpr bool renumber_k = G == 64;
+    static constexpr reduce_kind_t reduce_k = reduce_sum_k;
+
+    int lane, k;
+    bool active;
+    int32_t n, m;
+    uint8_t const *rows, *columns;
+    int64_t *results;
+    uint64_t output, mirror;
+    myers_tables_t *tables;
+    int32_t planes_used;
+
+    int32_t lane_top;
+    uint64_t planes[8];
+    uint64_t pv, mv;
+    uint32_t column_code, bottom_code;
+    int32_t vertical_sum;
+
+    __device__ __forceinline__ uint32_t code_of(uint32_t byte) const {
+        if constexpr (renumber_k) return tables->codes[byte];
+        else return byte;
+    }
+
+    __device__ __forceinline__ void load(item_args_t const &a, uint32_t task_index, bool is_active, uint32_t wave) {
+        myers_tables_t *wave_tables = a.myers + wave;
+        lane = (int)__lane_id();
+        k = lane % G;
+        active = is_active;
+        task_t const task = a.tasks[task_index];
+        segment_view_t const segment = a.segments[task.segment];
+        n = active ? (int32_t)task.rows_length : 0;
+        m = active ? (int32_t)task.columns_length : 0;
+        uint8_t const *query = (uint8_t const *)segment.query_symbols;
+        uint8_t const *candidate = (uint8_t const *)segment.candidate_symbols;
+        // Uniform costs are symmetric in the operands, so the transposition needs no compensation here.
+        rows = (task.transposed ? candidate : query) + task.rows_offset;
+        columns = (task.transposed ? query : candidate) + task.columns_offset;
+        results = segment.results;
+        output = task.output;
+        mirror = task.mirror;
+        tables = wave_tables;
+        vertical_sum = 0;
+        planes_used = 8;
+        if constexpr (renumber_k) {
+            // One pair per wave: collect the byte values of the rows, then number them 1..D in byte order.
+            if (lane < 8) tables->presence[lane] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            for (int32_t i = lane; i < n; i += 64) {
+                uint32_t const byte = rows[i];
+                atomicOr(&tables->presence[byte >> 5], 1u << (byte & 31));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            uint32_t words[8];
+            int32_t distinct = 0;
+#pragma unroll
+            for (int w = 0; w < 8; ++w) words[w] = tables->presence[w], distinct += __builtin_popcount(words[w]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t const value = (uint32_t)lane * 4 + q;
+                uint32_t const word = value >> 5, bit = value & 31;
+                int32_t below = 0;
+#pragma unroll
+                for (int w = 0; w < 8; ++w)
+                    if ((uint32_t)w < word) below += __builtin_popcount(words[w]);
+                below += __builtin_popcount(words[word] & ((1u << bit) - 1u));
+                bool const present = (words[word] >> bit) & 1u;
+                // With all 256 values present there is no absent code left: keep bytes as they are.
+                uint32_t const code = distinct == 256 ? value : present ? (uint32_t)below + 1 : 0u;
+                tables->codes[value] = (uint8_t)code;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            int32_t const largest = distinct == 256 ? 255 : distinct;
+            planes_used = largest ? 32 - __builtin_clz((uint32_t)largest) : 1;
+        }
+    }
+
+    __device__ __forceinline__ void begin_stripe(int32_t base) {
+        lane_top = base + k * 64;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) planes[b] = 0;
+        for (int32_t r = 0; r < 64; ++r) {
+            int32_t const i = lane_top + r; // 0-based row
+            uint32_t const code = i < n ? code_of(rows[i]) : 0u;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) planes[b] |= (uint64_t)((code >> b) & 1u) << r;
+        }
+        pv = ~0ull; // column 0: D[i][0] = i, every vertical delta is +1
+        mv = 0;
+        bottom_code = 1;
+        column_code = 0;
+    }
+
+    /** Horizontal deltas are coded as bit 0 = +1, bit 1 = -1. */
+    __device__ __forceinline__ void run_chunk(int32_t chunk, int32_t const *in_codes, int32_t const *,
+                                              int32_t *out_codes, int32_t *) {
+        int32_t const first_step = chunk * G;
+        int32_t const j_lane = first_step + k + 1;
+        uint32_t chunk_symbols = j_lane <= m ? code_of(columns[j_lane - 1]) : 0u;
+        uint32_t chunk_codes = in_codes ? (j_lane <= m ? (uint32_t)in_codes[j_lane] : 0u) : 1u;
+        int32_t stage = 0;
+        bool const first_in_group = k == 0;
+        int32_t const used = planes_used;
+
+#pragma unroll 2
+        for (int32_t step = 0; step < G; ++step) {
+            int32_t const t = first_step + step;
+            uint32_t const hin = group_shift_down<G>(chunk_codes, bottom_code, first_in_group);
+            column_code = group_shift_down<G>(chunk_symbols, column_code, first_in_group);
+            chunk_codes = group_rotate_up<G>(chunk_codes);
+            chunk_symbols = group_rotate_up<G>(chunk_symbols);
+
+            int32_t const j = t + 1 - k;
+            if ((uint32_t)(j - 1) < (uint32_t)m) {
+                // Eq = AND over planes of (plane XNOR column bit), one 32-bit half at a time. The all-ones or
+                // all-zeros mask is formed with a subtraction: a sign-extended 1-bit field widened to 64 bits
+                // was compiled as zero-extended (the upper half of Eq came out wrong).
+                uint32_t const inverted = ~column_code;
+                uint32_t eq_low = ~0u, eq_high = ~0u;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    if (b < used) {
+                        uint32_t const mask = 0u - ((inverted >> b) & 1u);
+                        eq_low &= (uint32_t)planes[b] ^ mask;
+                        eq_high &= (uint32_t)(planes[b] >> 32) ^ mask;
+                    }
+                }
+                uint64_t eq = (uint64_t)eq_high << 32 | eq_low;
+                uint64_t const hin_positive = hin & 1u, hin_negative = (hin >> 1) & 1u;
+                uint64_t const xv = eq | mv;
+                eq |= hin_negative;
+                uint64_t const xh = (((eq & pv) + pv) ^ pv) | eq;
+                uint64_t ph = mv | ~(xh | pv);
+                uint64_t mh = pv & xh;
+                bottom_code = (uint32_t)(ph >> 63) | (uint32_t)(mh >> 63) << 1;
+                ph = (ph << 1) | hin_positive;
+                mh = (mh << 1) | hin_negative;
+                pv = mh | ~(xv | ph);
+                mv = ph & xv;
+            }
+            if constexpr (G == 64) {
+                if (out_codes) stage = collect_from_last((int32_t)bottom_code, stage);
+            }
+        }
+        if constexpr (G == 64) {
+            if (out_codes) {
+                int32_t const j_out = first_step + lane - 62;
+                if (j_out >= 1 && j_out <= m) out_codes[j_out] = stage;
+            }
+        }
+    }
+
+    __device__ __forceinline__ void end_stripe(int32_t, bool) {
+        int32_t const rows_here = min(max(n - lane_top, 0), 64);
+        uint64_t const mask = rows_here == 64 ? ~0ull : ((1ull << rows_here) - 1ull);
+        vertical_sum += __builtin_popcountll(pv & mask) - __builtin_popcountll(mv & mask);
+    }
+
+    __device__ __forceinline__ int32_t partial() const {
+        int32_t value = vertical_sum;
+        for (int offset = 1; offset < G; offset <<= 1) value += __shfl_xor(value, offset, 64);
+        return value;
+    }
+
+    __device__ __forceinline__ void write_result(int32_t sum) const {
+        if (active && k == 0) {
+            int64_t const distance = (int64_t)m + sum;
+            results[output] = distance;
+            if (mirror != no_mirror_k) results[mirror] = distance;
+        }
+    }
+};
+
+} // namespace szs_amd

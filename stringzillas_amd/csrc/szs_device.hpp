@@ -67,22 +67,41 @@ constexpr uint64_t no_mirror_k = ~0ull;
 
 /**
  *  A kernel shape: `group` lanes share one pair (64 / group pairs per wave), each lane holds `rows_per_lane`
- *  consecutive rows, so one pass covers `group * rows_per_lane` rows. Only 64-lane groups run more than one pass.
+ *  consecutive rows, so one stripe covers `group * rows_per_lane` rows. A 64-lane group sweeps taller pairs
+ *  stripe after stripe (boundary row through HBM scratch). A `cooperative` shape gives one pair to all the
+ *  waves of a workgroup, which pipeline its stripes in lock-step phases.
  */
 struct shape_t {
     int32_t group;
     int32_t rows_per_lane;
+    int32_t cooperative;
+    int32_t bit_parallel; /**< unit-cost bytes: one 64-row Myers word per lane instead of R DP rows */
 };
 
-/** The shapes compiled for every family; index = shape id. */
-constexpr int shapes_count_k = 9;
+/** Waves per workgroup of the scoring kernels. */
+constexpr int waves_per_group_k = 4;
+
+/** The shapes compiled; index = shape id. Bit-parallel shapes exist for the unit-cost byte Levenshtein only. */
+constexpr int shapes_count_k = 15;
 constexpr shape_t shapes_k[shapes_count_k] = {
-    {16, 1}, {16, 2}, {16, 4},            // up to 16 / 32 / 64 rows, four pairs per wave
-    {64, 2}, {64, 3}, {64, 4}, {64, 6},   // up to 128 / 192 / 256 / 384 rows
-    {64, 8}, {64, 16},                    // up to 512 rows, then 1024-row stripes
+    {16, 1, 0, 0}, {16, 2, 0, 0}, {16, 4, 0, 0},    // up to 16 / 32 / 64 rows, four pairs per wave
+    {64, 2, 0, 0}, {64, 3, 0, 0}, {64, 4, 0, 0},    // up to 128 / 192 / 256 rows
+    {64, 6, 0, 0}, {64, 8, 0, 0}, {64, 16, 0, 0},   // up to 384 / 512 / 1024 rows per stripe
+    {64, 8, 1, 0}, {64, 16, 1, 0},                  // cooperative: 512- or 1024-row stripes, whole workgroup
+    {4, 64, 0, 1}, {16, 64, 0, 1},                  // Myers: up to 256 rows (16 pairs/wave), 1024 rows
+    {64, 64, 0, 1}, {64, 64, 1, 1},                 // Myers: 4096-row stripes on one wave, or cooperative
 };
 
-/** Columns-length buckets per shape for binning: half-octaves of `columns_length`. */
+/** Shape-table slot of the unit-cost byte Levenshtein (after the family x symbols slots). */
+constexpr int unit_bytes_slot_k = families_count_k * 2;
+constexpr int shape_slots_k = families_count_k * 2 + 1;
+
+/** Pairs one work item holds: one per wave slot of a non-cooperative shape, one pair for a cooperative one. */
+__host__ __device__ inline uint32_t pairs_per_item(shape_t shape) {
+    return shape.cooperative ? 1u : (uint32_t)(waves_per_group_k * (64 / shape.group));
+}
+
+/** Cost buckets per shape for binning: half-octaves of (stripes x (columns + group)). */
 constexpr int column_buckets_k = 64;
 /** Variant = (family, symbols, shape); bins = variants x column buckets. */
 constexpr int variants_count_k = families_count_k * 2 * shapes_count_k;
@@ -106,10 +125,10 @@ struct bins_view_t {
 
 /** Shape choice per (family, symbols): single-pass shapes by ascending capacity, then the multi-pass shape. */
 struct shape_table_t {
-    int32_t limits[families_count_k][2][shapes_count_k]; /**< rows capacity per shape, -1 = unused */
-    int32_t order[families_count_k][2][shapes_count_k];  /**< shapes to try, ascending capacity */
-    int32_t used[families_count_k][2];                   /**< entries in `order` */
-    int32_t multi_pass_shape[families_count_k][2];       /**< shape for rows beyond every single-pass capacity */
+    int32_t limits[shape_slots_k][shapes_count_k]; /**< most rows a shape takes in this slot, -1 = unused */
+    int32_t order[shape_slots_k][shapes_count_k];  /**< shapes to try, ascending capacity */
+    int32_t used[shape_slots_k];                   /**< entries in `order` */
+    int32_t multi_pass_shape[shape_slots_k];       /**< shape for rows beyond every listed limit */
 };
 
 /** Inputs of one segment for materialization. */
@@ -128,6 +147,7 @@ struct materialize_args_t {
     int32_t symbols;
     int32_t mode;                        /**< 0 pairs, 1 cross-product, 2 symmetric */
     int32_t open, extend;
+    int32_t unit_costs;                  /**< match 0, mismatch 1, gap 1: bit-parallel shapes apply to bytes */
 };
 
 } // namespace szs_amd

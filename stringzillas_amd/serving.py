@@ -1,0 +1,163 @@
+"""Serving benchmark: Poisson request arrivals into the native continuous-batching server.
+
+Each request is one pair list of one engine kind drawn like the mixed batch: kind 50% byte Levenshtein (ACGT),
+25% UTF-8 Levenshtein, 25% affine NW; pair count log-uniform in [1, max_pairs]; both lengths log-uniform in
+[16, max_length]. Requests are sampled (with replacement) from a fixed pool of pre-generated templates so host
+memory stays bounded. Latency = completion time - arrival time, measured on the host.
+
+The baseline serves the same arrivals with one plain launch per request, in arrival order, without batching
+across requests (``engine.pairs`` per request).
+"""
+
+from __future__ import annotations
+
+import math
+import threading
+import time
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+
+import stringzillas_amd as szs
+from stringzillas_amd import workloads as W
+
+
+@dataclass
+class Template:
+    kind: str
+    firsts: "szs.Strs"
+    seconds: "szs.Strs"
+    cells: int
+
+
+def _log_uniform_int(rng, low: int, high: int) -> int:
+    return int(min(high, max(low, math.exp(rng.uniform(math.log(low), math.log(high + 1))))))
+
+
+def make_templates(count: int = 48, max_pairs: int = 4096, max_length: int = 16384, seed: int = 0x535A0004):
+    """Request payloads following the mixed-batch length and kind mix (seeded, numpy)."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    templates = []
+    for i in range(count):
+        pairs = _log_uniform_int(rng, 1, max_pairs)
+        kind = rng.choice(["levenshtein", "levenshtein_utf8", "needleman_wunsch"], p=[0.5, 0.25, 0.25])
+        generator = torch.Generator()
+        generator.manual_seed(seed + i)
+        lengths = [torch.from_numpy(np.array([_log_uniform_int(rng, 16, max_length) for _ in range(pairs)]))
+                   for _ in range(2)]
+        if kind == "levenshtein_utf8":
+            tapes = [W._torch_utf8_tape(generator, l) for l in lengths]
+        else:
+            tapes = [W._torch_random_tape(generator, W.DNA if kind == "levenshtein" else W.AMINO, l) for l in lengths]
+        firsts = szs.Strs.from_tape(tapes[0][0].numpy(), tapes[0][1].numpy().astype(np.uint64))
+        seconds = szs.Strs.from_tape(tapes[1][0].numpy(), tapes[1][1].numpy().astype(np.uint64))
+        segment = W.Segment(kind, firsts.data, firsts.offsets, seconds.data, seconds.offsets)
+        templates.append(Template(kind, firsts, seconds, segment.cells()))
+    return templates
+
+
+def engines_by_kind():
+    b2c, costs = W.protein_matrix(np.random.default_rng(W.SEED_BASE + 1))
+    return {
+        "levenshtein": szs.LevenshteinDistances(),
+        "levenshtein_utf8": szs.LevenshteinDistancesUTF8(),
+        "needleman_wunsch": szs.NeedlemanWunschScores(b2c, costs, -11, -1),
+    }
+
+
+def _arrivals(rng, qps: float, duration: float) -> List[float]:
+    times, now = [], 0.0
+    while True:
+        now += rng.exponential(1.0 / qps)
+        if now >= duration:
+            return times
+        times.append(now)
+
+
+def _summary(latencies: List[float], cells: int, wall: float, offered: float, completed: int):
+    ms = np.array(latencies) * 1e3
+    return {
+        "offered_qps": offered,
+        "completed": completed,
+        "achieved_qps": round(completed / wall, 2),
+        "gcups": round(cells / wall / 1e9, 2),
+        "p50_ms": round(float(np.percentile(ms, 50)), 2) if len(ms) else None,
+        "p99_ms": round(float(np.percentile(ms, 99)), 2) if len(ms) else None,
+        "wall_s": round(wall, 2),
+    }
+
+
+def run_server(templates, engines, qps: float, duration: float, hbm_budget: int = 64 << 30,
+               inflight: int = 2, seed: int = 1, drain_timeout: float = 120.0):
+    """Continuous batching: a submitter thread follows the arrival schedule; the main thread polls tickets."""
+    rng = np.random.default_rng(seed)
+    arrivals = _arrivals(rng, qps, duration)
+    picks = rng.integers(0, len(templates), size=len(arrivals))
+    server = szs.Server(hbm_budget_bytes=hbm_budget, max_inflight_rounds=inflight)
+    pending = {}
+    lock = threading.Lock()
+    start = time.perf_counter()
+
+    def submit_all():
+        for at, pick in zip(arrivals, picks):
+            delay = start + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            template = templates[pick]
+            ticket = server.submit(engines[template.kind], template.firsts, template.seconds)
+            with lock:
+                pending[ticket] = (start + at, template)
+
+    submitter = threading.Thread(target=submit_all)
+    submitter.start()
+    latencies, cells, done = [], 0, 0
+    deadline = start + duration + drain_timeout
+    results = {}
+    while (submitter.is_alive() or pending) and time.perf_counter() < deadline:
+        with lock:
+            tickets = list(pending.items())
+        finished = [(t, v) for t, v in tickets if server.poll(t)]
+        now = time.perf_counter()
+        for ticket, (arrived, template) in finished:
+            out = server.wait(ticket)
+            latencies.append(now - arrived)
+            cells += template.cells
+            done += 1
+            results.setdefault(id(template), out)
+            with lock:
+                del pending[ticket]
+        if not finished:
+            time.sleep(0.0002)
+    wall = time.perf_counter() - start
+    submitter.join()
+    server.close()
+    summary = _summary(latencies, cells, wall, qps, done)
+    summary["dropped_after_drain_timeout"] = len(pending)
+    return summary, results
+
+
+def run_baseline(templates, engines, qps: float, duration: float, seed: int = 1, drain_timeout: float = 120.0):
+    """One plain launch per request, in arrival order, no cross-request batching."""
+    rng = np.random.default_rng(seed)
+    arrivals = _arrivals(rng, qps, duration)
+    picks = rng.integers(0, len(templates), size=len(arrivals))
+    start = time.perf_counter()
+    latencies, cells, done = [], 0, 0
+    for at, pick in zip(arrivals, picks):
+        if time.perf_counter() > start + duration + drain_timeout:
+            break
+        delay = start + at - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        template = templates[pick]
+        engines[template.kind].pairs(template.firsts, template.seconds)
+        latencies.append(time.perf_counter() - (start + at))
+        cells += template.cells
+        done += 1
+    wall = time.perf_counter() - start
+    summary = _summary(latencies, cells, wall, qps, done)
+    summary["dropped_after_drain_timeout"] = len(arrivals) - done
+    return summary

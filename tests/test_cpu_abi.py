@@ -1,0 +1,122 @@
+"""CPU-only checks of the C ABI library and its Python mirror: the library loads, exports every symbol the
+public header declares, validates arguments like the reference binding (python/stringzillas/similarities.c), and
+fails loudly (never silently computes on the CPU) when no GPU is visible. No kernel runs here.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.conftest import gpu_available
+
+szs = pytest.importorskip("stringzillas_amd")
+from stringzillas_amd import _native  # noqa: E402
+
+
+def test_library_exports_every_declared_symbol():
+    declared = _native.declared_symbols()
+    assert len(declared) >= 50
+    missing = [name for name in declared if not hasattr(_native.lib, name)]
+    assert not missing, missing
+    for name in ["szs_levenshtein_distances_u64tape", "szs_needleman_wunsch_scores_u32tape",
+                 "szs_smith_waterman_scores", "szs_device_scope_init_gpu_device", "szs_batch_init", "szs_server_init"]:
+        assert name in declared
+
+
+def test_version_and_capabilities():
+    assert szs.__version__ == "5.3.2"
+    comptime = _native.lib.szs_capabilities_comptime()
+    assert comptime & (1 << 25)  # sz_cap_rocm_k
+    if not gpu_available():
+        assert szs.__capabilities__ == ()
+
+
+def test_engine_argument_validation_mirrors_the_reference():
+    with pytest.raises(ValueError):
+        szs.LevenshteinDistances(match=200)
+    with pytest.raises(OverflowError):  # negative costs would wrap the reference's unsigned cells
+        szs.LevenshteinDistances(mismatch=-1)
+    b2c = np.zeros(256, dtype=np.uint8)
+    costs = np.zeros((32, 32), dtype=np.int8)
+    with pytest.raises(TypeError):
+        szs.NeedlemanWunschScores(b2c.astype(np.int32), costs)
+    with pytest.raises(ValueError):
+        szs.NeedlemanWunschScores(b2c[:100], costs)
+    with pytest.raises(TypeError):
+        szs.NeedlemanWunschScores(b2c, costs.astype(np.int16))
+    with pytest.raises(ValueError):
+        szs.SmithWatermanScores(b2c, np.zeros((16, 16), dtype=np.int8))
+    with pytest.raises(TypeError):
+        szs.NeedlemanWunschScores()
+    bad = b2c.copy()
+    bad[7] = 40
+    with pytest.raises(ValueError):
+        szs.NeedlemanWunschScores(bad, costs)
+    engine = szs.NeedlemanWunschScores(b2c, costs, open=300, extend=-1)  # silently truncated to int8
+    assert "open=44" in repr(engine)
+
+
+def test_inputs_must_be_tapes():
+    engine = szs.LevenshteinDistances()
+    with pytest.raises(TypeError):
+        engine(["a", "b"], ["c"])
+
+
+def test_out_argument_validation():
+    engine = szs.LevenshteinDistances()
+    q, c = szs.Strs(["ab", "c"]), szs.Strs(["a"])
+    with pytest.raises(TypeError):
+        engine(q, c, out=[[0], [0]])
+    with pytest.raises(ValueError):
+        engine(q, c, out=np.zeros(2, dtype=np.uint64))
+    with pytest.raises(ValueError):
+        engine(q, c, out=np.zeros((1, 1), dtype=np.uint64))
+    with pytest.raises(TypeError):
+        engine(q, c, out=np.zeros((2, 1), dtype=np.int64))
+    with pytest.raises(ValueError):
+        engine(q, c, out=np.zeros((2, 4), dtype=np.uint64)[:, ::2])
+
+
+def test_device_scopes_without_scoring():
+    cpu = szs.DeviceScope(cpu_cores=3)
+    cores = ctypes.c_size_t()
+    error = ctypes.c_char_p()
+    assert _native.lib.szs_device_scope_get_cpu_cores(cpu.handle, ctypes.byref(cores), ctypes.byref(error)) == 0
+    assert cores.value == 3
+    gpu_index = ctypes.c_size_t()
+    status = _native.lib.szs_device_scope_get_gpu_device(cpu.handle, ctypes.byref(gpu_index), ctypes.byref(error))
+    assert status == -17  # sz_device_code_mismatch_k
+    with pytest.raises(ValueError):
+        szs.DeviceScope(cpu_cores=1, gpu_device=0)
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-GPU error path")
+def test_scoring_without_a_gpu_fails_loudly():
+    engine = szs.LevenshteinDistances()
+    with pytest.raises(RuntimeError, match="GPU"):
+        engine(szs.Strs(["abc"]), szs.Strs(["abd"]))
+    with pytest.raises(RuntimeError, match="GPU"):
+        engine.pairs(szs.Strs(["abc"]), szs.Strs(["abd"]))
+    with pytest.raises(RuntimeError):
+        szs.DeviceScope(gpu_device=0)
+    with pytest.raises(RuntimeError):
+        szs.Batch([(engine, szs.Strs(["a"]), szs.Strs(["b"]))])
+    with pytest.raises(RuntimeError):
+        szs.Server(hbm_budget_bytes=1 << 20)
+
+
+def test_cpu_scope_rejects_gpu_engines_before_touching_a_device():
+    engine = szs.LevenshteinDistances()
+    with pytest.raises(RuntimeError, match="CPU"):
+        engine(szs.Strs(["a"]), szs.Strs(["b"]), device=szs.DeviceScope(cpu_cores=2))
+
+
+def test_arrow_inputs_are_viewed_as_tapes():
+    pa = pytest.importorskip("pyarrow")
+    strings = ["alpha", "", "βeta", "gamma" * 10]
+    for array in (pa.array(strings, type=pa.string()), pa.array(strings, type=pa.large_string()),
+                  pa.array(strings, type=pa.string()).slice(1, 3)):
+        tape = szs._as_tape(array, "queries")
+        expected = strings[1:4] if len(array) == 3 else strings
+        assert [tape[i].decode() for i in range(len(tape))] == expected

@@ -16,7 +16,8 @@ pytestmark = pytest.mark.gpu
 
 szs = pytest.importorskip("stringzillas_amd")
 
-RANGES = [(0, 20), (10, 70), (60, 300), (200, 600), (500, 1100), (1000, 2600)]
+# The last two ranges reach the cooperative workgroup tier (rows > 2048 for class costs, > 4096 for uniform).
+RANGES = [(0, 20), (10, 70), (60, 300), (200, 600), (500, 1100), (1000, 2600), (2100, 3600), (4200, 6000)]
 
 
 def _engines(rng):
@@ -60,7 +61,7 @@ def _strings(rng, alphabet, utf8, count, low, high):
 def test_pairs_match_oracle(name, low, high):
     rng = np.random.default_rng(zlib.crc32(f"{name}:{low}:{high}".encode()))
     engine, params, alphabet, utf8 = _engines(rng)[name]
-    count = 96 if high <= 600 else 24 if high <= 1100 else 8
+    count = 96 if high <= 600 else 24 if high <= 1100 else 8 if high <= 2600 else 3
     a = _strings(rng, alphabet, utf8, count, low, high)
     b = _strings(rng, alphabet, utf8, count, low, high)
     expected = O.pairs(params, O.tape(a), O.tape(b), threads=16)
@@ -144,3 +145,31 @@ def test_batch_modes_agree_and_graph_replays():
             for got, want in zip(outputs, expected):
                 np.testing.assert_array_equal(got.astype(np.int64), want, err_msg=f"mode {mode} replay {replay}")
         assert batch.cells > 0
+
+
+@pytest.mark.parametrize("alphabet_name", ["binary", "dna", "printable", "all_bytes"])
+def test_unit_levenshtein_bit_parallel_alphabets(alphabet_name):
+    """Unit costs take the bit-parallel tier above 64 rows; its symbol renumbering must not depend on the alphabet."""
+    alphabets = {"binary": b"ab", "dna": DNA, "printable": bytes(range(32, 127)), "all_bytes": bytes(range(256))}
+    rng = np.random.default_rng(zlib.crc32(alphabet_name.encode()))
+    alphabet = alphabets[alphabet_name]
+    lengths = [(50, 300), (250, 1100), (1000, 4200), (4000, 9000)]
+    a, b = [], []
+    for low, high in lengths:
+        for _ in range(6 if high < 5000 else 2):
+            a.append(random_bytes(rng, alphabet, low, high))
+            b.append(random_bytes(rng, alphabet, low, high))
+    expected = O.pairs(O.Params("levenshtein"), O.tape(a), O.tape(b), threads=16)
+    got = szs.LevenshteinDistances().pairs(szs.Strs(a), szs.Strs(b))
+    np.testing.assert_array_equal(got.astype(np.int64), expected)
+
+
+def test_unit_levenshtein_cooperative_tall_pair():
+    """Rows above 16,384 run on all four waves of a workgroup (cooperative bit-parallel stripes)."""
+    rng = np.random.default_rng(5)
+    a = [random_bytes(rng, DNA, 17000, 18000), random_bytes(rng, b"ACGTN", 16500, 17500)]
+    b = [random_bytes(rng, DNA, 17000, 19000), random_bytes(rng, b"ACGTN", 16500, 17000)]
+    b[0] = a[0][:9000] + b[0][:3000] + a[0][9000:]  # a related pair too
+    expected = O.pairs(O.Params("levenshtein"), O.tape(a), O.tape(b), threads=16)
+    got = szs.LevenshteinDistances().pairs(szs.Strs(a), szs.Strs(b))
+    np.testing.assert_array_equal(got.astype(np.int64), expected)
