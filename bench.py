@@ -37,6 +37,7 @@ def parse():
     parser.add_argument("--mode", choices=["graph", "persistent"], default="graph")
     parser.add_argument("--skip-cpu", action="store_true")
     parser.add_argument("--skip-per-call", action="store_true")
+    parser.add_argument("--breakdown", action="store_true", help="also time each kind alone")
     return parser.parse_args()
 
 
@@ -147,6 +148,24 @@ def main():
     log(f"batched: {ms_per_step:.1f} ms/step ({replay_ms_mean:.1f} ms replay), {gcups:.1f} GCUPS; "
         f"scoring kernel {kernel_ms_mean:.1f} ms")
 
+    # Each kind alone (persistent form): where the scoring time goes.
+    breakdown = {}
+    if args.breakdown:
+        for engine, (a, b), segment, output in zip(engines, tapes, workload.segments, outputs):
+            alone = szs.Batch([(engine, a, b)], mode=szs.BATCH_PERSISTENT, outputs=[output])
+            alone.run()
+            alone.run()
+            seg_cells = alone.cells
+            breakdown[segment.kind] = {
+                "pairs": segment.count,
+                "cells": seg_cells,
+                "kernel_ms": round(alone.last_kernel_ms, 2),
+                "gcups": round(seg_cells / (alone.last_kernel_ms / 1e3) / 1e9, 1),
+                "ops_per_cell": segment.ops_per_cell(),
+            }
+            del alone
+        log(f"per kind: {breakdown}")
+
     # Parity spot check against the oracle, including one of the longest pairs of each kind.
     from oracle import oracle as O
 
@@ -245,6 +264,8 @@ def main():
         "batched_over_per_call": round(gcups / per_call["gcups"], 1) if per_call else None,
         "parity": {"checked_pairs": checked, "mismatches": mismatches},
     }
+    if breakdown:
+        line["per_kind"] = breakdown
     print(json.dumps(line), flush=True)
 
 

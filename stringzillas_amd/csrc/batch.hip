@@ -395,7 +395,9 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
             lds_bytes = std::max<uint32_t>(lds_bytes, (uint32_t)engine->params.dense_count * 64u * 8u);
     }
     b.tasks_count = tasks_total;
-    b.lds_bytes = lds_bytes;
+    // Dynamic LDS per wave: the NW/SW profile or the Myers symbol tables, whichever is larger.
+    lds_bytes = std::max<uint32_t>(lds_bytes, 512u);
+    b.lds_bytes = (lds_bytes + 15u) & ~15u;
 
     // Device-side tables.
     SZS_TRY_HIP(b.params.reserve(sizeof(engine_params_t) * std::max<size_t>(segments_count, 1)), "allocation");

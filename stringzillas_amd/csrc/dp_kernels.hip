@@ -173,16 +173,14 @@ struct persistent_args_t {
     bins_view_t bins;
     int32_t *scratch;
     uint32_t scratch_capacity;
-    uint32_t profile_bytes;
+    uint32_t wave_lds_bytes;
 };
 
 /**
  *  Persistent scoring kernel: each 4-wave workgroup pops work items from a device counter, heaviest bins first,
  *  until the queue (`item_start[bins]` items) is drained; every workgroup exits once the counter passes it.
  */
-__global__ __launch_bounds__(256) void persistent_score_kernel(persistent_args_t args) {
-    extern __shared__ int8_t profiles_lds[];
-    __shared__ myers_tables_t myers_lds[waves_per_group_k];
+__global__ __launch_bounds__(256, 4) void persistent_score_kernel(persistent_args_t args) {
     __shared__ uint32_t taken_shared;
     __shared__ int32_t result_shared;
     uint32_t const total_items = args.bins.item_start[bins_count_k];
@@ -192,9 +190,7 @@ __global__ __launch_bounds__(256) void persistent_score_kernel(persistent_args_t
     item.order = args.bins.order;
     item.scratch = args.scratch + (size_t)blockIdx.x * waves_per_group_k * 2 * args.scratch_capacity;
     item.scratch_capacity = args.scratch_capacity;
-    item.profiles = profiles_lds;
-    item.profile_bytes = args.profile_bytes;
-    item.myers = myers_lds;
+    item.wave_lds_bytes = args.wave_lds_bytes;
     item.shared_result = &result_shared;
     for (;;) {
         if (threadIdx.x == 0) taken_shared = atomicAdd(args.bins.queue_head, 1u);
@@ -221,12 +217,10 @@ __global__ __launch_bounds__(256) void persistent_score_kernel(persistent_args_t
 }
 
 /** One-pair launch for the per-call baseline: the same scorers, one workgroup, no binning. */
-__global__ __launch_bounds__(256) void single_task_kernel(task_t const *tasks, segment_view_t const *segments,
+__global__ __launch_bounds__(256, 4) void single_task_kernel(task_t const *tasks, segment_view_t const *segments,
                                                           uint32_t const *identity, uint32_t task_index, int variant,
                                                           int32_t *scratch, uint32_t scratch_capacity,
-                                                          uint32_t profile_bytes) {
-    extern __shared__ int8_t profiles_lds[];
-    __shared__ myers_tables_t myers_lds[waves_per_group_k];
+                                                          uint32_t wave_lds_bytes) {
     __shared__ int32_t result_shared;
     item_args_t item;
     item.tasks = tasks;
@@ -236,9 +230,7 @@ __global__ __launch_bounds__(256) void single_task_kernel(task_t const *tasks, s
     item.count = 1;
     item.scratch = scratch;
     item.scratch_capacity = scratch_capacity;
-    item.profiles = profiles_lds;
-    item.profile_bytes = profile_bytes;
-    item.myers = myers_lds;
+    item.wave_lds_bytes = wave_lds_bytes;
     item.shared_result = &result_shared;
     run_variant(variant, item);
 }
@@ -488,7 +480,7 @@ hipError_t szs_amd_launch_persistent(szs_amd::task_t const *tasks, szs_amd::segm
     args.bins = *bins;
     args.scratch = scratch;
     args.scratch_capacity = scratch_capacity;
-    args.profile_bytes = profile_bytes;
+    args.wave_lds_bytes = profile_bytes;
     hipLaunchKernelGGL(szs_amd::persistent_score_kernel, dim3(workgroups), dim3(64 * szs_amd::waves_per_group_k),
                        profile_bytes * szs_amd::waves_per_group_k, stream, args);
     return hipGetLastError();

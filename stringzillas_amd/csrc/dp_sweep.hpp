@@ -93,11 +93,19 @@ struct item_args_t {
     uint32_t count;
     int32_t *scratch;      /**< this workgroup's 4 x 2 x `scratch_capacity` int32 rows */
     uint32_t scratch_capacity;
-    int8_t *profiles;      /**< this workgroup's LDS profiles, `profile_bytes` per wave */
-    uint32_t profile_bytes;
-    myers_tables_t *myers; /**< this workgroup's LDS symbol tables, one per wave */
+    uint32_t wave_lds_bytes; /**< dynamic LDS per wave: substitution profile or Myers symbol tables */
     int32_t *shared_result; /**< LDS word for cooperative reductions */
 };
+
+/**
+ *  This wave's slice of the dynamic LDS. It is addressed through the `extern __shared__` array in the function
+ *  that uses it, so the compiler emits `ds_*` instructions; a pointer handed over through memory would lose its
+ *  address space and turn every profile read into a flat load.
+ */
+__device__ __forceinline__ int8_t *wave_lds(uint32_t wave, uint32_t bytes) {
+    extern __shared__ __align__(16) int8_t szs_dynamic_lds[];
+    return szs_dynamic_lds + wave * bytes;
+}
 
 /** How a sweep's per-lane partial results combine into the pair's score. */
 enum reduce_kind_t { reduce_none_k = 0, reduce_max_k = 1, reduce_sum_k = 2 };
@@ -139,7 +147,7 @@ struct sweep_t {
     int32_t running_best;
 
     __device__ __forceinline__ void load(item_args_t const &a, uint32_t task_index, bool is_active, uint32_t wave) {
-        int8_t *wave_profile = a.profiles + wave * a.profile_bytes;
+        int8_t *wave_profile = wave_lds(wave, a.wave_lds_bytes);
         lane = (int)__lane_id();
         k = lane % G;
         active = is_active;
@@ -302,10 +310,12 @@ struct sweep_t {
         else if (last_stripe) {
             int32_t const last_row = n - 1 - base; // 0-based row within this stripe
             if (active && last_row / R == k) {
-                int32_t value = scores[0];
+                // A masked sum rather than `if (row == r) value = scores[r]`: the compiler folds that chain into
+                // a dynamic index, which demotes the whole row state to scratch memory.
+                int32_t const wanted = last_row % R;
+                int32_t value = 0;
 #pragma unroll
-                for (int r = 1; r < R; ++r)
-                    if (last_row % R == r) value = scores[r];
+                for (int r = 0; r < R; ++r) value += scores[r] & -(int32_t)(wanted == r);
                 results[output] = (int64_t)value;
                 if (mirror != no_mirror_k) results[mirror] = (int64_t)value;
             }

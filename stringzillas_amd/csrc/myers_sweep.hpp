@@ -80,7 +80,7 @@ struct myers_sweep_t {
     }
 
     __device__ __forceinline__ void load(item_args_t const &a, uint32_t task_index, bool is_active, uint32_t wave) {
-        myers_tables_t *wave_tables = a.myers + wave;
+        myers_tables_t *wave_tables = reinterpret_cast<myers_tables_t *>(wave_lds(wave, a.wave_lds_bytes));
         lane = (int)__lane_id();
         k = lane % G;
         active = is_active;

@@ -271,7 +271,7 @@ def _torch_utf8_tape(generator, budgets):
     keep = within <= budgets[owner]
     del running, within, base
     codepoints, widths, owner = codepoints[keep], widths[keep], owner[keep]
-    string_bytes = torch.bincount(owner, weights=widths.to(torch.float64), minlength=len(draws)).to(torch.int64)
+    string_bytes = torch.zeros(len(draws), dtype=torch.int64, device=device).index_add_(0, owner, widths)
     offsets = _torch_offsets(string_bytes)
     total = int(offsets[-1])
     out = torch.zeros(total + 1, dtype=torch.uint8, device=device)
