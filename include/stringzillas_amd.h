@@ -327,6 +327,20 @@ int szs_server_poll(szs_server_t server, szs_ticket_t ticket);
 sz_size_t szs_server_bytes_in_use(szs_server_t server);
 void szs_server_free(szs_server_t server);
 
+/**
+ *  Tuning (no reference counterpart): the kernel shape chosen for a pair depends on its rows (the shorter side)
+ *  through a per-slot ladder of (shape, most rows) steps plus a shape for anything taller. Slots are
+ *  `family * 2 + symbols` for the six recurrence families x {bytes, runes}, plus the unit-cost byte slot last.
+ *  An autotuner measures alternatives and installs the fastest ladder; changes apply to batches prepared later.
+ */
+int szs_tuning_slots(void);
+int szs_tuning_shapes(void);
+/** Shape `index` as (lanes per pair, rows per lane, cooperative, bit-parallel). */
+sz_status_t szs_tuning_shape(int index, int *group, int *rows_per_lane, int *cooperative, int *bit_parallel);
+sz_status_t szs_tuning_get_ladder(int slot, int *shapes, int *max_rows, int capacity, int *count, int *beyond_shape);
+sz_status_t szs_tuning_set_ladder(int slot, int const *shapes, int const *max_rows, int count, int beyond_shape,
+                                  char const **error_message);
+
 #pragma endregion
 
 #ifdef __cplusplus

@@ -463,6 +463,16 @@ class Server:
         self.close()
 
 
+def _apply_cached_tuning() -> None:
+    """Installs the autotuned shape ladders when they were measured on this GPU architecture."""
+    if os.path.exists(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_shapes.json")):
+        from . import autotune
+
+        autotune.apply()
+
+
+_apply_cached_tuning()
+
 __all__ = [
     "Strs", "DeviceStrs", "to_device", "DeviceScope", "LevenshteinDistances", "LevenshteinDistancesUTF8",
     "NeedlemanWunschScores", "SmithWatermanScores", "Batch", "Server", "reset_capabilities", "SzsError",

@@ -14,9 +14,13 @@ LIBRARY_PATH = os.path.join(HERE, "libstringzillas_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "stringzillas_amd.h")
 
 if not os.path.exists(LIBRARY_PATH):
-    raise ImportError(
-        f"{LIBRARY_PATH} is missing: build the HIP library first (python -m stringzillas_amd.build)"
-    )
+    # A fresh tree: compile it in place (hipcc is part of the toolchain); anything else is a hard error.
+    try:
+        from . import build as _build
+
+        _build.build(verbose=False)
+    except Exception as error:  # pragma: no cover - exercised only on trees without a built library
+        raise ImportError(f"{LIBRARY_PATH} is missing and building it failed: {error}") from error
 lib = ctypes.CDLL(LIBRARY_PATH)
 
 

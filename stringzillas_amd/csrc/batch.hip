@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <utility>
 #include <vector>
@@ -529,3 +530,82 @@ sz_status_t batch_wait(batch_t *batch, char const **error_message) {
 }
 
 } // namespace szs_amd
+
+#pragma region Tuning hooks
+
+namespace {
+std::mutex &tuning_mutex() {
+    static std::mutex mutex;
+    return mutex;
+}
+} // namespace
+
+extern "C" {
+
+int szs_tuning_slots(void) { return szs_amd::shape_slots_k; }
+int szs_tuning_shapes(void) { return szs_amd::shapes_count_k; }
+
+sz_status_t szs_tuning_shape(int index, int *group, int *rows_per_lane, int *cooperative, int *bit_parallel) {
+    if (index < 0 || index >= szs_amd::shapes_count_k || !group || !rows_per_lane || !cooperative || !bit_parallel)
+        return sz_unexpected_dimensions_k;
+    szs_amd::shape_t const shape = szs_amd::shapes_k[index];
+    *group = shape.group, *rows_per_lane = shape.rows_per_lane;
+    *cooperative = shape.cooperative, *bit_parallel = shape.bit_parallel;
+    return sz_success_k;
+}
+
+sz_status_t szs_tuning_get_ladder(int slot, int *shapes, int *max_rows, int capacity, int *count, int *beyond_shape) {
+    if (slot < 0 || slot >= szs_amd::shape_slots_k || !count || !beyond_shape) return sz_unexpected_dimensions_k;
+    std::lock_guard<std::mutex> lock(tuning_mutex());
+    szs_amd::shape_table_t const &table = szs_amd::active_shape_table();
+    *count = table.used[slot];
+    *beyond_shape = table.multi_pass_shape[slot];
+    for (int i = 0; i < table.used[slot] && i < capacity; ++i) {
+        shapes[i] = table.order[slot][i];
+        max_rows[i] = table.limits[slot][table.order[slot][i]];
+    }
+    return sz_success_k;
+}
+
+/** Validates that every shape can score the slot's family and that single-stripe shapes stay within capacity. */
+sz_status_t szs_tuning_set_ladder(int slot, int const *shapes, int const *max_rows, int count, int beyond_shape,
+                                  char const **error_message) {
+    using namespace szs_amd;
+    if (slot < 0 || slot >= shape_slots_k || count < 0 || count > shapes_count_k || (count && (!shapes || !max_rows)))
+        return fail(sz_unexpected_dimensions_k, error_message, "Invalid tuning slot or ladder size");
+    int const family = slot == unit_bytes_slot_k ? family_lev_linear_k : slot / 2;
+    bool const uniform = family == family_lev_linear_k || family == family_lev_affine_k;
+    bool const runes = slot != unit_bytes_slot_k && slot % 2 == 1;
+    if (!uniform && runes) return fail(sz_unexpected_dimensions_k, error_message, "Class costs never score runes");
+    auto allowed = [&](int shape) {
+        if (shape < 0 || shape >= shapes_count_k) return false;
+        shape_t const s = shapes_k[shape];
+        if (s.bit_parallel) return slot == unit_bytes_slot_k;
+        if (!uniform && s.rows_per_lane > 8) return false; // LDS profiles are sized for 8 rows per lane
+        return true;
+    };
+    if (!allowed(beyond_shape) || shapes_k[beyond_shape].group != 64)
+        return fail(sz_unexpected_dimensions_k, error_message, "The shape for tall pairs must be a 64-lane shape");
+    int previous = 0;
+    for (int i = 0; i < count; ++i) {
+        if (!allowed(shapes[i])) return fail(sz_unexpected_dimensions_k, error_message, "Shape not valid for this slot");
+        shape_t const s = shapes_k[shapes[i]];
+        if (max_rows[i] <= previous) return fail(sz_unexpected_dimensions_k, error_message, "Row limits must increase");
+        if (s.group < 64 && max_rows[i] > s.group * s.rows_per_lane)
+            return fail(sz_unexpected_dimensions_k, error_message, "A sub-wave shape holds a single stripe only");
+        for (int j = 0; j < i; ++j)
+            if (shapes[j] == shapes[i]) return fail(sz_unexpected_dimensions_k, error_message, "Shapes must be distinct");
+        previous = max_rows[i];
+    }
+    std::lock_guard<std::mutex> lock(tuning_mutex());
+    shape_table_t &table = active_shape_table();
+    for (int shape = 0; shape < shapes_count_k; ++shape) table.limits[slot][shape] = -1;
+    for (int i = 0; i < count; ++i) table.order[slot][i] = shapes[i], table.limits[slot][shapes[i]] = max_rows[i];
+    table.used[slot] = count;
+    table.multi_pass_shape[slot] = beyond_shape;
+    return ok(error_message);
+}
+
+} // extern "C"
+
+#pragma endregion

@@ -120,3 +120,31 @@ def test_arrow_inputs_are_viewed_as_tapes():
         tape = szs._as_tape(array, "queries")
         expected = strings[1:4] if len(array) == 3 else strings
         assert [tape[i].decode() for i in range(len(tape))] == expected
+
+
+def test_tuning_ladders_validate_and_round_trip():
+    from stringzillas_amd import autotune
+
+    table = autotune.shapes()
+    assert len(table) == _native.lib.szs_tuning_shapes()
+    unit = autotune.slot_of("lev_linear", unit=True)
+    nw = autotune.slot_of("nw_affine")
+    saved = {slot: autotune.get_ladder(slot) for slot in (unit, nw)}
+    try:
+        ladder, beyond = saved[nw]
+        assert ladder and all(b[1] > a[1] for a, b in zip(ladder, ladder[1:]))
+        autotune.set_ladder(nw, ladder[:2], beyond)
+        assert autotune.get_ladder(nw) == (ladder[:2], beyond)
+        myers = next(i for i, s in enumerate(table) if s[3] and s[0] == 64 and not s[2])
+        with pytest.raises(ValueError):
+            autotune.set_ladder(nw, [(myers, 64)], beyond)  # bit-parallel shapes score unit costs only
+        with pytest.raises(ValueError):
+            autotune.set_ladder(nw, [(0, 10_000)], beyond)  # a sub-wave shape holds one stripe
+        with pytest.raises(ValueError):
+            autotune.set_ladder(nw, [(3, 100), (4, 50)], beyond)  # limits must grow
+        with pytest.raises(ValueError):
+            autotune.set_ladder(autotune.slot_of("nw_affine", runes=True), [], beyond)
+        assert autotune.get_ladder(nw) == (ladder[:2], beyond)  # rejected ladders change nothing
+    finally:
+        for slot, (ladder, beyond) in saved.items():
+            autotune.set_ladder(slot, ladder, beyond)
