@@ -309,15 +309,17 @@ double szs_batch_last_kernel_ms(szs_batch_t batch);
 void szs_batch_free(szs_batch_t batch);
 
 /**
- *  Serving: a continuous-batching request loop. Requests (one segment each) are admitted while their tapes fit the
- *  HBM budget; a dispatcher thread coalesces everything admitted into the next device round while earlier rounds
- *  are still running, so requests that arrive during a launch overlap with it.
+ *  Serving: a continuous-batching request loop. Requests (one segment each) are admitted while their footprint fits
+ *  the HBM budget; a dispatcher thread coalesces everything admitted into the next device round while earlier rounds
+ *  are still running, so requests that arrive during a launch overlap with it. The server reserves its pool once
+ *  (the budget plus an eighth, at least 256 MiB) at init and prepares every round from it.
  */
 typedef void *szs_server_t;
 typedef sz_u64_t szs_ticket_t;
 sz_status_t szs_server_init(szs_device_scope_t device, sz_size_t hbm_budget_bytes, sz_size_t max_inflight_rounds,
                             szs_server_t *server, char const **error_message);
-/** Copies the request's tapes into the server's device pool; fails with `sz_bad_alloc_k` if over budget. */
+/** Queues a pair-list request. Device-resident tapes are scored in place; host tapes are copied into the pool with
+ *  the round. Fails with `sz_bad_alloc_k` if the request alone exceeds the budget. */
 sz_status_t szs_server_submit(szs_server_t server, szs_pairs_segment_t const *request, szs_ticket_t *ticket,
                               char const **error_message);
 /** Blocks until @p ticket completes; its results are then in the request's host `results` buffer. */

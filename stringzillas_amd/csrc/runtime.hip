@@ -57,9 +57,65 @@ sz_status_t from_hip(hipError_t error, char const **error_message, char const *c
 
 #pragma region Device buffers
 
+hipError_t arena_t::init(size_t bytes) {
+    capacity = (bytes + 4095) & ~size_t(4095);
+    void *pointer = nullptr;
+    hipError_t const error = hipMalloc(&pointer, capacity);
+    if (error != hipSuccess) {
+        (void)hipGetLastError();
+        capacity = 0;
+        return error;
+    }
+    base = static_cast<char *>(pointer);
+    free_blocks.clear();
+    free_blocks[0] = capacity;
+    return hipSuccess;
+}
+
+void *arena_t::allocate(size_t bytes) {
+    std::lock_guard<std::mutex> lock(mutex);
+    for (auto block = free_blocks.begin(); block != free_blocks.end(); ++block) {
+        if (block->second < bytes) continue;
+        size_t const offset = block->first, left = block->second - bytes;
+        free_blocks.erase(block);
+        if (left) free_blocks[offset + bytes] = left;
+        return base + offset;
+    }
+    return nullptr;
+}
+
+void arena_t::deallocate(void *pointer, size_t bytes) {
+    std::lock_guard<std::mutex> lock(mutex);
+    size_t offset = static_cast<char *>(pointer) - base;
+    auto next = free_blocks.lower_bound(offset);
+    if (next != free_blocks.end() && offset + bytes == next->first) { // merge with the following block
+        bytes += next->second;
+        next = free_blocks.erase(next);
+    }
+    if (next != free_blocks.begin()) { // merge with the preceding block
+        auto previous = std::prev(next);
+        if (previous->first + previous->second == offset) {
+            previous->second += bytes;
+            return;
+        }
+    }
+    free_blocks[offset] = bytes;
+}
+
+arena_t::~arena_t() {
+    if (base) (void)hipFree(base);
+}
+
+arena_t *&thread_arena() {
+    static thread_local arena_t *arena = nullptr;
+    return arena;
+}
+
 void device_buffer_t::release() {
-    if (data) (void)hipFree(data);
+    if (data && arena) arena->deallocate(data, capacity);
+    else if (data) (void)hipFree(data);
     data = nullptr;
+    arena = nullptr;
     capacity = 0;
 }
 
@@ -67,6 +123,14 @@ hipError_t device_buffer_t::reserve(size_t bytes) {
     if (bytes <= capacity) return hipSuccess;
     release();
     size_t const rounded = (bytes + 4095) & ~size_t(4095);
+    if (arena_t *pool = thread_arena()) {
+        data = pool->allocate(rounded);
+        if (data) {
+            arena = pool;
+            capacity = rounded;
+            return hipSuccess;
+        }
+    }
     hipError_t const error = hipMalloc(&data, rounded);
     if (error != hipSuccess) {
         data = nullptr;

@@ -6,11 +6,14 @@
  *  every admissible request, stages them as one multi-segment round (one persistent, length-binned launch), and
  *  queues it on one of `max_inflight_rounds` streams, so a round formed from requests that arrived during earlier
  *  launches runs while those are still on the device. Completed rounds release their budget and wake waiters.
+ *  Rounds draw their device buffers from a pool reserved once at init, so the loop never calls `hipMalloc` or
+ *  `hipFree` (which waits for every stream of the device) while other rounds run.
  *  Results do not depend on which round a request joins: each pair is scored independently.
  *  No reference counterpart (the reference has no serving layer).
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -30,7 +33,7 @@ namespace {
 struct request_t {
     szs_ticket_t ticket = 0;
     segment_input_t input;
-    sz_sequence_u64tape_t firsts, seconds; /**< the caller's tapes (host), kept alive by the caller until wait */
+    sz_sequence_u64tape_t firsts, seconds; /**< the caller's tapes (host or HBM), kept alive by the caller until wait */
     size_t footprint = 0;
 };
 
@@ -60,21 +63,37 @@ struct server_t {
     szs_ticket_t next_ticket = 1;
     bool stopping = false;
     std::thread dispatcher;
+    arena_t arena; /**< every round's device buffers, so rounds never allocate from the driver */
 
     void run();
 };
 
-/** HBM a request occupies while staged: tapes, offsets, runes for UTF-8, tasks and bin bookkeeping, results. */
-size_t footprint_of(engine_t const *engine, sz_sequence_u64tape_t const *a, sz_sequence_u64tape_t const *b) {
-    size_t const count = a->count;
-    size_t const bytes = (count ? a->offsets[count] : 0) + (b->count ? b->offsets[b->count] : 0);
-    size_t total = bytes + 16 * (count + 1) + count * (sizeof(task_t) + 4 * 3 + 8);
+/** Bytes of a tape's string data: `offsets[count]`, read from HBM for device-resident tapes. */
+sz_status_t tape_bytes(sz_sequence_u64tape_t const *tape, bool device, uint64_t *bytes, char const **error_message) {
+    *bytes = 0;
+    if (!tape->count) return ok(error_message);
+    if (!device) {
+        *bytes = tape->offsets[tape->count];
+        return ok(error_message);
+    }
+    return from_hip(hipMemcpy(bytes, tape->offsets + tape->count, 8, hipMemcpyDeviceToHost), error_message,
+                    "Reading the tape length failed");
+}
+
+/**
+ *  HBM a request occupies while staged: host tapes are copied (data and u64 offsets), device tapes are used in
+ *  place; UTF-8 requests add the decoded runes; every request adds its tasks, bin bookkeeping and results.
+ */
+size_t footprint_of(engine_t const *engine, uint64_t count, uint64_t bytes, bool staged) {
+    size_t total = count * (sizeof(task_t) + 4 * 3 + 8);
+    if (staged) total += bytes + 16 * (count + 1);
     if (engine->kind == szs_engine_levenshtein_utf8_k) total += bytes * 4 + count * 8;
     return total + 64 * 1024;
 }
 
 void server_t::run() {
     if (hipSetDevice(device) != hipSuccess) (void)hipGetLastError();
+    thread_arena() = arena.base ? &arena : nullptr;
     std::unique_lock<std::mutex> lock(mutex);
     while (true) {
         // Retire finished rounds.
@@ -172,6 +191,14 @@ sz_status_t szs_server_init(szs_device_scope_t device, sz_size_t hbm_budget_byte
     server->device = gpu;
     server->budget = hbm_budget_bytes;
     server->max_inflight = max_inflight_rounds ? max_inflight_rounds : 2;
+    // The pool holds the admitted footprints plus room for each in-flight round's boundary-row scratch; a round
+    // that still does not fit falls back to plain allocations.
+    size_t const margin = std::max<size_t>(size_t(256) << 20, hbm_budget_bytes / 8);
+    hipError_t const pooled = server->arena.init(hbm_budget_bytes + margin);
+    if (pooled != hipSuccess) {
+        delete server;
+        return from_hip(pooled, error_message, "Failed to reserve the server's HBM pool");
+    }
     for (size_t s = 0; s < server->max_inflight; ++s) {
         hipStream_t stream = nullptr;
         hipError_t const error = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
@@ -198,15 +225,23 @@ sz_status_t szs_server_submit(szs_server_t handle, szs_pairs_segment_t const *re
         return fail(sz_unexpected_dimensions_k, error_message, "Request kind does not match its engine");
     if (request->firsts->count != request->seconds->count)
         return fail(sz_unexpected_dimensions_k, error_message, "Pair tapes must hold equal counts");
-    if (is_device_pointer(request->firsts->offsets) || is_device_pointer(request->seconds->offsets))
-        return fail(sz_device_memory_mismatch_k, error_message, "Serving requests take host tapes");
+    // Device-resident tapes are scored in place; anything else is staged into HBM with the round.
+    bool const device_firsts = is_device_pointer(request->firsts->offsets);
+    bool const device_seconds = is_device_pointer(request->seconds->offsets);
+    bool const staged_firsts = !device_firsts || !is_device_pointer(request->firsts->data);
+    bool const staged_seconds = !device_seconds || !is_device_pointer(request->seconds->data);
+    uint64_t firsts_bytes = 0, seconds_bytes = 0;
+    sz_status_t status = tape_bytes(request->firsts, device_firsts, &firsts_bytes, error_message);
+    if (status == sz_success_k) status = tape_bytes(request->seconds, device_seconds, &seconds_bytes, error_message);
+    if (status != sz_success_k) return status;
     request_t r;
     r.firsts = *request->firsts;
     r.seconds = *request->seconds;
     r.input.engine = engine;
     r.input.mode = segment_pairs_k;
     r.input.results = request->results;
-    r.footprint = footprint_of(engine, request->firsts, request->seconds);
+    r.footprint = footprint_of(engine, request->firsts->count, firsts_bytes, staged_firsts) +
+                  footprint_of(engine, 0, seconds_bytes, staged_seconds);
     if (r.footprint > server->budget)
         return fail(sz_bad_alloc_k, error_message, "Request exceeds the server's HBM budget");
     std::lock_guard<std::mutex> lock(server->mutex);

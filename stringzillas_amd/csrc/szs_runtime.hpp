@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -19,10 +20,30 @@ sz_status_t ok(char const **error_message);
 sz_status_t from_hip(hipError_t error, char const **error_message, char const *context);
 bool is_device_pointer(void const *pointer);
 
+/**
+ *  A device memory pool carved from one allocation: first-fit over an offset-ordered free list, coalescing on
+ *  release. The serving loop prepares its rounds from a pool so that it never calls `hipMalloc` or `hipFree`
+ *  (which waits for the whole device) while other rounds are running.
+ */
+struct arena_t {
+    char *base = nullptr;
+    size_t capacity = 0;
+    std::map<size_t, size_t> free_blocks; /**< offset -> bytes */
+    std::mutex mutex;
+    hipError_t init(size_t bytes);
+    void *allocate(size_t bytes);
+    void deallocate(void *pointer, size_t bytes);
+    ~arena_t();
+};
+
+/** The pool `device_buffer_t::reserve` draws from on the calling thread (null: plain `hipMalloc`). */
+arena_t *&thread_arena();
+
 /** Grow-only device allocation. */
 struct device_buffer_t {
     void *data = nullptr;
     size_t capacity = 0;
+    arena_t *arena = nullptr; /**< the pool `data` came from, if any */
     hipError_t reserve(size_t bytes);
     void release();
     template <typename type_>

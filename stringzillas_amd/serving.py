@@ -7,11 +7,20 @@ memory stays bounded. Latency = completion time - arrival time, measured on the 
 
 The baseline serves the same arrivals with one plain launch per request, in arrival order, without batching
 across requests (``engine.pairs`` per request).
+
+Templates are uploaded to HBM once (``to_device``) unless ``--host-tapes`` is given, so both loops score
+device-resident tapes and only the results travel back.
+
+Run on the GPU: ``python -m stringzillas_amd.serving --qps 200 1000 5000 --seconds 20``.
 """
 
 from __future__ import annotations
 
+import argparse
+import itertools
+import json
 import math
+import sys
 import threading
 import time
 from dataclasses import dataclass
@@ -35,7 +44,8 @@ def _log_uniform_int(rng, low: int, high: int) -> int:
     return int(min(high, max(low, math.exp(rng.uniform(math.log(low), math.log(high + 1))))))
 
 
-def make_templates(count: int = 48, max_pairs: int = 4096, max_length: int = 16384, seed: int = 0x535A0004):
+def make_templates(count: int = 48, max_pairs: int = 4096, max_length: int = 16384, seed: int = 0x535A0004,
+                   device: bool = True):
     """Request payloads following the mixed-batch length and kind mix (seeded, numpy)."""
     import torch
 
@@ -55,6 +65,8 @@ def make_templates(count: int = 48, max_pairs: int = 4096, max_length: int = 163
         firsts = szs.Strs.from_tape(tapes[0][0].numpy(), tapes[0][1].numpy().astype(np.uint64))
         seconds = szs.Strs.from_tape(tapes[1][0].numpy(), tapes[1][1].numpy().astype(np.uint64))
         segment = W.Segment(kind, firsts.data, firsts.offsets, seconds.data, seconds.offsets)
+        if device:
+            firsts, seconds = szs.to_device(firsts), szs.to_device(seconds)
         templates.append(Template(kind, firsts, seconds, segment.cells()))
     return templates
 
@@ -117,8 +129,8 @@ def run_server(templates, engines, qps: float, duration: float, hbm_budget: int 
     deadline = start + duration + drain_timeout
     results = {}
     while (submitter.is_alive() or pending) and time.perf_counter() < deadline:
-        with lock:
-            tickets = list(pending.items())
+        with lock:  # rounds are admitted first-in first-out, so the oldest tickets finish first
+            tickets = list(itertools.islice(pending.items(), 512))
         finished = [(t, v) for t, v in tickets if server.poll(t)]
         now = time.perf_counter()
         for ticket, (arrived, template) in finished:
@@ -161,3 +173,48 @@ def run_baseline(templates, engines, qps: float, duration: float, seed: int = 1,
     summary = _summary(latencies, cells, wall, qps, done)
     summary["dropped_after_drain_timeout"] = len(arrivals) - done
     return summary
+
+
+def main(argv=None) -> None:
+    parser = argparse.ArgumentParser(description="Poisson serving benchmark of the continuous-batching server")
+    parser.add_argument("--qps", type=float, nargs="+", default=[200.0, 1000.0, 5000.0])
+    parser.add_argument("--seconds", type=float, default=20.0, help="arrival window per offered rate")
+    parser.add_argument("--drain", type=float, default=30.0, help="extra time allowed to finish queued requests")
+    parser.add_argument("--templates", type=int, default=96)
+    parser.add_argument("--max-pairs", type=int, default=4096)
+    parser.add_argument("--max-length", type=int, default=16384)
+    parser.add_argument("--inflight", type=int, default=2)
+    parser.add_argument("--budget-gib", type=float, default=64.0)
+    parser.add_argument("--host-tapes", action="store_true")
+    parser.add_argument("--skip-baseline", action="store_true")
+    args = parser.parse_args(argv)
+
+    def log(message):
+        print(f"[serving] {message}", file=sys.stderr, flush=True)
+
+    start = time.time()
+    templates = make_templates(args.templates, args.max_pairs, args.max_length, device=not args.host_tapes)
+    engines = engines_by_kind()
+    mean_cells = float(np.mean([t.cells for t in templates]))
+    log(f"{len(templates)} templates, mean {mean_cells / 1e9:.2f} G cells per request ({time.time() - start:.1f}s)")
+    # Warm both paths (kernel loading, first allocations).
+    run_server(templates[:4], engines, qps=50.0, duration=0.2, hbm_budget=int(args.budget_gib * (1 << 30)))
+    for template in templates[:4]:
+        engines[template.kind].pairs(template.firsts, template.seconds)
+    report = {"templates": len(templates), "mean_cells_per_request": mean_cells,
+              "tapes": "host" if args.host_tapes else "device", "runs": []}
+    for qps in args.qps:
+        summary, _ = run_server(templates, engines, qps, args.seconds, hbm_budget=int(args.budget_gib * (1 << 30)),
+                                inflight=args.inflight, drain_timeout=args.drain)
+        log(f"server at {qps:g} req/s: {summary}")
+        entry = {"offered_qps": qps, "server": summary}
+        if not args.skip_baseline:
+            baseline = run_baseline(templates, engines, qps, args.seconds, drain_timeout=args.drain)
+            log(f"per-request launches at {qps:g} req/s: {baseline}")
+            entry["per_request"] = baseline
+        report["runs"].append(entry)
+    print(json.dumps(report), flush=True)
+
+
+if __name__ == "__main__":
+    main()

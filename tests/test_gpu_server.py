@@ -1,0 +1,118 @@
+"""GPU tests of the continuous-batching server: every request's scores equal the same request scored alone (and the
+oracle), whichever round it joins, with host or HBM-resident tapes, under a budget that forces many rounds."""
+
+import threading
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import DNA, PROTEIN, dna_tables, protein_tables, random_bytes, random_utf8
+
+pytestmark = pytest.mark.gpu
+
+szs = pytest.importorskip("stringzillas_amd")
+
+
+def _kinds(rng):
+    b2c, costs = protein_tables(rng)
+    dna_b2c, dna_costs = dna_tables()
+    return [
+        (szs.LevenshteinDistances(), O.Params("levenshtein"), DNA, False),
+        (szs.LevenshteinDistances(0, 4, 3, 2), O.Params("levenshtein", 0, 4, 3, 2), DNA, False),
+        (szs.LevenshteinDistancesUTF8(), O.Params("levenshtein_utf8"), None, True),
+        (szs.NeedlemanWunschScores(b2c, costs, -11, -1),
+         O.Params("needleman_wunsch", open=-11, extend=-1, byte_to_class=b2c, class_costs=costs), PROTEIN, False),
+        (szs.SmithWatermanScores(dna_b2c, dna_costs, -2, -2),
+         O.Params("smith_waterman", open=-2, extend=-2, byte_to_class=dna_b2c, class_costs=dna_costs), DNA, False),
+    ]
+
+
+def _requests(seed, count):
+    rng = np.random.default_rng(seed)
+    kinds = _kinds(rng)
+    requests = []
+    for r in range(count):
+        engine, params, alphabet, utf8 = kinds[r % len(kinds)]
+        pairs = [0, 1, 3, 17, 40][r % 5]
+        high = [8, 90, 700, 1500][r % 4]
+
+        def make():
+            if utf8:
+                return random_utf8(rng, 0, high)
+            return random_bytes(rng, alphabet, 0, high)
+
+        firsts = szs.Strs([make() for _ in range(pairs)])
+        seconds = szs.Strs([make() for _ in range(pairs)])
+        requests.append((engine, params, firsts, seconds))
+    return requests
+
+
+def _expected(params, firsts, seconds):
+    return O.pairs(params, (firsts.data, firsts.offsets), (seconds.data, seconds.offsets), threads=8)
+
+
+@pytest.mark.parametrize("budget_mib,inflight", [(1024, 2), (4, 3)])
+def test_server_results_equal_results_alone(budget_mib, inflight):
+    requests = _requests(7 + budget_mib, 40)
+    server = szs.Server(hbm_budget_bytes=budget_mib << 20, max_inflight_rounds=inflight)
+    try:
+        tickets = []
+        for i, (engine, _, firsts, seconds) in enumerate(requests):
+            if i % 3 == 2:  # HBM-resident tapes are scored in place
+                firsts, seconds = szs.to_device(firsts), szs.to_device(seconds)
+            tickets.append((server.submit(engine, firsts, seconds), firsts, seconds))
+        for (ticket, _, _), (engine, params, firsts, seconds) in zip(tickets, requests):
+            out = server.wait(ticket)
+            alone = engine.pairs(firsts, seconds)
+            assert out.dtype == alone.dtype
+            np.testing.assert_array_equal(out, alone)
+            np.testing.assert_array_equal(out, _expected(params, firsts, seconds))
+        assert server.bytes_in_use == 0
+    finally:
+        server.close()
+
+
+def test_server_concurrent_submitters_and_polling():
+    requests = _requests(11, 30)
+    server = szs.Server(hbm_budget_bytes=64 << 20, max_inflight_rounds=2)
+    results = {}
+    errors = []
+
+    def submitter(part):
+        try:
+            for index in range(part, len(requests), 3):
+                engine, _, firsts, seconds = requests[index]
+                results[index] = server.submit(engine, firsts, seconds)
+        except Exception as error:  # surfaced below
+            errors.append(error)
+
+    threads = [threading.Thread(target=submitter, args=(p,)) for p in range(3)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    try:
+        assert not errors
+        for index, ticket in results.items():
+            engine, params, firsts, seconds = requests[index]
+            while not server.poll(ticket):
+                pass
+            np.testing.assert_array_equal(server.wait(ticket), _expected(params, firsts, seconds))
+    finally:
+        server.close()
+
+
+def test_server_rejects_requests_over_budget_and_bad_shapes():
+    rng = np.random.default_rng(3)
+    server = szs.Server(hbm_budget_bytes=1 << 20, max_inflight_rounds=1)
+    try:
+        big = szs.Strs([random_bytes(rng, DNA, 4000, 4000) for _ in range(300)])
+        with pytest.raises(MemoryError):
+            server.submit(szs.LevenshteinDistances(), big, big)
+        with pytest.raises(ValueError):
+            server.submit(szs.LevenshteinDistances(), szs.Strs(["a", "b"]), szs.Strs(["a"]))
+        ticket = server.submit(szs.LevenshteinDistances(), szs.Strs(["kitten"]), szs.Strs(["sitting"]))
+        assert server.wait(ticket).tolist() == [3]
+    finally:
+        server.close()
