@@ -81,7 +81,8 @@ def _engine_for(slot: int):
 def _batch_for(rng, alphabet, low: int, high: int, target_cells: float, max_pairs: int = 200_000):
     mean_rows = (low + high) / 2
     mean_cells = mean_rows * mean_rows * 2.5
-    pairs = int(min(max_pairs, max(64, target_cells / mean_cells)))
+    # Enough pairs to fill every wave slot of the device twice, as inside a large batch.
+    pairs = int(min(max_pairs, max(2048, target_cells / mean_cells)))
     rows = rng.integers(low, high + 1, size=pairs)
     columns = (rows * rng.uniform(1.0, 4.0, size=pairs)).astype(np.int64)
     a = W._random_tape(rng, alphabet, rows)
@@ -128,8 +129,7 @@ def tune(slots: List[int] = None, target_cells: float = 4e9, verbose: bool = Tru
     table = shapes()
     unit_slot = _native.lib.szs_tuning_slots() - 1
     if slots is None:
-        slots = [unit_slot, slot_of("lev_linear"), slot_of("lev_affine"), slot_of("lev_linear", runes=True),
-                 slot_of("nw_affine"), slot_of("nw_linear"), slot_of("sw_affine"), slot_of("sw_linear")]
+        slots = [unit_slot, slot_of("lev_linear", runes=True), slot_of("nw_affine"), slot_of("sw_linear")]
     result = {"arch": torch.cuda.get_device_properties(0).gcnArchName.split(":")[0], "slots": {}}
     for slot in slots:
         default_ladder, default_beyond = get_ladder(slot)
