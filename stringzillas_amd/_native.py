@@ -13,6 +13,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY_PATH = os.path.join(HERE, "libstringzillas_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "stringzillas_amd.h")
 
+try:
+    # PyTorch's ROCm wheels ship their own HIP runtime under the same soname as /opt/rocm's. Whichever loads first
+    # serves the whole process, and torch cannot start on the other one; loading torch first keeps one runtime, so
+    # device tensors and this library's device pointers are interchangeable.
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - torch is optional for host-only use
+    pass
+
 if not os.path.exists(LIBRARY_PATH):
     # A fresh tree: compile it in place (hipcc is part of the toolchain); anything else is a hard error.
     try:
