@@ -280,6 +280,10 @@ typedef struct szs_pairs_segment_t {
     sz_sequence_u64tape_t const *firsts;   /**< host or device tapes */
     sz_sequence_u64tape_t const *seconds;
     void *results;                         /**< `count` x 8-byte results (u64 or i64 by kind) */
+    /** Optional `offsets[count]` of each tape. For tapes in HBM it spares a device read per tape (0: read it).
+     *  When nonzero it must equal `offsets[count]`. */
+    sz_size_t firsts_bytes;
+    sz_size_t seconds_bytes;
 } szs_pairs_segment_t;
 
 /**
@@ -324,9 +328,26 @@ sz_status_t szs_server_submit(szs_server_t server, szs_pairs_segment_t const *re
                               char const **error_message);
 /** Blocks until @p ticket completes; its results are then in the request's host `results` buffer. */
 sz_status_t szs_server_wait(szs_server_t server, szs_ticket_t ticket, char const **error_message);
+/** Like `szs_server_wait`, and reports when the server saw the request's round finish, in CLOCK_MONOTONIC seconds
+ *  (the clock of Python's `time.perf_counter` on Linux), so latency needs no polling by the caller. */
+sz_status_t szs_server_wait_timed(szs_server_t server, szs_ticket_t ticket, double *completed_seconds,
+                                  char const **error_message);
 /** Non-blocking completion check: 1 if done, 0 if pending. */
 int szs_server_poll(szs_server_t server, szs_ticket_t ticket);
 sz_size_t szs_server_bytes_in_use(szs_server_t server);
+
+/** Serving counters since init. */
+typedef struct szs_server_stats_t {
+    sz_u64_t rounds;          /**< device rounds launched */
+    sz_u64_t requests;        /**< requests admitted into rounds */
+    sz_u64_t cells;           /**< DP cells scored by completed rounds */
+    sz_u64_t largest_round;   /**< most requests in one round */
+    sz_u64_t pending;         /**< requests waiting for admission now */
+    sz_u64_t inflight;        /**< rounds on the device now */
+    double prepare_seconds;   /**< host time spent staging, binning and queuing rounds */
+    double device_seconds;    /**< scoring-kernel time of completed rounds (HIP events) */
+} szs_server_stats_t;
+sz_status_t szs_server_get_stats(szs_server_t server, szs_server_stats_t *stats);
 void szs_server_free(szs_server_t server);
 
 /**

@@ -98,6 +98,11 @@ class Strs:
     def __len__(self) -> int:
         return len(self.offsets) - 1
 
+    @property
+    def nbytes(self) -> int:
+        """Bytes of string data, ``offsets[count]``."""
+        return int(self.offsets[-1]) if len(self.offsets) else 0
+
     def __getitem__(self, index: int) -> bytes:
         begin, end = int(self.offsets[index]), int(self.offsets[index + 1])
         return self.data[begin:end].tobytes()
@@ -123,6 +128,10 @@ class DeviceStrs:
 
     def __len__(self) -> int:
         return self.count
+
+    @property
+    def nbytes(self) -> int:
+        return self.host.nbytes
 
     def tape(self) -> _U64Tape:
         return _U64Tape(self._data.data_ptr(), self._offsets.data_ptr(), self.count)
@@ -375,6 +384,8 @@ class Batch:
             array[i].firsts = ctypes.pointer(a)
             array[i].seconds = ctypes.pointer(b)
             array[i].results = ctypes.c_void_p(pointer)
+            array[i].firsts_bytes = int(getattr(firsts, "nbytes", 0) or 0)
+            array[i].seconds_bytes = int(getattr(seconds, "nbytes", 0) or 0)
             self.outputs.append(out)
         self._engines = [s[0] for s in segments]
         handle, error = ctypes.c_void_p(), ctypes.c_char_p()
@@ -433,6 +444,8 @@ class Server:
         segment.firsts = ctypes.pointer(a)
         segment.seconds = ctypes.pointer(b)
         segment.results = ctypes.c_void_p(out.ctypes.data)
+        segment.firsts_bytes = int(getattr(firsts, "nbytes", 0) or 0)
+        segment.seconds_bytes = int(getattr(seconds, "nbytes", 0) or 0)
         ticket = ctypes.c_uint64()
         error = ctypes.c_char_p()
         _check(_lib.szs_server_submit(self.handle, ctypes.byref(segment), ctypes.byref(ticket), ctypes.byref(error)),
@@ -441,11 +454,22 @@ class Server:
         return ticket.value
 
     def wait(self, ticket: int) -> np.ndarray:
+        return self.wait_timed(ticket)[0]
+
+    def wait_timed(self, ticket: int) -> Tuple[np.ndarray, float]:
+        """Results and the time the request's round finished, on the clock of ``time.perf_counter``."""
         error = ctypes.c_char_p()
-        status = _lib.szs_server_wait(self.handle, ctypes.c_uint64(ticket), ctypes.byref(error))
+        completed = ctypes.c_double()
+        status = _lib.szs_server_wait_timed(self.handle, ctypes.c_uint64(ticket), ctypes.byref(completed),
+                                            ctypes.byref(error))
         kept = self._requests.pop(ticket)
         _check(status, error, "Server request")
-        return kept[4]
+        return kept[4], completed.value
+
+    def stats(self) -> dict:
+        stats = _native.ServerStats()
+        _lib.szs_server_get_stats(self.handle, ctypes.byref(stats))
+        return {name: getattr(stats, name) for name, _ in stats._fields_}
 
     def poll(self, ticket: int) -> bool:
         return bool(_lib.szs_server_poll(self.handle, ctypes.c_uint64(ticket)))

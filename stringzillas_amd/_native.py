@@ -47,6 +47,18 @@ class Segment(ctypes.Structure):
         ("firsts", ctypes.POINTER(U64Tape)),
         ("seconds", ctypes.POINTER(U64Tape)),
         ("results", ctypes.c_void_p),
+        ("firsts_bytes", ctypes.c_size_t),
+        ("seconds_bytes", ctypes.c_size_t),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    """``szs_server_stats_t`` (include/stringzillas_amd.h)."""
+
+    _fields_ = [
+        ("rounds", ctypes.c_uint64), ("requests", ctypes.c_uint64), ("cells", ctypes.c_uint64),
+        ("largest_round", ctypes.c_uint64), ("pending", ctypes.c_uint64), ("inflight", ctypes.c_uint64),
+        ("prepare_seconds", ctypes.c_double), ("device_seconds", ctypes.c_double),
     ]
 
 

@@ -81,8 +81,9 @@ def _engine_for(slot: int):
 def _batch_for(rng, alphabet, low: int, high: int, target_cells: float, max_pairs: int = 200_000):
     mean_rows = (low + high) / 2
     mean_cells = mean_rows * mean_rows * 2.5
-    # Enough pairs to fill every wave slot of the device twice, as inside a large batch.
-    pairs = int(min(max_pairs, max(2048, target_cells / mean_cells)))
+    # Enough pairs to fill every wave slot of the device (256 CUs x 4 workgroups x 4 waves) twice, as inside a
+    # large batch, so shapes are compared at full occupancy.
+    pairs = int(min(max_pairs, max(8192, target_cells / mean_cells)))
     rows = rng.integers(low, high + 1, size=pairs)
     columns = (rows * rng.uniform(1.0, 4.0, size=pairs)).astype(np.int64)
     a = W._random_tape(rng, alphabet, rows)
@@ -112,11 +113,14 @@ def candidates_for(slot: int, low: int, high: int, table) -> List[int]:
             continue
         capacity = group * rows_per_lane
         if cooperative:
-            if high > 4 * capacity:  # needs several stripes per wave to pay off
+            if high > 2 * capacity:  # needs stripes for several waves to pay off
                 out.append(index)
             continue
         if group < 64:
             if high <= capacity:
+                out.append(index)
+        elif bit_parallel:
+            if high > 256:  # a partly filled 64-word stripe still scores 64 cells per word operation
                 out.append(index)
         elif high <= 4 * capacity and (high > capacity // 2 or capacity <= 128):
             out.append(index)

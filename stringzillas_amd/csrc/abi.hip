@@ -408,6 +408,8 @@ sz_status_t szs_batch_init(szs_device_scope_t device, szs_pairs_segment_t const 
         inputs[s].queries = segment.firsts;
         inputs[s].candidates = segment.seconds;
         inputs[s].results = segment.results;
+        inputs[s].queries_bytes = segment.firsts_bytes;
+        inputs[s].candidates_bytes = segment.seconds_bytes;
     }
     batch_t *prepared = nullptr;
     sz_status_t const status = batch_prepare(as_scope(device), inputs, mode, &prepared, error_message);

@@ -24,9 +24,11 @@ hipError_t szs_amd_launch_fill(uint32_t *data, uint32_t count, uint32_t value, h
 hipError_t szs_amd_launch_utf8_decode(uint8_t const *bytes, uint64_t const *offsets, uint64_t count, uint32_t *runes,
                                       uint32_t *rune_counts, hipStream_t stream);
 hipError_t szs_amd_launch_widen_offsets(uint32_t const *narrow, uint64_t *wide, uint64_t count, hipStream_t stream);
-hipError_t szs_amd_launch_materialize(szs_amd::materialize_args_t const *args, szs_amd::shape_table_t const *table,
-                                      szs_amd::task_t *tasks, uint32_t *bin_of_task, uint32_t *histogram,
-                                      szs_amd::segment_view_t const *segments, hipStream_t stream);
+hipError_t szs_amd_launch_materialize(szs_amd::materialize_args_t const *args, uint32_t segments_count,
+                                      uint64_t tasks_total, szs_amd::shape_table_t const *table, szs_amd::task_t *tasks,
+                                      uint32_t *bin_of_task, uint32_t *histogram,
+                                      szs_amd::segment_view_t const *segments, szs_amd::prepare_stats_t *stats,
+                                      uint32_t *segment_overflow, hipStream_t stream);
 hipError_t szs_amd_launch_scan_bins(szs_amd::bins_view_t const *bins, hipStream_t stream);
 hipError_t szs_amd_launch_scatter_bins(szs_amd::bins_view_t const *bins, uint32_t const *bin_of_task,
                                        uint32_t const *position_of_bin, uint32_t tasks_count, hipStream_t stream);
@@ -120,9 +122,11 @@ static std::vector<uint32_t> const &bin_priority() {
 
 #pragma region Tape staging
 
+/** `bytes_hint` is `offsets[count]` when the caller knows it (0: read it back from device offsets). */
 template <typename offset_type_>
 static sz_status_t stage_tape(sz_cptr_t data, offset_type_ const *offsets, uint64_t count, bool runes,
-                              staged_tape_t &tape, hipStream_t stream, char const **error_message) {
+                              uint64_t bytes_hint, staged_tape_t &tape, hipStream_t stream,
+                              char const **error_message) {
     tape.count = count;
     bool const device_offsets = is_device_pointer(offsets);
     // Offsets as u64 on the device.
@@ -134,9 +138,11 @@ static sz_status_t stage_tape(sz_cptr_t data, offset_type_ const *offsets, uint6
     }
     else if (sizeof(offset_type_) == 8 && device_offsets) {
         tape.offsets = reinterpret_cast<uint64_t const *>(offsets);
-        uint64_t last = 0;
-        SZS_TRY_HIP(hipMemcpyAsync(&last, offsets + count, 8, hipMemcpyDeviceToHost, stream), "offset readback");
-        SZS_TRY_HIP(hipStreamSynchronize(stream), "offset readback");
+        uint64_t last = bytes_hint;
+        if (!last) {
+            SZS_TRY_HIP(hipMemcpyAsync(&last, offsets + count, 8, hipMemcpyDeviceToHost, stream), "offset readback");
+            SZS_TRY_HIP(hipStreamSynchronize(stream), "offset readback");
+        }
         tape.bytes = last;
     }
     else if (device_offsets) { // u32 offsets on the device
@@ -145,17 +151,20 @@ static sz_status_t stage_tape(sz_cptr_t data, offset_type_ const *offsets, uint6
                                                  tape.offsets_copy.as<uint64_t>(), count + 1, stream),
                     "offset widening");
         tape.offsets = tape.offsets_copy.as<uint64_t>();
-        uint32_t last = 0;
-        SZS_TRY_HIP(hipMemcpyAsync(&last, offsets + count, 4, hipMemcpyDeviceToHost, stream), "offset readback");
-        SZS_TRY_HIP(hipStreamSynchronize(stream), "offset readback");
+        uint32_t last = (uint32_t)bytes_hint;
+        if (!last) {
+            SZS_TRY_HIP(hipMemcpyAsync(&last, offsets + count, 4, hipMemcpyDeviceToHost, stream), "offset readback");
+            SZS_TRY_HIP(hipStreamSynchronize(stream), "offset readback");
+        }
         tape.bytes = last;
     }
     else {
         std::vector<uint64_t> wide(count + 1);
         for (uint64_t i = 0; i <= count; ++i) wide[i] = (uint64_t)offsets[i];
         SZS_TRY_HIP(tape.offsets_copy.reserve((count + 1) * 8), "Failed to allocate tape offsets");
-        SZS_TRY_HIP(hipMemcpy(tape.offsets_copy.data, wide.data(), (count + 1) * 8, hipMemcpyHostToDevice),
-                    "tape offsets upload");
+        SZS_TRY_HIP(hipMemcpyAsync(tape.offsets_copy.data, wide.data(), (count + 1) * 8, hipMemcpyHostToDevice,
+                                   stream), "tape offsets upload");
+        SZS_TRY_HIP(hipStreamSynchronize(stream), "tape offsets upload"); // `wide` is about to go out of scope
         tape.offsets = tape.offsets_copy.as<uint64_t>();
         tape.bytes = wide[count];
     }
@@ -163,7 +172,8 @@ static sz_status_t stage_tape(sz_cptr_t data, offset_type_ const *offsets, uint6
     if (tape.bytes == 0 || is_device_pointer(data)) tape.data = reinterpret_cast<uint8_t const *>(data);
     else {
         SZS_TRY_HIP(tape.data_copy.reserve(tape.bytes + 16), "Failed to allocate the tape in HBM");
-        SZS_TRY_HIP(hipMemcpy(tape.data_copy.data, data, tape.bytes, hipMemcpyHostToDevice), "tape upload");
+        SZS_TRY_HIP(hipMemcpyAsync(tape.data_copy.data, data, tape.bytes, hipMemcpyHostToDevice, stream),
+                    "tape upload");
         tape.data = tape.data_copy.as<uint8_t>();
     }
     if (!tape.data) { // empty strings only: any valid device address works, nothing is read
@@ -183,16 +193,16 @@ static sz_status_t stage_input(segment_input_t const &input, bool queries, bool 
                                hipStream_t stream, char const **error_message) {
     if (queries) {
         if (input.queries)
-            return stage_tape(input.queries->data, input.queries->offsets, input.queries->count, runes, tape, stream,
-                              error_message);
-        return stage_tape(input.queries32->data, input.queries32->offsets, input.queries32->count, runes, tape,
-                          stream, error_message);
+            return stage_tape(input.queries->data, input.queries->offsets, input.queries->count, runes,
+                              input.queries_bytes, tape, stream, error_message);
+        return stage_tape(input.queries32->data, input.queries32->offsets, input.queries32->count, runes,
+                          input.queries_bytes, tape, stream, error_message);
     }
     if (input.candidates)
-        return stage_tape(input.candidates->data, input.candidates->offsets, input.candidates->count, runes, tape,
-                          stream, error_message);
-    return stage_tape(input.candidates32->data, input.candidates32->offsets, input.candidates32->count, runes, tape,
-                      stream, error_message);
+        return stage_tape(input.candidates->data, input.candidates->offsets, input.candidates->count, runes,
+                          input.candidates_bytes, tape, stream, error_message);
+    return stage_tape(input.candidates32->data, input.candidates32->offsets, input.candidates32->count, runes,
+                      input.candidates_bytes, tape, stream, error_message);
 }
 
 #pragma endregion
@@ -230,12 +240,15 @@ static sz_status_t enqueue_front(batch_t &b, bool bin, char const **error_messag
             SZS_TRY_HIP(szs_amd_launch_utf8_decode(tape->data, tape->offsets, tape->count, tape->runes,
                                                    tape->rune_counts, stream),
                         "UTF-8 decoding");
-    auto const *args = reinterpret_cast<materialize_args_t const *>(b.materialize_args.data());
-    for (size_t s = 0; s < b.inputs.size(); ++s)
-        SZS_TRY_HIP(szs_amd_launch_materialize(&args[s], b.shape_table.as<shape_table_t>(), b.tasks.as<task_t>(),
-                                               b.bin_of_task.as<uint32_t>(), b.histogram.as<uint32_t>(),
-                                               b.segments.as<segment_view_t>(), stream),
-                    "materialization");
+    uint32_t const segments_count = (uint32_t)b.inputs.size();
+    SZS_TRY_HIP(szs_amd_launch_fill(b.stats.as<uint32_t>(), sizeof(prepare_stats_t) / 4, 0, stream), "fill");
+    SZS_TRY_HIP(szs_amd_launch_fill(b.segment_overflow.as<uint32_t>(), segments_count, 0, stream), "fill");
+    SZS_TRY_HIP(szs_amd_launch_materialize(b.materialize_device.as<materialize_args_t>(), segments_count,
+                                           b.tasks_count, b.shape_table.as<shape_table_t>(), b.tasks.as<task_t>(),
+                                           b.bin_of_task.as<uint32_t>(), b.histogram.as<uint32_t>(),
+                                           b.segments.as<segment_view_t>(), b.stats.as<prepare_stats_t>(),
+                                           b.segment_overflow.as<uint32_t>(), stream),
+                "materialization");
     if (bin) {
         bins_view_t const bins = bins_view(b);
         SZS_TRY_HIP(szs_amd_launch_scan_bins(&bins, stream), "bin scan");
@@ -271,6 +284,7 @@ static sz_status_t enqueue_scoring(batch_t &b, bool timed, char const **error_me
 static sz_status_t enqueue_results(batch_t &b, char const **error_message) {
     for (size_t s = 0; s < b.inputs.size(); ++s) {
         if (!b.host_results[s] || !b.results_elements[s]) continue;
+        if (b.pinned_offsets[s] != SIZE_MAX) continue; // copied together below
         if (b.inputs[s].mode == segment_pairs_k)
             SZS_TRY_HIP(hipMemcpyAsync(b.host_results[s], b.device_results[s], b.results_elements[s] * 8,
                                        hipMemcpyDeviceToHost, b.stream),
@@ -281,16 +295,20 @@ static sz_status_t enqueue_results(batch_t &b, char const **error_message) {
                                          b.result_rows[s], hipMemcpyDeviceToHost, b.stream),
                         "results download");
     }
+    if (b.pinned_results && b.results_block.capacity) // every staged segment sits in one device block
+        SZS_TRY_HIP(hipMemcpyAsync(b.pinned_results, b.results_block.data, b.pinned_bytes, hipMemcpyDeviceToHost,
+                                   b.stream),
+                    "results download");
     return ok(error_message);
 }
 
 sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> const &inputs, szs_batch_mode_t mode,
-                          batch_t **out, char const **error_message, hipStream_t stream_override) {
+                          batch_t **out, char const **error_message, batch_options_t const &options) {
     int device = 0;
     hipStream_t stream = nullptr;
     sz_status_t status = resolve_gpu(scope, &device, &stream, error_message);
     if (status != sz_success_k) return status;
-    if (stream_override) stream = stream_override;
+    if (options.stream) stream = options.stream;
     batch_t *batch = new (std::nothrow) batch_t();
     if (!batch) return fail(sz_bad_alloc_k, error_message, "Failed to allocate a batch");
     struct guard_t {
@@ -308,8 +326,28 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
     b.result_rows.assign(segments_count, 0);
     b.result_columns.assign(segments_count, 0);
     b.device_results.assign(segments_count, nullptr);
+    b.pinned_offsets.assign(segments_count, SIZE_MAX);
     b.materialize_args.resize(sizeof(materialize_args_t) * std::max<size_t>(segments_count, 1));
     auto *args = reinterpret_cast<materialize_args_t *>(b.materialize_args.data());
+
+    // Pair-list results headed for host memory are staged together: one device block, one copy into the
+    // caller's page-locked staging, then a host copy per segment when the batch is joined.
+    size_t staged_bytes = 0;
+    if (options.pinned_results)
+        for (size_t s = 0; s < segments_count; ++s) {
+            segment_input_t const &input = inputs[s];
+            uint64_t const count = input.queries ? input.queries->count : input.queries32 ? input.queries32->count : 0;
+            if (input.mode != segment_pairs_k || !count || !input.results || is_device_pointer(input.results))
+                continue;
+            if (staged_bytes + count * 8 > options.pinned_capacity) break;
+            b.pinned_offsets[s] = staged_bytes;
+            staged_bytes += count * 8;
+        }
+    if (staged_bytes) {
+        SZS_TRY_HIP(b.results_block.reserve(staged_bytes), "Failed to allocate results in HBM");
+        b.pinned_results = options.pinned_results;
+        b.pinned_bytes = staged_bytes;
+    }
 
     std::vector<engine_params_t> params(segments_count);
     std::vector<segment_view_t> views(segments_count);
@@ -348,7 +386,11 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
         }
         b.results_elements[s] = elements;
         if (!input.results && elements) return fail(sz_unexpected_dimensions_k, error_message, "Results are null");
-        if (elements && !is_device_pointer(input.results)) {
+        if (b.pinned_offsets[s] != SIZE_MAX) {
+            b.host_results[s] = input.results;
+            b.device_results[s] = reinterpret_cast<int64_t *>(b.results_block.as<char>() + b.pinned_offsets[s]);
+        }
+        else if (elements && !is_device_pointer(input.results)) {
             auto *buffer = new (std::nothrow) device_buffer_t();
             if (!buffer) return fail(sz_bad_alloc_k, error_message, "Failed to allocate results");
             b.result_buffers.push_back(buffer);
@@ -356,8 +398,8 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
             b.host_results[s] = input.results;
             b.device_results[s] = buffer->as<int64_t>();
             if (input.mode != segment_pairs_k) // cells outside the matrix stay as the caller left them
-                SZS_TRY_HIP(hipMemcpy2D(buffer->data, input.results_row_stride * 8, input.results,
-                                        input.results_row_stride * 8, c * 8, q, hipMemcpyHostToDevice),
+                SZS_TRY_HIP(hipMemcpy2DAsync(buffer->data, input.results_row_stride * 8, input.results,
+                                             input.results_row_stride * 8, c * 8, q, hipMemcpyHostToDevice, stream),
                             "results upload");
         }
         else b.device_results[s] = reinterpret_cast<int64_t *>(input.results);
@@ -389,6 +431,7 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
         a.extend = engine->params.extend;
         a.unit_costs = engine->params.match == 0 && engine->params.mismatch == 1 && engine->params.open == 1 &&
                        engine->params.extend == 1;
+        a.reach_magnitude = std::max(1, std::max(engine->substitution_magnitude, engine->gap_magnitude));
         tasks_total += tasks;
         if (tasks_total >= 0xFFFFFFF0ull)
             return fail(sz_overflow_risk_k, error_message, "A batch holds at most 2^32 - 16 scored cells");
@@ -400,24 +443,31 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
     lds_bytes = std::max<uint32_t>(lds_bytes, 512u);
     b.lds_bytes = (lds_bytes + 15u) & ~15u;
 
-    // Device-side tables.
-    SZS_TRY_HIP(b.params.reserve(sizeof(engine_params_t) * std::max<size_t>(segments_count, 1)), "allocation");
-    SZS_TRY_HIP(hipMemcpy(b.params.data, params.data(), sizeof(engine_params_t) * segments_count,
-                          hipMemcpyHostToDevice), "upload");
+    // Device-side tables, uploaded on the batch stream (the host vectors outlive the synchronization below).
+    size_t const table_count = std::max<size_t>(segments_count, 1);
+    SZS_TRY_HIP(b.params.reserve(sizeof(engine_params_t) * table_count), "allocation");
+    SZS_TRY_HIP(hipMemcpyAsync(b.params.data, params.data(), sizeof(engine_params_t) * segments_count,
+                               hipMemcpyHostToDevice, stream), "upload");
     for (size_t s = 0; s < segments_count; ++s) views[s].params = b.params.as<engine_params_t>() + s;
-    SZS_TRY_HIP(b.segments.reserve(sizeof(segment_view_t) * std::max<size_t>(segments_count, 1)), "allocation");
-    SZS_TRY_HIP(hipMemcpy(b.segments.data, views.data(), sizeof(segment_view_t) * segments_count,
-                          hipMemcpyHostToDevice), "upload");
+    SZS_TRY_HIP(b.segments.reserve(sizeof(segment_view_t) * table_count), "allocation");
+    SZS_TRY_HIP(hipMemcpyAsync(b.segments.data, views.data(), sizeof(segment_view_t) * segments_count,
+                               hipMemcpyHostToDevice, stream), "upload");
+    SZS_TRY_HIP(b.materialize_device.reserve(sizeof(materialize_args_t) * table_count), "allocation");
+    SZS_TRY_HIP(hipMemcpyAsync(b.materialize_device.data, args, sizeof(materialize_args_t) * segments_count,
+                               hipMemcpyHostToDevice, stream), "upload");
+    shape_table_t const shape_table = active_shape_table();
     SZS_TRY_HIP(b.shape_table.reserve(sizeof(shape_table_t)), "allocation");
-    SZS_TRY_HIP(hipMemcpy(b.shape_table.data, &active_shape_table(), sizeof(shape_table_t), hipMemcpyHostToDevice),
+    SZS_TRY_HIP(hipMemcpyAsync(b.shape_table.data, &shape_table, sizeof(shape_table_t), hipMemcpyHostToDevice, stream),
                 "upload");
     std::vector<uint32_t> const &priority = bin_priority();
     std::vector<uint32_t> position(bins_count_k);
     for (uint32_t p = 0; p < bins_count_k; ++p) position[priority[p]] = p;
     SZS_TRY_HIP(b.priority.reserve(bins_count_k * 4), "allocation");
-    SZS_TRY_HIP(hipMemcpy(b.priority.data, priority.data(), bins_count_k * 4, hipMemcpyHostToDevice), "upload");
+    SZS_TRY_HIP(hipMemcpyAsync(b.priority.data, priority.data(), bins_count_k * 4, hipMemcpyHostToDevice, stream),
+                "upload");
     SZS_TRY_HIP(b.position_of_bin.reserve(bins_count_k * 4), "allocation");
-    SZS_TRY_HIP(hipMemcpy(b.position_of_bin.data, position.data(), bins_count_k * 4, hipMemcpyHostToDevice), "upload");
+    SZS_TRY_HIP(hipMemcpyAsync(b.position_of_bin.data, position.data(), bins_count_k * 4, hipMemcpyHostToDevice,
+                               stream), "upload");
     uint64_t const slots = std::max<uint64_t>(tasks_total, 1);
     SZS_TRY_HIP(b.tasks.reserve(slots * sizeof(task_t)), "Failed to allocate tasks");
     SZS_TRY_HIP(b.bin_of_task.reserve(slots * 4), "Failed to allocate tasks");
@@ -428,48 +478,47 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
     SZS_TRY_HIP(b.task_start.reserve(bins_count_k * 4), "allocation");
     SZS_TRY_HIP(b.item_start.reserve((bins_count_k + 1) * 4), "allocation");
     SZS_TRY_HIP(b.queue_head.reserve(64), "allocation");
+    SZS_TRY_HIP(b.stats.reserve(sizeof(prepare_stats_t)), "allocation");
+    SZS_TRY_HIP(b.segment_overflow.reserve(table_count * 4), "allocation");
     SZS_TRY_HIP(hipEventCreate(&b.kernel_begin), "event");
     SZS_TRY_HIP(hipEventCreate(&b.kernel_end), "event");
 
-    // Materialize once to size the scratch rows, check the reach bound and count the cells.
+    // Materialize once: the stats size the scratch rows, carry the reach check and count the cells.
     status = enqueue_front(b, false, error_message);
     if (status != sz_success_k) return status;
-    std::vector<task_t> tasks(tasks_total);
-    std::vector<uint32_t> bins(tasks_total);
-    if (tasks_total) {
+    prepare_stats_t stats;
+    std::memset(&stats, 0, sizeof(stats));
+    b.overflowed.assign(segments_count, 0);
+    SZS_TRY_HIP(hipMemcpyAsync(&stats, b.stats.data, sizeof(stats), hipMemcpyDeviceToHost, stream), "readback");
+    std::vector<task_t> tasks;
+    std::vector<uint32_t> bins;
+    if (mode == szs_batch_per_pair_k && tasks_total) { // the per-pair baseline launches task by task
+        tasks.resize(tasks_total);
+        bins.resize(tasks_total);
         SZS_TRY_HIP(hipMemcpyAsync(tasks.data(), b.tasks.data, tasks_total * sizeof(task_t), hipMemcpyDeviceToHost,
                                    stream), "task readback");
         SZS_TRY_HIP(hipMemcpyAsync(bins.data(), b.bin_of_task.data, tasks_total * 4, hipMemcpyDeviceToHost, stream),
                     "task readback");
     }
     SZS_TRY_HIP(hipStreamSynchronize(stream), "materialization");
-    uint64_t widest_multi_pass = 0;
-    for (uint64_t t = 0; t < tasks_total; ++t) {
-        if (bins[t] == ~0u) continue;
-        task_t const &task = tasks[t];
-        engine_t const *engine = inputs[task.segment].engine;
-        int const variant = (int)(bins[t] / column_buckets_k);
-        shape_t const shape = shapes_k[variant % shapes_count_k];
-        uint64_t const n = task.rows_length, m = task.columns_length;
-        b.cells += n * m;
-        if (n > (uint64_t)shape.group * shape.rows_per_lane) widest_multi_pass = std::max(widest_multi_pass, m);
-        (void)shape;
-        // Reach bound of serial.hpp:117-140, applied to int32 cells.
-        int const family = engine->params.family;
-        bool const affine = family == family_lev_affine_k || family == family_nw_affine_k ||
-                            family == family_sw_affine_k;
-        uint64_t const magnitude = (uint64_t)std::max(1, std::max(engine->substitution_magnitude, engine->gap_magnitude));
-        uint64_t const span = family <= family_lev_affine_k ? std::max(n, m) : n + m;
-        if ((span + (affine ? 3 : 1)) * magnitude >= 0x7FFFFFFFull)
+    if (stats.overflow) {
+        if (!options.defer_overflow)
             return fail(sz_overflow_risk_k, error_message, "Scores of this pair would overflow 32-bit cells");
-        if (mode == szs_batch_per_pair_k) {
-            b.per_pair_tasks.push_back((uint32_t)t);
-            b.per_pair_variants.push_back(variant);
+        if (segments_count) {
+            SZS_TRY_HIP(hipMemcpyAsync(b.overflowed.data(), b.segment_overflow.data, segments_count * 4,
+                                       hipMemcpyDeviceToHost, stream), "readback");
+            SZS_TRY_HIP(hipStreamSynchronize(stream), "readback");
         }
     }
+    b.cells = stats.cells;
+    for (uint64_t t = 0; t < tasks.size(); ++t) {
+        if (bins[t] == ~0u) continue;
+        b.per_pair_tasks.push_back((uint32_t)t);
+        b.per_pair_variants.push_back((int)(bins[t] / column_buckets_k));
+    }
+    uint64_t const widest_multi_pass = stats.widest_multi_pass;
     b.scratch_capacity = (uint32_t)(widest_multi_pass + 128);
-    int device_count = 0, compute_units = 0, blocks_per_cu = 0;
-    (void)device_count;
+    int compute_units = 0, blocks_per_cu = 0;
     SZS_TRY_HIP(hipDeviceGetAttribute(&compute_units, hipDeviceAttributeMultiprocessorCount, device), "attributes");
     SZS_TRY_HIP(szs_amd_persistent_occupancy(&blocks_per_cu, lds_bytes), "occupancy");
     blocks_per_cu = std::max(1, std::min(blocks_per_cu, 8));
@@ -523,6 +572,9 @@ sz_status_t batch_wait(batch_t *batch, char const **error_message) {
     batch_t &b = *batch;
     SZS_TRY_HIP(hipSetDevice(b.device), "hipSetDevice failed");
     SZS_TRY_HIP(hipStreamSynchronize(b.stream), "scoring failed on the device");
+    for (size_t s = 0; s < b.inputs.size(); ++s)
+        if (b.pinned_offsets[s] != SIZE_MAX)
+            std::memcpy(b.host_results[s], b.pinned_results + b.pinned_offsets[s], b.results_elements[s] * 8);
     float ms = 0;
     if (hipEventElapsedTime(&ms, b.kernel_begin, b.kernel_end) == hipSuccess) b.last_kernel_ms = ms;
     else (void)hipGetLastError();

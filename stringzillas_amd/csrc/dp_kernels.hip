@@ -284,13 +284,33 @@ __device__ __forceinline__ uint32_t half_octave(uint64_t x) {
 }
 
 /**
- *  Builds one task per scored cell and counts it into its bin. Degenerate cells (an empty side) are scored here
- *  with the closed forms of serial.hpp:142-151 and 2493-2496 and never reach the scoring kernel.
+ *  Builds one task per scored cell of every segment (one launch for the whole batch) and counts it into its bin.
+ *  Degenerate cells (an empty side) are scored here with the closed forms of serial.hpp:142-151 and 2493-2496 and
+ *  never reach the scoring kernel. A pair whose scores could leave int32 (the reach bound of serial.hpp:117-140)
+ *  is not scored: its segment is flagged instead. Cells and the widest multi-stripe pair are reduced per block
+ *  into `stats`, so the host reads back a few bytes rather than the task list.
  */
-__global__ void materialize_kernel(materialize_args_t a, shape_table_t const *table, task_t *tasks,
-                                   uint32_t *bin_of_task, uint32_t *histogram, segment_view_t const *segments) {
-    for (uint64_t index = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; index < a.tasks_count;
-         index += (uint64_t)gridDim.x * blockDim.x) {
+__global__ void materialize_kernel(materialize_args_t const *all, uint32_t segments_count, uint64_t tasks_total,
+                                   shape_table_t const *table, task_t *tasks, uint32_t *bin_of_task,
+                                   uint32_t *histogram, segment_view_t const *segments, prepare_stats_t *stats,
+                                   uint32_t *segment_overflow) {
+    __shared__ unsigned long long block_cells;
+    __shared__ uint32_t block_widest, block_overflow;
+    if (threadIdx.x == 0) block_cells = 0, block_widest = 0, block_overflow = 0;
+    __syncthreads();
+    unsigned long long cells = 0;
+    uint32_t widest = 0, overflow = 0;
+    for (uint64_t global = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; global < tasks_total;
+         global += (uint64_t)gridDim.x * blockDim.x) {
+        // The segment holding this task: the last one whose first task is at or before it.
+        uint32_t low = 0, high = segments_count - 1;
+        while (low < high) {
+            uint32_t const middle = (low + high + 1) / 2;
+            if (all[middle].task_base <= global) low = middle;
+            else high = middle - 1;
+        }
+        materialize_args_t const &a = all[low];
+        uint64_t const index = global - a.task_base;
         uint64_t q, c;
         if (a.mode == 0) q = c = index;
         else if (a.mode == 1) q = index / a.candidates_count, c = index % a.candidates_count;
@@ -322,16 +342,24 @@ __global__ void materialize_kernel(materialize_args_t a, shape_table_t const *ta
         task.rows_length = (uint32_t)(transposed ? c_length : q_length);
         task.columns_length = (uint32_t)(transposed ? q_length : c_length);
         uint32_t bin = ~0u;
+        bool const local = a.family == family_sw_linear_k || a.family == family_sw_affine_k;
+        bool const affine = a.family == family_lev_affine_k || a.family == family_nw_affine_k ||
+                            a.family == family_sw_affine_k;
+        uint64_t const longest = q_length > c_length ? q_length : c_length;
+        uint64_t const span = a.family <= family_lev_affine_k ? longest : q_length + c_length;
+        bool const overflows = (span + (affine ? 3 : 1)) * (uint64_t)a.reach_magnitude >= 0x7FFFFFFFull;
         if (q_length == 0 || c_length == 0) {
-            bool const local = a.family == family_sw_linear_k || a.family == family_sw_affine_k;
-            bool const affine = a.family == family_lev_affine_k || a.family == family_nw_affine_k ||
-                                a.family == family_sw_affine_k;
-            int64_t const longest = (int64_t)(q_length > c_length ? q_length : c_length);
             int64_t value = 0;
-            if (!local) value = affine ? (longest ? a.open + (int64_t)a.extend * (longest - 1) : 0) : a.open * longest;
+            if (!local)
+                value = affine ? (longest ? a.open + (int64_t)a.extend * ((int64_t)longest - 1) : 0)
+                               : a.open * (int64_t)longest;
             int64_t *results = segments[a.segment].results;
             results[task.output] = value;
             if (task.mirror != no_mirror_k) results[task.mirror] = value;
+        }
+        else if (overflows) {
+            overflow = 1;
+            segment_overflow[a.segment] = 1;
         }
         else {
             uint32_t const rows = task.rows_length;
@@ -355,9 +383,20 @@ __global__ void materialize_kernel(materialize_args_t a, shape_table_t const *ta
             uint64_t const work = passes * ((uint64_t)task.columns_length + sh.group) * per_step;
             bin = (uint32_t)variant_of(family, symbols, shape) * column_buckets_k + half_octave(work);
             atomicAdd(&histogram[bin], 1u);
+            cells += (unsigned long long)rows * task.columns_length;
+            if (rows > stripe && task.columns_length > widest) widest = task.columns_length;
         }
-        tasks[a.task_base + index] = task;
-        bin_of_task[a.task_base + index] = bin;
+        tasks[global] = task;
+        bin_of_task[global] = bin;
+    }
+    if (cells) atomicAdd(&block_cells, cells);
+    if (widest) atomicMax(&block_widest, widest);
+    if (overflow) atomicOr(&block_overflow, overflow);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (block_cells) atomicAdd(&stats->cells, block_cells);
+        if (block_widest) atomicMax(&stats->widest_multi_pass, block_widest);
+        if (block_overflow) atomicOr(&stats->overflow, block_overflow);
     }
 }
 
@@ -446,14 +485,16 @@ hipError_t szs_amd_launch_widen_offsets(uint32_t const *narrow, uint64_t *wide, 
     return hipGetLastError();
 }
 
-hipError_t szs_amd_launch_materialize(szs_amd::materialize_args_t const *args, szs_amd::shape_table_t const *table,
-                                      szs_amd::task_t *tasks, uint32_t *bin_of_task, uint32_t *histogram,
-                                      szs_amd::segment_view_t const *segments, hipStream_t stream) {
-    if (!args->tasks_count) return hipSuccess;
-    uint64_t const blocks64 = (args->tasks_count + 255) / 256;
+hipError_t szs_amd_launch_materialize(szs_amd::materialize_args_t const *args, uint32_t segments_count,
+                                      uint64_t tasks_total, szs_amd::shape_table_t const *table, szs_amd::task_t *tasks,
+                                      uint32_t *bin_of_task, uint32_t *histogram,
+                                      szs_amd::segment_view_t const *segments, szs_amd::prepare_stats_t *stats,
+                                      uint32_t *segment_overflow, hipStream_t stream) {
+    if (!tasks_total || !segments_count) return hipSuccess;
+    uint64_t const blocks64 = (tasks_total + 255) / 256;
     uint32_t const blocks = blocks64 < 65536 ? (uint32_t)blocks64 : 65536;
-    hipLaunchKernelGGL(szs_amd::materialize_kernel, dim3(blocks), dim3(256), 0, stream, *args, table, tasks,
-                       bin_of_task, histogram, segments);
+    hipLaunchKernelGGL(szs_amd::materialize_kernel, dim3(blocks), dim3(256), 0, stream, args, segments_count,
+                       tasks_total, table, tasks, bin_of_task, histogram, segments, stats, segment_overflow);
     return hipGetLastError();
 }
 

@@ -148,6 +148,14 @@ struct materialize_args_t {
     int32_t mode;                        /**< 0 pairs, 1 cross-product, 2 symmetric */
     int32_t open, extend;
     int32_t unit_costs;                  /**< match 0, mismatch 1, gap 1: bit-parallel shapes apply to bytes */
+    int32_t reach_magnitude;             /**< max |cost| of the engine, for the int32 reach bound */
+};
+
+/** What materialization reports back to the host: one small readback instead of the whole task list. */
+struct prepare_stats_t {
+    unsigned long long cells;            /**< sum of n*m over non-empty tasks */
+    uint32_t widest_multi_pass;          /**< most columns of a pair that needs several stripes */
+    uint32_t overflow;                   /**< nonzero if some pair's scores could leave int32 */
 };
 
 } // namespace szs_amd

@@ -105,6 +105,16 @@ struct segment_input_t {
     sz_sequence_u32tape_t const *candidates32 = nullptr;
     void *results = nullptr;
     uint64_t results_row_stride = 0;
+    uint64_t queries_bytes = 0;    /**< `offsets[count]` of the queries when known (0: read it from the offsets) */
+    uint64_t candidates_bytes = 0; /**< the same for the candidates */
+};
+
+/** How a batch is prepared beyond its inputs and mode. */
+struct batch_options_t {
+    hipStream_t stream = nullptr;   /**< overrides the scope's stream when set */
+    bool defer_overflow = false;    /**< flag segments whose pairs could overflow instead of failing the batch */
+    char *pinned_results = nullptr; /**< page-locked host staging for pair-list results (null: copy directly) */
+    size_t pinned_capacity = 0;     /**< bytes at `pinned_results` */
 };
 
 /** A batch staged on one device: everything `run` needs, ready to be launched (or replayed from a graph). */
@@ -122,7 +132,12 @@ struct batch_t {
 
     device_buffer_t params, segments, shape_table, tasks, bin_of_task, order, shape_of_task, identity;
     device_buffer_t histogram, cursor, task_start, item_start, priority, position_of_bin, queue_head, scratch, stats;
+    device_buffer_t materialize_device, segment_overflow, results_block;
     std::vector<char> materialize_args;        /**< one `materialize_args_t` per segment (opaque here) */
+    std::vector<uint32_t> overflowed;          /**< per segment: nonzero if a pair could overflow (deferred mode) */
+    char *pinned_results = nullptr;            /**< staging of pair-list results, see `batch_options_t` */
+    std::vector<size_t> pinned_offsets;        /**< per segment: byte offset in the staging, or SIZE_MAX */
+    size_t pinned_bytes = 0;                   /**< bytes of results staged through `pinned_results` */
     uint64_t tasks_count = 0;
     uint64_t cells = 0;                        /**< sum of n*m over non-empty tasks */
     uint32_t scratch_capacity = 0;
@@ -138,9 +153,9 @@ struct batch_t {
     ~batch_t();
 };
 
-/** Stages and bins a batch on the scope's device; `stream` overrides the scope's stream when given. */
+/** Stages and bins a batch on the scope's device (or on `options.stream`). */
 sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> const &inputs, szs_batch_mode_t mode,
-                          batch_t **out, char const **error_message, hipStream_t stream = nullptr);
+                          batch_t **out, char const **error_message, batch_options_t const &options = {});
 sz_status_t batch_launch(batch_t *batch, char const **error_message);
 sz_status_t batch_wait(batch_t *batch, char const **error_message);
 

@@ -17,9 +17,9 @@ Run on the GPU: ``python -m stringzillas_amd.serving --qps 200 1000 5000 --secon
 from __future__ import annotations
 
 import argparse
-import itertools
 import json
 import math
+import queue
 import sys
 import threading
 import time
@@ -104,13 +104,14 @@ def _summary(latencies: List[float], cells: int, wall: float, offered: float, co
 
 def run_server(templates, engines, qps: float, duration: float, hbm_budget: int = 64 << 30,
                inflight: int = 2, seed: int = 1, drain_timeout: float = 120.0):
-    """Continuous batching: a submitter thread follows the arrival schedule; the main thread polls tickets."""
+    """Continuous batching: a submitter thread follows the arrival schedule; the main thread joins tickets in
+    arrival order, blocking in the library (the interpreter lock is released), and takes each request's latency
+    from the completion time the server stamped on it."""
     rng = np.random.default_rng(seed)
     arrivals = _arrivals(rng, qps, duration)
     picks = rng.integers(0, len(templates), size=len(arrivals))
     server = szs.Server(hbm_budget_bytes=hbm_budget, max_inflight_rounds=inflight)
-    pending = {}
-    lock = threading.Lock()
+    submitted = queue.Queue()
     start = time.perf_counter()
 
     def submit_all():
@@ -120,34 +121,39 @@ def run_server(templates, engines, qps: float, duration: float, hbm_budget: int 
                 time.sleep(delay)
             template = templates[pick]
             ticket = server.submit(engines[template.kind], template.firsts, template.seconds)
-            with lock:
-                pending[ticket] = (start + at, template)
+            submitted.put((ticket, start + at, template))
+        submitted.put(None)
 
     submitter = threading.Thread(target=submit_all)
     submitter.start()
-    latencies, cells, done = [], 0, 0
+    latencies, cells, done, late = [], 0, 0, 0
     deadline = start + duration + drain_timeout
     results = {}
-    while (submitter.is_alive() or pending) and time.perf_counter() < deadline:
-        with lock:  # rounds are admitted first-in first-out, so the oldest tickets finish first
-            tickets = list(itertools.islice(pending.items(), 512))
-        finished = [(t, v) for t, v in tickets if server.poll(t)]
-        now = time.perf_counter()
-        for ticket, (arrived, template) in finished:
-            out = server.wait(ticket)
-            latencies.append(now - arrived)
-            cells += template.cells
-            done += 1
-            results.setdefault(id(template), out)
-            with lock:
-                del pending[ticket]
-        if not finished:
-            time.sleep(0.0002)
-    wall = time.perf_counter() - start
+    last_completion = start
+    while True:
+        item = submitted.get()
+        if item is None:
+            break
+        ticket, arrived, template = item
+        out, completed = server.wait_timed(ticket)
+        last_completion = max(last_completion, completed)
+        if completed > deadline:
+            late += 1
+            continue
+        latencies.append(completed - arrived)
+        cells += template.cells
+        done += 1
+        results.setdefault(id(template), out)
     submitter.join()
+    stats = server.stats()
     server.close()
+    wall = min(last_completion, deadline) - start
     summary = _summary(latencies, cells, wall, qps, done)
-    summary["dropped_after_drain_timeout"] = len(pending)
+    summary["dropped_after_drain_timeout"] = late
+    summary["rounds"] = stats["rounds"]
+    summary["largest_round"] = stats["largest_round"]
+    summary["prepare_s"] = round(stats["prepare_seconds"], 3)
+    summary["device_s"] = round(stats["device_seconds"], 3)
     return summary, results
 
 

@@ -116,3 +116,31 @@ def test_server_rejects_requests_over_budget_and_bad_shapes():
         assert server.wait(ticket).tolist() == [3]
     finally:
         server.close()
+
+
+def test_server_stats_completion_times_and_overflow_isolation():
+    import time
+
+    rng = np.random.default_rng(5)
+    server = szs.Server(hbm_budget_bytes=512 << 20, max_inflight_rounds=2)
+    try:
+        heavy = szs.LevenshteinDistances(0, 127, 127, 127)  # reach bound: max(n, m) * 127 must stay below 2^31
+        long_side = szs.Strs([b"A" * (17 << 20)])
+        short_side = szs.Strs([b"C"])
+        normal = szs.Strs([random_bytes(rng, DNA, 10, 300) for _ in range(20)])
+        others = szs.Strs([random_bytes(rng, DNA, 10, 300) for _ in range(20)])
+        before = time.perf_counter()
+        overflowing = server.submit(heavy, long_side, short_side)
+        fine = server.submit(heavy, normal, others)
+        with pytest.raises(OverflowError):
+            server.wait(overflowing)
+        out, completed = server.wait_timed(fine)
+        assert before <= completed <= time.perf_counter()
+        expected = O.pairs(O.Params("levenshtein", 0, 127, 127, 127), (normal.data, normal.offsets),
+                           (others.data, others.offsets), threads=4)
+        np.testing.assert_array_equal(out, expected)
+        stats = server.stats()
+        assert stats["requests"] == 2 and stats["rounds"] >= 1 and stats["cells"] > 0
+        assert stats["pending"] == 0 and stats["inflight"] == 0
+    finally:
+        server.close()
