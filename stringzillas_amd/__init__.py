@@ -182,7 +182,7 @@ class DeviceScope:
         self.handle = handle
 
     def __del__(self):
-        if getattr(self, "handle", None):
+        if getattr(self, "handle", None) and _lib is not None:  # module globals are gone at interpreter exit
             _lib.szs_device_scope_free(self.handle)
             self.handle = None
 
@@ -225,7 +225,7 @@ class _Engine:
         self.description = description
 
     def __del__(self):
-        if getattr(self, "handle", None):
+        if getattr(self, "handle", None) and _lib is not None:
             getattr(_lib, f"{self._prefix}_free")(self.handle)
             self.handle = None
 
@@ -416,7 +416,7 @@ class Batch:
         return self.outputs
 
     def __del__(self):
-        if getattr(self, "handle", None):
+        if getattr(self, "handle", None) and _lib is not None:
             _lib.szs_batch_free(self.handle)
             self.handle = None
 
@@ -479,7 +479,7 @@ class Server:
         return int(_lib.szs_server_bytes_in_use(self.handle))
 
     def close(self) -> None:
-        if getattr(self, "handle", None):
+        if getattr(self, "handle", None) and _lib is not None:
             _lib.szs_server_free(self.handle)
             self.handle = None
 

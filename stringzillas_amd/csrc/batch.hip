@@ -21,8 +21,8 @@
 
 extern "C" {
 hipError_t szs_amd_launch_fill(uint32_t *data, uint32_t count, uint32_t value, hipStream_t stream);
-hipError_t szs_amd_launch_utf8_decode(uint8_t const *bytes, uint64_t const *offsets, uint64_t count, uint32_t *runes,
-                                      uint32_t *rune_counts, hipStream_t stream);
+hipError_t szs_amd_launch_utf8_decode(szs_amd::utf8_tape_t const *tapes, uint32_t tapes_count, uint64_t strings_total,
+                                      hipStream_t stream);
 hipError_t szs_amd_launch_widen_offsets(uint32_t const *narrow, uint64_t *wide, uint64_t count, hipStream_t stream);
 hipError_t szs_amd_launch_materialize(szs_amd::materialize_args_t const *args, uint32_t segments_count,
                                       uint64_t tasks_total, szs_amd::shape_table_t const *table, szs_amd::task_t *tasks,
@@ -235,11 +235,9 @@ static sz_status_t enqueue_front(batch_t &b, bool bin, char const **error_messag
     SZS_TRY_HIP(szs_amd_launch_fill(b.histogram.as<uint32_t>(), bins_count_k, 0, stream), "fill");
     SZS_TRY_HIP(szs_amd_launch_fill(b.cursor.as<uint32_t>(), bins_count_k, 0, stream), "fill");
     SZS_TRY_HIP(szs_amd_launch_fill(b.queue_head.as<uint32_t>(), 1, 0, stream), "fill");
-    for (auto *tape : b.tapes)
-        if (tape && tape->runes)
-            SZS_TRY_HIP(szs_amd_launch_utf8_decode(tape->data, tape->offsets, tape->count, tape->runes,
-                                                   tape->rune_counts, stream),
-                        "UTF-8 decoding");
+    SZS_TRY_HIP(szs_amd_launch_utf8_decode(b.decode_tapes.as<utf8_tape_t>(), b.decode_tapes_count, b.decode_strings,
+                                           stream),
+                "UTF-8 decoding");
     uint32_t const segments_count = (uint32_t)b.inputs.size();
     SZS_TRY_HIP(szs_amd_launch_fill(b.stats.as<uint32_t>(), sizeof(prepare_stats_t) / 4, 0, stream), "fill");
     SZS_TRY_HIP(szs_amd_launch_fill(b.segment_overflow.as<uint32_t>(), segments_count, 0, stream), "fill");
@@ -455,6 +453,25 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
     SZS_TRY_HIP(b.materialize_device.reserve(sizeof(materialize_args_t) * table_count), "allocation");
     SZS_TRY_HIP(hipMemcpyAsync(b.materialize_device.data, args, sizeof(materialize_args_t) * segments_count,
                                hipMemcpyHostToDevice, stream), "upload");
+    // Every UTF-8 tape is transcoded by one launch.
+    std::vector<utf8_tape_t> decode;
+    for (auto *tape : b.tapes)
+        if (tape && tape->runes && tape->count) {
+            utf8_tape_t entry;
+            entry.bytes = tape->data;
+            entry.offsets = tape->offsets;
+            entry.runes = tape->runes;
+            entry.rune_counts = tape->rune_counts;
+            entry.string_base = b.decode_strings;
+            decode.push_back(entry);
+            b.decode_strings += tape->count;
+        }
+    b.decode_tapes_count = (uint32_t)decode.size();
+    if (!decode.empty()) {
+        SZS_TRY_HIP(b.decode_tapes.reserve(sizeof(utf8_tape_t) * decode.size()), "allocation");
+        SZS_TRY_HIP(hipMemcpyAsync(b.decode_tapes.data, decode.data(), sizeof(utf8_tape_t) * decode.size(),
+                                   hipMemcpyHostToDevice, stream), "upload");
+    }
     shape_table_t const shape_table = active_shape_table();
     SZS_TRY_HIP(b.shape_table.reserve(sizeof(shape_table_t)), "allocation");
     SZS_TRY_HIP(hipMemcpyAsync(b.shape_table.data, &shape_table, sizeof(shape_table_t), hipMemcpyHostToDevice, stream),

@@ -240,32 +240,71 @@ __global__ void fill_u32_kernel(uint32_t *data, uint32_t count, uint32_t value) 
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) data[i] = value;
 }
 
+/** Byte `k` (0..15) of a 16-byte block held in registers. */
+__device__ __forceinline__ uint32_t block_byte(uint4 block, uint32_t k) {
+    uint32_t const word = (k & 8) ? ((k & 4) ? block.w : block.z) : ((k & 4) ? block.y : block.x);
+    return (word >> ((k & 3) * 8)) & 0xFFu;
+}
+
 /**
- *  UTF-8 to runes, one thread per string, with `sz_rune_decode_unchecked` semantics
- *  (include/stringzilla/utf8_runes/serial.h:101-114); continuation bytes missing at the end of a string read as 0.
- *  Runes of string `s` start at index `offsets[s]` of `runes`; their count goes to `rune_counts[s]`.
+ *  UTF-8 to runes for every tape of a batch in one launch, one thread per string, with
+ *  `sz_rune_decode_unchecked` semantics (include/stringzilla/utf8_runes/serial.h:101-114): the lead byte fixes
+ *  the width, continuation bytes are not validated, and continuation bytes missing at the end of a string read
+ *  as 0. Runes of string `s` start at index `offsets[s]` of the tape's `runes`; their count goes to
+ *  `rune_counts[s]`.
+ *
+ *  The walk reads the string through two 16-byte aligned blocks held in registers instead of one dependent byte
+ *  load per rune. A block is loaded only if it holds a byte of the string, so it never leaves the pages of the
+ *  tape.
  */
-__global__ void utf8_decode_kernel(uint8_t const *bytes, uint64_t const *offsets, uint64_t count, uint32_t *runes,
-                                   uint32_t *rune_counts) {
-    for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < count; s += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t const begin = offsets[s], end = offsets[s + 1];
-        uint64_t at = begin, written = 0;
-        while (at < end) {
-            uint32_t const lead = bytes[at];
-            uint32_t const width = 1 + (lead >= 0xC0) + (lead >= 0xE0) + (lead >= 0xF0);
-            uint32_t t0 = 0, t1 = 0, t2 = 0;
-            if (width > 1 && at + 1 < end) t0 = bytes[at + 1] & 0x3Fu;
-            if (width > 2 && at + 2 < end) t1 = bytes[at + 2] & 0x3Fu;
-            if (width > 3 && at + 3 < end) t2 = bytes[at + 3] & 0x3Fu;
-            uint32_t rune;
-            if (width == 1) rune = lead;
-            else if (width == 2) rune = (lead & 0x1Fu) << 6 | t0;
-            else if (width == 3) rune = (lead & 0x0Fu) << 12 | t0 << 6 | t1;
-            else rune = (lead & 0x07u) << 18 | t0 << 12 | t1 << 6 | t2;
-            runes[begin + written++] = rune;
-            at += width;
+__global__ void utf8_decode_kernel(utf8_tape_t const *tapes, uint32_t tapes_count, uint64_t strings_total) {
+    for (uint64_t global = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; global < strings_total;
+         global += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t low = 0, high = tapes_count - 1;
+        while (low < high) {
+            uint32_t const middle = (low + high + 1) / 2;
+            if (tapes[middle].string_base <= global) low = middle;
+            else high = middle - 1;
         }
-        rune_counts[s] = (uint32_t)written;
+        utf8_tape_t const &tape = tapes[low];
+        uint64_t const s = global - tape.string_base;
+        uint64_t const begin = tape.offsets[s], end = tape.offsets[s + 1];
+        uint8_t const *const first = tape.bytes + begin, *const last = tape.bytes + end;
+        uint32_t *const out = tape.runes + begin;
+        uint64_t written = 0;
+        if (first < last) {
+            uint8_t const *block_start = (uint8_t const *)((uintptr_t)first & ~(uintptr_t)15);
+            uint4 current = *(uint4 const *)block_start, next = make_uint4(0, 0, 0, 0);
+            if (block_start + 16 < last) next = *(uint4 const *)(block_start + 16);
+            uint8_t const *at = first;
+            while (at < last) {
+                while (at >= block_start + 16) { // slide the pair of blocks forward
+                    block_start += 16;
+                    current = next;
+                    next = make_uint4(0, 0, 0, 0);
+                    if (block_start + 16 < last) next = *(uint4 const *)(block_start + 16);
+                }
+                uint32_t const k = (uint32_t)(at - block_start); // 0..15; at + 3 stays within the two blocks
+                auto byte_at = [&](uint32_t d) {
+                    uint32_t const j = k + d;
+                    return j < 16 ? block_byte(current, j) : block_byte(next, j - 16);
+                };
+                uint32_t const lead = byte_at(0);
+                uint32_t const width = 1 + (lead >= 0xC0) + (lead >= 0xE0) + (lead >= 0xF0);
+                uint32_t t0 = 0, t1 = 0, t2 = 0;
+                if (width > 1 && at + 1 < last) t0 = byte_at(1) & 0x3Fu;
+                if (width > 2 && at + 2 < last) t1 = byte_at(2) & 0x3Fu;
+                if (width > 3 && at + 3 < last) t2 = byte_at(3) & 0x3Fu;
+                uint32_t rune;
+                if (width == 1) rune = lead;
+                else if (width == 2) rune = (lead & 0x1Fu) << 6 | t0;
+                else if (width == 3) rune = (lead & 0x0Fu) << 12 | t0 << 6 | t1;
+                else rune = (lead & 0x07u) << 18 | t0 << 12 | t1 << 6 | t2;
+                out[written++] = rune;
+                at += width;
+            }
+        }
+        tape.rune_counts[s] = (uint32_t)written;
     }
 }
 
@@ -467,13 +506,13 @@ hipError_t szs_amd_launch_fill(uint32_t *data, uint32_t count, uint32_t value, h
     return hipGetLastError();
 }
 
-hipError_t szs_amd_launch_utf8_decode(uint8_t const *bytes, uint64_t const *offsets, uint64_t count, uint32_t *runes,
-                                      uint32_t *rune_counts, hipStream_t stream) {
-    if (!count) return hipSuccess;
-    uint64_t const blocks64 = (count + 255) / 256;
+hipError_t szs_amd_launch_utf8_decode(szs_amd::utf8_tape_t const *tapes, uint32_t tapes_count, uint64_t strings_total,
+                                      hipStream_t stream) {
+    if (!strings_total || !tapes_count) return hipSuccess;
+    uint64_t const blocks64 = (strings_total + 255) / 256;
     uint32_t const blocks = blocks64 < 65536 ? (uint32_t)blocks64 : 65536;
-    hipLaunchKernelGGL(szs_amd::utf8_decode_kernel, dim3(blocks), dim3(256), 0, stream, bytes, offsets, count, runes,
-                       rune_counts);
+    hipLaunchKernelGGL(szs_amd::utf8_decode_kernel, dim3(blocks), dim3(256), 0, stream, tapes, tapes_count,
+                       strings_total);
     return hipGetLastError();
 }
 

@@ -25,8 +25,6 @@ using namespace szs_amd;
 
 extern "C" {
 hipError_t szs_amd_launch_fill(uint32_t *data, uint32_t count, uint32_t value, hipStream_t stream);
-hipError_t szs_amd_launch_utf8_decode(uint8_t const *bytes, uint64_t const *offsets, uint64_t count, uint32_t *runes,
-                                      uint32_t *rune_counts, hipStream_t stream);
 hipError_t szs_amd_launch_widen_offsets(uint32_t const *narrow, uint64_t *wide, uint64_t count, hipStream_t stream);
 }
 

@@ -151,6 +151,15 @@ struct materialize_args_t {
     int32_t reach_magnitude;             /**< max |cost| of the engine, for the int32 reach bound */
 };
 
+/** One UTF-8 tape to transcode: its strings are numbered from `string_base` across the batch. */
+struct utf8_tape_t {
+    uint8_t const *bytes;
+    uint64_t const *offsets;
+    uint32_t *runes;       /**< u32[offsets[count]]: runes of string s start at index offsets[s] */
+    uint32_t *rune_counts; /**< u32[count] */
+    uint64_t string_base;
+};
+
 /** What materialization reports back to the host: one small readback instead of the whole task list. */
 struct prepare_stats_t {
     unsigned long long cells;            /**< sum of n*m over non-empty tasks */

@@ -132,7 +132,9 @@ struct batch_t {
 
     device_buffer_t params, segments, shape_table, tasks, bin_of_task, order, shape_of_task, identity;
     device_buffer_t histogram, cursor, task_start, item_start, priority, position_of_bin, queue_head, scratch, stats;
-    device_buffer_t materialize_device, segment_overflow, results_block;
+    device_buffer_t materialize_device, segment_overflow, results_block, decode_tapes;
+    uint32_t decode_tapes_count = 0;           /**< UTF-8 tapes transcoded by each run */
+    uint64_t decode_strings = 0;               /**< strings across those tapes */
     std::vector<char> materialize_args;        /**< one `materialize_args_t` per segment (opaque here) */
     std::vector<uint32_t> overflowed;          /**< per segment: nonzero if a pair could overflow (deferred mode) */
     char *pinned_results = nullptr;            /**< staging of pair-list results, see `batch_options_t` */
