@@ -41,6 +41,9 @@ def parse():
     parser.add_argument("--skip-cpu", action="store_true")
     parser.add_argument("--skip-per-call", action="store_true")
     parser.add_argument("--breakdown", action="store_true", help="also time each kind alone")
+    parser.add_argument("--serve-qps", type=float, nargs="*", default=[1000.0, 2000.0],
+                        help="offered rates of the short serving run (none to skip)")
+    parser.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per serving rate")
     return parser.parse_args()
 
 
@@ -226,6 +229,10 @@ def main():
         }
         log(f"per-call: {per_call['gcups']} GCUPS; batched on the same sample: {per_call['batched_same_sample_gcups']}")
 
+    serving = None
+    if args.serve_qps:
+        serving = serving_run(args.serve_qps, args.serve_seconds)
+
     cpu = None
     if not args.skip_cpu:
         cpu = cpu_baseline(host_segments, args.cpu_seconds)
@@ -268,10 +275,37 @@ def main():
         "per_call": per_call,
         "batched_over_per_call": round(gcups / per_call["gcups"], 1) if per_call else None,
         "parity": {"checked_pairs": checked, "mismatches": mismatches},
+        "serving": serving,
     }
     if breakdown:
         line["per_kind"] = breakdown
     print(json.dumps(line), flush=True)
+
+
+def serving_run(rates, seconds: float):
+    """Poisson arrivals into the continuous-batching server, and one plain launch per request at the first rate.
+
+    Requests follow the mixed batch's kind and length mix with 1..4096 pairs each, from HBM-resident templates;
+    latency runs from a request's arrival to the completion of its round.
+    """
+    from stringzillas_amd import serving as S
+
+    templates = S.make_templates(96)
+    engines = S.engines_by_kind()
+    S.run_server(templates[:4], engines, qps=50.0, duration=0.2, hbm_budget=16 << 30)  # warm-up
+    report = {"requests": "log-uniform 1..4096 pairs of the mixed kinds, lengths log-uniform 16..16384",
+              "mean_cells_per_request": round(float(np.mean([t.cells for t in templates]))),
+              "seconds_per_rate": seconds, "server": [], "per_request": None}
+    for qps in rates:
+        summary, _ = S.run_server(templates, engines, qps, seconds, hbm_budget=16 << 30, drain_timeout=10.0)
+        log(f"serving at {qps:g} req/s: {summary}")
+        report["server"].append({k: summary[k] for k in ("offered_qps", "achieved_qps", "gcups", "p50_ms", "p99_ms",
+                                                         "completed", "rounds")})
+    baseline = S.run_baseline(templates, engines, rates[0], seconds, drain_timeout=4.0)
+    log(f"one launch per request at {rates[0]:g} req/s: {baseline}")
+    report["per_request"] = {k: baseline[k] for k in ("offered_qps", "achieved_qps", "gcups", "p50_ms", "p99_ms",
+                                                      "completed", "dropped_after_drain_timeout")}
+    return report
 
 
 def cpu_baseline(host_segments, target_seconds: float):

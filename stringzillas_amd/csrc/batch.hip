@@ -423,7 +423,10 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
         a.task_base = (uint32_t)tasks_total;
         a.segment = (uint32_t)s;
         a.family = engine->params.family;
+        a.runes = runes;
         a.symbols = views[s].symbols;
+        if (engine->params.family >= family_nw_linear_k && engine->params.shift != engine->params.open)
+            a.symbols = class_unshifted_k;
         a.mode = input.mode;
         a.open = engine->params.open;
         a.extend = engine->params.extend;
@@ -437,9 +440,9 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
             lds_bytes = std::max<uint32_t>(lds_bytes, (uint32_t)engine->params.dense_count * 64u * 8u);
     }
     b.tasks_count = tasks_total;
-    // Dynamic LDS per wave: the NW/SW profile or the Myers symbol tables, whichever is larger.
+    // Dynamic LDS per wave: the NW/SW profile or the Myers symbol tables, whichever is larger, then the ring.
     lds_bytes = std::max<uint32_t>(lds_bytes, 512u);
-    b.lds_bytes = (lds_bytes + 15u) & ~15u;
+    b.lds_bytes = ((lds_bytes + 15u) & ~15u) + inject_ring_bytes_k;
 
     // Device-side tables, uploaded on the batch stream (the host vectors outlive the synchronization below).
     size_t const table_count = std::max<size_t>(segments_count, 1);
@@ -537,7 +540,7 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
     b.scratch_capacity = (uint32_t)(widest_multi_pass + 128);
     int compute_units = 0, blocks_per_cu = 0;
     SZS_TRY_HIP(hipDeviceGetAttribute(&compute_units, hipDeviceAttributeMultiprocessorCount, device), "attributes");
-    SZS_TRY_HIP(szs_amd_persistent_occupancy(&blocks_per_cu, lds_bytes), "occupancy");
+    SZS_TRY_HIP(szs_amd_persistent_occupancy(&blocks_per_cu, b.lds_bytes), "occupancy");
     blocks_per_cu = std::max(1, std::min(blocks_per_cu, 8));
     b.waves = mode == szs_batch_per_pair_k ? 1u : (uint32_t)(compute_units * blocks_per_cu); // workgroups
     uint64_t const scratch_groups = widest_multi_pass ? b.waves : 1;

@@ -42,12 +42,22 @@ __device__ void run_independent(item_args_t const &a) {
     int32_t const chunks = (longest + 2 * G - 2) / G; // steps = longest + G - 1
     int32_t *scores_row = a.scratch + (size_t)wave * 2 * a.scratch_capacity;
     int32_t *gaps_row = scores_row + a.scratch_capacity;
+    // With one pair per wave, chunks whose 64 columns are valid for every lane skip the per-step column check.
+    int32_t const columns = G == 64 ? __builtin_amdgcn_readfirstlane(sweep.m) : 0;
     for (int32_t pass = 0; pass < passes; ++pass) {
         int32_t const base = pass * stripe_rows;
         bool const more = pass + 1 < passes;
+        int32_t const *in_row = pass ? scores_row : nullptr;
         sweep.begin_stripe(base);
-        for (int32_t chunk = 0; chunk < chunks; ++chunk)
-            sweep.run_chunk(chunk, pass ? scores_row : nullptr, gaps_row, more ? scores_row : nullptr, gaps_row);
+        for (int32_t chunk = 0; chunk < chunks; ++chunk) {
+            bool const steady = G == 64 && chunk >= 1 && chunk * 64 + 64 <= columns;
+            if (more) {
+                if (steady) sweep.template run_chunk<true, true>(chunk, in_row, gaps_row, scores_row, gaps_row);
+                else sweep.template run_chunk<false, true>(chunk, in_row, gaps_row, scores_row, gaps_row);
+            }
+            else if (steady) sweep.template run_chunk<true, false>(chunk, in_row, gaps_row, nullptr, nullptr);
+            else sweep.template run_chunk<false, false>(chunk, in_row, gaps_row, nullptr, nullptr);
+        }
         if (more) __threadfence_block(); // lane 63's stores before lane 0's loads in the next pass
         sweep.end_stripe(base, !more);
     }
@@ -65,6 +75,7 @@ __device__ void run_cooperative(item_args_t const &a) {
     int32_t const stripe_rows = 64 * sweep_type_::rows_per_lane;
     int32_t const stripes = (sweep.n + stripe_rows - 1) / stripe_rows;
     int32_t const chunks = (sweep.m + 126) / 64;
+    int32_t const columns = __builtin_amdgcn_readfirstlane(sweep.m);
     int32_t const delay = max(2, (chunks + W - 1) / W);
     int32_t const phases = (stripes - 1) * delay + chunks;
     size_t const row_pitch = 2 * (size_t)a.scratch_capacity;
@@ -84,8 +95,15 @@ __device__ void run_cooperative(item_args_t const &a) {
             if (chunk == 0) sweep.begin_stripe(base);
             int32_t *in_row = stripe ? a.scratch + ((stripe - 1) % W) * row_pitch : nullptr;
             int32_t *out_row = stripe + 1 < stripes ? a.scratch + (stripe % W) * row_pitch : nullptr;
-            sweep.run_chunk(chunk, in_row, in_row ? in_row + a.scratch_capacity : nullptr, out_row,
-                            out_row ? out_row + a.scratch_capacity : nullptr);
+            int32_t const *in_gaps = in_row ? in_row + a.scratch_capacity : nullptr;
+            bool const steady = chunk >= 1 && chunk * 64 + 64 <= columns;
+            if (out_row) {
+                int32_t *out_gaps = out_row + a.scratch_capacity;
+                if (steady) sweep.template run_chunk<true, true>(chunk, in_row, in_gaps, out_row, out_gaps);
+                else sweep.template run_chunk<false, true>(chunk, in_row, in_gaps, out_row, out_gaps);
+            }
+            else if (steady) sweep.template run_chunk<true, false>(chunk, in_row, in_gaps, nullptr, nullptr);
+            else sweep.template run_chunk<false, false>(chunk, in_row, in_gaps, nullptr, nullptr);
             if (chunk == chunks - 1) sweep.end_stripe(base, stripe == stripes - 1);
         }
         __syncthreads(); // publishes this phase's boundary rows to the next stripe's wave
@@ -105,7 +123,7 @@ __device__ void run_cooperative(item_args_t const &a) {
 
 #pragma region Dispatch
 
-template <int family_, typename symbol_type_, int shape_>
+template <int family_, typename symbol_type_, int shape_, bool shifted_>
 __device__ __forceinline__ void run_shape(item_args_t const &a) {
     constexpr shape_t shape = shapes_k[shape_];
     constexpr bool uniform = family_traits<family_>::uniform_k;
@@ -119,28 +137,28 @@ __device__ __forceinline__ void run_shape(item_args_t const &a) {
     // Class-cost families keep their LDS profile at <= 8 rows per lane; larger shapes are never binned for them.
     else if constexpr (!uniform && shape.rows_per_lane > 8) return;
     else if constexpr (shape.cooperative)
-        run_cooperative<sweep_t<family_, symbol_type_, 64, shape.rows_per_lane>>(a);
-    else run_independent<sweep_t<family_, symbol_type_, shape.group, shape.rows_per_lane>>(a);
+        run_cooperative<sweep_t<family_, symbol_type_, 64, shape.rows_per_lane, shifted_>>(a);
+    else run_independent<sweep_t<family_, symbol_type_, shape.group, shape.rows_per_lane, shifted_>>(a);
 }
 
-template <int family_, typename symbol_type_>
+template <int family_, typename symbol_type_, bool shifted_ = true>
 __device__ void run_family(int shape, item_args_t const &a) {
     switch (shape) {
-    case 0: run_shape<family_, symbol_type_, 0>(a); break;
-    case 1: run_shape<family_, symbol_type_, 1>(a); break;
-    case 2: run_shape<family_, symbol_type_, 2>(a); break;
-    case 3: run_shape<family_, symbol_type_, 3>(a); break;
-    case 4: run_shape<family_, symbol_type_, 4>(a); break;
-    case 5: run_shape<family_, symbol_type_, 5>(a); break;
-    case 6: run_shape<family_, symbol_type_, 6>(a); break;
-    case 7: run_shape<family_, symbol_type_, 7>(a); break;
-    case 8: run_shape<family_, symbol_type_, 8>(a); break;
-    case 9: run_shape<family_, symbol_type_, 9>(a); break;
-    case 10: run_shape<family_, symbol_type_, 10>(a); break;
-    case 11: run_shape<family_, symbol_type_, 11>(a); break;
-    case 12: run_shape<family_, symbol_type_, 12>(a); break;
-    case 13: run_shape<family_, symbol_type_, 13>(a); break;
-    case 14: run_shape<family_, symbol_type_, 14>(a); break;
+    case 0: run_shape<family_, symbol_type_, 0, shifted_>(a); break;
+    case 1: run_shape<family_, symbol_type_, 1, shifted_>(a); break;
+    case 2: run_shape<family_, symbol_type_, 2, shifted_>(a); break;
+    case 3: run_shape<family_, symbol_type_, 3, shifted_>(a); break;
+    case 4: run_shape<family_, symbol_type_, 4, shifted_>(a); break;
+    case 5: run_shape<family_, symbol_type_, 5, shifted_>(a); break;
+    case 6: run_shape<family_, symbol_type_, 6, shifted_>(a); break;
+    case 7: run_shape<family_, symbol_type_, 7, shifted_>(a); break;
+    case 8: run_shape<family_, symbol_type_, 8, shifted_>(a); break;
+    case 9: run_shape<family_, symbol_type_, 9, shifted_>(a); break;
+    case 10: run_shape<family_, symbol_type_, 10, shifted_>(a); break;
+    case 11: run_shape<family_, symbol_type_, 11, shifted_>(a); break;
+    case 12: run_shape<family_, symbol_type_, 12, shifted_>(a); break;
+    case 13: run_shape<family_, symbol_type_, 13, shifted_>(a); break;
+    case 14: run_shape<family_, symbol_type_, 14, shifted_>(a); break;
     default: break;
     }
 }
@@ -158,7 +176,12 @@ __device__ void run_variant(int variant, item_args_t const &a) {
     case family_nw_affine_k * 2 + symbols_u8_k: run_family<family_nw_affine_k, uint8_t>(shape, a); break;
     case family_sw_linear_k * 2 + symbols_u8_k: run_family<family_sw_linear_k, uint8_t>(shape, a); break;
     case family_sw_affine_k * 2 + symbols_u8_k: run_family<family_sw_affine_k, uint8_t>(shape, a); break;
-    default: break; // class-cost families never run over runes
+    // Class-cost families never run over runes; their second code is the unshifted-profile sweep.
+    case family_nw_linear_k * 2 + class_unshifted_k: run_family<family_nw_linear_k, uint8_t, false>(shape, a); break;
+    case family_nw_affine_k * 2 + class_unshifted_k: run_family<family_nw_affine_k, uint8_t, false>(shape, a); break;
+    case family_sw_linear_k * 2 + class_unshifted_k: run_family<family_sw_linear_k, uint8_t, false>(shape, a); break;
+    case family_sw_affine_k * 2 + class_unshifted_k: run_family<family_sw_affine_k, uint8_t, false>(shape, a); break;
+    default: break;
     }
 }
 
@@ -365,7 +388,7 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
         uint64_t const q_begin = a.query_offsets[q], c_begin = c_offsets[c];
         uint64_t q_length = a.query_offsets[q + 1] - q_begin;
         uint64_t c_length = c_offsets[c + 1] - c_begin;
-        if (a.symbols == symbols_u32_k) q_length = a.query_rune_counts[q], c_length = c_runes[c];
+        if (a.runes) q_length = a.query_rune_counts[q], c_length = c_runes[c];
 
         task_t task;
         task.segment = a.segment;

@@ -16,6 +16,13 @@
  *  vertical-gap value and the column symbol to lane k+1 with one DPP shift each. Lane 0 receives the row above
  *  the stripe: the matrix border, or the previous stripe's bottom row read back from HBM scratch. Lane G-1's
  *  bottom row is collected 64 columns at a time (G = 64) and written to scratch for the next stripe.
+ *
+ *  Scores are kept shifted by S = open (H + S): the gap terms `left + open`, `up + open` and linear
+ *  `opt(up, left) + g` then read the stored values directly, and the substitution term reads `cost - S` from the
+ *  profile, so each cell pays one add for the shift instead of two for the gaps. With G = 64 lane 0's inputs for a
+ *  64-step chunk (row above the stripe, its vertical gaps, column symbols) are staged in a 1 KB LDS ring, one
+ *  16-byte entry per step read back as a broadcast, so each step injects them with the DPP shift alone. Chunks in
+ *  which every lane has a valid column skip the per-step column check.
  */
 #pragma once
 #include <hip/hip_runtime.h>
@@ -93,7 +100,7 @@ struct item_args_t {
     uint32_t count;
     int32_t *scratch;      /**< this workgroup's 4 x 2 x `scratch_capacity` int32 rows */
     uint32_t scratch_capacity;
-    uint32_t wave_lds_bytes; /**< dynamic LDS per wave: substitution profile or Myers symbol tables */
+    uint32_t wave_lds_bytes; /**< dynamic LDS per wave: profile or Myers tables, then the injection ring */
     int32_t *shared_result; /**< LDS word for cooperative reductions */
 };
 
@@ -110,8 +117,12 @@ __device__ __forceinline__ int8_t *wave_lds(uint32_t wave, uint32_t bytes) {
 /** How a sweep's per-lane partial results combine into the pair's score. */
 enum reduce_kind_t { reduce_none_k = 0, reduce_max_k = 1, reduce_sum_k = 2 };
 
-/** Per-lane state of one group sweeping one stripe of one pair. */
-template <int family_, typename symbol_type_, int G, int R>
+/**
+ *  Per-lane state of one group sweeping one stripe of one pair. `shifted_` keeps scores as H + open (the profile
+ *  then holds `cost - open`); without it S = 0 and the gap terms add `open` explicitly, for engines whose shifted
+ *  costs would not fit int8.
+ */
+template <int family_, typename symbol_type_, int G, int R, bool shifted_ = true>
 struct sweep_t {
     using traits = family_traits<family_>;
     static constexpr bool minimize_k = traits::minimize_k;
@@ -128,13 +139,16 @@ struct sweep_t {
     bool active;
     int32_t n, m;
     int32_t open, extend, match, mismatch, dense_count;
+    int32_t shift;                  /**< S: stored scores are H + S */
+    int32_t match_shifted, mismatch_shifted;
     bool transposed;
     engine_params_t const *params;
     symbol_type_ const *rows;
     symbol_type_ const *columns;
     int64_t *results;
     uint64_t output, mirror;
-    int8_t *profile; /**< this wave's LDS profile: [dense class][lane][RP] int8 */
+    int8_t *profile; /**< this wave's LDS profile: [dense class][lane][RP] int8 of `cost - S` */
+    int8_t *ring;    /**< this wave's injection ring (G = 64) */
 
     // The stripe.
     int32_t lane_top;
@@ -159,6 +173,8 @@ struct sweep_t {
         open = params->open, extend = params->extend;
         match = params->match, mismatch = params->mismatch;
         dense_count = params->dense_count;
+        shift = shifted_ ? params->shift : 0;
+        match_shifted = match - shift, mismatch_shifted = mismatch - shift;
         transposed = task.transposed != 0;
         symbol_type_ const *query = (symbol_type_ const *)segment.query_symbols;
         symbol_type_ const *candidate = (symbol_type_ const *)segment.candidate_symbols;
@@ -168,6 +184,7 @@ struct sweep_t {
         output = task.output;
         mirror = task.mirror;
         profile = wave_profile;
+        ring = wave_profile + (a.wave_lds_bytes - inject_ring_bytes_k);
         running_best = 0; // serial.hpp:781 starts the local best at 0
     }
 
@@ -180,8 +197,9 @@ struct sweep_t {
             uint32_t symbol = 0xFFFFFFFFu;
             if (i <= n) symbol = (uint32_t)rows[i - 1];
             row_symbols[r] = symbol;
-            scores[r] = border_score<family_>(i, open, extend);
-            horizontal[r] = (open + extend) + scores[r]; // discard value of the horizontal track on column 0
+            int32_t const border = border_score<family_>(i, open, extend);
+            scores[r] = border + shift;
+            horizontal[r] = (open + extend) + border; // discard value of the horizontal track on column 0
             best[r] = 0;
         }
         if constexpr (!uniform_k) {
@@ -193,109 +211,137 @@ struct sweep_t {
                         row_symbols[r] == 0xFFFFFFFFu ? 0u : params->byte_to_dense[row_symbols[r] & 0xFF];
                     int8_t const cost = transposed ? params->dense_costs[c * max_dense_classes_k + row_class]
                                                    : params->dense_costs[row_class * max_dense_classes_k + c];
-                    mine[c * 64 * RP + r] = cost;
+                    mine[c * 64 * RP + r] = (int8_t)(cost - shift); // fits: checked when the engine was made
                 }
             }
         }
-        diagonal_top = border_score<family_>(lane_top, open, extend); // H[top - 1][0]
+        diagonal_top = border_score<family_>(lane_top, open, extend) + shift; // H[top - 1][0] + S
         bottom_score = scores[R - 1];
         bottom_gap = 0;
         column_symbol = 0;
     }
 
+    /** Lane-0 inputs of column `j` (1-based): the row above the stripe (H + S), its vertical gap, the symbol. */
+    __device__ __forceinline__ void column_inputs(int32_t j, int32_t const *in_scores, int32_t const *in_gaps,
+                                                  int32_t &score, int32_t &gap, uint32_t &symbol) const {
+        symbol = 0;
+        if (j <= m) {
+            uint32_t const raw = (uint32_t)columns[j - 1];
+            symbol = uniform_k ? raw : (uint32_t)params->byte_to_dense[raw & 0xFF];
+        }
+        if (in_scores) {
+            score = j <= m ? in_scores[j] : 0;
+            gap = (affine_k && j <= m) ? in_gaps[j] : 0;
+        }
+        else {
+            int32_t const border = border_score<family_>(j, open, extend);
+            score = border + shift;
+            gap = (open + extend) + border; // discard value of the vertical track on row 0
+        }
+    }
+
+    /** One column of this lane's R rows: `up_score` is H + S of the cell above, `up_gap` its vertical gap. */
+    __device__ __forceinline__ void update_rows(int32_t up_score, int32_t up_gap) {
+        int32_t diagonal = diagonal_top;
+        int32_t up = up_score;
+        int32_t vertical = up_gap;
+        uint32_t costs_packed[RP / 4];
+        if constexpr (!uniform_k) {
+            uint32_t const *row_costs = (uint32_t const *)(profile + (column_symbol * 64 + lane) * RP);
+#pragma unroll
+            for (int w = 0; w < RP / 4; ++w) costs_packed[w] = row_costs[w];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            int32_t cost; // cost - S
+            if constexpr (uniform_k) cost = row_symbols[r] == column_symbol ? match_shifted : mismatch_shifted;
+            else cost = (int32_t)(int8_t)(costs_packed[r / 4] >> (8 * (r % 4)));
+            int32_t const left = scores[r];                      // H[i][j-1] + S
+            int32_t const if_substitution = diagonal + cost;     // H[i-1][j-1] + s(i, j), unshifted
+            int32_t cell;                                        // H[i][j], unshifted
+            if constexpr (affine_k) {
+                int32_t const open_left = shifted_ ? left : left + open; // H[i][j-1] + open
+                int32_t const open_up = shifted_ ? up : up + open;
+                int32_t const h = opt<minimize_k>(open_left, horizontal[r] + extend);
+                int32_t const v = opt<minimize_k>(open_up, vertical + extend);
+                horizontal[r] = h;
+                vertical = v;
+                if constexpr (local_k) cell = max(max(h, v), max(if_substitution, 0));
+                else cell = opt<minimize_k>(opt<minimize_k>(h, v), if_substitution);
+            }
+            else {
+                int32_t gap = opt<minimize_k>(up, left); // + g, folded into the stored scores when shifted
+                if constexpr (!shifted_) gap += open;
+                if constexpr (local_k) cell = max(gap, max(if_substitution, 0));
+                else cell = opt<minimize_k>(gap, if_substitution);
+            }
+            if constexpr (local_k) best[r] = max(best[r], cell);
+            int32_t const stored = cell + shift;
+            diagonal = left;
+            up = stored;
+            scores[r] = stored;
+        }
+        bottom_score = up;
+        bottom_gap = vertical;
+    }
+
     /**
      *  Steps `chunk * G .. + G`. The row above the stripe comes from `in_scores`/`in_gaps` (indexed by 1-based
-     *  column) or, when null, from the matrix border. When `out_scores` is set (G = 64 only), lane 63's bottom
-     *  row is stored there for the next stripe.
+     *  column, holding H + S) or, when null, from the matrix border. With `collect_` (G = 64 only), lane 63's
+     *  bottom row is stored to `out_scores`/`out_gaps` for the next stripe. `steady_` promises that every lane's
+     *  column is within 1..m for all steps of the chunk.
      */
+    template <bool steady_, bool collect_>
     __device__ __forceinline__ void run_chunk(int32_t chunk, int32_t const *in_scores, int32_t const *in_gaps,
                                               int32_t *out_scores, int32_t *out_gaps) {
         int32_t const first_step = chunk * G;
-        int32_t const j_lane = first_step + k + 1; // 1-based column this lane loads for its group
-        uint32_t chunk_symbols = 0;
-        if (j_lane <= m) {
-            uint32_t const raw = (uint32_t)columns[j_lane - 1];
-            chunk_symbols = uniform_k ? raw : (uint32_t)params->byte_to_dense[raw & 0xFF];
-        }
+        int32_t const j_lane = first_step + k + 1; // 1-based column this lane stages for its group
         int32_t chunk_scores, chunk_gaps;
-        if (in_scores) {
-            chunk_scores = j_lane <= m ? in_scores[j_lane] : 0;
-            chunk_gaps = (affine_k && j_lane <= m) ? in_gaps[j_lane] : 0;
-        }
-        else {
-            chunk_scores = border_score<family_>(j_lane, open, extend);
-            chunk_gaps = (open + extend) + chunk_scores; // discard value of the vertical track on row 0
-        }
+        uint32_t chunk_symbols;
+        column_inputs(j_lane, in_scores, in_gaps, chunk_scores, chunk_gaps, chunk_symbols);
         int32_t stage_scores = 0, stage_gaps = 0;
 
+        if constexpr (G == 64) {
+            *(int4 *)(ring + lane * 16) = make_int4(chunk_scores, chunk_gaps, (int32_t)chunk_symbols, 0);
+            int4 inject = *(int4 const *)ring;
 #pragma unroll 2
-        for (int32_t step = 0; step < G; ++step) {
-            int32_t const t = first_step + step;
-            int32_t const up_score = shift_down<G>(chunk_scores, bottom_score);
-            int32_t up_gap = 0;
-            if constexpr (affine_k) up_gap = shift_down<G>(chunk_gaps, bottom_gap);
-            column_symbol = (uint32_t)shift_down<G>((int32_t)chunk_symbols, (int32_t)column_symbol);
-            chunk_scores = rotate_up<G>(chunk_scores);
-            if constexpr (affine_k) chunk_gaps = rotate_up<G>(chunk_gaps);
-            chunk_symbols = (uint32_t)rotate_up<G>((int32_t)chunk_symbols);
-
-            int32_t const j = t + 1 - k;
-            if ((uint32_t)(j - 1) < (uint32_t)m) {
-                int32_t diagonal = diagonal_top;
-                int32_t up = up_score;
-                int32_t vertical = up_gap;
-                uint32_t costs_packed[RP / 4];
-                if constexpr (!uniform_k) {
-                    uint32_t const *row_costs = (uint32_t const *)(profile + (column_symbol * 64 + lane) * RP);
-#pragma unroll
-                    for (int w = 0; w < RP / 4; ++w) costs_packed[w] = row_costs[w];
-                }
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    int32_t cost;
-                    if constexpr (uniform_k) cost = row_symbols[r] == column_symbol ? match : mismatch;
-                    else cost = (int32_t)(int8_t)(costs_packed[r / 4] >> (8 * (r % 4)));
-                    int32_t const left = scores[r];
-                    int32_t const if_substitution = diagonal + cost;
-                    int32_t cell;
-                    if constexpr (affine_k) {
-                        int32_t const h = opt<minimize_k>(left + open, horizontal[r] + extend);
-                        int32_t const v = opt<minimize_k>(up + open, vertical + extend);
-                        horizontal[r] = h;
-                        vertical = v;
-                        int32_t const gap = opt<minimize_k>(h, v);
-                        if constexpr (local_k) cell = max(gap, max(if_substitution, 0));
-                        else cell = opt<minimize_k>(gap, if_substitution);
-                    }
-                    else {
-                        int32_t const gap = opt<minimize_k>(up, left) + open;
-                        if constexpr (local_k) cell = max(gap, max(if_substitution, 0));
-                        else cell = opt<minimize_k>(gap, if_substitution);
-                    }
-                    if constexpr (local_k) best[r] = max(best[r], cell);
-                    diagonal = left;
-                    up = cell;
-                    scores[r] = cell;
-                }
-                bottom_score = up;
-                bottom_gap = vertical;
-            }
-            diagonal_top = up_score;
-            if constexpr (G == 64) {
-                if (out_scores) {
+            for (int32_t step = 0; step < 64; ++step) {
+                int4 const next = *(int4 const *)(ring + ((step + 1) & 63) * 16); // prefetch the next entry
+                int32_t const up_score = shift_down<64>(inject.x, bottom_score);
+                int32_t up_gap = 0;
+                if constexpr (affine_k) up_gap = shift_down<64>(inject.y, bottom_gap);
+                column_symbol = (uint32_t)shift_down<64>(inject.z, (int32_t)column_symbol);
+                inject = next;
+                int32_t const j = first_step + step + 1 - k;
+                if (steady_ || (uint32_t)(j - 1) < (uint32_t)m) update_rows(up_score, up_gap);
+                diagonal_top = up_score;
+                if constexpr (collect_) {
                     stage_scores = collect_from_last(bottom_score, stage_scores);
                     if constexpr (affine_k) stage_gaps = collect_from_last(bottom_gap, stage_gaps);
                 }
             }
-        }
-        // Lane 63's bottom row of these 64 steps, oldest in lane 0: columns first_step - 62 .. first_step + 1.
-        if constexpr (G == 64) {
-            if (out_scores) {
+            // Lane 63's bottom row of these 64 steps, oldest in lane 0: columns first_step - 62 .. first_step + 1.
+            if constexpr (collect_) {
                 int32_t const j_out = first_step + lane - 62;
                 if (j_out >= 1 && j_out <= m) {
                     out_scores[j_out] = stage_scores;
                     if constexpr (affine_k) out_gaps[j_out] = stage_gaps;
                 }
+            }
+        }
+        else {
+#pragma unroll 2
+            for (int32_t step = 0; step < G; ++step) {
+                int32_t const up_score = shift_down<G>(chunk_scores, bottom_score);
+                int32_t up_gap = 0;
+                if constexpr (affine_k) up_gap = shift_down<G>(chunk_gaps, bottom_gap);
+                column_symbol = (uint32_t)shift_down<G>((int32_t)chunk_symbols, (int32_t)column_symbol);
+                chunk_scores = rotate_up<G>(chunk_scores);
+                if constexpr (affine_k) chunk_gaps = rotate_up<G>(chunk_gaps);
+                chunk_symbols = (uint32_t)rotate_up<G>((int32_t)chunk_symbols);
+                int32_t const j = first_step + step + 1 - k;
+                if ((uint32_t)(j - 1) < (uint32_t)m) update_rows(up_score, up_gap);
+                diagonal_top = up_score;
             }
         }
     }
@@ -316,6 +362,7 @@ struct sweep_t {
                 int32_t value = 0;
 #pragma unroll
                 for (int r = 0; r < R; ++r) value += scores[r] & -(int32_t)(wanted == r);
+                value -= shift;
                 results[output] = (int64_t)value;
                 if (mirror != no_mirror_k) results[mirror] = (int64_t)value;
             }

@@ -148,7 +148,11 @@ struct myers_sweep_t {
         column_code = 0;
     }
 
-    /** Horizontal deltas are coded as bit 0 = +1, bit 1 = -1. */
+    /**
+     *  Horizontal deltas are coded as bit 0 = +1, bit 1 = -1. `steady_` promises every lane a valid column on
+     *  every step of the chunk; `collect_` (G = 64) stores lane 63's bottom codes for the next stripe.
+     */
+    template <bool steady_, bool collect_>
     __device__ __forceinline__ void run_chunk(int32_t chunk, int32_t const *in_codes, int32_t const *,
                                               int32_t *out_codes, int32_t *) {
         int32_t const first_step = chunk * G;
@@ -168,7 +172,7 @@ struct myers_sweep_t {
             chunk_symbols = group_rotate_up<G>(chunk_symbols);
 
             int32_t const j = t + 1 - k;
-            if ((uint32_t)(j - 1) < (uint32_t)m) {
+            if (steady_ || (uint32_t)(j - 1) < (uint32_t)m) {
                 // Eq = AND over planes of (plane XNOR column bit), one 32-bit half at a time. The all-ones or
                 // all-zeros mask is formed with a subtraction: a sign-extended 1-bit field widened to 64 bits
                 // was compiled as zero-extended (the upper half of Eq came out wrong).
@@ -195,15 +199,11 @@ struct myers_sweep_t {
                 pv = mh | ~(xv | ph);
                 mv = ph & xv;
             }
-            if constexpr (G == 64) {
-                if (out_codes) stage = collect_from_last((int32_t)bottom_code, stage);
-            }
+            if constexpr (G == 64 && collect_) stage = collect_from_last((int32_t)bottom_code, stage);
         }
-        if constexpr (G == 64) {
-            if (out_codes) {
-                int32_t const j_out = first_step + lane - 62;
-                if (j_out >= 1 && j_out <= m) out_codes[j_out] = stage;
-            }
+        if constexpr (G == 64 && collect_) {
+            int32_t const j_out = first_step + lane - 62;
+            if (j_out >= 1 && j_out <= m) out_codes[j_out] = stage;
         }
     }
 

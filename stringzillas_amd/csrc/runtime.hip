@@ -243,6 +243,13 @@ sz_status_t make_engine(int kind, int match, int mismatch, int open, int extend,
         if (kind == szs_engine_needleman_wunsch_k) engine->params.family = linear ? family_nw_linear_k : family_nw_affine_k;
         else engine->params.family = linear ? family_sw_linear_k : family_sw_affine_k;
     }
+    // The LDS profile holds `cost - shift` as int8; fall back to no shift when some used cost would not fit.
+    engine->params.shift = open;
+    for (int x = 0; x < engine->params.dense_count; ++x)
+        for (int y = 0; y < engine->params.dense_count; ++y) {
+            int const shifted = engine->params.dense_costs[x * max_dense_classes_k + y] - open;
+            if (shifted < -128 || shifted > 127) engine->params.shift = 0;
+        }
     *handle = engine;
     return ok(error_message);
 }

@@ -27,6 +27,11 @@ enum family_t : int32_t {
 
 /** Symbol width of a segment: bytes, or runes decoded from UTF-8. */
 enum symbols_t : int32_t { symbols_u8_k = 0, symbols_u32_k = 1 };
+/**
+ *  Class-cost families never score runes, so their variant code 1 marks byte pairs of an engine whose
+ *  `cost - open` does not fit the int8 profile: those run a sweep that keeps scores unshifted.
+ */
+constexpr int32_t class_unshifted_k = 1;
 
 constexpr int max_dense_classes_k = 32;
 
@@ -36,6 +41,9 @@ struct engine_params_t {
     int32_t open, extend;     /**< gap costs (sign as given: positive for distances, negative for scores) */
     int32_t match, mismatch;  /**< uniform substitution costs */
     int32_t dense_count;      /**< number of distinct classes actually used by `byte_to_class` */
+    /** Offset S the sweep adds to every stored score: scores kept as H + S let one add serve both gap uses of a
+     *  cell. `open` when every `dense_costs - open` fits int8 (and always for uniform costs), else 0. */
+    int32_t shift;
     uint8_t byte_to_dense[256];
     int8_t dense_costs[max_dense_classes_k * max_dense_classes_k]; /**< [dense(query)][dense(candidate)] */
 };
@@ -148,8 +156,13 @@ struct materialize_args_t {
     int32_t mode;                        /**< 0 pairs, 1 cross-product, 2 symmetric */
     int32_t open, extend;
     int32_t unit_costs;                  /**< match 0, mismatch 1, gap 1: bit-parallel shapes apply to bytes */
+    int32_t runes;                       /**< lengths come from the rune counts (UTF-8) */
     int32_t reach_magnitude;             /**< max |cost| of the engine, for the int32 reach bound */
 };
+
+/** Bytes of the per-wave LDS ring that stages lane 0's inputs of a 64-step chunk (64 entries of 16 bytes); it
+ *  follows the wave's profile or Myers tables in dynamic LDS. */
+constexpr uint32_t inject_ring_bytes_k = 64 * 16;
 
 /** One UTF-8 tape to transcode: its strings are numbered from `string_base` across the batch. */
 struct utf8_tape_t {

@@ -20,10 +20,18 @@ szs = pytest.importorskip("stringzillas_amd")
 RANGES = [(0, 20), (10, 70), (60, 300), (200, 600), (500, 1100), (1000, 2600), (2100, 3600), (4200, 6000)]
 
 
+def _wide_tables(rng):
+    """Costs whose difference to the gap-open cost leaves int8: these engines run the unshifted-profile sweep."""
+    byte_to_class, _ = protein_tables(rng)
+    costs = rng.integers(-100, 101, size=(32, 32)).astype(np.int8)
+    return byte_to_class, np.ascontiguousarray(costs)
+
+
 def _engines(rng):
     b2c, costs = protein_tables(rng)
     b2c_a, costs_a = protein_tables(rng, asymmetric=True)
     dna_b2c, dna_costs = dna_tables()
+    b2c_w, costs_w = _wide_tables(rng)
     return {
         "lev_unit": (szs.LevenshteinDistances(), O.Params("levenshtein"), DNA, False),
         "lev_affine": (szs.LevenshteinDistances(0, 4, 3, 2), O.Params("levenshtein", 0, 4, 3, 2), DNA, False),
@@ -44,6 +52,12 @@ def _engines(rng):
                       O.Params("smith_waterman", open=-5, extend=-1, byte_to_class=b2c, class_costs=costs), PROTEIN, False),
         "sw_odd_signs": (szs.SmithWatermanScores(b2c_a, costs_a, 3, 1),
                          O.Params("smith_waterman", open=3, extend=1, byte_to_class=b2c_a, class_costs=costs_a), PROTEIN, False),
+        "nw_wide_affine": (szs.NeedlemanWunschScores(b2c_w, costs_w, -100, -40),
+                           O.Params("needleman_wunsch", open=-100, extend=-40, byte_to_class=b2c_w, class_costs=costs_w),
+                           PROTEIN, False),
+        "sw_wide_linear": (szs.SmithWatermanScores(b2c_w, costs_w, -90, -90),
+                           O.Params("smith_waterman", open=-90, extend=-90, byte_to_class=b2c_w, class_costs=costs_w),
+                           PROTEIN, False),
     }
 
 
