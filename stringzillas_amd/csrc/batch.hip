@@ -22,7 +22,7 @@
 extern "C" {
 hipError_t szs_amd_launch_fill(uint32_t *data, uint32_t count, uint32_t value, hipStream_t stream);
 hipError_t szs_amd_launch_utf8_decode(szs_amd::utf8_tape_t const *tapes, uint32_t tapes_count, uint64_t strings_total,
-                                      hipStream_t stream);
+                                      szs_amd::prepare_stats_t *stats, uint32_t *segment_flags, hipStream_t stream);
 hipError_t szs_amd_launch_widen_offsets(uint32_t const *narrow, uint64_t *wide, uint64_t count, hipStream_t stream);
 hipError_t szs_amd_launch_materialize(szs_amd::materialize_args_t const *args, uint32_t segments_count,
                                       uint64_t tasks_total, szs_amd::shape_table_t const *table, szs_amd::task_t *tasks,
@@ -235,12 +235,12 @@ static sz_status_t enqueue_front(batch_t &b, bool bin, char const **error_messag
     SZS_TRY_HIP(szs_amd_launch_fill(b.histogram.as<uint32_t>(), bins_count_k, 0, stream), "fill");
     SZS_TRY_HIP(szs_amd_launch_fill(b.cursor.as<uint32_t>(), bins_count_k, 0, stream), "fill");
     SZS_TRY_HIP(szs_amd_launch_fill(b.queue_head.as<uint32_t>(), 1, 0, stream), "fill");
-    SZS_TRY_HIP(szs_amd_launch_utf8_decode(b.decode_tapes.as<utf8_tape_t>(), b.decode_tapes_count, b.decode_strings,
-                                           stream),
-                "UTF-8 decoding");
     uint32_t const segments_count = (uint32_t)b.inputs.size();
     SZS_TRY_HIP(szs_amd_launch_fill(b.stats.as<uint32_t>(), sizeof(prepare_stats_t) / 4, 0, stream), "fill");
     SZS_TRY_HIP(szs_amd_launch_fill(b.segment_overflow.as<uint32_t>(), segments_count, 0, stream), "fill");
+    SZS_TRY_HIP(szs_amd_launch_utf8_decode(b.decode_tapes.as<utf8_tape_t>(), b.decode_tapes_count, b.decode_strings,
+                                           b.stats.as<prepare_stats_t>(), b.segment_overflow.as<uint32_t>(), stream),
+                "UTF-8 decoding");
     SZS_TRY_HIP(szs_amd_launch_materialize(b.materialize_device.as<materialize_args_t>(), segments_count,
                                            b.tasks_count, b.shape_table.as<shape_table_t>(), b.tasks.as<task_t>(),
                                            b.bin_of_task.as<uint32_t>(), b.histogram.as<uint32_t>(),
@@ -358,6 +358,7 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
         auto *queries = new (std::nothrow) staged_tape_t();
         if (!queries) return fail(sz_bad_alloc_k, error_message, "Failed to allocate a tape");
         b.tapes.push_back(queries);
+        queries->segment = (uint32_t)s;
         status = stage_input(input, true, runes, *queries, stream, error_message);
         if (status != sz_success_k) return status;
         staged_tape_t *candidates = queries;
@@ -365,6 +366,7 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
             candidates = new (std::nothrow) staged_tape_t();
             if (!candidates) return fail(sz_bad_alloc_k, error_message, "Failed to allocate a tape");
             b.tapes.push_back(candidates);
+            candidates->segment = (uint32_t)s;
             status = stage_input(input, false, runes, *candidates, stream, error_message);
             if (status != sz_success_k) return status;
         }
@@ -466,6 +468,9 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
             entry.runes = tape->runes;
             entry.rune_counts = tape->rune_counts;
             entry.string_base = b.decode_strings;
+            entry.capacity = std::max<uint64_t>(tape->bytes, 1);
+            entry.segment = tape->segment;
+            entry.reserved = 0;
             decode.push_back(entry);
             b.decode_strings += tape->count;
         }
@@ -521,7 +526,9 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
                     "task readback");
     }
     SZS_TRY_HIP(hipStreamSynchronize(stream), "materialization");
-    if (stats.overflow) {
+    if (stats.overflow || stats.bad_tape) {
+        if (!options.defer_overflow && stats.bad_tape)
+            return fail(sz_unexpected_dimensions_k, error_message, "A tape's offsets run past its stated byte count");
         if (!options.defer_overflow)
             return fail(sz_overflow_risk_k, error_message, "Scores of this pair would overflow 32-bit cells");
         if (segments_count) {

@@ -278,9 +278,11 @@ __device__ __forceinline__ uint32_t block_byte(uint4 block, uint32_t k) {
  *
  *  The walk reads the string through two 16-byte aligned blocks held in registers instead of one dependent byte
  *  load per rune. A block is loaded only if it holds a byte of the string, so it never leaves the pages of the
- *  tape.
+ *  tape. Runes never go past the buffer's capacity: a tape whose offsets run further than the byte count the
+ *  host sized it for keeps its strings truncated there and raises its segment's flag.
  */
-__global__ void utf8_decode_kernel(utf8_tape_t const *tapes, uint32_t tapes_count, uint64_t strings_total) {
+__global__ void utf8_decode_kernel(utf8_tape_t const *tapes, uint32_t tapes_count, uint64_t strings_total,
+                                   prepare_stats_t *stats, uint32_t *segment_flags) {
     for (uint64_t global = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; global < strings_total;
          global += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t low = 0, high = tapes_count - 1;
@@ -294,6 +296,7 @@ __global__ void utf8_decode_kernel(utf8_tape_t const *tapes, uint32_t tapes_coun
         uint64_t const begin = tape.offsets[s], end = tape.offsets[s + 1];
         uint8_t const *const first = tape.bytes + begin, *const last = tape.bytes + end;
         uint32_t *const out = tape.runes + begin;
+        uint64_t const room = begin < tape.capacity ? tape.capacity - begin : 0;
         uint64_t written = 0;
         if (first < last) {
             uint8_t const *block_start = (uint8_t const *)((uintptr_t)first & ~(uintptr_t)15);
@@ -301,6 +304,11 @@ __global__ void utf8_decode_kernel(utf8_tape_t const *tapes, uint32_t tapes_coun
             if (block_start + 16 < last) next = *(uint4 const *)(block_start + 16);
             uint8_t const *at = first;
             while (at < last) {
+                if (written == room) { // the offsets disagree with the byte count the buffer was sized for
+                    atomicOr(&stats->bad_tape, 1u);
+                    atomicOr(&segment_flags[tape.segment], (uint32_t)segment_bad_tape_k);
+                    break;
+                }
                 while (at >= block_start + 16) { // slide the pair of blocks forward
                     block_start += 16;
                     current = next;
@@ -421,7 +429,7 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
         }
         else if (overflows) {
             overflow = 1;
-            segment_overflow[a.segment] = 1;
+            atomicOr(&segment_overflow[a.segment], (uint32_t)segment_overflow_k);
         }
         else {
             uint32_t const rows = task.rows_length;
@@ -530,12 +538,12 @@ hipError_t szs_amd_launch_fill(uint32_t *data, uint32_t count, uint32_t value, h
 }
 
 hipError_t szs_amd_launch_utf8_decode(szs_amd::utf8_tape_t const *tapes, uint32_t tapes_count, uint64_t strings_total,
-                                      hipStream_t stream) {
+                                      szs_amd::prepare_stats_t *stats, uint32_t *segment_flags, hipStream_t stream) {
     if (!strings_total || !tapes_count) return hipSuccess;
     uint64_t const blocks64 = (strings_total + 255) / 256;
     uint32_t const blocks = blocks64 < 65536 ? (uint32_t)blocks64 : 65536;
     hipLaunchKernelGGL(szs_amd::utf8_decode_kernel, dim3(blocks), dim3(256), 0, stream, tapes, tapes_count,
-                       strings_total);
+                       strings_total, stats, segment_flags);
     return hipGetLastError();
 }
 

@@ -143,12 +143,16 @@ void server_t::retire(round_t &round, sz_status_t status, char const *error) {
     for (size_t i = 0; i < round.tickets.size(); ++i) {
         ticket_t &ticket = tickets[round.tickets[i]];
         ticket.completed = now;
-        bool const overflowed = status == sz_success_k && round.batch && i < round.batch->overflowed.size() &&
-                                round.batch->overflowed[i];
-        if (status != sz_success_k || overflowed) {
+        uint32_t const flags = status == sz_success_k && round.batch && i < round.batch->overflowed.size()
+                                   ? round.batch->overflowed[i]
+                                   : 0u;
+        if (status != sz_success_k || flags) {
             ticket.state = ticket_failed_k;
-            ticket.status = overflowed ? sz_overflow_risk_k : status;
-            ticket.error = overflowed ? "Scores of a pair would overflow 32-bit cells" : error;
+            if (status != sz_success_k) ticket.status = status, ticket.error = error;
+            else if (flags & segment_bad_tape_k)
+                ticket.status = sz_unexpected_dimensions_k,
+                ticket.error = "A tape's offsets run past its stated byte count";
+            else ticket.status = sz_overflow_risk_k, ticket.error = "Scores of a pair would overflow 32-bit cells";
         }
         else ticket.state = ticket_done_k;
     }

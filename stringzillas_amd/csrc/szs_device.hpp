@@ -168,16 +168,24 @@ constexpr uint32_t inject_ring_bytes_k = 64 * 16;
 struct utf8_tape_t {
     uint8_t const *bytes;
     uint64_t const *offsets;
-    uint32_t *runes;       /**< u32[offsets[count]]: runes of string s start at index offsets[s] */
+    uint32_t *runes;       /**< u32[capacity]: runes of string s start at index offsets[s] */
     uint32_t *rune_counts; /**< u32[count] */
     uint64_t string_base;
+    uint64_t capacity;     /**< runes the buffer holds: the tape's byte count as the host knows it */
+    uint32_t segment;
+    uint32_t reserved;
 };
 
-/** What materialization reports back to the host: one small readback instead of the whole task list. */
+/** What transcoding and materialization report back to the host: a few bytes instead of the task list. */
 struct prepare_stats_t {
     unsigned long long cells;            /**< sum of n*m over non-empty tasks */
     uint32_t widest_multi_pass;          /**< most columns of a pair that needs several stripes */
     uint32_t overflow;                   /**< nonzero if some pair's scores could leave int32 */
+    uint32_t bad_tape;                   /**< nonzero if a UTF-8 tape's offsets run past its stated byte count */
+    uint32_t reserved;
 };
+
+/** Per-segment flags raised on the device (the batch fails, or only that segment's request when serving). */
+enum segment_flag_t : uint32_t { segment_overflow_k = 1, segment_bad_tape_k = 2 };
 
 } // namespace szs_amd

@@ -86,7 +86,8 @@ struct staged_tape_t {
     uint8_t const *data = nullptr;
     uint64_t const *offsets = nullptr;
     uint64_t count = 0;
-    uint64_t bytes = 0;             /**< offsets[count] */
+    uint64_t bytes = 0;             /**< offsets[count] (or the caller's stated byte count) */
+    uint32_t segment = 0;           /**< the batch segment this tape belongs to */
     uint32_t *runes = nullptr;      /**< UTF-8 only: u32[bytes] */
     uint32_t *rune_counts = nullptr;
     device_buffer_t data_copy, offsets_copy, runes_buffer, rune_counts_buffer;
@@ -112,7 +113,7 @@ struct segment_input_t {
 /** How a batch is prepared beyond its inputs and mode. */
 struct batch_options_t {
     hipStream_t stream = nullptr;   /**< overrides the scope's stream when set */
-    bool defer_overflow = false;    /**< flag segments whose pairs could overflow instead of failing the batch */
+    bool defer_overflow = false;    /**< flag failing segments (overflow, bad tape) instead of failing the batch */
     char *pinned_results = nullptr; /**< page-locked host staging for pair-list results (null: copy directly) */
     size_t pinned_capacity = 0;     /**< bytes at `pinned_results` */
 };
@@ -136,7 +137,7 @@ struct batch_t {
     uint32_t decode_tapes_count = 0;           /**< UTF-8 tapes transcoded by each run */
     uint64_t decode_strings = 0;               /**< strings across those tapes */
     std::vector<char> materialize_args;        /**< one `materialize_args_t` per segment (opaque here) */
-    std::vector<uint32_t> overflowed;          /**< per segment: nonzero if a pair could overflow (deferred mode) */
+    std::vector<uint32_t> overflowed;          /**< per segment: `segment_flag_t` bits raised on the device */
     char *pinned_results = nullptr;            /**< staging of pair-list results, see `batch_options_t` */
     std::vector<size_t> pinned_offsets;        /**< per segment: byte offset in the staging, or SIZE_MAX */
     size_t pinned_bytes = 0;                   /**< bytes of results staged through `pinned_results` */

@@ -144,3 +144,35 @@ def test_server_stats_completion_times_and_overflow_isolation():
         assert stats["pending"] == 0 and stats["inflight"] == 0
     finally:
         server.close()
+
+
+class _Understated:
+    """An HBM tape whose stated byte count is below what its offsets span."""
+
+    def __init__(self, device_strs, stated):
+        self._inner, self.nbytes = device_strs, stated
+
+    def __len__(self):
+        return len(self._inner)
+
+    def tape(self):
+        return self._inner.tape()
+
+
+def test_understated_tape_bytes_fail_only_their_request():
+    rng = np.random.default_rng(9)
+    strings = [random_utf8(rng, 200, 400) for _ in range(30)]
+    honest = szs.to_device(szs.Strs(strings))
+    understated = _Understated(honest, honest.nbytes // 3)
+    engine = szs.LevenshteinDistancesUTF8()
+    with pytest.raises(ValueError):
+        szs.Batch([(engine, understated, understated)])
+    server = szs.Server(hbm_budget_bytes=256 << 20, max_inflight_rounds=1)
+    try:
+        bad = server.submit(engine, understated, understated)
+        good = server.submit(engine, honest, honest)
+        with pytest.raises(ValueError):
+            server.wait(bad)
+        assert (server.wait(good) == 0).all()  # every string against itself
+    finally:
+        server.close()
