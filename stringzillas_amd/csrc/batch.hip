@@ -440,6 +440,8 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
             return fail(sz_overflow_risk_k, error_message, "A batch holds at most 2^32 - 16 scored cells");
         if (engine->params.family >= family_nw_linear_k) // per-wave profile: classes x 64 lanes x 8 rows
             lds_bytes = std::max<uint32_t>(lds_bytes, (uint32_t)engine->params.dense_count * 64u * 8u);
+        if (a.unit_costs && !runes && engine->params.family == family_lev_linear_k) // Myers tables + match table
+            lds_bytes = std::max<uint32_t>(lds_bytes, myers_lds_bytes_k - inject_ring_bytes_k);
     }
     b.tasks_count = tasks_total;
     // Dynamic LDS per wave: the NW/SW profile or the Myers symbol tables, whichever is larger, then the ring.

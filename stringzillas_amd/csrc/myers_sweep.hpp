@@ -13,6 +13,10 @@
  *  bit-planes: plane b holds bit b of every row's symbol code, and Eq = AND_b (plane_b XNOR bit b of the column
  *  code), one `v_bitop3` per plane and half-word. For 64-lane groups the symbols are first renumbered to a dense
  *  code (0 = absent from the rows) so that only ceil(log2(distinct + 1)) planes are needed, e.g. 3 for DNA.
+ *  With at most 15 distinct symbols in the rows, each stripe instead writes the match word of every code to an
+ *  8 KB LDS table ([16 codes][64 lanes] u64), and a step reads its word back (prefetched one step ahead) in
+ *  place of the plane arithmetic. For 64-lane groups lane 0's inputs of a 64-step chunk (horizontal deltas from
+ *  the stripe above, column codes) are staged in the LDS ring and read back as one broadcast entry per step.
  *  The distance is m + the sum of the vertical deltas of the last column over all rows.
  */
 #pragma once
@@ -50,6 +54,7 @@ struct myers_tables_t {
     uint32_t presence[8]; /**< 256-bit set of the byte values in the rows string */
     uint8_t codes[256];   /**< byte -> dense code, 0 for bytes absent from the rows */
 };
+static_assert(sizeof(myers_tables_t) <= myers_tables_bytes_k, "the match table follows the renumbering tables");
 
 template <int G>
 struct myers_sweep_t {
@@ -66,7 +71,10 @@ struct myers_sweep_t {
     int64_t *results;
     uint64_t output, mirror;
     myers_tables_t *tables;
+    uint64_t *eq_table;  /**< [16 codes][64 lanes] match words of the current stripe (table mode) */
+    int8_t *ring;        /**< lane-0 inputs of a 64-step chunk (G = 64) */
     int32_t planes_used;
+    bool table_mode;     /**< wave-uniform: at most 15 distinct symbols, match words come from `eq_table` */
 
     int32_t lane_top;
     uint64_t planes[8];
@@ -97,8 +105,11 @@ struct myers_sweep_t {
         output = task.output;
         mirror = task.mirror;
         tables = wave_tables;
+        eq_table = reinterpret_cast<uint64_t *>(wave_lds(wave, a.wave_lds_bytes) + myers_tables_bytes_k);
+        ring = wave_lds(wave, a.wave_lds_bytes) + (a.wave_lds_bytes - inject_ring_bytes_k);
         vertical_sum = 0;
         planes_used = 8;
+        table_mode = false;
         if constexpr (renumber_k) {
             // One pair per wave: collect the byte values of the rows, then number them 1..D in byte order.
             if (lane < 8) tables->presence[lane] = 0;
@@ -129,6 +140,7 @@ struct myers_sweep_t {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             int32_t const largest = distinct == 256 ? 255 : distinct;
             planes_used = largest ? 32 - __builtin_clz((uint32_t)largest) : 1;
+            table_mode = distinct < myers_eq_codes_k && a.wave_lds_bytes >= myers_lds_bytes_k;
         }
     }
 
@@ -146,6 +158,42 @@ struct myers_sweep_t {
         mv = 0;
         bottom_code = 1;
         column_code = 0;
+        if (table_mode) { // the match word of every code for this lane's 64 rows
+            for (uint32_t code = 0; code < (uint32_t)myers_eq_codes_k; ++code)
+                eq_table[code * 64 + lane] = planes_eq(code);
+        }
+    }
+
+    /** Eq of `code` over this lane's 64 rows: AND over the used planes of (plane XNOR code bit). */
+    __device__ __forceinline__ uint64_t planes_eq(uint32_t code) const {
+        // One 32-bit half at a time; the all-ones or all-zeros mask is formed with a subtraction: a sign-extended
+        // 1-bit field widened to 64 bits was compiled as zero-extended (the upper half of Eq came out wrong).
+        uint32_t const inverted = ~code;
+        uint32_t eq_low = ~0u, eq_high = ~0u;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            if (b < planes_used) {
+                uint32_t const mask = 0u - ((inverted >> b) & 1u);
+                eq_low &= (uint32_t)planes[b] ^ mask;
+                eq_high &= (uint32_t)(planes[b] >> 32) ^ mask;
+            }
+        }
+        return (uint64_t)eq_high << 32 | eq_low;
+    }
+
+    /** One column of this lane's word (Hyyro's step); `hin` is the horizontal delta entering its top row. */
+    __device__ __forceinline__ void step_word(uint64_t eq, uint32_t hin) {
+        uint64_t const hin_positive = hin & 1u, hin_negative = (hin >> 1) & 1u;
+        uint64_t const xv = eq | mv;
+        eq |= hin_negative;
+        uint64_t const xh = (((eq & pv) + pv) ^ pv) | eq;
+        uint64_t ph = mv | ~(xh | pv);
+        uint64_t mh = pv & xh;
+        bottom_code = (uint32_t)(ph >> 63) | (uint32_t)(mh >> 63) << 1;
+        ph = (ph << 1) | hin_positive;
+        mh = (mh << 1) | hin_negative;
+        pv = mh | ~(xv | ph);
+        mv = ph & xv;
     }
 
     /**
@@ -155,55 +203,61 @@ struct myers_sweep_t {
     template <bool steady_, bool collect_>
     __device__ __forceinline__ void run_chunk(int32_t chunk, int32_t const *in_codes, int32_t const *,
                                               int32_t *out_codes, int32_t *) {
+        if constexpr (G == 64) {
+            if (table_mode) chunk_64<steady_, collect_, true>(chunk, in_codes, out_codes);
+            else chunk_64<steady_, collect_, false>(chunk, in_codes, out_codes);
+        }
+        else chunk_small(chunk, in_codes);
+    }
+
+    template <bool steady_, bool collect_, bool table_>
+    __device__ __forceinline__ void chunk_64(int32_t chunk, int32_t const *in_codes, int32_t *out_codes) {
+        int32_t const first_step = chunk * 64;
+        int32_t const j_lane = first_step + lane + 1;
+        uint32_t const symbol = j_lane <= m ? code_of(columns[j_lane - 1]) : 0u;
+        uint32_t const entering = in_codes ? (j_lane <= m ? (uint32_t)in_codes[j_lane] : 0u) : 1u;
+        *(uint2 *)(ring + lane * 8) = make_uint2(entering, symbol);
+        int32_t stage = 0;
+        // Column codes do not depend on the scores, so step s + 1's code and match word are fetched during step s.
+        uint2 now = *(uint2 const *)ring, after = *(uint2 const *)(ring + 8);
+        uint32_t code = (uint32_t)shift_down<64>((int32_t)now.y, (int32_t)column_code);
+        uint64_t eq_next = 0;
+        if constexpr (table_) eq_next = eq_table[code * 64 + lane];
+#pragma unroll 2
+        for (int32_t step = 0; step < 64; ++step) {
+            uint2 const later = *(uint2 const *)(ring + ((step + 2) & 63) * 8);
+            uint32_t const hin = (uint32_t)shift_down<64>((int32_t)now.x, (int32_t)bottom_code);
+            uint32_t const code_after = (uint32_t)shift_down<64>((int32_t)after.y, (int32_t)code);
+            uint64_t const eq_now = eq_next;
+            if constexpr (table_) eq_next = eq_table[code_after * 64 + lane];
+            column_code = code;
+            int32_t const j = first_step + step + 1 - lane;
+            if (steady_ || (uint32_t)(j - 1) < (uint32_t)m) step_word(table_ ? eq_now : planes_eq(code), hin);
+            if constexpr (collect_) stage = collect_from_last((int32_t)bottom_code, stage);
+            code = code_after;
+            now = after;
+            after = later;
+        }
+        if constexpr (collect_) {
+            int32_t const j_out = first_step + lane - 62;
+            if (j_out >= 1 && j_out <= m) out_codes[j_out] = stage;
+        }
+    }
+
+    __device__ __forceinline__ void chunk_small(int32_t chunk, int32_t const *in_codes) {
         int32_t const first_step = chunk * G;
         int32_t const j_lane = first_step + k + 1;
         uint32_t chunk_symbols = j_lane <= m ? code_of(columns[j_lane - 1]) : 0u;
         uint32_t chunk_codes = in_codes ? (j_lane <= m ? (uint32_t)in_codes[j_lane] : 0u) : 1u;
-        int32_t stage = 0;
         bool const first_in_group = k == 0;
-        int32_t const used = planes_used;
-
 #pragma unroll 2
         for (int32_t step = 0; step < G; ++step) {
-            int32_t const t = first_step + step;
             uint32_t const hin = group_shift_down<G>(chunk_codes, bottom_code, first_in_group);
             column_code = group_shift_down<G>(chunk_symbols, column_code, first_in_group);
             chunk_codes = group_rotate_up<G>(chunk_codes);
             chunk_symbols = group_rotate_up<G>(chunk_symbols);
-
-            int32_t const j = t + 1 - k;
-            if (steady_ || (uint32_t)(j - 1) < (uint32_t)m) {
-                // Eq = AND over planes of (plane XNOR column bit), one 32-bit half at a time. The all-ones or
-                // all-zeros mask is formed with a subtraction: a sign-extended 1-bit field widened to 64 bits
-                // was compiled as zero-extended (the upper half of Eq came out wrong).
-                uint32_t const inverted = ~column_code;
-                uint32_t eq_low = ~0u, eq_high = ~0u;
-#pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    if (b < used) {
-                        uint32_t const mask = 0u - ((inverted >> b) & 1u);
-                        eq_low &= (uint32_t)planes[b] ^ mask;
-                        eq_high &= (uint32_t)(planes[b] >> 32) ^ mask;
-                    }
-                }
-                uint64_t eq = (uint64_t)eq_high << 32 | eq_low;
-                uint64_t const hin_positive = hin & 1u, hin_negative = (hin >> 1) & 1u;
-                uint64_t const xv = eq | mv;
-                eq |= hin_negative;
-                uint64_t const xh = (((eq & pv) + pv) ^ pv) | eq;
-                uint64_t ph = mv | ~(xh | pv);
-                uint64_t mh = pv & xh;
-                bottom_code = (uint32_t)(ph >> 63) | (uint32_t)(mh >> 63) << 1;
-                ph = (ph << 1) | hin_positive;
-                mh = (mh << 1) | hin_negative;
-                pv = mh | ~(xv | ph);
-                mv = ph & xv;
-            }
-            if constexpr (G == 64 && collect_) stage = collect_from_last((int32_t)bottom_code, stage);
-        }
-        if constexpr (G == 64 && collect_) {
-            int32_t const j_out = first_step + lane - 62;
-            if (j_out >= 1 && j_out <= m) out_codes[j_out] = stage;
+            int32_t const j = first_step + step + 1 - k;
+            if ((uint32_t)(j - 1) < (uint32_t)m) step_word(planes_eq(column_code), hin);
         }
     }
 

@@ -163,6 +163,10 @@ struct materialize_args_t {
 /** Bytes of the per-wave LDS ring that stages lane 0's inputs of a 64-step chunk (64 entries of 16 bytes); it
  *  follows the wave's profile or Myers tables in dynamic LDS. */
 constexpr uint32_t inject_ring_bytes_k = 64 * 16;
+/** Myers LDS per wave: renumbering tables, then a [16 codes][64 lanes] u64 match table, then the ring. */
+constexpr uint32_t myers_tables_bytes_k = 512;
+constexpr int32_t myers_eq_codes_k = 16;
+constexpr uint32_t myers_lds_bytes_k = myers_tables_bytes_k + myers_eq_codes_k * 64 * 8 + inject_ring_bytes_k;
 
 /** One UTF-8 tape to transcode: its strings are numbered from `string_base` across the batch. */
 struct utf8_tape_t {
