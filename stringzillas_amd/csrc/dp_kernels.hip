@@ -269,73 +269,185 @@ __device__ __forceinline__ uint32_t block_byte(uint4 block, uint32_t k) {
     return (word >> ((k & 3) * 8)) & 0xFFu;
 }
 
+/** The rune of a lead byte of `width` bytes and the continuation bytes that exist (missing ones are 0). */
+__device__ __forceinline__ uint32_t rune_of(uint32_t lead, uint32_t width, uint32_t t0, uint32_t t1, uint32_t t2) {
+    if (width == 1) return lead;
+    if (width == 2) return (lead & 0x1Fu) << 6 | (t0 & 0x3Fu);
+    if (width == 3) return (lead & 0x0Fu) << 12 | (t0 & 0x3Fu) << 6 | (t1 & 0x3Fu);
+    return (lead & 0x07u) << 18 | (t0 & 0x3Fu) << 12 | (t1 & 0x3Fu) << 6 | (t2 & 0x3Fu);
+}
+
+__device__ __forceinline__ uint32_t utf8_width(uint32_t lead) {
+    return 1 + (lead >= 0xC0) + (lead >= 0xE0) + (lead >= 0xF0);
+}
+
 /**
- *  UTF-8 to runes for every tape of a batch in one launch, one thread per string, with
- *  `sz_rune_decode_unchecked` semantics (include/stringzilla/utf8_runes/serial.h:101-114): the lead byte fixes
- *  the width, continuation bytes are not validated, and continuation bytes missing at the end of a string read
- *  as 0. Runes of string `s` start at index `offsets[s]` of the tape's `runes`; their count goes to
- *  `rune_counts[s]`.
+ *  One lane walks one string, the way `sz_rune_decode_unchecked` is applied in a loop, reading it through two
+ *  16-byte aligned blocks held in registers. A block is loaded only if it holds a byte of the string, so it never
+ *  leaves the pages of the tape. At most `room` runes are written; returns the count, or `room + 1` if the
+ *  string needed more.
+ */
+__device__ uint64_t decode_walk(uint8_t const *first, uint8_t const *last, uint32_t *out, uint64_t room) {
+    uint64_t written = 0;
+    if (first >= last) return 0;
+    uint8_t const *block_start = (uint8_t const *)((uintptr_t)first & ~(uintptr_t)15);
+    uint4 current = *(uint4 const *)block_start, next = make_uint4(0, 0, 0, 0);
+    if (block_start + 16 < last) next = *(uint4 const *)(block_start + 16);
+    uint8_t const *at = first;
+    while (at < last) {
+        if (written == room) return room + 1;
+        while (at >= block_start + 16) { // slide the pair of blocks forward
+            block_start += 16;
+            current = next;
+            next = make_uint4(0, 0, 0, 0);
+            if (block_start + 16 < last) next = *(uint4 const *)(block_start + 16);
+        }
+        uint32_t const k = (uint32_t)(at - block_start); // 0..15; at + 3 stays within the two blocks
+        auto byte_at = [&](uint32_t d) {
+            uint32_t const j = k + d;
+            return j < 16 ? block_byte(current, j) : block_byte(next, j - 16);
+        };
+        uint32_t const lead = byte_at(0);
+        uint32_t const width = utf8_width(lead);
+        uint32_t const t0 = width > 1 && at + 1 < last ? byte_at(1) : 0u;
+        uint32_t const t1 = width > 2 && at + 2 < last ? byte_at(2) : 0u;
+        uint32_t const t2 = width > 3 && at + 3 < last ? byte_at(3) : 0u;
+        out[written++] = rune_of(lead, width, t0, t1, t2);
+        at += width;
+    }
+    return written;
+}
+
+/**
+ *  The whole wave decodes one string (the same arguments in every lane), 256 bytes per pass, 4 per lane. When the
+ *  string is well formed (every lead byte followed by exactly its continuation bytes, no stray continuation
+ *  byte; a sequence cut by the end of the string is allowed), the walk above visits exactly the bytes that are
+ *  not continuation bytes, so a rune's index is the count of such bytes before it: a wave prefix sum. Returns
+ *  the rune count, or -1 if some byte breaks the rule (the caller then walks the string in one lane). The output
+ *  must have room for one rune per byte.
+ */
+__device__ int64_t decode_wave(uint8_t const *first, uint8_t const *last, uint32_t *out) {
+    uint32_t const lane = __lane_id();
+    int64_t const length = last - first;
+    int64_t written = 0;
+    uint32_t carry = 0; // the previous pass's last 4 bytes (zero bytes before the string: ASCII, cover nothing)
+    for (int64_t base = 0; base < length; base += 256) {
+        int64_t const mine = base + 4 * (int64_t)lane;
+        int32_t const valid = (int32_t)max((int64_t)0, min((int64_t)4, length - mine));
+        uint32_t word = 0;
+        for (int32_t i = 0; i < valid; ++i) word |= (uint32_t)first[mine + i] << (8 * i);
+        uint32_t const previous_lane = (uint32_t)__shfl_up((int)word, 1, 64);
+        uint32_t const before = lane == 0 ? carry : previous_lane;   // bytes mine-4 .. mine-1
+        uint32_t following = (uint32_t)__shfl_down((int)word, 1, 64); // bytes mine+4 .. mine+7
+        if (lane == 63) {
+            following = 0;
+            for (int32_t i = 0; i < 3; ++i)
+                if (mine + 4 + i < length) following |= (uint32_t)first[mine + 4 + i] << (8 * i);
+        }
+        // Continuation bytes still owed by the last lead among the 3 bytes before this lane.
+        int32_t need = 0;
+        for (int32_t d = 3; d >= 1; --d) {
+            uint32_t const byte = (before >> (8 * (4 - d))) & 0xFFu;
+            if ((byte & 0xC0u) != 0x80u) need = max(0, (int32_t)utf8_width(byte) - d);
+        }
+        bool broken = false;
+        uint32_t leads = 0;
+        for (int32_t i = 0; i < valid; ++i) {
+            uint32_t const byte = (word >> (8 * i)) & 0xFFu;
+            bool const continuation = (byte & 0xC0u) == 0x80u;
+            if (need > 0) {
+                broken |= !continuation;
+                --need;
+            }
+            else if (continuation) broken = true;
+            else {
+                leads |= 1u << i;
+                need = (int32_t)utf8_width(byte) - 1;
+            }
+        }
+        if (__ballot(broken)) return -1;
+        int32_t const count = __builtin_popcount(leads);
+        int32_t inclusive = count;
+        for (int32_t d = 1; d < 64; d <<= 1) {
+            int32_t const other = __shfl_up(inclusive, d, 64);
+            if ((int32_t)lane >= d) inclusive += other;
+        }
+        int64_t rank = written + inclusive - count;
+        uint64_t const window = (uint64_t)following << 32 | word; // 8 bytes from `mine`
+        for (int32_t i = 0; i < valid; ++i) {
+            if (!((leads >> i) & 1u)) continue;
+            uint32_t const lead = (uint32_t)(window >> (8 * i)) & 0xFFu;
+            uint32_t const width = utf8_width(lead);
+            auto tail = [&](int32_t d) { // continuation byte d of this lead, 0 past the end of the string
+                return mine + i + d < length ? (uint32_t)(window >> (8 * (i + d))) & 0xFFu : 0u;
+            };
+            out[rank++] = rune_of(lead, width, width > 1 ? tail(1) : 0u, width > 2 ? tail(2) : 0u,
+                                  width > 3 ? tail(3) : 0u);
+        }
+        written += __shfl(inclusive, 63, 64);
+        carry = (uint32_t)__shfl((int)word, 63, 64);
+    }
+    return written;
+}
+
+/**
+ *  UTF-8 to runes for every tape of a batch in one launch, with `sz_rune_decode_unchecked` semantics
+ *  (include/stringzilla/utf8_runes/serial.h:101-114): the lead byte fixes the width, continuation bytes are not
+ *  validated, and continuation bytes missing at the end of a string read as 0. Runes of string `s` start at
+ *  index `offsets[s]` of the tape's `runes`; their count goes to `rune_counts[s]`.
  *
- *  The walk reads the string through two 16-byte aligned blocks held in registers instead of one dependent byte
- *  load per rune. A block is loaded only if it holds a byte of the string, so it never leaves the pages of the
- *  tape. Runes never go past the buffer's capacity: a tape whose offsets run further than the byte count the
- *  host sized it for keeps its strings truncated there and raises its segment's flag.
+ *  Each wave takes 64 strings: the short ones are walked one per lane, then the whole wave decodes each long one
+ *  (`decode_wave`, falling back to a one-lane walk for malformed strings). Runes never go past the buffer's
+ *  capacity: a tape whose offsets run further than the byte count the host sized it for keeps its strings
+ *  truncated there and raises its segment's flag.
  */
 __global__ void utf8_decode_kernel(utf8_tape_t const *tapes, uint32_t tapes_count, uint64_t strings_total,
                                    prepare_stats_t *stats, uint32_t *segment_flags) {
-    for (uint64_t global = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; global < strings_total;
-         global += (uint64_t)gridDim.x * blockDim.x) {
+    constexpr int64_t long_string_k = 256;
+    uint32_t const lane = __lane_id();
+    uint64_t const waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+    uint64_t const wave = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    for (uint64_t base = wave * 64; base < strings_total; base += waves * 64) {
+        uint64_t const global = base + lane;
+        bool const mine = global < strings_total;
         uint32_t low = 0, high = tapes_count - 1;
-        while (low < high) {
+        while (mine && low < high) {
             uint32_t const middle = (low + high + 1) / 2;
             if (tapes[middle].string_base <= global) low = middle;
             else high = middle - 1;
         }
         utf8_tape_t const &tape = tapes[low];
-        uint64_t const s = global - tape.string_base;
-        uint64_t const begin = tape.offsets[s], end = tape.offsets[s + 1];
-        uint8_t const *const first = tape.bytes + begin, *const last = tape.bytes + end;
-        uint32_t *const out = tape.runes + begin;
+        uint64_t const s = mine ? global - tape.string_base : 0;
+        uint64_t const begin = mine ? tape.offsets[s] : 0, end = mine ? tape.offsets[s + 1] : 0;
         uint64_t const room = begin < tape.capacity ? tape.capacity - begin : 0;
+        bool const is_long = mine && (int64_t)(end - begin) >= long_string_k && room >= end - begin;
         uint64_t written = 0;
-        if (first < last) {
-            uint8_t const *block_start = (uint8_t const *)((uintptr_t)first & ~(uintptr_t)15);
-            uint4 current = *(uint4 const *)block_start, next = make_uint4(0, 0, 0, 0);
-            if (block_start + 16 < last) next = *(uint4 const *)(block_start + 16);
-            uint8_t const *at = first;
-            while (at < last) {
-                if (written == room) { // the offsets disagree with the byte count the buffer was sized for
-                    atomicOr(&stats->bad_tape, 1u);
-                    atomicOr(&segment_flags[tape.segment], (uint32_t)segment_bad_tape_k);
-                    break;
-                }
-                while (at >= block_start + 16) { // slide the pair of blocks forward
-                    block_start += 16;
-                    current = next;
-                    next = make_uint4(0, 0, 0, 0);
-                    if (block_start + 16 < last) next = *(uint4 const *)(block_start + 16);
-                }
-                uint32_t const k = (uint32_t)(at - block_start); // 0..15; at + 3 stays within the two blocks
-                auto byte_at = [&](uint32_t d) {
-                    uint32_t const j = k + d;
-                    return j < 16 ? block_byte(current, j) : block_byte(next, j - 16);
-                };
-                uint32_t const lead = byte_at(0);
-                uint32_t const width = 1 + (lead >= 0xC0) + (lead >= 0xE0) + (lead >= 0xF0);
-                uint32_t t0 = 0, t1 = 0, t2 = 0;
-                if (width > 1 && at + 1 < last) t0 = byte_at(1) & 0x3Fu;
-                if (width > 2 && at + 2 < last) t1 = byte_at(2) & 0x3Fu;
-                if (width > 3 && at + 3 < last) t2 = byte_at(3) & 0x3Fu;
-                uint32_t rune;
-                if (width == 1) rune = lead;
-                else if (width == 2) rune = (lead & 0x1Fu) << 6 | t0;
-                else if (width == 3) rune = (lead & 0x0Fu) << 12 | t0 << 6 | t1;
-                else rune = (lead & 0x07u) << 18 | t0 << 12 | t1 << 6 | t2;
-                out[written++] = rune;
-                at += width;
-            }
+        if (mine && !is_long) written = decode_walk(tape.bytes + begin, tape.bytes + end, tape.runes + begin, room);
+        uint64_t pending = __ballot(is_long);
+        while (pending) {
+            int const source = __builtin_ctzll(pending);
+            pending &= pending - 1;
+            auto broadcast = [&](auto *pointer) { // a pointer of lane `source`, in every lane
+                uint64_t const bits = (uint64_t)pointer;
+                uint32_t const lo = (uint32_t)__shfl((int)(uint32_t)bits, source, 64);
+                uint32_t const hi = (uint32_t)__shfl((int)(uint32_t)(bits >> 32), source, 64);
+                return (decltype(pointer))((uint64_t)hi << 32 | lo);
+            };
+            uint8_t const *first = broadcast(tape.bytes + begin);
+            uint8_t const *last = broadcast(tape.bytes + end);
+            uint32_t *out = broadcast(tape.runes + begin);
+            int64_t const decoded = decode_wave(first, last, out);
+            if ((int)lane == source)
+                written = decoded >= 0 ? (uint64_t)decoded : decode_walk(first, last, out, room);
         }
-        tape.rune_counts[s] = (uint32_t)written;
+        if (mine) {
+            if (written > room) { // the offsets disagree with the byte count the buffer was sized for
+                written = room;
+                atomicOr(&stats->bad_tape, 1u);
+                atomicOr(&segment_flags[tape.segment], (uint32_t)segment_bad_tape_k);
+            }
+            tape.rune_counts[s] = (uint32_t)written;
+        }
     }
 }
 

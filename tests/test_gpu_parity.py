@@ -187,3 +187,28 @@ def test_unit_levenshtein_cooperative_tall_pair():
     expected = O.pairs(O.Params("levenshtein"), O.tape(a), O.tape(b), threads=16)
     got = szs.LevenshteinDistances().pairs(szs.Strs(a), szs.Strs(b))
     np.testing.assert_array_equal(got.astype(np.int64), expected)
+
+
+@pytest.mark.parametrize("low,high", [(1, 40), (200, 400), (900, 3000)])
+def test_utf8_malformed_and_truncated_input(low, high):
+    """Stray continuation bytes, leads without their continuations and sequences cut by the end of a string:
+    the unchecked decode (utf8_runes/serial.h:101-114) fixes the runes, on the one-lane and whole-wave paths."""
+    rng = np.random.default_rng(low + high)
+    pieces = [b"a", b"Z", b"\xd0\x96", b"\xe4\xb8\x80", b"\xf0\x9f\x8c\x80", b"\x80", b"\xbf", b"\xc3", b"\xe2\x82",
+              b"\xf0\x9f", b"\xff", b"\xfe"]
+
+    def make(clean: bool):
+        size = int(rng.integers(low, high + 1))
+        out = bytearray()
+        while len(out) < size:
+            out += pieces[int(rng.integers(0, 5 if clean else len(pieces)))]
+        return bytes(out[:size])  # the cut may split a sequence at the end
+
+    firsts = [make(i % 3 == 0) for i in range(60)]
+    seconds = [make(i % 4 == 0) for i in range(60)]
+    for params, engine in ((O.Params("levenshtein_utf8"), szs.LevenshteinDistancesUTF8()),
+                           (O.Params("levenshtein_utf8", 0, 4, 3, 2), szs.LevenshteinDistancesUTF8(0, 4, 3, 2))):
+        a, b = szs.Strs(firsts), szs.Strs(seconds)
+        expected = O.pairs(params, (a.data, a.offsets), (b.data, b.offsets), threads=8)
+        np.testing.assert_array_equal(engine.pairs(a, b), expected)
+        np.testing.assert_array_equal(engine.pairs(szs.to_device(a), szs.to_device(b)), expected)
