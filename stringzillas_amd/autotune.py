@@ -80,12 +80,14 @@ def _engine_for(slot: int):
 
 def _batch_for(rng, alphabet, low: int, high: int, target_cells: float, max_pairs: int = 200_000):
     mean_rows = (low + high) / 2
-    mean_cells = mean_rows * mean_rows * 2.5
+    mean_cells = mean_rows * max(mean_rows * 2.5, (16384 - mean_rows) / np.log(16384 / mean_rows))
     # Enough pairs to fill every wave slot of the device (256 CUs x 4 workgroups x 4 waves) twice, as inside a
     # large batch, so shapes are compared at full occupancy.
     pairs = int(min(max_pairs, max(8192, target_cells / mean_cells)))
     rows = rng.integers(low, high + 1, size=pairs)
-    columns = (rows * rng.uniform(1.0, 4.0, size=pairs)).astype(np.int64)
+    # The longer side log-uniform from the rows up to 16,384 (4x the rows at least), like the mixed batch's pairs.
+    top = np.maximum(rows * 4, 16384)
+    columns = np.exp(rng.uniform(np.log(rows), np.log(top))).astype(np.int64)
     a = W._random_tape(rng, alphabet, rows)
     b = W._random_tape(rng, alphabet, columns)
     return szs.Strs.from_tape(*a), szs.Strs.from_tape(*b)
