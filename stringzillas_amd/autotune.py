@@ -124,12 +124,12 @@ def candidates_for(slot: int, low: int, high: int, table) -> List[int]:
         elif bit_parallel:
             if high > 256:  # a partly filled 64-word stripe still scores 64 cells per word operation
                 out.append(index)
-        elif high <= 4 * capacity and (high > capacity // 2 or capacity <= 128):
-            out.append(index)
+        elif high <= 16 * capacity and (high > capacity // 2 or capacity <= 128):
+            out.append(index)  # up to 16 stripes swept one after another by one wave
     return out
 
 
-def tune(slots: List[int] = None, target_cells: float = 4e9, verbose: bool = True) -> Dict:
+def tune(slots: List[int] = None, target_cells: float = 4e9, verbose: bool = True, bands=None) -> Dict:
     import torch
 
     table = shapes()
@@ -143,7 +143,7 @@ def tune(slots: List[int] = None, target_cells: float = 4e9, verbose: bool = Tru
         rng = np.random.default_rng(slot)
         measurements, chosen = {}, []
         low = 1
-        for high in BANDS:
+        for high in bands or BANDS:
             options = candidates_for(slot, low, high, table)
             if not options:
                 low = high + 1
@@ -209,8 +209,11 @@ def apply(path: str = CACHE) -> bool:
 
 
 if __name__ == "__main__":
+    import sys
+
     start = time.time()
-    tuned = tune()
+    chosen_slots = [int(x) for x in sys.argv[1:]] or None
+    tuned = tune(chosen_slots)
     with open(CACHE, "w") as f:
         json.dump(tuned, f, indent=1)
     print(f"wrote {CACHE} in {time.time() - start:.0f}s")
