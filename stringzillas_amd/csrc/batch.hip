@@ -427,7 +427,10 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
         a.family = engine->params.family;
         a.runes = runes;
         a.symbols = views[s].symbols;
-        if (engine->params.family >= family_nw_linear_k && engine->params.shift != engine->params.open)
+        bool const local_family = engine->params.family == family_sw_linear_k ||
+                                  engine->params.family == family_sw_affine_k;
+        int32_t const fast_bias = local_family ? engine->params.open : engine->params.open + engine->params.extend;
+        if (engine->params.family >= family_nw_linear_k && engine->params.shift != fast_bias)
             a.symbols = class_unshifted_k;
         a.mode = input.mode;
         a.open = engine->params.open;

@@ -529,7 +529,10 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
                             a.family == family_sw_affine_k;
         uint64_t const longest = q_length > c_length ? q_length : c_length;
         uint64_t const span = a.family <= family_lev_affine_k ? longest : q_length + c_length;
-        bool const overflows = (span + (affine ? 3 : 1)) * (uint64_t)a.reach_magnitude >= 0x7FFFFFFFull;
+        // The reach bound of serial.hpp:117-140, plus the anti-diagonal offset e*(n+m) that global families
+        // carry in their cells.
+        uint64_t const carried = local ? 0 : q_length + c_length + 2;
+        bool const overflows = (span + (affine ? 3 : 1) + carried) * (uint64_t)a.reach_magnitude >= 0x7FFFFFFFull;
         if (q_length == 0 || c_length == 0) {
             int64_t value = 0;
             if (!local)

@@ -17,9 +17,15 @@
  *  the stripe: the matrix border, or the previous stripe's bottom row read back from HBM scratch. Lane G-1's
  *  bottom row is collected 64 columns at a time (G = 64) and written to scratch for the next stripe.
  *
- *  Scores are kept shifted by S = open (H + S): the gap terms `left + open`, `up + open` and linear
- *  `opt(up, left) + g` then read the stored values directly, and the substitution term reads `cost - S` from the
- *  profile, so each cell pays one add for the shift instead of two for the gaps. With G = 64 lane 0's inputs for a
+ *  Global families keep every value relative to its anti-diagonal: Ĉ = H - e*(i+j), ĥ = E - e*(i+j) + (e - o),
+ *  v̂ = F - e*(i+j) + (e - o) (e = extend, o = open; for linear gaps e = o = g). In that domain the recurrences lose
+ *  their gap constants: ĥ = opt(Ĉ_left, ĥ_left), v̂ = opt(Ĉ_up, v̂_up), Ĉ = opt(ĥ, v̂, Ĉ_diag + s - o - e) + (o - e),
+ *  and linear Ĉ = opt(Ĉ_up, Ĉ_left, Ĉ_diag + s - 2g): 5 and 2 VALU per cell. Each value lives in its own cell's
+ *  domain, so lanes, stripes and the HBM boundary rows exchange them unchanged; the border values enter in the
+ *  same domain and the result leaves it with + e*(n+m). Local families cannot (their zero floor is not constant
+ *  along anti-diagonals) and keep scores shifted by S = open instead: the gap terms read the stored value, the
+ *  substitution term reads `cost - S`, one add per cell stores `cell + S`. The LDS profile holds `cost - bias`
+ *  with the bias of the form used (o + e, 2g or o). With G = 64 lane 0's inputs for a
  *  64-step chunk (row above the stripe, its vertical gaps, column symbols) are staged in a 1 KB LDS ring, one
  *  16-byte entry per step read back as a broadcast, so each step injects them with the DPP shift alone. Chunks in
  *  which every lane has a valid column skip the per-step column check.
@@ -118,9 +124,9 @@ __device__ __forceinline__ int8_t *wave_lds(uint32_t wave, uint32_t bytes) {
 enum reduce_kind_t { reduce_none_k = 0, reduce_max_k = 1, reduce_sum_k = 2 };
 
 /**
- *  Per-lane state of one group sweeping one stripe of one pair. `shifted_` keeps scores as H + open (the profile
- *  then holds `cost - open`); without it S = 0 and the gap terms add `open` explicitly, for engines whose shifted
- *  costs would not fit int8.
+ *  Per-lane state of one group sweeping one stripe of one pair. `shifted_` selects the anti-diagonal domain
+ *  (global families) or the `H + open` shift (local families), whose profiles hold `cost - bias`; without it the
+ *  plain recurrences run with explicit gap adds, for engines whose biased costs would not fit int8.
  */
 template <int family_, typename symbol_type_, int G, int R, bool shifted_ = true>
 struct sweep_t {
@@ -133,14 +139,16 @@ struct sweep_t {
     static constexpr int group_k = G;
     static constexpr int rows_per_lane = R;
     static constexpr reduce_kind_t reduce_k = local_k ? reduce_max_k : reduce_none_k;
+    static constexpr bool diagonal_k = shifted_ && !local_k; /**< values relative to their anti-diagonal */
 
     // The pair.
     int lane, k;
     bool active;
     int32_t n, m;
     int32_t open, extend, match, mismatch, dense_count;
-    int32_t shift;                  /**< S: stored scores are H + S */
+    int32_t shift;                  /**< cost bias of the profile and of `match`/`mismatch` below */
     int32_t match_shifted, mismatch_shifted;
+    int32_t open_minus_extend;
     bool transposed;
     engine_params_t const *params;
     symbol_type_ const *rows;
@@ -175,6 +183,7 @@ struct sweep_t {
         dense_count = params->dense_count;
         shift = shifted_ ? params->shift : 0;
         match_shifted = match - shift, mismatch_shifted = mismatch - shift;
+        open_minus_extend = open - extend;
         transposed = task.transposed != 0;
         symbol_type_ const *query = (symbol_type_ const *)segment.query_symbols;
         symbol_type_ const *candidate = (symbol_type_ const *)segment.candidate_symbols;
@@ -188,6 +197,18 @@ struct sweep_t {
         running_best = 0; // serial.hpp:781 starts the local best at 0
     }
 
+    /** Offset between a stored score and H at cell (i, j): -e*(i+j), S = open (local families) or 0. */
+    __device__ __forceinline__ int32_t domain(int32_t i, int32_t j) const {
+        if constexpr (diagonal_k) return -extend * (i + j);
+        else return shift;
+    }
+
+    /** Offset between a stored gap value and its track at cell (i, j). */
+    __device__ __forceinline__ int32_t gap_domain(int32_t i, int32_t j) const {
+        if constexpr (diagonal_k) return -extend * (i + j) + (extend - open);
+        else return 0;
+    }
+
     /** Loads the rows `base + k*R .. + R` of this lane and resets the stripe state to the column-0 border. */
     __device__ __forceinline__ void begin_stripe(int32_t base) {
         lane_top = base + k * R;
@@ -198,8 +219,9 @@ struct sweep_t {
             if (i <= n) symbol = (uint32_t)rows[i - 1];
             row_symbols[r] = symbol;
             int32_t const border = border_score<family_>(i, open, extend);
-            scores[r] = border + shift;
-            horizontal[r] = (open + extend) + border; // discard value of the horizontal track on column 0
+            scores[r] = border + domain(i, 0);
+            // The horizontal track starts from its discard value on column 0.
+            horizontal[r] = (open + extend) + border + gap_domain(i, 0);
             best[r] = 0;
         }
         if constexpr (!uniform_k) {
@@ -215,13 +237,14 @@ struct sweep_t {
                 }
             }
         }
-        diagonal_top = border_score<family_>(lane_top, open, extend) + shift; // H[top - 1][0] + S
+        diagonal_top = border_score<family_>(lane_top, open, extend) + domain(lane_top, 0); // the cell above-left
         bottom_score = scores[R - 1];
         bottom_gap = 0;
         column_symbol = 0;
     }
 
-    /** Lane-0 inputs of column `j` (1-based): the row above the stripe (H + S), its vertical gap, the symbol. */
+    /** Lane-0 inputs of column `j` (1-based): the row above the stripe and its vertical gap (stored form), and
+     *  the column symbol. */
     __device__ __forceinline__ void column_inputs(int32_t j, int32_t const *in_scores, int32_t const *in_gaps,
                                                   int32_t &score, int32_t &gap, uint32_t &symbol) const {
         symbol = 0;
@@ -235,8 +258,8 @@ struct sweep_t {
         }
         else {
             int32_t const border = border_score<family_>(j, open, extend);
-            score = border + shift;
-            gap = (open + extend) + border; // discard value of the vertical track on row 0
+            score = border + domain(0, j);
+            gap = (open + extend) + border + gap_domain(0, j); // discard value of the vertical track on row 0
         }
     }
 
@@ -253,30 +276,43 @@ struct sweep_t {
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            int32_t cost; // cost - S
+            int32_t cost; // cost - bias
             if constexpr (uniform_k) cost = row_symbols[r] == column_symbol ? match_shifted : mismatch_shifted;
             else cost = (int32_t)(int8_t)(costs_packed[r / 4] >> (8 * (r % 4)));
-            int32_t const left = scores[r];                      // H[i][j-1] + S
-            int32_t const if_substitution = diagonal + cost;     // H[i-1][j-1] + s(i, j), unshifted
-            int32_t cell;                                        // H[i][j], unshifted
-            if constexpr (affine_k) {
-                int32_t const open_left = shifted_ ? left : left + open; // H[i][j-1] + open
-                int32_t const open_up = shifted_ ? up : up + open;
-                int32_t const h = opt<minimize_k>(open_left, horizontal[r] + extend);
-                int32_t const v = opt<minimize_k>(open_up, vertical + extend);
-                horizontal[r] = h;
-                vertical = v;
-                if constexpr (local_k) cell = max(max(h, v), max(if_substitution, 0));
-                else cell = opt<minimize_k>(opt<minimize_k>(h, v), if_substitution);
+            int32_t const left = scores[r];
+            int32_t const if_substitution = diagonal + cost;
+            int32_t stored;
+            if constexpr (diagonal_k) {
+                if constexpr (affine_k) {
+                    int32_t const h = opt<minimize_k>(left, horizontal[r]);
+                    int32_t const v = opt<minimize_k>(up, vertical);
+                    horizontal[r] = h;
+                    vertical = v;
+                    stored = opt<minimize_k>(opt<minimize_k>(h, v), if_substitution) + open_minus_extend;
+                }
+                else stored = opt<minimize_k>(opt<minimize_k>(up, left), if_substitution);
             }
             else {
-                int32_t gap = opt<minimize_k>(up, left); // + g, folded into the stored scores when shifted
-                if constexpr (!shifted_) gap += open;
-                if constexpr (local_k) cell = max(gap, max(if_substitution, 0));
-                else cell = opt<minimize_k>(gap, if_substitution);
+                int32_t cell; // H[i][j]; `left`, `up` and `diagonal` hold H + S, `if_substitution` is unshifted
+                if constexpr (affine_k) {
+                    int32_t const open_left = shifted_ ? left : left + open; // H[i][j-1] + open
+                    int32_t const open_up = shifted_ ? up : up + open;
+                    int32_t const h = opt<minimize_k>(open_left, horizontal[r] + extend);
+                    int32_t const v = opt<minimize_k>(open_up, vertical + extend);
+                    horizontal[r] = h;
+                    vertical = v;
+                    if constexpr (local_k) cell = max(max(h, v), max(if_substitution, 0));
+                    else cell = opt<minimize_k>(opt<minimize_k>(h, v), if_substitution);
+                }
+                else {
+                    int32_t gap = opt<minimize_k>(up, left); // + g, folded into the stored scores when shifted
+                    if constexpr (!shifted_) gap += open;
+                    if constexpr (local_k) cell = max(gap, max(if_substitution, 0));
+                    else cell = opt<minimize_k>(gap, if_substitution);
+                }
+                if constexpr (local_k) best[r] = max(best[r], cell);
+                stored = cell + shift;
             }
-            if constexpr (local_k) best[r] = max(best[r], cell);
-            int32_t const stored = cell + shift;
             diagonal = left;
             up = stored;
             scores[r] = stored;
@@ -362,7 +398,7 @@ struct sweep_t {
                 int32_t value = 0;
 #pragma unroll
                 for (int r = 0; r < R; ++r) value += scores[r] & -(int32_t)(wanted == r);
-                value -= shift;
+                value -= domain(n, m);
                 results[output] = (int64_t)value;
                 if (mirror != no_mirror_k) results[mirror] = (int64_t)value;
             }

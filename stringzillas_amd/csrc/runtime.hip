@@ -243,12 +243,16 @@ sz_status_t make_engine(int kind, int match, int mismatch, int open, int extend,
         if (kind == szs_engine_needleman_wunsch_k) engine->params.family = linear ? family_nw_linear_k : family_nw_affine_k;
         else engine->params.family = linear ? family_sw_linear_k : family_sw_affine_k;
     }
-    // The LDS profile holds `cost - shift` as int8; fall back to no shift when some used cost would not fit.
-    engine->params.shift = open;
+    // The sweep's cost bias: o + e (affine global, anti-diagonal domain), 2g (linear global), o (local, scores
+    // shifted by the open cost). The LDS profile holds `cost - bias` as int8; engines with a used cost that would
+    // not fit fall back to the plain recurrences (bias 0).
+    int const family = engine->params.family;
+    bool const local = family == family_sw_linear_k || family == family_sw_affine_k;
+    engine->params.shift = local ? open : open + extend; // 2g when open == extend
     for (int x = 0; x < engine->params.dense_count; ++x)
         for (int y = 0; y < engine->params.dense_count; ++y) {
-            int const shifted = engine->params.dense_costs[x * max_dense_classes_k + y] - open;
-            if (shifted < -128 || shifted > 127) engine->params.shift = 0;
+            int const biased = engine->params.dense_costs[x * max_dense_classes_k + y] - engine->params.shift;
+            if (biased < -128 || biased > 127) engine->params.shift = 0;
         }
     *handle = engine;
     return ok(error_message);

@@ -41,8 +41,9 @@ struct engine_params_t {
     int32_t open, extend;     /**< gap costs (sign as given: positive for distances, negative for scores) */
     int32_t match, mismatch;  /**< uniform substitution costs */
     int32_t dense_count;      /**< number of distinct classes actually used by `byte_to_class` */
-    /** Offset S the sweep adds to every stored score: scores kept as H + S let one add serve both gap uses of a
-     *  cell. `open` when every `dense_costs - open` fits int8 (and always for uniform costs), else 0. */
+    /** Cost bias of the sweep's fast forms: `open + extend` for global families (their values live relative to
+     *  the anti-diagonal), `open` for local ones (scores kept as H + open). The LDS profile holds
+     *  `dense_costs - shift` as int8; 0 when some used cost would not fit (the plain recurrences then run). */
     int32_t shift;
     uint8_t byte_to_dense[256];
     int8_t dense_costs[max_dense_classes_k * max_dense_classes_k]; /**< [dense(query)][dense(candidate)] */
