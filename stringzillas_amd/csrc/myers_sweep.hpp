@@ -12,10 +12,11 @@
  *  or the previous stripe's bottom deltas from HBM scratch. The match vector of a column symbol is built from
  *  bit-planes: plane b holds bit b of every row's symbol code, and Eq = AND_b (plane_b XNOR bit b of the column
  *  code), one `v_bitop3` per plane and half-word. For 64-lane groups the symbols are first renumbered to a dense
- *  code (0 = absent from the rows) so that only ceil(log2(distinct + 1)) planes are needed, e.g. 3 for DNA.
- *  With at most 15 distinct symbols in the rows, each stripe instead writes the match word of every code to an
- *  8 KB LDS table ([16 codes][64 lanes] u64), and a step reads its word back (prefetched one step ahead) in
- *  place of the plane arithmetic. For 64-lane groups lane 0's inputs of a 64-step chunk (horizontal deltas from
+ *  code (0 = absent from the rows) so that only ceil(log2(distinct + 1)) planes are needed, e.g. 3 for DNA; every
+ *  group renumbers its own pair (4- and 16-lane groups compute codes from a presence set and its prefix counts).
+ *  When every group of the wave has at most 15 distinct symbols, each stripe instead writes the match word of
+ *  every code to an 8 KB LDS table ([16 codes][lanes] u64 per group), and a step reads its word back (for 64-lane
+ *  groups prefetched one step ahead) in place of the plane arithmetic. For 64-lane groups lane 0's inputs of a 64-step chunk (horizontal deltas from
  *  the stripe above, column codes) are staged in the LDS ring and read back as one broadcast entry per step.
  *  The distance is m + the sum of the vertical deltas of the last column over all rows.
  */
@@ -49,6 +50,13 @@ __device__ __forceinline__ uint32_t group_rotate_up(uint32_t src) {
     else return (uint32_t)rotate_up<G>((int32_t)src);
 }
 
+/** LDS a smaller group uses to renumber its pair's symbols: codes are computed from these on demand. */
+struct myers_group_tables_t {
+    uint32_t presence[8]; /**< 256-bit set of the byte values in the rows string */
+    uint32_t below[8];    /**< values present in the words before each word */
+};
+static_assert(sizeof(myers_group_tables_t) * 16 <= myers_tables_bytes_k, "16 four-lane groups per wave");
+
 /** LDS a 64-lane Myers group uses to renumber its pair's symbols. */
 struct myers_tables_t {
     uint32_t presence[8]; /**< 256-bit set of the byte values in the rows string */
@@ -61,7 +69,7 @@ struct myers_sweep_t {
     static constexpr int group_k = G;
     static constexpr int rows_per_lane = 64;
     static constexpr bool local_k = false;
-    static constexpr bool renumber_k = G == 64;
+    static constexpr bool renumber_k = G == 64; /**< one pair per wave: a full byte -> code table */
     static constexpr reduce_kind_t reduce_k = reduce_sum_k;
 
     int lane, k;
@@ -71,7 +79,9 @@ struct myers_sweep_t {
     int64_t *results;
     uint64_t output, mirror;
     myers_tables_t *tables;
-    uint64_t *eq_table;  /**< [16 codes][64 lanes] match words of the current stripe (table mode) */
+    myers_group_tables_t *group_tables; /**< this lane's group (G < 64) */
+    int32_t distinct;    /**< distinct byte values in the rows */
+    uint64_t *eq_table;  /**< [16 codes][G lanes] match words of this group's current stripe (table mode) */
     int8_t *ring;        /**< lane-0 inputs of a 64-step chunk (G = 64) */
     int32_t planes_used;
     bool table_mode;     /**< wave-uniform: at most 15 distinct symbols, match words come from `eq_table` */
@@ -84,7 +94,12 @@ struct myers_sweep_t {
 
     __device__ __forceinline__ uint32_t code_of(uint32_t byte) const {
         if constexpr (renumber_k) return tables->codes[byte];
-        else return byte;
+        else {
+            if (distinct == 256) return byte; // no absent code left: bytes stay as they are
+            uint32_t const word = group_tables->presence[byte >> 5], bit = byte & 31u;
+            if (!((word >> bit) & 1u)) return 0u;
+            return group_tables->below[byte >> 5] + __builtin_popcount(word & ((1u << bit) - 1u)) + 1u;
+        }
     }
 
     __device__ __forceinline__ void load(item_args_t const &a, uint32_t task_index, bool is_active, uint32_t wave) {
@@ -105,7 +120,10 @@ struct myers_sweep_t {
         output = task.output;
         mirror = task.mirror;
         tables = wave_tables;
-        eq_table = reinterpret_cast<uint64_t *>(wave_lds(wave, a.wave_lds_bytes) + myers_tables_bytes_k);
+        uint32_t const group = (uint32_t)lane / G;
+        group_tables = reinterpret_cast<myers_group_tables_t *>(wave_lds(wave, a.wave_lds_bytes)) + group;
+        eq_table = reinterpret_cast<uint64_t *>(wave_lds(wave, a.wave_lds_bytes) + myers_tables_bytes_k) +
+                   group * (myers_eq_codes_k * G);
         ring = wave_lds(wave, a.wave_lds_bytes) + (a.wave_lds_bytes - inject_ring_bytes_k);
         vertical_sum = 0;
         planes_used = 8;
@@ -140,7 +158,31 @@ struct myers_sweep_t {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             int32_t const largest = distinct == 256 ? 255 : distinct;
             planes_used = largest ? 32 - __builtin_clz((uint32_t)largest) : 1;
+            this->distinct = distinct;
             table_mode = distinct < myers_eq_codes_k && a.wave_lds_bytes >= myers_lds_bytes_k;
+        }
+        else {
+            // Several pairs per wave: each group collects its rows' byte values and keeps the per-word prefix
+            // counts; codes 1..D follow byte order, as for the 64-lane groups.
+            for (int w = k; w < 8; w += G) group_tables->presence[w] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            for (int32_t i = k; i < n; i += G) {
+                uint32_t const byte = rows[i];
+                atomicOr(&group_tables->presence[byte >> 5], 1u << (byte & 31));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            int32_t count = 0;
+            for (int w = 0; w < 8; ++w) {
+                uint32_t const word = group_tables->presence[w];
+                if ((w % G) == k) group_tables->below[w] = (uint32_t)count;
+                count += __builtin_popcount(word);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            distinct = count;
+            int32_t const largest = distinct == 256 ? 255 : distinct;
+            planes_used = largest ? 32 - __builtin_clz((uint32_t)largest) : 1;
+            bool const wide = distinct >= myers_eq_codes_k;
+            table_mode = __ballot(wide) == 0 && a.wave_lds_bytes >= myers_lds_bytes_k;
         }
     }
 
@@ -160,7 +202,7 @@ struct myers_sweep_t {
         column_code = 0;
         if (table_mode) { // the match word of every code for this lane's 64 rows
             for (uint32_t code = 0; code < (uint32_t)myers_eq_codes_k; ++code)
-                eq_table[code * 64 + lane] = planes_eq(code);
+                eq_table[code * G + k] = planes_eq(code);
         }
     }
 
@@ -207,7 +249,8 @@ struct myers_sweep_t {
             if (table_mode) chunk_64<steady_, collect_, true>(chunk, in_codes, out_codes);
             else chunk_64<steady_, collect_, false>(chunk, in_codes, out_codes);
         }
-        else chunk_small(chunk, in_codes);
+        else if (table_mode) chunk_small<true>(chunk, in_codes);
+        else chunk_small<false>(chunk, in_codes);
     }
 
     template <bool steady_, bool collect_, bool table_>
@@ -244,6 +287,7 @@ struct myers_sweep_t {
         }
     }
 
+    template <bool table_>
     __device__ __forceinline__ void chunk_small(int32_t chunk, int32_t const *in_codes) {
         int32_t const first_step = chunk * G;
         int32_t const j_lane = first_step + k + 1;
@@ -257,7 +301,8 @@ struct myers_sweep_t {
             chunk_codes = group_rotate_up<G>(chunk_codes);
             chunk_symbols = group_rotate_up<G>(chunk_symbols);
             int32_t const j = first_step + step + 1 - k;
-            if ((uint32_t)(j - 1) < (uint32_t)m) step_word(planes_eq(column_code), hin);
+            if ((uint32_t)(j - 1) < (uint32_t)m)
+                step_word(table_ ? eq_table[column_code * G + k] : planes_eq(column_code), hin);
         }
     }
 

@@ -164,8 +164,9 @@ struct materialize_args_t {
 /** Bytes of the per-wave LDS ring that stages lane 0's inputs of a 64-step chunk (64 entries of 16 bytes); it
  *  follows the wave's profile or Myers tables in dynamic LDS. */
 constexpr uint32_t inject_ring_bytes_k = 64 * 16;
-/** Myers LDS per wave: renumbering tables, then a [16 codes][64 lanes] u64 match table, then the ring. */
-constexpr uint32_t myers_tables_bytes_k = 512;
+/** Myers LDS per wave: renumbering tables (one set for a 64-lane group, a 64-byte set per smaller group), then a
+ *  [16 codes][64 lanes] u64 match table (split among the groups), then the ring. */
+constexpr uint32_t myers_tables_bytes_k = 1024;
 constexpr int32_t myers_eq_codes_k = 16;
 constexpr uint32_t myers_lds_bytes_k = myers_tables_bytes_k + myers_eq_codes_k * 64 * 8 + inject_ring_bytes_k;
 
