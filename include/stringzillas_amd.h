@@ -351,6 +351,20 @@ sz_status_t szs_server_get_stats(szs_server_t server, szs_server_stats_t *stats)
 void szs_server_free(szs_server_t server);
 
 /**
+ *  On-disk tapes (no reference counterpart): a 64-byte header ("SZSTAPE1", u64 count, u64 bytes), then u64
+ *  offsets[count + 1] and the u8 data, each section 64-byte aligned; the two sections are an Arrow `large_binary`
+ *  array's buffers. `szs_tape_load` maps the file and streams it into HBM through page-locked staging; the view
+ *  it fills holds device pointers that every scoring call and the server take in place.
+ */
+typedef void *szs_device_tape_t;
+sz_status_t szs_tape_save(char const *path, sz_sequence_u64tape_t const *tape, char const **error_message);
+sz_status_t szs_tape_load(szs_device_scope_t device, char const *path, szs_device_tape_t *tape,
+                          sz_sequence_u64tape_t *view, char const **error_message);
+/** Bytes of string data of a loaded tape (`offsets[count]`), e.g. for `szs_pairs_segment_t.firsts_bytes`. */
+sz_size_t szs_tape_bytes(szs_device_tape_t tape);
+void szs_tape_free(szs_device_tape_t tape);
+
+/**
  *  Tuning (no reference counterpart): the kernel shape chosen for a pair depends on its rows (the shorter side)
  *  through a per-slot ladder of (shape, most rows) steps plus a shape for anything taller. Slots are
  *  `family * 2 + symbols` for the six recurrence families x {bytes, runes}, plus the unit-cost byte slot last.

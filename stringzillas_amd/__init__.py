@@ -141,6 +141,68 @@ def to_device(strs: Strs, device: int = 0) -> DeviceStrs:
     return DeviceStrs(strs if isinstance(strs, Strs) else Strs(strs), device)
 
 
+TAPE_MAGIC = b"SZSTAPE1"
+
+
+def save_tape(strs, path: str) -> None:
+    """Writes a tape file: 64-byte header, u64 offsets, u8 data (the layout of an Arrow ``large_binary`` array)."""
+    tape = _as_tape(strs, "strs")
+    if not isinstance(tape, Strs):
+        raise TypeError("save_tape takes host strings (Strs, a pyarrow array or a list)")
+    error = ctypes.c_char_p()
+    view = tape.tape()
+    _check(_lib.szs_tape_save(path.encode(), ctypes.byref(view), ctypes.byref(error)), error, "Saving a tape")
+
+
+def read_tape(path: str) -> Strs:
+    """Maps a tape file into host memory (no copy): the same layout `save_tape` and `load_tape` use."""
+    raw = np.memmap(path, dtype=np.uint8, mode="r")
+    if raw.size < 72 or raw[:8].tobytes() != TAPE_MAGIC:
+        raise ValueError(f"{path} is not a tape file")
+    count, size = (int(x) for x in raw[8:24].view(np.uint64))
+    data_start = (64 + (count + 1) * 8 + 63) // 64 * 64
+    if data_start + size > raw.size:
+        raise ValueError(f"{path} is shorter than its header says")
+    offsets = raw[64:64 + (count + 1) * 8].view(np.uint64)
+    if int(offsets[0]) != 0 or int(offsets[-1]) != size or (np.diff(offsets.astype(np.int64)) < 0).any():
+        raise ValueError(f"{path} has offsets that do not span its data")
+    strs = Strs.__new__(Strs)
+    strs.data, strs.offsets, strs._keep = raw[data_start:data_start + size], offsets, raw
+    return strs
+
+
+class DeviceTape:
+    """A tape file streamed into HBM by the library (``load_tape``); scored in place like ``DeviceStrs``."""
+
+    def __init__(self, path: str, device: Optional[DeviceScope] = None):
+        self.scope = _scope(device)
+        handle, error = ctypes.c_void_p(), ctypes.c_char_p()
+        self._view = _U64Tape()
+        _check(_lib.szs_tape_load(self.scope.handle, path.encode(), ctypes.byref(handle), ctypes.byref(self._view),
+                                  ctypes.byref(error)), error, "Loading a tape")
+        self.handle = handle
+
+    def __len__(self) -> int:
+        return int(self._view.count)
+
+    @property
+    def nbytes(self) -> int:
+        return int(_lib.szs_tape_bytes(self.handle))
+
+    def tape(self) -> _U64Tape:
+        return self._view
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.szs_tape_free(self.handle)
+            self.handle = None
+
+
+def load_tape(path: str, device: Optional[DeviceScope] = None) -> DeviceTape:
+    """Streams a tape file into HBM without reading it whole into host memory."""
+    return DeviceTape(path, device)
+
+
 def _as_tape(obj, name: str):
     if isinstance(obj, (Strs, DeviceStrs)) or (hasattr(obj, "tape") and hasattr(obj, "__len__")):
         return obj
@@ -500,5 +562,6 @@ _apply_cached_tuning()
 __all__ = [
     "Strs", "DeviceStrs", "to_device", "DeviceScope", "LevenshteinDistances", "LevenshteinDistancesUTF8",
     "NeedlemanWunschScores", "SmithWatermanScores", "Batch", "Server", "reset_capabilities", "SzsError",
+    "save_tape", "read_tape", "load_tape", "DeviceTape",
     "BATCH_PERSISTENT", "BATCH_GRAPH", "BATCH_PER_PAIR", "__capabilities__", "__version__",
 ]

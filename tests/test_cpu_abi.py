@@ -148,3 +148,27 @@ def test_tuning_ladders_validate_and_round_trip():
     finally:
         for slot, (ladder, beyond) in saved.items():
             autotune.set_ladder(slot, ladder, beyond)
+
+
+def test_tape_files_round_trip_and_reject_damage(tmp_path):
+    pa = pytest.importorskip("pyarrow")
+    strings = [b"", b"ACGT" * 40, "Ж中😀".encode(), b"x"]
+    path = str(tmp_path / "strings.tape")
+    szs.save_tape(szs.Strs(strings), path)
+    back = szs.read_tape(path)
+    assert [back[i] for i in range(len(back))] == strings
+    # The sections are an Arrow large_binary array's buffers, and a slice saves as a standalone tape.
+    arrow = pa.array(strings, type=pa.large_binary()).slice(1, 2)
+    szs.save_tape(arrow, path)
+    assert [szs.read_tape(path)[i] for i in range(2)] == strings[1:3]
+    raw = bytearray(open(path, "rb").read())
+    raw[0] ^= 1
+    open(path, "wb").write(bytes(raw))
+    with pytest.raises(ValueError):
+        szs.read_tape(path)
+    if not gpu_available():
+        with pytest.raises(ValueError):  # the native loader validates before it touches a device... or fails on it
+            try:
+                szs.load_tape(path)
+            except RuntimeError as error:  # no GPU in this container
+                raise ValueError(str(error))
