@@ -97,6 +97,9 @@ def _oracle():
         _lib.szo_pairs.restype = ctypes.c_int
         _lib.szo_cross.restype = ctypes.c_int
         _lib.szo_utf8_decode.restype = ctypes.c_size_t
+        _lib.szo_splitmix64.restype = ctypes.c_uint64
+        _lib.szo_fingerprint_hashers.restype = ctypes.c_int
+        _lib.szo_fingerprints.restype = ctypes.c_int
     return _lib
 
 
@@ -177,3 +180,79 @@ def reference_pairs(params: Params, a_tape, b_tape, threads: int = 1, variant: s
     if status != 0:
         raise RuntimeError(f"reference scorer failed with status {status}")
     return out
+
+
+# --- rolling fingerprints -------------------------------------------------------------------------------------
+
+DEFAULT_WINDOW_WIDTHS = (3, 4, 5, 7, 9, 11, 15, 31)
+
+
+class Hasher(ctypes.Structure):
+    """One dimension of the reference's f64 rolling hasher (fingerprints/serial.hpp:364-431)."""
+
+    _fields_ = [
+        ("multiplier", ctypes.c_double),
+        ("modulo", ctypes.c_double),
+        ("inverse_modulo", ctypes.c_double),
+        ("negative_discarding_multiplier", ctypes.c_double),
+        ("discarding_multiplier", ctypes.c_double),
+        ("window_width", ctypes.c_uint64),
+    ]
+
+
+def splitmix64(state: int) -> int:
+    return int(_oracle().szo_splitmix64(ctypes.c_uint64(state)))
+
+
+def fingerprint_hashers(ndim: int, window_widths=None, seed: int = 0):
+    """Per-dimension hashers in output order and whether the sliced layout applies."""
+    hashers = (Hasher * max(ndim, 1))()
+    widths = None if window_widths is None else np.ascontiguousarray(window_widths, dtype=np.uint64)
+    sliced = _oracle().szo_fingerprint_hashers(ctypes.c_uint64(ndim), None if widths is None else _ptr(widths),
+                                               ctypes.c_uint64(0 if widths is None else len(widths)),
+                                               ctypes.c_uint64(seed), hashers)
+    return hashers, bool(sliced)
+
+
+def fingerprints(texts_tape, ndim: int, window_widths=None, seed: int = 0, threads: int = 1):
+    """(hashes, counts), two uint32 matrices of shape (texts, ndim), from the oracle restatement."""
+    data, offsets = texts_tape
+    count = len(offsets) - 1
+    hashers, _ = fingerprint_hashers(ndim, window_widths, seed)
+    hashes = np.zeros((count, ndim), dtype=np.uint32)
+    counts = np.zeros((count, ndim), dtype=np.uint32)
+    if count and ndim:
+        _oracle().szo_fingerprints(hashers, ctypes.c_uint64(ndim), _ptr(data), _ptr(offsets), ctypes.c_uint64(count),
+                                   _ptr(hashes), ctypes.c_uint64(ndim * 4), _ptr(counts), ctypes.c_uint64(ndim * 4),
+                                   ctypes.c_int(threads))
+    return hashes, counts
+
+
+def reference_fingerprints_available(variant: str = "serial") -> bool:
+    if variant == "skylake" and not _cpu_has_icelake_features():
+        return False
+    return os.path.exists(os.path.join(HERE, "_ref", f"libszref_fp_{variant}.so"))
+
+
+def reference_fingerprints(texts_tape, ndim: int, window_widths=None, seed: int = 0, alphabet_size: int = 256,
+                           per_dimension: bool = False, threads: int = 1, variant: str = "serial"):
+    """(hashes, counts) from the reference's own engines (sliced or per-dimension), compiled into oracle/_ref/."""
+    key = f"fp_{variant}"
+    if key not in _refs:
+        lib = ctypes.CDLL(os.path.join(HERE, "_ref", f"libszref_fp_{variant}.so"))
+        lib.szref_fingerprints.restype = ctypes.c_int
+        _refs[key] = lib
+    data, offsets = texts_tape
+    count = len(offsets) - 1
+    widths = None if window_widths is None else np.ascontiguousarray(window_widths, dtype=np.uint64)
+    hashes = np.zeros((count, ndim), dtype=np.uint32)
+    counts = np.zeros((count, ndim), dtype=np.uint32)
+    if count and ndim:
+        status = _refs[key].szref_fingerprints(
+            ctypes.c_uint64(ndim), ctypes.c_uint64(alphabet_size), None if widths is None else _ptr(widths),
+            ctypes.c_uint64(0 if widths is None else len(widths)), ctypes.c_uint64(seed), _ptr(data), _ptr(offsets),
+            ctypes.c_uint64(count), _ptr(hashes), _ptr(counts), ctypes.c_uint64(ndim), ctypes.c_int(int(per_dimension)),
+            ctypes.c_int(threads))
+        if status != 0:
+            raise RuntimeError(f"reference fingerprints failed with status {status}")
+    return hashes, counts

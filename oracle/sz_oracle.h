@@ -1,6 +1,6 @@
 /**
  *  @file oracle/sz_oracle.h
- *  @brief TEST INFRASTRUCTURE ONLY: the CPU oracle for the similarity scores.
+ *  @brief TEST INFRASTRUCTURE ONLY: the CPU oracle for the similarity scores and the rolling fingerprints.
  *
  *  This is a plain-C restatement of the reference's scoring semantics, used by `tests/`, by the package smoke
  *  check and by the CPU-baseline leg of `bench.py` as the checker. The shipped library never links, loads or
@@ -66,6 +66,39 @@ int szo_pairs(szo_params_t const *params, uint8_t const *a_data, uint64_t const 
 int szo_cross(szo_params_t const *params, uint8_t const *q_data, uint64_t const *q_offsets, uint64_t q_count,
               uint8_t const *c_data, uint64_t const *c_offsets, uint64_t c_count, int64_t *results, uint64_t stride,
               int threads);
+
+/*
+ *  Rolling MinHash / Count-Min fingerprints (sz_fingerprints.c), restating
+ *  include/stringzillas/fingerprints/serial.hpp and c/stringzillas/fingerprints.cuh.
+ *
+ *  Pinning: published splitmix64 outputs; a closed form (each window's hash is the Rabin-Karp polynomial
+ *  sum of (byte + 1) * multiplier^(window - 1 - k) mod modulo, evaluated with exact integers and a brute-force
+ *  minimum and count) in tests/test_oracle.py; differential checks against the reference's serial sliced,
+ *  serial per-dimension and AVX-512 sliced engines compiled into oracle/_ref/.
+ */
+
+/** One dimension of the reference's f64 rolling hasher (serial.hpp:364-431). */
+typedef struct szo_hasher_t {
+    double multiplier, modulo, inverse_modulo;
+    double negative_discarding_multiplier; /**< -(multiplier^(window-1) mod modulo) */
+    double discarding_multiplier;          /**< (-multiplier^window) mod modulo, in [0, modulo) */
+    uint64_t window_width;
+} szo_hasher_t;
+
+uint64_t szo_splitmix64(uint64_t state);
+
+/** Fills `hashers[dimensions]` in output order; returns 1 for the sliced layout, 0 for the per-dimension one. */
+int szo_fingerprint_hashers(uint64_t dimensions, uint64_t const *widths, uint64_t widths_count, uint64_t seed,
+                            szo_hasher_t *hashers);
+
+/** Min-hashes and their counts of one text over all dimensions. */
+void szo_fingerprint(szo_hasher_t const *hashers, uint64_t dimensions, uint8_t const *text, size_t length,
+                     uint32_t *min_hashes, uint32_t *min_counts);
+
+/** Fingerprints of a u64-offset tape into strided rows (strides in bytes) on @p threads. */
+int szo_fingerprints(szo_hasher_t const *hashers, uint64_t dimensions, uint8_t const *data, uint64_t const *offsets,
+                     uint64_t count, uint32_t *min_hashes, uint64_t hashes_stride, uint32_t *min_counts,
+                     uint64_t counts_stride, int threads);
 
 #ifdef __cplusplus
 }
