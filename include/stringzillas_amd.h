@@ -234,6 +234,78 @@ void szs_smith_waterman_scores_free(szs_smith_waterman_scores_t engine);
 
 #pragma endregion
 
+#pragma region Rolling fingerprints: MinHash + Count-Min (stringzillas.h:436-536)
+
+typedef void *szs_fingerprints_t;      /* stringzillas.h:457 */
+typedef void *szs_fingerprints_utf8_t; /* stringzillas.h:458 */
+
+/**
+ *  Replaces stringzillas.h:476. `dimensions` hash dimensions per text; `window_widths` (NULL or count 0: the
+ *  defaults {3, 4, 5, 7, 9, 11, 15, 31}); `seed` derives every dimension's multiplier and modulo. When every width
+ *  gets the same multiple of 64 dimensions, dimension `d` uses `window_widths[(d / 64) % count]`, otherwise
+ *  `window_widths[d % count]` (c/stringzillas/fingerprints.cuh:39-140). `alphabet_size`, `alloc` and
+ *  `capabilities` are accepted and, as in the reference, do not change the results. A zero window width is
+ *  rejected with `sz_unexpected_dimensions_k`.
+ */
+sz_status_t szs_fingerprints_init(sz_size_t dimensions, sz_size_t alphabet_size, sz_size_t const *window_widths,
+                                  sz_size_t window_widths_count, sz_u64_t seed, sz_memory_allocator_t const *alloc,
+                                  sz_capability_t capabilities, szs_fingerprints_t *engine,
+                                  char const **error_message);
+/**
+ *  Replaces stringzillas.h:492. Row `i` of `min_hashes` (at `i * min_hashes_stride` bytes) receives the low 32
+ *  bits of each dimension's smallest window hash, or 0xFFFFFFFF for a text shorter than that dimension's window;
+ *  `min_counts` the number of windows that reach it (0 when skipped). Outputs may be host or device memory.
+ */
+sz_status_t szs_fingerprints_sequence(szs_fingerprints_t engine, szs_device_scope_t device, sz_sequence_t const *texts,
+                                      sz_u32_t *min_hashes, sz_size_t min_hashes_stride, sz_u32_t *min_counts,
+                                      sz_size_t min_counts_stride, char const **error_message);
+/** Replaces stringzillas.h:509. */
+sz_status_t szs_fingerprints_u64tape(szs_fingerprints_t engine, szs_device_scope_t device,
+                                     sz_sequence_u64tape_t const *texts, sz_u32_t *min_hashes,
+                                     sz_size_t min_hashes_stride, sz_u32_t *min_counts, sz_size_t min_counts_stride,
+                                     char const **error_message);
+/** Replaces stringzillas.h:526. */
+sz_status_t szs_fingerprints_u32tape(szs_fingerprints_t engine, szs_device_scope_t device,
+                                     sz_sequence_u32tape_t const *texts, sz_u32_t *min_hashes,
+                                     sz_size_t min_hashes_stride, sz_u32_t *min_counts, sz_size_t min_counts_stride,
+                                     char const **error_message);
+/** Replaces stringzillas.h:536. */
+void szs_fingerprints_free(szs_fingerprints_t engine);
+
+/* The UTF-8 engine hashes the same bytes (c/stringzillas/fingerprints.cuh:144-190 forwards every call). */
+sz_status_t szs_fingerprints_utf8_init(sz_size_t dimensions, sz_size_t alphabet_size, sz_size_t const *window_widths,
+                                       sz_size_t window_widths_count, sz_u64_t seed,
+                                       sz_memory_allocator_t const *alloc, sz_capability_t capabilities,
+                                       szs_fingerprints_utf8_t *engine, char const **error_message);
+sz_status_t szs_fingerprints_utf8_sequence(szs_fingerprints_utf8_t engine, szs_device_scope_t device,
+                                           sz_sequence_t const *texts, sz_u32_t *min_hashes,
+                                           sz_size_t min_hashes_stride, sz_u32_t *min_counts,
+                                           sz_size_t min_counts_stride, char const **error_message);
+sz_status_t szs_fingerprints_utf8_u32tape(szs_fingerprints_utf8_t engine, szs_device_scope_t device,
+                                          sz_sequence_u32tape_t const *texts, sz_u32_t *min_hashes,
+                                          sz_size_t min_hashes_stride, sz_u32_t *min_counts,
+                                          sz_size_t min_counts_stride, char const **error_message);
+sz_status_t szs_fingerprints_utf8_u64tape(szs_fingerprints_utf8_t engine, szs_device_scope_t device,
+                                          sz_sequence_u64tape_t const *texts, sz_u32_t *min_hashes,
+                                          sz_size_t min_hashes_stride, sz_u32_t *min_counts,
+                                          sz_size_t min_counts_stride, char const **error_message);
+void szs_fingerprints_utf8_free(szs_fingerprints_utf8_t engine);
+
+/** New: dimensions of an engine (0 for a null handle). */
+sz_size_t szs_fingerprints_dimensions(szs_fingerprints_t engine);
+/** New: HIP-event time of the last call's hashing kernels on the device, in milliseconds. */
+double szs_fingerprints_last_kernel_ms(szs_fingerprints_t engine);
+/**
+ *  New, tuning: `kernel` 0 picks automatically (the wave kernel when every window width holds at least 64
+ *  dimensions, else the lane kernel), 1 forces the wave-per-text kernel, 2 the lane-per-text kernel;
+ *  `segment_bytes` (0: automatic, else within [64, 2^30]) caps the window starts one work unit covers; longer
+ *  texts are split and merged. Results never depend on either setting.
+ */
+sz_status_t szs_fingerprints_tune(szs_fingerprints_t engine, int kernel, sz_size_t segment_bytes,
+                                  char const **error_message);
+
+#pragma endregion
+
 #pragma region Unified memory (stringzillas.h:545-551)
 
 void *szs_unified_alloc(sz_size_t size_bytes);           /* replaces stringzillas.h:545 */

@@ -1,10 +1,11 @@
-"""MI355X-native drop-in for the ``stringzillas`` similarity classes.
+"""MI355X-native drop-in for the ``stringzillas`` similarity and fingerprint classes.
 
-Mirrors the reference's Python surface (python/stringzillas/similarities.c, device_scope.c, stringzillas.c):
-``LevenshteinDistances``, ``LevenshteinDistancesUTF8``, ``NeedlemanWunschScores``, ``SmithWatermanScores``,
-``DeviceScope``, ``Strs``, ``__capabilities__``, ``__version__``, ``reset_capabilities`` and ``to_device``, with
-the same argument names, defaults, dtypes and exception types. Every score is computed by the HIP kernels in
-``libstringzillas_amd.so``; importing this package fails loudly when that library is missing.
+Mirrors the reference's Python surface (python/stringzillas/similarities.c, fingerprints.c, device_scope.c,
+stringzillas.c): ``LevenshteinDistances``, ``LevenshteinDistancesUTF8``, ``NeedlemanWunschScores``,
+``SmithWatermanScores``, ``Fingerprints``, ``DeviceScope``, ``Strs``, ``__capabilities__``, ``__version__``,
+``reset_capabilities`` and ``to_device``, with the same argument names, defaults, dtypes and exception types.
+Every score and fingerprint is computed by the HIP kernels in ``libstringzillas_amd.so``; importing this package
+fails loudly when that library is missing.
 
 Extensions beyond the reference: ``engine.pairs(firsts, seconds)`` scores independent pairs, ``Batch`` scores a
 mixed list of (engine, firsts, seconds) segments in one persistent launch, ``Server`` is the continuous-batching
@@ -412,6 +413,107 @@ class SmithWatermanScores(NeedlemanWunschScores):
 
     _kind = 3
     _prefix = "szs_smith_waterman_scores"
+
+
+#  Fingerprints -----------------------------------------------------------------------------------------------
+
+
+class Fingerprints:
+    """``Fingerprints(ndim, window_widths=None, alphabet_size=256, seed=0, capabilities=None)``
+    (python/stringzillas/fingerprints.c:34-83, 212-245): rolling MinHash + Count-Min sketches.
+
+    ``engine(texts, device=None, out=None)`` returns ``(hashes, counts)``, two ``uint32`` matrices of shape
+    ``(len(texts), ndim)``: per dimension the low 32 bits of the smallest window hash (0xFFFFFFFF when the text is
+    shorter than that dimension's window) and how many windows reach it. ``out`` (an extension: the reference
+    accepts and ignores it) may be a pair of ``uint32`` arrays or device tensors of that shape to write into.
+    """
+
+    def __init__(self, ndim: int, window_widths=None, alphabet_size: int = 256, seed: int = 0, capabilities=None):
+        ndim = int(ndim)
+        if ndim < 0:
+            raise ValueError("ndim must be non-negative")
+        widths_pointer, widths_count = None, 0
+        if window_widths is not None:
+            if not isinstance(window_widths, np.ndarray):
+                raise TypeError("window_widths must be a numpy array of uint64")
+            if window_widths.dtype != np.uint64:
+                raise TypeError("window_widths must have dtype uint64")
+            if window_widths.ndim != 1:
+                raise ValueError("window_widths must be a 1D array")
+            if not window_widths.flags.c_contiguous:
+                raise ValueError("window_widths must be a contiguous C-style array (no strides)")
+            widths_pointer, widths_count = ctypes.c_void_p(window_widths.ctypes.data), window_widths.size
+        handle, error = ctypes.c_void_p(), ctypes.c_char_p()
+        status = _lib.szs_fingerprints_init(
+            ctypes.c_size_t(ndim), ctypes.c_size_t(int(alphabet_size)), widths_pointer, ctypes.c_size_t(widths_count),
+            ctypes.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), None, ctypes.c_int(_active_capabilities),
+            ctypes.byref(handle), ctypes.byref(error))
+        _check(status, error, "Fingerprints initialization")
+        self.handle = handle
+        self.ndim = ndim
+        self.description = f"ndim={ndim},window_widths={widths_count},alphabet_size={alphabet_size},seed={seed}"
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.szs_fingerprints_free(self.handle)
+            self.handle = None
+
+    def __repr__(self) -> str:
+        return f"Fingerprints({self.description})"
+
+    @property
+    def capabilities(self) -> Tuple[str, ...]:
+        return _caps_to_tuple(_active_capabilities)
+
+    @property
+    def last_kernel_ms(self) -> float:
+        """Device time of the last call's hashing kernels (HIP events), in milliseconds."""
+        return float(_lib.szs_fingerprints_last_kernel_ms(self.handle))
+
+    KERNEL_AUTO, KERNEL_WAVE, KERNEL_LANE = 0, 1, 2
+
+    def tune(self, kernel: int = 0, segment_bytes: int = 0) -> None:
+        """Tuning (results never change): force the wave-per-text (1) or lane-per-text (2) kernel, or pick
+        automatically (0); cap the window starts of one work unit (0: automatic)."""
+        error = ctypes.c_char_p()
+        _check(_lib.szs_fingerprints_tune(self.handle, ctypes.c_int(kernel), ctypes.c_size_t(segment_bytes),
+                                          ctypes.byref(error)), error, "Fingerprints tuning")
+
+    @staticmethod
+    def _output(array, rows: int, columns: int, name: str):
+        if hasattr(array, "data_ptr"):  # a torch tensor, typically in HBM
+            if tuple(array.shape) != (rows, columns) or array.element_size() != 4 or array.stride(1) != 1:
+                raise ValueError(f"{name} must be a ({rows}, {columns}) 32-bit tensor with contiguous rows")
+            return array.data_ptr(), array.stride(0) * 4
+        if not isinstance(array, np.ndarray) or array.dtype != np.uint32:
+            raise TypeError(f"{name} must be a uint32 NumPy array or a device tensor")
+        if array.shape != (rows, columns) or (columns and array.strides[1] != 4):
+            raise ValueError(f"{name} must have shape ({rows}, {columns}) with contiguous rows")
+        return array.ctypes.data, array.strides[0] if rows > 1 else columns * 4
+
+    def __call__(self, texts, device: Optional[DeviceScope] = None, out=None):
+        if isinstance(texts, (list, tuple)) and len(texts) == 0:
+            texts = Strs([])
+        texts = _as_tape(texts, "texts")
+        scope = _scope(device)
+        rows = len(texts)
+        if out is None:
+            hashes = np.empty((rows, self.ndim), dtype=np.uint32)
+            counts = np.empty((rows, self.ndim), dtype=np.uint32)
+        else:
+            hashes, counts = out
+        if rows == 0 or self.ndim == 0:
+            return hashes, counts
+        hashes_pointer, hashes_stride = self._output(hashes, rows, self.ndim, "hashes")
+        counts_pointer, counts_stride = self._output(counts, rows, self.ndim, "counts")
+        tape = texts.tape()
+        error = ctypes.c_char_p()
+        status = _lib.szs_fingerprints_u64tape(
+            self.handle, scope.handle, ctypes.byref(tape), ctypes.c_void_p(hashes_pointer),
+            ctypes.c_size_t(max(hashes_stride, self.ndim * 4)), ctypes.c_void_p(counts_pointer),
+            ctypes.c_size_t(max(counts_stride, self.ndim * 4)), ctypes.byref(error))
+        _check(status, error, "Fingerprints computation")
+        return hashes, counts
 
 
 #  Batches and serving (extensions) ----------------------------------------------------------------------------

@@ -108,10 +108,12 @@ for name, restype in {
     "szs_capabilities": _c.c_int, "szs_capabilities_comptime": _c.c_int, "szs_capabilities_runtime": _c.c_int,
     "szs_batch_cells": _c.c_size_t, "szs_batch_last_kernel_ms": _c.c_double, "szs_server_poll": _c.c_int,
     "szs_server_bytes_in_use": _c.c_size_t, "szs_unified_alloc": _c.c_void_p, "szs_tape_bytes": _c.c_size_t,
+    "szs_fingerprints_dimensions": _c.c_size_t, "szs_fingerprints_last_kernel_ms": _c.c_double,
 }.items():
     getattr(lib, name).restype = restype
 for name in ("szs_device_scope_free", "szs_levenshtein_distances_free", "szs_levenshtein_distances_utf8_free",
              "szs_needleman_wunsch_scores_free", "szs_smith_waterman_scores_free", "szs_batch_free",
-             "szs_server_free", "szs_unified_free", "szs_tape_free"):
+             "szs_server_free", "szs_unified_free", "szs_tape_free", "szs_fingerprints_free",
+             "szs_fingerprints_utf8_free"):
     getattr(lib, name).restype = None
     getattr(lib, name).argtypes = [_c.c_void_p] if name != "szs_unified_free" else [_c.c_void_p, _c.c_size_t]

@@ -189,6 +189,16 @@ static sz_status_t stage_tape(sz_cptr_t data, offset_type_ const *offsets, uint6
     return ok(error_message);
 }
 
+sz_status_t stage_bytes_tape(sz_sequence_u64tape_t const *input, staged_tape_t &tape, hipStream_t stream,
+                             char const **error_message) {
+    return stage_tape(input->data, input->offsets, input->count, false, 0, tape, stream, error_message);
+}
+
+sz_status_t stage_bytes_tape(sz_sequence_u32tape_t const *input, staged_tape_t &tape, hipStream_t stream,
+                             char const **error_message) {
+    return stage_tape(input->data, input->offsets, input->count, false, 0, tape, stream, error_message);
+}
+
 static sz_status_t stage_input(segment_input_t const &input, bool queries, bool runes, staged_tape_t &tape,
                                hipStream_t stream, char const **error_message) {
     if (queries) {

@@ -93,6 +93,12 @@ struct staged_tape_t {
     device_buffer_t data_copy, offsets_copy, runes_buffer, rune_counts_buffer;
 };
 
+/** Stages a host or device tape of raw bytes on `stream` (device data and u64 offsets; copies only what is needed). */
+sz_status_t stage_bytes_tape(sz_sequence_u64tape_t const *input, staged_tape_t &tape, hipStream_t stream,
+                             char const **error_message);
+sz_status_t stage_bytes_tape(sz_sequence_u32tape_t const *input, staged_tape_t &tape, hipStream_t stream,
+                             char const **error_message);
+
 /** How a segment's cells are enumerated. */
 enum segment_mode_t { segment_pairs_k = 0, segment_cross_k = 1, segment_symmetric_k = 2 };
 

@@ -172,3 +172,28 @@ def test_tape_files_round_trip_and_reject_damage(tmp_path):
                 szs.load_tape(path)
             except RuntimeError as error:  # no GPU in this container
                 raise ValueError(str(error))
+
+
+def test_fingerprints_argument_validation_mirrors_the_reference():
+    """python/stringzillas/fingerprints.c:34-83: widths must be a contiguous 1-D uint64 array."""
+    engine = szs.Fingerprints(ndim=1024)
+    assert engine.ndim == 1024 and "ndim=1024" in repr(engine)
+    assert _native.lib.szs_fingerprints_dimensions(engine.handle) == 1024
+    szs.Fingerprints(ndim=7, window_widths=np.array([3, 5], dtype=np.uint64), alphabet_size=4, seed=9)
+    with pytest.raises(TypeError):
+        szs.Fingerprints(ndim=64, window_widths=[3, 5])
+    with pytest.raises(TypeError):
+        szs.Fingerprints(ndim=64, window_widths=np.array([3, 5], dtype=np.int64))
+    with pytest.raises(ValueError):
+        szs.Fingerprints(ndim=64, window_widths=np.array([[3, 5]], dtype=np.uint64))
+    with pytest.raises(ValueError):
+        szs.Fingerprints(ndim=64, window_widths=np.array([3, 0, 5, 0], dtype=np.uint64)[::2])
+    with pytest.raises(ValueError):  # a zero-width window hashes nothing
+        szs.Fingerprints(ndim=64, window_widths=np.array([3, 0], dtype=np.uint64))
+    hashes, counts = engine(szs.Strs([]))  # empty input: no device work at all
+    assert hashes.shape == (0, 1024) and counts.dtype == np.uint32
+    if not gpu_available():
+        with pytest.raises(RuntimeError, match="GPU"):
+            engine(szs.Strs(["hello"]))
+    with pytest.raises(RuntimeError, match="CPU"):
+        engine(szs.Strs(["hello"]), device=szs.DeviceScope(cpu_cores=2))
