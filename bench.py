@@ -47,6 +47,10 @@ def parse():
     parser.add_argument("--serve-qps", type=float, nargs="*", default=[1000.0, 2000.0],
                         help="offered rates of the short serving run (none to skip)")
     parser.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per serving rate")
+    parser.add_argument("--skip-fingerprints", action="store_true", help="skip the fingerprint engine's measurement")
+    parser.add_argument("--fingerprint-mb", type=int, default=200, help="corpus size per fingerprint shape, MB")
+    parser.add_argument("--only-fingerprints", action="store_true",
+                        help="measure only the fingerprint engine and print its report (for profiling)")
     return parser.parse_args()
 
 
@@ -109,6 +113,10 @@ def main():
     import stringzillas_amd as szs
     from stringzillas_amd import workloads as W
 
+    if args.only_fingerprints:
+        print(json.dumps({"fingerprints": fingerprints_run(args.fingerprint_mb * 1_000_000, args.steps,
+                                                           not args.skip_cpu)}), flush=True)
+        return
     device = torch.device("cuda", 0)
     t0 = time.time()
     workload = W.mixed_c3(args.pairs, args.long_pairs, device="cuda")
@@ -241,6 +249,10 @@ def main():
         cpu = cpu_baseline(host_segments, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
 
+    fingerprints = None
+    if not args.skip_fingerprints:
+        fingerprints = fingerprints_run(args.fingerprint_mb * 1_000_000, args.steps, not args.skip_cpu)
+
     achieved = cells * ops_per_cell / (kernel_ms_mean / 1e3) / 1e12  # T lane-ops/s over the scoring kernel
     line = {
         "metric": METRIC,
@@ -282,10 +294,112 @@ def main():
         "batched_over_per_call": round(gcups / per_call["gcups"], 1) if per_call else None,
         "parity": {"checked_pairs": checked, "mismatches": mismatches},
         "serving": serving,
+        "fingerprints": fingerprints,
     }
     if breakdown:
         line["per_kind"] = breakdown
     print(json.dumps(line), flush=True)
+
+
+# The reference's published fingerprint throughput on an H100 (include/stringzillas/fingerprints/README.md:15-42):
+# equal-length ACGT documents, 64 dimensions, default window widths; MB there is 2^20 bytes.
+FINGERPRINT_H100_MIB_S = {100: 438.7, 1000: 3400.0 * 1.024, 10000: 11200.0 * 1.024, 100000: 5600.0 * 1.024}
+# VALU issue rate of the lane kernel on the 10,000-byte, 64-dimension shape, from SQ_INSTS_VALU
+# (profiles/fingerprint_kernels_acgt_documents.md).
+MEASURED_FINGERPRINT_VALU_ISSUE = {"lane_64": None, "wave_1024": None}
+
+
+def fingerprints_run(corpus_bytes: int, steps: int, with_cpu: bool):
+    """The fingerprint engine on equal-length ACGT documents resident in HBM (the reference README's shapes): 64
+    dimensions over the default 8 window widths (the lane kernel), and 1,024 dimensions (128 per width, the wave
+    kernel) on the 10,000-byte documents. Throughput counts text bytes; a hash update is one window of one
+    dimension. Spot-checked against the oracle."""
+    import torch
+
+    import stringzillas_amd as szs
+    from oracle import oracle as O
+
+    letters = torch.tensor(list(b"ACGT"), dtype=torch.uint8, device="cuda")
+    generator = torch.Generator(device="cuda")
+    generator.manual_seed(64)
+    report = {"corpus_bytes": corpus_bytes, "documents": "equal-length ACGT, generated on the device", "shapes": []}
+    shapes = [(64, 100), (64, 1000), (64, 10000), (64, 100000), (1024, 10000)]
+    for ndim, length in shapes:
+        count = corpus_bytes // length
+        data = letters[torch.randint(0, 4, (count * length + 16,), device="cuda", generator=generator)]
+        offsets = torch.arange(count + 1, dtype=torch.int64, device="cuda") * length
+        texts = TensorTape(data, offsets)
+        torch.cuda.synchronize()
+        engine = szs.Fingerprints(ndim=ndim)
+        hashes = torch.empty((count, ndim), dtype=torch.int32, device="cuda")
+        counts = torch.empty((count, ndim), dtype=torch.int32, device="cuda")
+        engine(texts, out=(hashes, counts))
+        torch.cuda.synchronize()
+        kernel_ms, start = [], time.perf_counter()
+        for _ in range(max(1, steps)):
+            engine(texts, out=(hashes, counts))
+            kernel_ms.append(engine.last_kernel_ms)
+        torch.cuda.synchronize()
+        wall_ms = (time.perf_counter() - start) / max(1, steps) * 1e3
+        kernel = float(np.mean(kernel_ms))
+        # Oracle check on 4 documents spread over the corpus.
+        picks = np.linspace(0, count - 1, 4).astype(np.int64)
+        host = data[: count * length].view(count, length)[torch.from_numpy(picks).cuda()].cpu().numpy()
+        expected_hashes, expected_counts = O.fingerprints(O.tape([row.tobytes() for row in host]), ndim, threads=8)
+        got_hashes = hashes[torch.from_numpy(picks).cuda()].cpu().numpy().view(np.uint32)
+        got_counts = counts[torch.from_numpy(picks).cuda()].cpu().numpy().view(np.uint32)
+        if not (np.array_equal(got_hashes, expected_hashes) and np.array_equal(got_counts, expected_counts)):
+            raise SystemExit(f"fingerprints differ from the oracle (ndim {ndim}, {length}-byte documents)")
+        total = count * length
+        entry = {
+            "ndim": ndim, "document_bytes": length, "documents": count,
+            "kernel": "lane" if ndim == 64 else "wave",
+            "kernel_ms": round(kernel, 3), "call_ms": round(wall_ms, 3),
+            "gb_per_s": round(total / kernel / 1e6, 2),
+            "mhash_per_s": round(count * ndim / kernel / 1e3, 1),
+            "t_updates_per_s": round(total * ndim / kernel / 1e9, 3),
+            "checked_documents": len(picks),
+        }
+        if ndim == 64 and length in FINGERPRINT_H100_MIB_S:
+            h100 = FINGERPRINT_H100_MIB_S[length] * 1.048576  # MB/s
+            entry["reference_h100_mb_per_s"] = round(h100, 1)
+            entry["over_reference_h100"] = round(total / kernel / 1e3 / h100, 2)
+        log(f"fingerprints {entry}")
+        report["shapes"].append(entry)
+        del data, offsets, texts, hashes, counts, engine
+    if with_cpu:
+        report["cpu_baseline"] = fingerprint_cpu_baseline()
+        log(f"fingerprint cpu baseline: {report['cpu_baseline']}")
+    return report
+
+
+def fingerprint_cpu_baseline(target_seconds: float = 6.0):
+    """The reference's own CPU fingerprint engine (AVX-512 sliced when the host has it, else serial) on 10,000-byte
+    ACGT documents at 64 dimensions, sharded over threads; a bounded sample."""
+    from oracle import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    variant = next((v for v in ("skylake", "serial") if O.reference_fingerprints_available(v)), None)
+    rng = np.random.default_rng(5)
+    documents = 2 * threads
+    while True:
+        texts = [bytes(rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), 10000)) for _ in range(documents)]
+        tape = O.tape(texts)
+        start = time.perf_counter()
+        if variant:
+            O.reference_fingerprints(tape, 64, threads=threads, variant=variant)
+        else:
+            O.fingerprints(tape, 64, threads=threads)
+        seconds = time.perf_counter() - start
+        if seconds >= target_seconds / 3 or documents >= 1 << 16:
+            break
+        documents = int(min(documents * max(2.0, 0.9 * target_seconds / max(seconds, 1e-3)), 1 << 16))
+    return {
+        "value": round(documents * 10000 / seconds / 1e6, 2), "unit": "MB/s", "cores": threads,
+        "kind": "reference" if variant else "port",
+        "sample": f"{documents} ACGT documents of 10,000 bytes at 64 dimensions ({seconds:.1f}s), engine="
+                  f"{'reference ' + variant if variant else 'oracle'}",
+    }
 
 
 def serving_run(rates, seconds: float):

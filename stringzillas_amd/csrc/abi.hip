@@ -59,13 +59,15 @@ struct gathered_tape_t {
 
 sz_status_t run_segments(szs_device_scope_t scope, std::vector<segment_input_t> const &inputs,
                          char const **error_message) {
-    batch_t *batch = nullptr;
-    sz_status_t status = batch_prepare(as_scope(scope), inputs, szs_batch_persistent_k, &batch, error_message);
-    if (status != sz_success_k) return status;
-    status = batch_launch(batch, error_message);
-    if (status == sz_success_k) status = batch_wait(batch, error_message);
-    delete batch;
-    return status;
+    return guarded(error_message, [&]() {
+        batch_t *batch = nullptr;
+        sz_status_t status = batch_prepare(as_scope(scope), inputs, szs_batch_persistent_k, &batch, error_message);
+        if (status != sz_success_k) return status;
+        status = batch_launch(batch, error_message);
+        if (status == sz_success_k) status = batch_wait(batch, error_message);
+        delete batch;
+        return status;
+    });
 }
 
 template <typename tape_type_>
@@ -97,10 +99,13 @@ sz_status_t sequence_call(int kind, void *engine, szs_device_scope_t scope, sz_s
                           sz_sequence_t const *candidates, void *results, sz_size_t stride,
                           char const **error_message) {
     if (!queries) return fail(sz_unexpected_dimensions_k, error_message, "Query texts cannot be null");
-    gathered_tape_t q, c;
-    q.gather(queries);
-    if (candidates) c.gather(candidates);
-    return cross_call(kind, engine, scope, &q.view, candidates ? &c.view : nullptr, results, stride, error_message);
+    return guarded(error_message, [&]() {
+        gathered_tape_t q, c;
+        q.gather(queries);
+        if (candidates) c.gather(candidates);
+        return cross_call(kind, engine, scope, &q.view, candidates ? &c.view : nullptr, results, stride,
+                          error_message);
+    });
 }
 
 sz_status_t pairs_call(int kind, void *engine_handle, szs_device_scope_t scope, sz_sequence_u64tape_t const *firsts,
@@ -118,6 +123,31 @@ sz_status_t pairs_call(int kind, void *engine_handle, szs_device_scope_t scope, 
     input.results = results;
     if (!scope) return fail(sz_unexpected_dimensions_k, error_message, "Device must be initialized");
     return run_segments(scope, {input}, error_message);
+}
+
+sz_status_t batch_init(szs_device_scope_t device, szs_pairs_segment_t const *segments, sz_size_t segments_count,
+                       szs_batch_mode_t mode, szs_batch_t *batch, char const **error_message) {
+    std::vector<segment_input_t> inputs(segments_count);
+    for (size_t s = 0; s < segments_count; ++s) {
+        szs_pairs_segment_t const &segment = segments[s];
+        if (!segment.engine || !segment.firsts || !segment.seconds)
+            return fail(sz_unexpected_dimensions_k, error_message, "Segment fields must not be null");
+        engine_t *engine = reinterpret_cast<engine_t *>(segment.engine);
+        if (engine->kind != (int)segment.kind)
+            return fail(sz_unexpected_dimensions_k, error_message, "Segment kind does not match its engine");
+        inputs[s].engine = engine;
+        inputs[s].mode = segment_pairs_k;
+        inputs[s].queries = segment.firsts;
+        inputs[s].candidates = segment.seconds;
+        inputs[s].results = segment.results;
+        inputs[s].queries_bytes = segment.firsts_bytes;
+        inputs[s].candidates_bytes = segment.seconds_bytes;
+    }
+    batch_t *prepared = nullptr;
+    sz_status_t const status = batch_prepare(as_scope(device), inputs, mode, &prepared, error_message);
+    if (status != sz_success_k) return status;
+    *batch = prepared;
+    return ok(error_message);
 }
 
 } // namespace
@@ -395,37 +425,17 @@ sz_status_t szs_batch_init(szs_device_scope_t device, szs_pairs_segment_t const 
     if (!batch || (!segments && segments_count))
         return fail(sz_unexpected_dimensions_k, error_message, "Batch arguments must not be null");
     if (!device) return fail(sz_unexpected_dimensions_k, error_message, "Device must be initialized");
-    std::vector<segment_input_t> inputs(segments_count);
-    for (size_t s = 0; s < segments_count; ++s) {
-        szs_pairs_segment_t const &segment = segments[s];
-        if (!segment.engine || !segment.firsts || !segment.seconds)
-            return fail(sz_unexpected_dimensions_k, error_message, "Segment fields must not be null");
-        engine_t *engine = reinterpret_cast<engine_t *>(segment.engine);
-        if (engine->kind != (int)segment.kind)
-            return fail(sz_unexpected_dimensions_k, error_message, "Segment kind does not match its engine");
-        inputs[s].engine = engine;
-        inputs[s].mode = segment_pairs_k;
-        inputs[s].queries = segment.firsts;
-        inputs[s].candidates = segment.seconds;
-        inputs[s].results = segment.results;
-        inputs[s].queries_bytes = segment.firsts_bytes;
-        inputs[s].candidates_bytes = segment.seconds_bytes;
-    }
-    batch_t *prepared = nullptr;
-    sz_status_t const status = batch_prepare(as_scope(device), inputs, mode, &prepared, error_message);
-    if (status != sz_success_k) return status;
-    *batch = prepared;
-    return ok(error_message);
+    return guarded(error_message, [&]() { return batch_init(device, segments, segments_count, mode, batch, error_message); });
 }
 
 sz_status_t szs_batch_launch(szs_batch_t batch, char const **error_message) {
     if (!batch) return fail(sz_unexpected_dimensions_k, error_message, "Batch must be initialized");
-    return batch_launch(reinterpret_cast<batch_t *>(batch), error_message);
+    return guarded(error_message, [&]() { return batch_launch(reinterpret_cast<batch_t *>(batch), error_message); });
 }
 
 sz_status_t szs_batch_wait(szs_batch_t batch, char const **error_message) {
     if (!batch) return fail(sz_unexpected_dimensions_k, error_message, "Batch must be initialized");
-    return batch_wait(reinterpret_cast<batch_t *>(batch), error_message);
+    return guarded(error_message, [&]() { return batch_wait(reinterpret_cast<batch_t *>(batch), error_message); });
 }
 
 sz_status_t szs_batch_run(szs_batch_t batch, char const **error_message) {

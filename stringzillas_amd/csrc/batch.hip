@@ -317,6 +317,7 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
     sz_status_t status = resolve_gpu(scope, &device, &stream, error_message);
     if (status != sz_success_k) return status;
     if (options.stream) stream = options.stream;
+    SZS_TRY_HIP(order_after_default_stream(stream), "Ordering after the default stream");
     batch_t *batch = new (std::nothrow) batch_t();
     if (!batch) return fail(sz_bad_alloc_k, error_message, "Failed to allocate a batch");
     struct guard_t {
@@ -596,6 +597,7 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
 sz_status_t batch_launch(batch_t *batch, char const **error_message) {
     batch_t &b = *batch;
     SZS_TRY_HIP(hipSetDevice(b.device), "hipSetDevice failed");
+    SZS_TRY_HIP(order_after_default_stream(b.stream), "Ordering after the default stream");
     if (b.mode == szs_batch_graph_k) { // the events then time the whole replay
         SZS_TRY_HIP(hipEventRecord(b.kernel_begin, b.stream), "event");
         SZS_TRY_HIP(hipGraphLaunch(b.graph_exec, b.stream), "graph launch");

@@ -17,10 +17,10 @@
  *  (4 bytes per lane), and every byte reaches all lanes through `v_readlane` into a scalar register, so the
  *  per-byte cost is the per-dimension arithmetic alone:
  *      x = fma(state, multiplier, new + 1); x = fma(discard, old + 1, x)          (exact: x < 2^52)
- *      q = floor(x * inverse_low); r = fma(-q, modulo, x); state = r >= modulo ? r - modulo : r
- *  where `discard = (-multiplier^w) mod modulo` and `inverse_low` is 1/modulo rounded down by two ulps, so that
- *  `q` is never above the true quotient and at most one below it: one conditional subtraction yields the
- *  canonical residue. The minimum and count update follow serial.hpp:1012-1016.
+ *      r = x - floor(x * inverse_low) * modulo;  state = r - floor((r + 1) * inverse_low) * modulo
+ *  where `discard = (-multiplier^w) mod modulo` and `inverse_low` is 1/modulo rounded down by two ulps: the first
+ *  quotient is never above the true one and at most one below it, and the second is exactly the missing one
+ *  (the proof is at `reduce`). The minimum and count update follow serial.hpp:1012-1016.
  *
  *  Texts with few dimensions per width (the reference's per-dimension layout, e.g. 64 dimensions over 8 widths)
  *  go to a second kernel instead, where each lane walks its own text and updates 8 dimensions of one width, whose
@@ -91,14 +91,16 @@ struct fp_launch_t {
     uint64_t segment;           /**< window starts per unit */
     uint32_t *hashes, *counts;  /**< rows at these byte strides */
     uint64_t hashes_stride, counts_stride;
-    double *partial_minimums;   /**< [partial][dimension] */
+    double *partial_minimums;   /**< [partial][position]: position = lane-slot index in `hashers` */
     uint32_t *partial_counts;
+    uint32_t positions;         /**< lane-slots of the layout */
 };
 
 struct fp_merge_t {
     fp_long_text_t const *texts;
     uint64_t texts_count;
-    uint32_t dimensions;
+    uint32_t positions;
+    uint32_t const *dims; /**< output dimension of each position (or none) */
     double const *partial_minimums;
     uint32_t const *partial_counts;
     uint32_t *hashes, *counts;
@@ -107,12 +109,17 @@ struct fp_merge_t {
 
 __device__ __forceinline__ uint32_t low_word(double value) { return (uint32_t)(uint64_t)value; }
 
-/** Canonical residue of an integer-valued `x < 2^53`; exact because `q * modulo` and `x - q * modulo` are. */
+/**
+ *  Canonical residue of an integer-valued `0 <= x < 2^52`, in six VALU operations and no compare.
+ *  `inverse_low` lies 2^-53 to 2^-51 (relative) below 1/modulo. Then `x * inverse_low` rounds strictly below
+ *  x/modulo and within 2^-40 of it, so the first quotient is the true one or one less and `r` lands in
+ *  [0, 2 * modulo). For an integer `r` there, `(r + 1) * inverse_low` (one fma, one rounding) is below 1 exactly
+ *  when `r < modulo` and below 2 always, so the second quotient is exactly `r >= modulo`. Every product
+ *  `q * modulo` and difference is an integer below 2^53, hence exact.
+ */
 __device__ __forceinline__ double reduce(double x, double modulo, double inverse_low) {
-    double const q = floor(x * inverse_low);
-    double const r = fma(-q, modulo, x);
-    double const lowered = r - modulo;
-    return lowered >= 0.0 ? lowered : r;
+    double const r = fma(-floor(x * inverse_low), modulo, x);
+    return fma(-floor(fma(r, inverse_low, inverse_low)), modulo, r);
 }
 
 /** A value every lane holds alike, moved into scalar registers. */
@@ -144,26 +151,40 @@ struct fp_lanes_t {
         for (int s = 0; s < slots_; ++s) minimum[s] = state[s], count[s] = 1;
     }
 
+    /**
+     *  One window step of every slot. A new minimum or a tie gets rarer with every window (about 1/j at the j-th),
+     *  so the hot path only compares; each slot's minimum and count update runs behind its own branch, taken
+     *  only when some lane of the wave hits in that slot.
+     */
     __device__ __forceinline__ void roll(double fresh, double stale) {
 #pragma unroll
+        for (int s = 0; s < slots_; ++s) { // all slots first, so their independent chains interleave
+            double const x = fma(discard[s], stale, fma(state[s], multiplier[s], fresh));
+            state[s] = reduce(x, modulo[s], inverse[s]);
+        }
+        // Wave-wide hit masks first (scalar registers): the chains above then stay in one block and interleave.
+        uint64_t hits[slots_];
+#pragma unroll
+        for (int s = 0; s < slots_; ++s) hits[s] = __ballot(state[s] <= minimum[s]);
+#pragma unroll
         for (int s = 0; s < slots_; ++s) {
-            double x = fma(state[s], multiplier[s], fresh);
-            x = fma(discard[s], stale, x);
-            double const value = reduce(x, modulo[s], inverse[s]);
-            state[s] = value;
-            count[s] = value < minimum[s] ? 0u : count[s];
-            count[s] += value <= minimum[s] ? 1u : 0u;
-            minimum[s] = __builtin_fmin(value, minimum[s]);
+            if (__builtin_expect(hits[s] != 0, 0) && state[s] <= minimum[s]) {
+                count[s] = state[s] < minimum[s] ? 1u : count[s] + 1u;
+                minimum[s] = state[s];
+            }
         }
     }
 };
 
-/** Writes one dimension's result: to the caller's rows for a whole text, else to the partial arrays. */
-__device__ __forceinline__ void store(fp_launch_t const &a, uint64_t text, uint64_t partial, uint32_t dim, bool any,
-                                      double minimum, uint32_t count) {
+/**
+ *  Writes one lane-slot's result: to the caller's rows for a whole text, else to the partial arrays, where the
+ *  lane-slots of one unit are adjacent (a lane's slots, or a slot's lanes, fill whole sectors).
+ */
+__device__ __forceinline__ void store(fp_launch_t const &a, uint64_t text, uint64_t partial, uint32_t position,
+                                      uint32_t dim, bool any, double minimum, uint32_t count) {
     if (partial != direct_k) {
-        a.partial_minimums[partial * a.dimensions + dim] = any ? minimum : DBL_MAX;
-        a.partial_counts[partial * a.dimensions + dim] = any ? count : 0u;
+        a.partial_minimums[partial * a.positions + position] = any ? minimum : DBL_MAX;
+        a.partial_counts[partial * a.positions + position] = any ? count : 0u;
         return;
     }
     *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(a.hashes) + text * a.hashes_stride + dim * 4ull) =
@@ -262,7 +283,9 @@ __global__ __launch_bounds__(wave_k *waves_per_block_k) void fingerprint_wave_ke
         }
 #pragma unroll
         for (int s = 0; s < slots_; ++s)
-            if (dim[s] != no_dimension_k) store(a, unit.text, unit.partial, dim[s], any, h.minimum[s], h.count[s]);
+            if (dim[s] != no_dimension_k)
+                store(a, unit.text, unit.partial, run.first + (uint32_t)s * wave_k + lane, dim[s], any, h.minimum[s],
+                      h.count[s]);
     }
 }
 
@@ -319,21 +342,24 @@ __global__ __launch_bounds__(wave_k *waves_per_block_k) void fingerprint_lane_ke
         }
 #pragma unroll
         for (int s = 0; s < lane_slots_k; ++s)
-            if (dim[s] != no_dimension_k) store(a, unit.text, unit.partial, dim[s], any, h.minimum[s], h.count[s]);
+            if (dim[s] != no_dimension_k)
+                store(a, unit.text, unit.partial, group * lane_slots_k + s, dim[s], any, h.minimum[s], h.count[s]);
     }
 }
 
-/** One thread per (long text, dimension): folds the segments in order, equal minima adding their counts. */
+/** One thread per (long text, lane-slot position): folds the segments, equal minima adding their counts. */
 __global__ __launch_bounds__(256) void fingerprint_merge_kernel(fp_merge_t m) {
     uint64_t const index = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (index >= m.texts_count * m.dimensions) return;
-    fp_long_text_t const text = m.texts[index / m.dimensions];
-    uint32_t const dim = (uint32_t)(index % m.dimensions);
+    if (index >= m.texts_count * m.positions) return;
+    fp_long_text_t const text = m.texts[index / m.positions];
+    uint32_t const position = (uint32_t)(index % m.positions);
+    uint32_t const dim = m.dims[position];
+    if (dim == no_dimension_k) return;
     double best = DBL_MAX;
     uint32_t count = 0;
     for (uint64_t p = text.first_partial; p < text.first_partial + text.partials; ++p) {
-        double const value = m.partial_minimums[p * m.dimensions + dim];
-        uint32_t const found = m.partial_counts[p * m.dimensions + dim];
+        double const value = m.partial_minimums[p * m.positions + position];
+        uint32_t const found = m.partial_counts[p * m.positions + position];
         if (value < best) best = value, count = found;
         else if (value == best) count += found;
     }
@@ -571,9 +597,9 @@ void plan_units(std::vector<uint64_t> const &lengths, uint64_t segment, bool sor
 }
 
 template <typename tape_type_>
-sz_status_t fingerprint_call(void *engine_handle, szs_device_scope_t device, tape_type_ const *texts,
-                             sz_u32_t *min_hashes, sz_size_t min_hashes_stride, sz_u32_t *min_counts,
-                             sz_size_t min_counts_stride, char const **error_message) {
+sz_status_t fingerprint_call_unguarded(void *engine_handle, szs_device_scope_t device, tape_type_ const *texts,
+                                       sz_u32_t *min_hashes, sz_size_t min_hashes_stride, sz_u32_t *min_counts,
+                                       sz_size_t min_counts_stride, char const **error_message) {
     if (!engine_handle) return fail(sz_unexpected_dimensions_k, error_message, "Engine must be initialized");
     if (!device) return fail(sz_unexpected_dimensions_k, error_message, "Device must be initialized");
     if (!texts) return fail(sz_unexpected_dimensions_k, error_message, "Input texts cannot be null");
@@ -590,6 +616,8 @@ sz_status_t fingerprint_call(void *engine_handle, szs_device_scope_t device, tap
     hipStream_t stream = nullptr;
     sz_status_t status = resolve_gpu(reinterpret_cast<device_scope_t *>(device), &gpu, &stream, error_message);
     if (status != sz_success_k) return status;
+    hipError_t const ordered = order_after_default_stream(stream);
+    if (ordered != hipSuccess) return from_hip(ordered, error_message, "Ordering after the default stream");
     fp_device_tables_t *tables = nullptr;
     status = device_tables(engine, gpu, stream, &tables, error_message);
     if (status != sz_success_k) return status;
@@ -605,15 +633,19 @@ sz_status_t fingerprint_call(void *engine_handle, szs_device_scope_t device, tap
     } while (0)
     // Text lengths on the host: from the caller's offsets, or read back when they live on the device.
     std::vector<uint64_t> lengths(count);
+    std::vector<uint64_t> offsets;
     if (is_device_pointer(texts->offsets)) {
-        std::vector<uint64_t> offsets(count + 1);
+        offsets.resize(count + 1);
         SZS_FP_TRY(hipMemcpyAsync(offsets.data(), tape.offsets, (count + 1) * 8, hipMemcpyDeviceToHost, stream),
                    "Reading back text offsets");
         SZS_FP_TRY(hipStreamSynchronize(stream), "Reading back text offsets");
-        for (uint64_t i = 0; i < count; ++i) lengths[i] = offsets[i + 1] - offsets[i];
     }
-    else
-        for (uint64_t i = 0; i < count; ++i) lengths[i] = (uint64_t)texts->offsets[i + 1] - (uint64_t)texts->offsets[i];
+    for (uint64_t i = 0; i < count; ++i) {
+        uint64_t const begin = offsets.empty() ? (uint64_t)texts->offsets[i] : offsets[i];
+        uint64_t const end = offsets.empty() ? (uint64_t)texts->offsets[i + 1] : offsets[i + 1];
+        if (end < begin) return fail(sz_unexpected_dimensions_k, error_message, "Text offsets must not decrease");
+        lengths[i] = end - begin;
+    }
 
     // The wave kernel fills its lanes when every width holds at least 64 dimensions; otherwise lanes walk texts.
     bool const lanes = engine.kernel == fp_kernel_lane_k ||
@@ -622,8 +654,9 @@ sz_status_t fingerprint_call(void *engine_handle, szs_device_scope_t device, tap
     if (lanes) {
         uint64_t total = 0;
         for (uint64_t length : lengths) total += length;
-        // About 16K waves of work: several per SIMD at the lane kernel's occupancy, so the tail stays short.
-        uint64_t const target_units = std::max<uint64_t>(1, 16384ull * wave_k / engine.lane_groups.size());
+        // About 32K waves of work, several per SIMD at the lane kernel's occupancy, so the tail stays short: on
+        // 200 MB of 10 KB documents at 64 dimensions this picks 1 KB units, the fastest of 1-64 KB measured.
+        uint64_t const target_units = std::max<uint64_t>(1, 32768ull * wave_k / engine.lane_groups.size());
         segment = 256;
         while (segment < 16384 && segment * target_units < total) segment *= 2;
     }
@@ -647,8 +680,9 @@ sz_status_t fingerprint_call(void *engine_handle, szs_device_scope_t device, tap
     }
     if (!identity) SZS_FP_TRY(upload(units_buffer, units, stream), "Uploading text units");
     if (partials) {
-        SZS_FP_TRY(partial_minimums.reserve(partials * dims * 8), "Failed to allocate segment partials");
-        SZS_FP_TRY(partial_counts.reserve(partials * dims * 4), "Failed to allocate segment partials");
+        uint64_t const positions = lanes ? engine.lane_dims.size() : engine.wave_dims.size();
+        SZS_FP_TRY(partial_minimums.reserve(partials * positions * 8), "Failed to allocate segment partials");
+        SZS_FP_TRY(partial_counts.reserve(partials * positions * 4), "Failed to allocate segment partials");
         SZS_FP_TRY(upload(long_buffer, long_texts, stream), "Uploading long texts");
     }
 
@@ -673,12 +707,13 @@ sz_status_t fingerprint_call(void *engine_handle, szs_device_scope_t device, tap
     args.segment = segment;
     args.hashes = hashes, args.counts = counts, args.hashes_stride = hashes_stride, args.counts_stride = counts_stride;
     args.partial_minimums = partial_minimums.as<double>(), args.partial_counts = partial_counts.as<uint32_t>();
+    args.positions = (uint32_t)(lanes ? engine.lane_dims.size() : engine.wave_dims.size());
     if (error == hipSuccess) error = launch_kernel(engine, lanes, args, stream);
     if (error == hipSuccess && partials) {
-        fp_merge_t merge {long_buffer.as<fp_long_text_t>(), long_texts.size(), (uint32_t)dims,
+        fp_merge_t merge {long_buffer.as<fp_long_text_t>(), long_texts.size(), args.positions, args.dims,
                           partial_minimums.as<double>(), partial_counts.as<uint32_t>(), hashes, counts,
                           hashes_stride, counts_stride};
-        uint64_t const threads = long_texts.size() * dims;
+        uint64_t const threads = long_texts.size() * args.positions;
         hipLaunchKernelGGL(fingerprint_merge_kernel, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, stream,
                            merge);
         error = hipGetLastError();
@@ -702,6 +737,17 @@ sz_status_t fingerprint_call(void *engine_handle, szs_device_scope_t device, tap
     return ok(error_message);
 }
 
+/** The C ABI boundary: no C++ exception (a failed host allocation, for one) crosses it. */
+template <typename tape_type_>
+sz_status_t fingerprint_call(void *engine_handle, szs_device_scope_t device, tape_type_ const *texts,
+                             sz_u32_t *min_hashes, sz_size_t min_hashes_stride, sz_u32_t *min_counts,
+                             sz_size_t min_counts_stride, char const **error_message) {
+    return guarded(error_message, [&]() {
+        return fingerprint_call_unguarded(engine_handle, device, texts, min_hashes, min_hashes_stride, min_counts,
+                                          min_counts_stride, error_message);
+    });
+}
+
 #pragma endregion
 
 } // namespace
@@ -717,8 +763,10 @@ sz_status_t szs_fingerprints_init(sz_size_t dimensions, sz_size_t alphabet_size,
     fp_engine_t *created = new (std::nothrow) fp_engine_t();
     if (!created) return fail(sz_bad_alloc_k, error_message, "Failed to allocate Fingerprints engine");
     static_assert(sizeof(sz_size_t) == sizeof(uint64_t), "64-bit sizes expected");
-    sz_status_t const status = build_engine(*created, dimensions, reinterpret_cast<uint64_t const *>(window_widths),
-                                            window_widths_count, seed, error_message);
+    sz_status_t const status = guarded(error_message, [&]() {
+        return build_engine(*created, dimensions, reinterpret_cast<uint64_t const *>(window_widths),
+                            window_widths_count, seed, error_message);
+    });
     if (status != sz_success_k) {
         delete created;
         return status;
@@ -731,10 +779,12 @@ sz_status_t szs_fingerprints_sequence(szs_fingerprints_t engine, szs_device_scop
                                       sz_u32_t *min_hashes, sz_size_t min_hashes_stride, sz_u32_t *min_counts,
                                       sz_size_t min_counts_stride, char const **error_message) {
     if (!texts) return fail(sz_unexpected_dimensions_k, error_message, "Input texts cannot be null");
-    gathered_t gathered;
-    gathered.gather(texts);
-    return fingerprint_call(engine, device, &gathered.view, min_hashes, min_hashes_stride, min_counts,
-                            min_counts_stride, error_message);
+    return guarded(error_message, [&]() {
+        gathered_t gathered;
+        gathered.gather(texts);
+        return fingerprint_call(engine, device, &gathered.view, min_hashes, min_hashes_stride, min_counts,
+                                min_counts_stride, error_message);
+    });
 }
 
 sz_status_t szs_fingerprints_u64tape(szs_fingerprints_t engine, szs_device_scope_t device,

@@ -139,6 +139,24 @@ hipError_t device_buffer_t::reserve(size_t bytes) {
     return hipSuccess;
 }
 
+hipError_t order_after_default_stream(hipStream_t stream) {
+    int device = 0;
+    hipError_t error = hipGetDevice(&device);
+    if (error != hipSuccess) return error;
+    static thread_local std::map<int, hipEvent_t> events; // one per device, kept for the thread's life
+    hipEvent_t &event = events[device];
+    if (!event) {
+        error = hipEventCreateWithFlags(&event, hipEventDisableTiming);
+        if (error != hipSuccess) {
+            event = nullptr;
+            return error;
+        }
+    }
+    error = hipEventRecord(event, nullptr);
+    if (error == hipSuccess) error = hipStreamWaitEvent(stream, event, 0);
+    return error;
+}
+
 bool is_device_pointer(void const *pointer) {
     if (!pointer) return false;
     hipPointerAttribute_t attributes;

@@ -225,8 +225,11 @@ void server_t::run() {
                 options.pinned_capacity = pinned[free_stream].capacity;
             }
             char const *error = nullptr;
-            sz_status_t status = batch_prepare(scope, inputs, szs_batch_persistent_k, &round.batch, &error, options);
-            if (status == sz_success_k) status = batch_launch(round.batch, &error);
+            sz_status_t const status = guarded(&error, [&]() {
+                sz_status_t prepared = batch_prepare(scope, inputs, szs_batch_persistent_k, &round.batch, &error,
+                                                     options);
+                return prepared == sz_success_k ? batch_launch(round.batch, &error) : prepared;
+            });
             double const prepared = monotonic_seconds();
             lock.lock();
             stats.rounds += 1;
@@ -320,13 +323,15 @@ sz_status_t szs_server_submit(szs_server_t handle, szs_pairs_segment_t const *re
                   footprint_of(engine, 0, seconds_bytes, staged_seconds);
     if (r.footprint > server->budget)
         return fail(sz_bad_alloc_k, error_message, "Request exceeds the server's HBM budget");
-    std::lock_guard<std::mutex> lock(server->mutex);
-    r.ticket = server->next_ticket++;
-    server->tickets[r.ticket] = ticket_t();
-    *ticket = r.ticket;
-    server->pending.push_back(std::move(r));
-    server->wake_dispatcher.notify_one();
-    return ok(error_message);
+    return guarded(error_message, [&]() {
+        std::lock_guard<std::mutex> lock(server->mutex);
+        server->tickets[server->next_ticket] = ticket_t();
+        r.ticket = server->next_ticket++;
+        server->pending.push_back(std::move(r));
+        *ticket = server->pending.back().ticket;
+        server->wake_dispatcher.notify_one();
+        return ok(error_message);
+    });
 }
 
 sz_status_t szs_server_wait_timed(szs_server_t handle, szs_ticket_t ticket, double *completed_seconds,

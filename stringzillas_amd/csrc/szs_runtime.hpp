@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <map>
 #include <mutex>
+#include <new>
 #include <vector>
 
 #include "../../include/stringzillas_amd.h"
@@ -19,6 +20,27 @@ sz_status_t fail(sz_status_t status, char const **error_message, char const *tex
 sz_status_t ok(char const **error_message);
 sz_status_t from_hip(hipError_t error, char const **error_message, char const *context);
 bool is_device_pointer(void const *pointer);
+
+/** Runs `body` so that no C++ exception crosses the C ABI: a failed host allocation becomes `sz_bad_alloc_k`. */
+template <typename callable_type_>
+sz_status_t guarded(char const **error_message, callable_type_ &&body) noexcept {
+    try {
+        return body();
+    }
+    catch (std::bad_alloc const &) {
+        return fail(sz_bad_alloc_k, error_message, "Host memory allocation failed");
+    }
+    catch (...) {
+        return fail(sz_status_unknown_k, error_message, "Unexpected failure on the host");
+    }
+}
+
+/**
+ *  Makes `stream` wait for the work already queued on the device's default (null) stream: device-resident inputs
+ *  written there (by torch's default stream, for one) are complete before this library's non-blocking stream
+ *  reads them. Called at the start of every call that may read caller-owned device memory.
+ */
+hipError_t order_after_default_stream(hipStream_t stream);
 
 /**
  *  A device memory pool carved from one allocation: first-fit over an offset-ordered free list, coalescing on

@@ -146,3 +146,38 @@ def test_u32_tapes_strided_host_outputs_and_utf8_entry_points():
         np.testing.assert_array_equal(counts[:, :ndim], expected_counts)
         assert (hashes[:, ndim:] == 7).all() and (counts[:, ndim:] == 7).all()
         getattr(lib, free)(handle)
+
+
+def test_inputs_written_on_the_default_stream_are_complete_before_hashing():
+    """Offsets and bytes produced by torch on its default stream, handed over with no synchronization: the
+    library's own stream waits for that work (a read racing it once saw stale offsets)."""
+    torch = pytest.importorskip("torch")
+    letters = torch.tensor(list(b"ACGT"), dtype=torch.uint8, device="cuda")
+    engine = szs.Fingerprints(ndim=64)
+    for length, count in ((100000, 40), (700, 3000)):
+        scratch = torch.full((count + 1,), 0x4147, dtype=torch.int64, device="cuda")  # stale-looking contents
+        del scratch
+        data = letters[torch.randint(0, 4, (count * length,), device="cuda")]
+        offsets = torch.arange(count + 1, dtype=torch.int64, device="cuda") * length
+        tape = szs._native.U64Tape(data.data_ptr(), offsets.data_ptr(), count)
+
+        class Texts:
+            def __len__(self):
+                return count
+
+            def tape(self):
+                return tape
+
+        hashes, counts = engine(Texts())
+        host = data.view(count, length)[:3].cpu().numpy()
+        expected_hashes, expected_counts = O.fingerprints(O.tape([row.tobytes() for row in host]), 64)
+        np.testing.assert_array_equal(hashes[:3], expected_hashes)
+        np.testing.assert_array_equal(counts[:3], expected_counts)
+
+
+def test_decreasing_offsets_are_rejected():
+    data = np.frombuffer(b"abcdefgh", dtype=np.uint8)
+    for offsets in (np.array([0, 5, 3, 8], dtype=np.uint64), np.array([4, 0], dtype=np.uint64)):
+        strs = szs.Strs.from_tape(data, offsets)
+        with pytest.raises(ValueError, match="decrease"):
+            szs.Fingerprints(ndim=64)(strs)
