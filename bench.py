@@ -304,9 +304,13 @@ def main():
 # The reference's published fingerprint throughput on an H100 (include/stringzillas/fingerprints/README.md:15-42):
 # equal-length ACGT documents, 64 dimensions, default window widths; MB there is 2^20 bytes.
 FINGERPRINT_H100_MIB_S = {100: 438.7, 1000: 3400.0 * 1.024, 10000: 11200.0 * 1.024, 100000: 5600.0 * 1.024}
-# VALU issue rate of the lane kernel on the 10,000-byte, 64-dimension shape, from SQ_INSTS_VALU
-# (profiles/fingerprint_kernels_acgt_documents.md).
-MEASURED_FINGERPRINT_VALU_ISSUE = {"lane_64": None, "wave_1024": None}
+# Per (ndim, document bytes): VALU wave instructions per hash update (SQ_INSTS_VALU x 64 / updates) and HBM bytes
+# per launch of the hashing kernel (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors of a plain device copy),
+# recorded in profiles/fingerprint_kernels_acgt_documents.md.
+FINGERPRINT_VALU_PER_UPDATE = {(64, 100): 13.6, (64, 1000): 11.3, (64, 10000): 11.6, (64, 100000): 11.4,
+                               (1024, 10000): 10.4}
+FINGERPRINT_TRAFFIC_BYTES = {(64, 10000): 673.7e6, (1024, 10000): 571.1e6}
+VALU_ISSUE_PEAK = 256 * 2.4e9  # wave64 VALU instructions per second: one per cycle per CU
 
 
 def fingerprints_run(corpus_bytes: int, steps: int, with_cpu: bool):
@@ -360,6 +364,14 @@ def fingerprints_run(corpus_bytes: int, steps: int, with_cpu: bool):
             "t_updates_per_s": round(total * ndim / kernel / 1e9, 3),
             "checked_documents": len(picks),
         }
+        per_update = FINGERPRINT_VALU_PER_UPDATE.get((ndim, length))
+        if per_update:
+            issue = total * ndim * per_update / 64 / (kernel / 1e3)
+            entry["roofline"] = {
+                "bound": "valu", "achieved": round(issue / 1e9, 1), "peak": round(VALU_ISSUE_PEAK / 1e9, 1),
+                "unit": "G wave64 VALU instructions/s", "frac": round(issue / VALU_ISSUE_PEAK, 3),
+                "valu_per_update": per_update, "traffic": FINGERPRINT_TRAFFIC_BYTES.get((ndim, length)),
+            }
         if ndim == 64 and length in FINGERPRINT_H100_MIB_S:
             h100 = FINGERPRINT_H100_MIB_S[length] * 1.048576  # MB/s
             entry["reference_h100_mb_per_s"] = round(h100, 1)
@@ -374,8 +386,9 @@ def fingerprints_run(corpus_bytes: int, steps: int, with_cpu: bool):
 
 
 def fingerprint_cpu_baseline(target_seconds: float = 6.0):
-    """The reference's own CPU fingerprint engine (AVX-512 sliced when the host has it, else serial) on 10,000-byte
-    ACGT documents at 64 dimensions, sharded over threads; a bounded sample."""
+    """The reference's own CPU fingerprint engine on 10,000-byte ACGT documents at 64 dimensions, sharded over
+    threads; a bounded sample. At 64 dimensions over 8 widths `szs_fingerprints_init` builds the per-dimension
+    `basic_rolling_hashers` engine (serial code), whichever SIMD build the shim comes from."""
     from oracle import oracle as O
 
     threads = min(16, os.cpu_count() or 1)
@@ -398,7 +411,7 @@ def fingerprint_cpu_baseline(target_seconds: float = 6.0):
         "value": round(documents * 10000 / seconds / 1e6, 2), "unit": "MB/s", "cores": threads,
         "kind": "reference" if variant else "port",
         "sample": f"{documents} ACGT documents of 10,000 bytes at 64 dimensions ({seconds:.1f}s), engine="
-                  f"{'reference ' + variant if variant else 'oracle'}",
+                  f"{'reference basic_rolling_hashers (' + variant + ' build)' if variant else 'oracle'}",
     }
 
 
