@@ -1,7 +1,7 @@
 /**
  *  @file include/stringzillas_amd.h
- *  @brief C ABI of the MI355X-native similarity engine: a drop-in for the `szs_*` similarity symbols of
- *         StringZillas (include/stringzillas/stringzillas.h) plus batch and serving extensions.
+ *  @brief C ABI of the MI355X-native similarity and fingerprint engines: a drop-in for the `szs_*` symbols of
+ *         StringZillas (include/stringzillas/stringzillas.h) plus batch, serving and on-disk tape extensions.
  *
  *  Every reference symbol keeps its name, signature, argument meaning and status codes. The type layouts below
  *  are ABI-identical to include/stringzilla/types.h so a caller compiled against the reference header can link
