@@ -44,7 +44,7 @@ def parse():
     parser.add_argument("--skip-cpu", action="store_true")
     parser.add_argument("--skip-per-call", action="store_true")
     parser.add_argument("--breakdown", action="store_true", help="also time each kind alone")
-    parser.add_argument("--serve-qps", type=float, nargs="*", default=[1000.0, 2000.0],
+    parser.add_argument("--serve-qps", type=float, nargs="*", default=[200.0, 1000.0, 5000.0],
                         help="offered rates of the short serving run (none to skip)")
     parser.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per serving rate")
     parser.add_argument("--skip-fingerprints", action="store_true", help="skip the fingerprint engine's measurement")
