@@ -444,8 +444,16 @@ void szs_tape_free(szs_device_tape_t tape);
  */
 int szs_tuning_slots(void);
 int szs_tuning_shapes(void);
-/** Shape `index` as (lanes per pair, rows per lane, cooperative, bit-parallel). */
-sz_status_t szs_tuning_shape(int index, int *group, int *rows_per_lane, int *cooperative, int *bit_parallel);
+/**
+ *  Shape `index` as (lanes per pair, rows per lane, cooperative, bit-parallel, packed). Packed shapes keep two
+ *  16-bit cells per register; they score the global class-cost families (slots of `nw_linear`/`nw_affine` bytes)
+ *  when the engine's range bound admits their rows per lane, and their int32 twin otherwise.
+ */
+sz_status_t szs_tuning_shape(int index, int *group, int *rows_per_lane, int *cooperative, int *bit_parallel,
+                             int *packed);
+/** Most rows per lane of a packed 16-bit shape an engine handle admits: 8, 6, 4, 2, or 0 when scores could leave
+ *  the 16-bit window (the engine then runs int32 cells everywhere). */
+int szs_tuning_packed_rows(void const *engine);
 sz_status_t szs_tuning_get_ladder(int slot, int *shapes, int *max_rows, int capacity, int *count, int *beyond_shape);
 sz_status_t szs_tuning_set_ladder(int slot, int const *shapes, int const *max_rows, int count, int beyond_shape,
                                   char const **error_message);

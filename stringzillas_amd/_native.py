@@ -111,6 +111,8 @@ for name, restype in {
     "szs_fingerprints_dimensions": _c.c_size_t, "szs_fingerprints_last_kernel_ms": _c.c_double,
 }.items():
     getattr(lib, name).restype = restype
+lib.szs_tuning_packed_rows.restype = _c.c_int
+lib.szs_tuning_packed_rows.argtypes = [_c.c_void_p]
 for name in ("szs_device_scope_free", "szs_levenshtein_distances_free", "szs_levenshtein_distances_utf8_free",
              "szs_needleman_wunsch_scores_free", "szs_smith_waterman_scores_free", "szs_batch_free",
              "szs_server_free", "szs_unified_free", "szs_tape_free", "szs_fingerprints_free",

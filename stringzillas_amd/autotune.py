@@ -29,10 +29,11 @@ FAMILIES = ["lev_linear", "lev_affine", "nw_linear", "nw_affine", "sw_linear", "
 BANDS = [16, 32, 64, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192, 16384]
 
 
-def shapes() -> List[Tuple[int, int, int, int]]:
+def shapes() -> List[Tuple[int, int, int, int, int]]:
+    """Every compiled shape as (lanes per pair, rows per lane, cooperative, bit-parallel, packed 16-bit)."""
     out = []
     for index in range(_native.lib.szs_tuning_shapes()):
-        values = [ctypes.c_int() for _ in range(4)]
+        values = [ctypes.c_int() for _ in range(5)]
         _native.lib.szs_tuning_shape(index, *[ctypes.byref(v) for v in values])
         out.append(tuple(v.value for v in values))
     return out
@@ -108,8 +109,10 @@ def candidates_for(slot: int, low: int, high: int, table) -> List[int]:
     family = "lev_linear" if slot == unit_slot else FAMILIES[slot // 2]
     uniform = family.startswith("lev")
     out = []
-    for index, (group, rows_per_lane, cooperative, bit_parallel) in enumerate(table):
+    for index, (group, rows_per_lane, cooperative, bit_parallel, packed) in enumerate(table):
         if bit_parallel and slot != unit_slot:
+            continue
+        if packed and (slot == unit_slot or family not in ("nw_linear", "nw_affine") or slot % 2 == 1):
             continue
         if not uniform and rows_per_lane > 8:
             continue

@@ -449,6 +449,7 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
         a.unit_costs = engine->params.match == 0 && engine->params.mismatch == 1 && engine->params.open == 1 &&
                        engine->params.extend == 1;
         a.reach_magnitude = std::max(1, std::max(engine->substitution_magnitude, engine->gap_magnitude));
+        a.packed_rows = a.symbols == symbols_u8_k ? engine->packed_rows : 0;
         tasks_total += tasks;
         if (tasks_total >= 0xFFFFFFF0ull)
             return fail(sz_overflow_risk_k, error_message, "A batch holds at most 2^32 - 16 scored cells");
@@ -641,13 +642,19 @@ extern "C" {
 int szs_tuning_slots(void) { return szs_amd::shape_slots_k; }
 int szs_tuning_shapes(void) { return szs_amd::shapes_count_k; }
 
-sz_status_t szs_tuning_shape(int index, int *group, int *rows_per_lane, int *cooperative, int *bit_parallel) {
-    if (index < 0 || index >= szs_amd::shapes_count_k || !group || !rows_per_lane || !cooperative || !bit_parallel)
+sz_status_t szs_tuning_shape(int index, int *group, int *rows_per_lane, int *cooperative, int *bit_parallel,
+                             int *packed) {
+    if (index < 0 || index >= szs_amd::shapes_count_k || !group || !rows_per_lane || !cooperative || !bit_parallel ||
+        !packed)
         return sz_unexpected_dimensions_k;
     szs_amd::shape_t const shape = szs_amd::shapes_k[index];
     *group = shape.group, *rows_per_lane = shape.rows_per_lane;
-    *cooperative = shape.cooperative, *bit_parallel = shape.bit_parallel;
+    *cooperative = shape.cooperative, *bit_parallel = shape.bit_parallel, *packed = shape.packed;
     return sz_success_k;
+}
+
+int szs_tuning_packed_rows(void const *engine) {
+    return engine ? static_cast<szs_amd::engine_t const *>(engine)->packed_rows : 0;
 }
 
 sz_status_t szs_tuning_get_ladder(int slot, int *shapes, int *max_rows, int capacity, int *count, int *beyond_shape) {
@@ -677,6 +684,7 @@ sz_status_t szs_tuning_set_ladder(int slot, int const *shapes, int const *max_ro
         if (shape < 0 || shape >= shapes_count_k) return false;
         shape_t const s = shapes_k[shape];
         if (s.bit_parallel) return slot == unit_bytes_slot_k;
+        if (s.packed) return family == family_nw_linear_k || family == family_nw_affine_k;
         if (!uniform && s.rows_per_lane > 8) return false; // LDS profiles are sized for 8 rows per lane
         return true;
     };

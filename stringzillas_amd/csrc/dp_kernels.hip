@@ -7,19 +7,36 @@
  *  Workgroups have 4 waves. A work item of a non-cooperative shape gives each wave 64/G pairs of the same bin;
  *  a cooperative item gives the whole workgroup one tall pair, whose stripes the waves sweep in a lock-step
  *  pipeline: stripe s runs on wave s mod 4 during phases [s*D, s*D + C) (C = 64-column chunks per stripe,
- *  D = max(2, ceil(C / 4))), reading stripe s-1's bottom row from an HBM ring of 4 rows that stripe s-1 wrote at
- *  least one phase (one workgroup barrier) earlier.
+ *  D = max(skew + 1, ceil(C / 4)), skew = 1, or 2 for the packed 16-bit sweep), reading stripe s-1's bottom row
+ *  from an HBM ring of 4 rows that stripe s-1 wrote at least one phase (one workgroup barrier) earlier.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "dp_sweep.hpp"
 #include "myers_sweep.hpp"
+#include "packed_sweep.hpp"
 #include "szs_device.hpp"
 
 namespace szs_amd {
 
 #pragma region Work items
+
+/** Chunks of G steps that sweep `columns` columns: lane G - 1 finishes column `columns` at step
+ *  columns + skew * G - 2 (the packed sweep's lower half trails its upper half by one more column). */
+template <typename sweep_type_>
+__device__ __forceinline__ int32_t chunks_for(int32_t columns) {
+    constexpr int32_t G = sweep_type_::group_k, skew = sweep_type_::skew_k;
+    return (columns + skew * G - 1 + G - 1) / G;
+}
+
+/** A 64-step chunk in which every lane's columns stay within 1 .. columns - (skew - 1) on every step, so the
+ *  per-step column check (and the packed sweep's edge fix-ups) can be skipped. */
+template <typename sweep_type_>
+__device__ __forceinline__ bool is_steady(int32_t chunk, int32_t columns) {
+    constexpr int32_t skew = sweep_type_::skew_k;
+    return chunk >= skew && chunk * 64 + 64 + (skew - 1) <= columns;
+}
 
 /** Each wave scores 64/G pairs of the item on its own, stripe after stripe for tall pairs (G = 64). */
 template <typename sweep_type_>
@@ -39,7 +56,7 @@ __device__ void run_independent(item_args_t const &a) {
     }
     int32_t const stripe_rows = G * R;
     int32_t const passes = G == 64 ? (sweep.n + stripe_rows - 1) / stripe_rows : 1;
-    int32_t const chunks = (longest + 2 * G - 2) / G; // steps = longest + G - 1
+    int32_t const chunks = chunks_for<sweep_type_>(longest);
     int32_t *scores_row = a.scratch + (size_t)wave * 2 * a.scratch_capacity;
     int32_t *gaps_row = scores_row + a.scratch_capacity;
     // With one pair per wave, chunks whose 64 columns are valid for every lane skip the per-step column check.
@@ -50,7 +67,7 @@ __device__ void run_independent(item_args_t const &a) {
         int32_t const *in_row = pass ? scores_row : nullptr;
         sweep.begin_stripe(base);
         for (int32_t chunk = 0; chunk < chunks; ++chunk) {
-            bool const steady = G == 64 && chunk >= 1 && chunk * 64 + 64 <= columns;
+            bool const steady = G == 64 && is_steady<sweep_type_>(chunk, columns);
             if (more) {
                 if (steady) sweep.template run_chunk<true, true>(chunk, in_row, gaps_row, scores_row, gaps_row);
                 else sweep.template run_chunk<false, true>(chunk, in_row, gaps_row, scores_row, gaps_row);
@@ -74,9 +91,11 @@ __device__ void run_cooperative(item_args_t const &a) {
     sweep.load(a, a.order[a.first], true, (uint32_t)wave);
     int32_t const stripe_rows = 64 * sweep_type_::rows_per_lane;
     int32_t const stripes = (sweep.n + stripe_rows - 1) / stripe_rows;
-    int32_t const chunks = (sweep.m + 126) / 64;
+    int32_t const chunks = chunks_for<sweep_type_>(sweep.m);
     int32_t const columns = __builtin_amdgcn_readfirstlane(sweep.m);
-    int32_t const delay = max(2, (chunks + W - 1) / W);
+    // Stripe s - 1 has written the bottom-row columns that stripe s's chunk x reads by the end of its own chunk
+    // x + skew: a delay of skew + 1 phases keeps the reader behind.
+    int32_t const delay = max(sweep_type_::skew_k + 1, (chunks + W - 1) / W);
     int32_t const phases = (stripes - 1) * delay + chunks;
     size_t const row_pitch = 2 * (size_t)a.scratch_capacity;
     if constexpr (sweep_type_::reduce_k != reduce_none_k) {
@@ -96,7 +115,7 @@ __device__ void run_cooperative(item_args_t const &a) {
             int32_t *in_row = stripe ? a.scratch + ((stripe - 1) % W) * row_pitch : nullptr;
             int32_t *out_row = stripe + 1 < stripes ? a.scratch + (stripe % W) * row_pitch : nullptr;
             int32_t const *in_gaps = in_row ? in_row + a.scratch_capacity : nullptr;
-            bool const steady = chunk >= 1 && chunk * 64 + 64 <= columns;
+            bool const steady = is_steady<sweep_type_>(chunk, columns);
             if (out_row) {
                 int32_t *out_gaps = out_row + a.scratch_capacity;
                 if (steady) sweep.template run_chunk<true, true>(chunk, in_row, in_gaps, out_row, out_gaps);
@@ -134,6 +153,15 @@ __device__ __forceinline__ void run_shape(item_args_t const &a) {
             else run_independent<myers_sweep_t<shape.group>>(a);
         }
     }
+    else if constexpr (shape.packed) {
+        // Packed 16-bit shapes exist for the global class-cost families in their anti-diagonal form only (the
+        // binning never picks them otherwise).
+        if constexpr ((family_ == family_nw_linear_k || family_ == family_nw_affine_k) && sizeof(symbol_type_) == 1 &&
+                      shifted_) {
+            if constexpr (shape.cooperative) run_cooperative<packed_sweep_t<family_, shape.rows_per_lane>>(a);
+            else run_independent<packed_sweep_t<family_, shape.rows_per_lane>>(a);
+        }
+    }
     // Class-cost families keep their LDS profile at <= 8 rows per lane; larger shapes are never binned for them.
     else if constexpr (!uniform && shape.rows_per_lane > 8) return;
     else if constexpr (shape.cooperative)
@@ -159,10 +187,14 @@ __device__ void run_family(int shape, item_args_t const &a) {
     case 12: run_shape<family_, symbol_type_, 12, shifted_>(a); break;
     case 13: run_shape<family_, symbol_type_, 13, shifted_>(a); break;
     case 14: run_shape<family_, symbol_type_, 14, shifted_>(a); break;
+    case 15: run_shape<family_, symbol_type_, 15, shifted_>(a); break;
+    case 16: run_shape<family_, symbol_type_, 16, shifted_>(a); break;
+    case 17: run_shape<family_, symbol_type_, 17, shifted_>(a); break;
+    case 18: run_shape<family_, symbol_type_, 18, shifted_>(a); break;
     default: break;
     }
 }
-static_assert(shapes_count_k == 15, "dispatch switch lists every shape");
+static_assert(shapes_count_k == 19, "dispatch switch lists every shape");
 
 __device__ void run_variant(int variant, item_args_t const &a) {
     int const shape = variant % shapes_count_k;
@@ -559,13 +591,20 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
                     break;
                 }
             }
+            // Packed 16-bit shapes only for engines whose score range provably fits them; else the int32 twin.
+            if (shapes_k[shape].packed && shapes_k[shape].rows_per_lane > a.packed_rows) shape = unpacked_twin(shape);
             shape_t const sh = shapes_k[shape];
             // Latency of the item in lane-row-steps, the key for heaviest-first ordering.
             uint64_t const stripe = (uint64_t)sh.group * sh.rows_per_lane;
             uint64_t passes = (rows + stripe - 1) / stripe;
             if (sh.cooperative) passes = (passes + waves_per_group_k - 1) / waves_per_group_k + 1;
-            uint64_t const per_step = sh.bit_parallel ? 8u : (uint64_t)sh.rows_per_lane; // ~DP rows of VALU work
-            uint64_t const work = passes * ((uint64_t)task.columns_length + sh.group) * per_step;
+            // ~DP rows of VALU work per step; a packed step costs about 3/4 of an int32 one and its stripes are
+            // swept in columns + 2 * group steps.
+            uint64_t const per_step = sh.bit_parallel ? 8u
+                                      : sh.packed     ? (uint64_t)(sh.rows_per_lane * 3 + 3) / 4
+                                                      : (uint64_t)sh.rows_per_lane;
+            uint64_t const ramp = (uint64_t)sh.group * (sh.packed ? 2u : 1u);
+            uint64_t const work = passes * ((uint64_t)task.columns_length + ramp) * per_step;
             bin = (uint32_t)variant_of(family, symbols, shape) * column_buckets_k + half_octave(work);
             atomicAdd(&histogram[bin], 1u);
             cells += (unsigned long long)rows * task.columns_length;

@@ -15,7 +15,8 @@
  *  in VGPRs. At step t lane k updates its R rows at column j = t + 1 - k, then passes its bottom score, bottom
  *  vertical-gap value and the column symbol to lane k+1 with one DPP shift each. Lane 0 receives the row above
  *  the stripe: the matrix border, or the previous stripe's bottom row read back from HBM scratch. Lane G-1's
- *  bottom row is collected 64 columns at a time (G = 64) and written to scratch for the next stripe.
+ *  bottom row is stored step by step into the consumed entries of the LDS ring and written to scratch 64 columns
+ *  at a time (G = 64) for the next stripe.
  *
  *  Global families keep every value relative to its anti-diagonal: Ĉ = H - e*(i+j), ĥ = E - e*(i+j) + (e - o),
  *  v̂ = F - e*(i+j) + (e - o) (e = extend, o = open; for linear gaps e = o = g). In that domain the recurrences lose
@@ -138,6 +139,7 @@ struct sweep_t {
     static constexpr int RP = (R + 3) / 4 * 4; // profile bytes per lane and class, padded to whole dwords
     static constexpr int group_k = G;
     static constexpr int rows_per_lane = R;
+    static constexpr int skew_k = 1; // lane k + 1 runs one column behind lane k
     static constexpr reduce_kind_t reduce_k = local_k ? reduce_max_k : reduce_none_k;
     static constexpr bool diagonal_k = shifted_ && !local_k; /**< values relative to their anti-diagonal */
 
@@ -335,8 +337,6 @@ struct sweep_t {
         int32_t chunk_scores, chunk_gaps;
         uint32_t chunk_symbols;
         column_inputs(j_lane, in_scores, in_gaps, chunk_scores, chunk_gaps, chunk_symbols);
-        int32_t stage_scores = 0, stage_gaps = 0;
-
         if constexpr (G == 64) {
             *(int4 *)(ring + lane * 16) = make_int4(chunk_scores, chunk_gaps, (int32_t)chunk_symbols, 0);
             int4 inject = *(int4 const *)ring;
@@ -351,17 +351,22 @@ struct sweep_t {
                 int32_t const j = first_step + step + 1 - k;
                 if (steady_ || (uint32_t)(j - 1) < (uint32_t)m) update_rows(up_score, up_gap);
                 diagonal_top = up_score;
-                if constexpr (collect_) {
-                    stage_scores = collect_from_last(bottom_score, stage_scores);
-                    if constexpr (affine_k) stage_gaps = collect_from_last(bottom_gap, stage_gaps);
+                // Lane 63's bottom row goes to ring entry `step`, read during the previous step: an LDS store
+                // under a one-lane mask instead of shift registers through every lane. Ring words are accessed as
+                // 32-bit integers throughout, so the compiler sees these stores alias the reads below.
+                if (collect_ && lane == 63) {
+                    int32_t *entry = (int32_t *)(ring + step * 16);
+                    entry[0] = bottom_score;
+                    if constexpr (affine_k) entry[1] = bottom_gap;
                 }
             }
-            // Lane 63's bottom row of these 64 steps, oldest in lane 0: columns first_step - 62 .. first_step + 1.
+            // Entry l holds step l's bottom row: column first_step + l - 62.
             if constexpr (collect_) {
                 int32_t const j_out = first_step + lane - 62;
+                int32_t const *entry = (int32_t const *)(ring + lane * 16);
                 if (j_out >= 1 && j_out <= m) {
-                    out_scores[j_out] = stage_scores;
-                    if constexpr (affine_k) out_gaps[j_out] = stage_gaps;
+                    out_scores[j_out] = entry[0];
+                    if constexpr (affine_k) out_gaps[j_out] = entry[1];
                 }
             }
         }

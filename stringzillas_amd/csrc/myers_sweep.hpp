@@ -68,6 +68,7 @@ template <int G>
 struct myers_sweep_t {
     static constexpr int group_k = G;
     static constexpr int rows_per_lane = 64;
+    static constexpr int skew_k = 1; // lane k + 1 runs one column behind lane k
     static constexpr bool local_k = false;
     static constexpr bool renumber_k = G == 64; /**< one pair per wave: a full byte -> code table */
     static constexpr reduce_kind_t reduce_k = reduce_sum_k;
@@ -260,7 +261,6 @@ struct myers_sweep_t {
         uint32_t const symbol = j_lane <= m ? code_of(columns[j_lane - 1]) : 0u;
         uint32_t const entering = in_codes ? (j_lane <= m ? (uint32_t)in_codes[j_lane] : 0u) : 1u;
         *(uint2 *)(ring + lane * 8) = make_uint2(entering, symbol);
-        int32_t stage = 0;
         // Column codes do not depend on the scores, so step s + 1's code and match word are fetched during step s.
         uint2 now = *(uint2 const *)ring, after = *(uint2 const *)(ring + 8);
         uint32_t code = (uint32_t)shift_down<64>((int32_t)now.y, (int32_t)column_code);
@@ -276,14 +276,17 @@ struct myers_sweep_t {
             column_code = code;
             int32_t const j = first_step + step + 1 - lane;
             if (steady_ || (uint32_t)(j - 1) < (uint32_t)m) step_word(table_ ? eq_now : planes_eq(code), hin);
-            if constexpr (collect_) stage = collect_from_last((int32_t)bottom_code, stage);
+            // Lane 63's bottom code goes to ring entry `step`, read two steps earlier: an LDS store under a
+            // one-lane mask instead of a shift register through every lane.
+            if (collect_ && lane == 63) *(uint32_t *)(ring + step * 8) = bottom_code;
             code = code_after;
             now = after;
             after = later;
         }
-        if constexpr (collect_) {
+        if constexpr (collect_) { // entry l holds step l's bottom code: column first_step + l - 62
             int32_t const j_out = first_step + lane - 62;
-            if (j_out >= 1 && j_out <= m) out_codes[j_out] = stage;
+            uint32_t const bottom = *(uint32_t const *)(ring + lane * 8);
+            if (j_out >= 1 && j_out <= m) out_codes[j_out] = (int32_t)bottom;
         }
     }
 

@@ -1,0 +1,343 @@
+/**
+ *  @file packed_sweep.hpp
+ *  @brief Needleman-Wunsch sweeps with two 16-bit cells per 32-bit register (packed integer VALU).
+ *
+ *  The recurrences are the global ones of include/stringzillas/similarities/serial.hpp:669-749 (linear) and
+ *  854-964 (affine, Gotoh), in the anti-diagonal domain of dp_sweep.hpp: Ĉ = H - e*(i+j), ĥ and v̂ likewise plus
+ *  (e - o), so that
+ *      ĥ = max(Ĉ_left, ĥ_left),  v̂ = max(Ĉ_up, v̂_up),  Ĉ = max(ĥ, v̂, Ĉ_diag + s - o - e) + (o - e)
+ *      linear: Ĉ = max(Ĉ_up, Ĉ_left, Ĉ_diag + s - 2g).
+ *
+ *  Packing. The rows of one lane form a serial chain (each row needs the row above at the same column), so two
+ *  cells of one register must come from independent chains: lane k holds R rows, and its register p carries row
+ *  p of the lane at column c in the low half and row p + R/2 at column c - 1 in the high half. The lower half
+ *  runs one column behind the upper one, so its inputs are the upper half's last row from the previous step; a
+ *  lane's bottom row (high half of the last register) reaches lane k + 1 one column late, hence lane k + 1 runs two
+ *  columns behind lane k (lane k is at column t + 1 - 2k at step t) and a stripe needs m + 127 steps.
+ *  Per two cells the affine step costs one `v_perm_b32` (both halves' costs from the int8 LDS profile: this
+ *  column's byte for the upper row, the previous column's for the lower row), one `v_pk_add_u16` (diagonal +
+ *  cost), four `v_pk_max_u16` and one `v_pk_add_u16` (+ o - e); linear three fewer. Per step add three DPP moves
+ *  and two `v_alignbit_b32` that assemble the packed top inputs (the lane above's bottom row in the low half, this
+ *  lane's upper-half bottom row in the high half).
+ *
+ *  Value range. Cells are stored as u16 = value - base + 32768, where `base` is one wave-uniform int32 value chosen
+ *  at the start of every 64-step chunk (the row above the stripe at the chunk's first column) and every register is
+ *  moved to the new base with one packed add. The engine admits this form only when every value the wave holds
+ *  during a chunk provably stays within 32767 of that base: adjacent cells differ by at most
+ *  delta = max(|o|, max(0, s_max - min(o, e)) + max(0, o - e)) + |e| in this domain, the cells a wave holds during
+ *  a chunk lie within a Manhattan distance of 64R + 320 of each other, and gap tracks and intermediate sums stay
+ *  within a few more deltas of their cell (runtime.hip, `packed_rows_limit`). Otherwise pairs run the int32 sweep.
+ *  Profile bytes hold `cost - (o + e)`, which the engine requires in [0, 127], so they read as unsigned bytes.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dp_sweep.hpp"
+#include "szs_device.hpp"
+
+namespace szs_amd {
+
+typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pk_max_u16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t,
+                              __builtin_elementwise_max(__builtin_bit_cast(u16x2_t, a), __builtin_bit_cast(u16x2_t, b)));
+}
+
+__device__ __forceinline__ uint32_t pk_add_u16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2_t, a) + __builtin_bit_cast(u16x2_t, b));
+}
+
+/** The u16 that stands for int32 `value - base`, centred at 32768. */
+constexpr int32_t packed_centre_k = 32768;
+
+template <int family_, int R>
+struct packed_sweep_t {
+    static_assert(family_ == family_nw_linear_k || family_ == family_nw_affine_k, "global class-cost families only");
+    static_assert(R % 2 == 0 && R >= 2 && R <= 8, "two half-stripes of R/2 rows; profiles hold at most 8 rows");
+    static constexpr int P = R / 2;               // packed registers per lane
+    static constexpr int RP = (R + 3) / 4 * 4;    // profile bytes per lane and class (the int32 sweep's layout)
+    static constexpr bool affine_k = family_ == family_nw_affine_k;
+    static constexpr int group_k = 64;
+    static constexpr int rows_per_lane = R;
+    static constexpr int skew_k = 2;              // lane k + 1 runs two columns behind lane k
+    static constexpr reduce_kind_t reduce_k = reduce_none_k;
+
+    // The pair.
+    int lane;
+    int32_t n, m;
+    int32_t open, extend, dense_count, shift;
+    bool transposed;
+    engine_params_t const *params;
+    uint8_t const *rows;
+    uint8_t const *columns;
+    int64_t *results;
+    uint64_t output, mirror;
+    int8_t *profile; /**< this wave's LDS profile: [dense class][lane][RP] bytes of `cost - (o + e)` */
+    int8_t *ring;    /**< this wave's injection ring: 64 entries of 16 bytes */
+
+    // The stripe.
+    int32_t lane_top;
+    int32_t base;                     /**< wave-uniform: the int32 value that the u16 32768 stands for */
+    uint32_t scores[P], horizontal[P];
+    uint32_t diagonal;                /**< the packed top input of the previous step: this step's diagonal */
+    uint32_t vertical;                /**< vertical gaps below the last register after the previous step */
+    uint32_t symbol_now, symbol_low;  /**< dense classes of this step's upper and lower columns */
+    uint32_t costs_now[RP / 4], costs_low[RP / 4];
+    bool result_upper;                /**< the pair's last row is in this lane's upper half */
+    int32_t result_register;
+    int32_t captured;                 /**< the last row's score at column m, when it sits in the upper half */
+
+    __device__ __forceinline__ void load(item_args_t const &a, uint32_t task_index, bool, uint32_t wave) {
+        int8_t *wave_profile = wave_lds(wave, a.wave_lds_bytes);
+        lane = (int)__lane_id();
+        task_t const task = a.tasks[task_index];
+        segment_view_t const segment = a.segments[task.segment];
+        params = segment.params;
+        n = (int32_t)task.rows_length;
+        m = (int32_t)task.columns_length;
+        open = params->open, extend = params->extend;
+        dense_count = params->dense_count;
+        shift = params->shift;
+        transposed = task.transposed != 0;
+        uint8_t const *query = (uint8_t const *)segment.query_symbols;
+        uint8_t const *candidate = (uint8_t const *)segment.candidate_symbols;
+        rows = (transposed ? candidate : query) + task.rows_offset;
+        columns = (transposed ? query : candidate) + task.columns_offset;
+        results = segment.results;
+        output = task.output;
+        mirror = task.mirror;
+        profile = wave_profile;
+        ring = wave_profile + (a.wave_lds_bytes - inject_ring_bytes_k);
+        captured = 0;
+    }
+
+    /** H at cell (i, j) minus its stored int32 form: -e*(i+j) (linear: e = g). */
+    __device__ __forceinline__ int32_t domain(int32_t i, int32_t j) const { return -extend * (i + j); }
+    __device__ __forceinline__ int32_t gap_domain(int32_t i, int32_t j) const {
+        return -extend * (i + j) + (extend - open);
+    }
+
+    __device__ __forceinline__ uint32_t to_u16(int32_t value) const {
+        return (uint32_t)(value - base + packed_centre_k) & 0xFFFFu;
+    }
+    __device__ __forceinline__ int32_t from_u16(uint32_t half) const {
+        return (int32_t)(half & 0xFFFFu) - packed_centre_k + base;
+    }
+
+    /** Column-0 values of any stripe row (constant in the anti-diagonal domain), as stored int32 values. */
+    __device__ __forceinline__ int32_t border_cell() const {
+        return border_score<family_>(1, open, extend) + domain(1, 0);
+    }
+    __device__ __forceinline__ int32_t border_gap() const {
+        return (open + extend) + border_score<family_>(1, open, extend) + gap_domain(1, 0);
+    }
+
+    __device__ __forceinline__ void read_costs(uint32_t symbol, uint32_t (&out)[RP / 4]) const {
+        uint32_t const *row_costs = (uint32_t const *)(profile + (symbol * 64 + (uint32_t)lane) * RP);
+#pragma unroll
+        for (int w = 0; w < RP / 4; ++w) out[w] = row_costs[w];
+    }
+
+    /** Loads rows `base_row + lane*R .. + R`, builds the profile and resets the stripe to the column-0 border. */
+    __device__ __forceinline__ void begin_stripe(int32_t base_row) {
+        lane_top = base_row + lane * R;
+        uint32_t row_classes[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            int32_t const i = lane_top + r + 1; // 1-based matrix row
+            row_classes[r] = i <= n ? params->byte_to_dense[rows[i - 1]] : 0u;
+        }
+        int8_t *mine = profile + lane * RP;
+        for (int32_t c = 0; c < dense_count; ++c) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                int8_t const cost = transposed ? params->dense_costs[c * max_dense_classes_k + row_classes[r]]
+                                               : params->dense_costs[row_classes[r] * max_dense_classes_k + c];
+                mine[c * 64 * RP + r] = (int8_t)(cost - shift); // in [0, 127]: checked when the engine was made
+            }
+        }
+        base = border_cell();
+        uint32_t const cell = to_u16(border_cell()) * 0x10001u;
+        uint32_t const gap = to_u16(border_gap()) * 0x10001u;
+#pragma unroll
+        for (int p = 0; p < P; ++p) scores[p] = cell, horizontal[p] = gap;
+        // Lane 0's first diagonal is the cell above the stripe at column 0 (the origin for the first stripe).
+        int32_t const above = border_score<family_>(lane_top, open, extend) + domain(lane_top, 0);
+        diagonal = to_u16(above) | (cell & 0xFFFF0000u);
+        vertical = gap;
+        symbol_now = symbol_low = 0;
+#pragma unroll
+        for (int w = 0; w < RP / 4; ++w) costs_now[w] = costs_low[w] = 0;
+        int32_t const last_row = n - 1 - base_row; // beyond this stripe unless it is the last one
+        result_upper = last_row >= 0 && last_row / R == lane && last_row % R < P;
+        result_register = (last_row % R) % P;
+    }
+
+    /** Lane-0 inputs of column `j`: the row above the stripe (stored int32 form), its vertical gap, its symbol. */
+    __device__ __forceinline__ void column_inputs(int32_t j, int32_t const *in_scores, int32_t const *in_gaps,
+                                                  int32_t &score, int32_t &gap, uint32_t &symbol) const {
+        symbol = j <= m ? (uint32_t)params->byte_to_dense[columns[j - 1]] : 0u;
+        if (in_scores) {
+            score = j <= m ? in_scores[j] : 0;
+            gap = (affine_k && j <= m) ? in_gaps[j] : 0;
+        }
+        else {
+            int32_t const border = border_score<family_>(j, open, extend);
+            score = border + domain(0, j);
+            gap = (open + extend) + border + gap_domain(0, j); // discard value of the vertical track on row 0
+        }
+    }
+
+    /** Moves every held value to a new base (exact modulo 2^16; the range bound keeps the result in range). */
+    __device__ __forceinline__ void rebase(int32_t new_base) {
+        uint32_t const delta = ((uint32_t)(base - new_base) & 0xFFFFu) * 0x10001u;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            scores[p] = pk_add_u16(scores[p], delta);
+            if constexpr (affine_k) horizontal[p] = pk_add_u16(horizontal[p], delta);
+        }
+        diagonal = pk_add_u16(diagonal, delta);
+        if constexpr (affine_k) vertical = pk_add_u16(vertical, delta);
+        base = new_base;
+    }
+
+    /** One step of both half-stripes: `up` and `vertical_in` are the packed top inputs. */
+    __device__ __forceinline__ void update_rows(uint32_t up, uint32_t vertical_in, uint32_t kappa) {
+        uint32_t diag = diagonal;
+        uint32_t vert = vertical_in;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            // Upper row p reads this column's profile byte p, lower row P + p the previous column's byte P + p;
+            // selector bytes 0x0C give zeros, so both bytes widen to u16 unsigned.
+            uint32_t const upper_byte = p % 4, lower_byte = (P + p) % 4; // constants once unrolled
+            uint32_t const selector = upper_byte | 0x0C00u | ((4u + lower_byte) << 16) | 0x0C000000u;
+            uint32_t const cost = __builtin_amdgcn_perm(costs_low[(P + p) / 4], costs_now[p / 4], selector);
+            uint32_t const left = scores[p];
+            uint32_t const if_substitution = pk_add_u16(diag, cost);
+            uint32_t stored;
+            if constexpr (affine_k) {
+                uint32_t const h = pk_max_u16(left, horizontal[p]);
+                uint32_t const v = pk_max_u16(up, vert);
+                horizontal[p] = h;
+                vert = v;
+                stored = pk_add_u16(pk_max_u16(pk_max_u16(h, v), if_substitution), kappa);
+            }
+            else stored = pk_max_u16(pk_max_u16(up, left), if_substitution);
+            diag = left;
+            up = stored;
+            scores[p] = stored;
+        }
+        if constexpr (affine_k) vertical = vert;
+    }
+
+    /** The masked sum reads register `index` without a dynamic index (which would demote the array to scratch). */
+    __device__ __forceinline__ uint32_t register_at(int32_t index) const {
+        uint32_t value = 0;
+#pragma unroll
+        for (int p = 0; p < P; ++p) value += scores[p] & -(uint32_t)(index == p);
+        return value;
+    }
+
+    /**
+     *  Steps `chunk * 64 .. + 64`. The row above the stripe comes from `in_scores`/`in_gaps` (1-based columns, int32
+     *  anti-diagonal form) or the matrix border; with `collect_`, lane 63's bottom row goes to `out_scores` and
+     *  `out_gaps`. `steady_` promises both halves of every lane a column within 1..m - 1 on every step.
+     */
+    template <bool steady_, bool collect_>
+    __device__ __forceinline__ void run_chunk(int32_t chunk, int32_t const *in_scores, int32_t const *in_gaps,
+                                              int32_t *out_scores, int32_t *out_gaps) {
+        int32_t const first_step = chunk * 64;
+        // The chunk's base: the row above the stripe at its first column (or the last one), a cell every value
+        // the wave holds during the chunk is close to.
+        int32_t const base_column = min(first_step + 1, m);
+        int32_t base_score, base_gap;
+        uint32_t base_symbol;
+        column_inputs(base_column, in_scores, in_gaps, base_score, base_gap, base_symbol);
+        rebase(__builtin_amdgcn_readfirstlane(base_score));
+
+        int32_t const j_lane = first_step + lane + 1;
+        int32_t chunk_score, chunk_gap;
+        uint32_t chunk_symbol;
+        column_inputs(j_lane, in_scores, in_gaps, chunk_score, chunk_gap, chunk_symbol);
+        *(int4 *)(ring + lane * 16) = make_int4((int32_t)(to_u16(chunk_score) << 16),
+                                                (int32_t)(to_u16(chunk_gap) << 16), (int32_t)chunk_symbol, 0);
+        int4 inject = *(int4 const *)ring;
+        // Lane 0's column symbol comes from the ring; the other lanes fetched theirs during the previous step.
+        if (lane == 0) symbol_now = (uint32_t)inject.z;
+        read_costs(symbol_now, costs_now);
+        uint32_t const kappa = ((uint32_t)(open - extend) & 0xFFFFu) * 0x10001u;
+        uint32_t const border_high = to_u16(border_cell()) << 16, border_gap_high = to_u16(border_gap()) << 16;
+
+#pragma unroll 2
+        for (int32_t step = 0; step < 64; ++step) {
+            int4 const next = *(int4 const *)(ring + ((step + 1) & 63) * 16);
+            // Packed top inputs: the lane above's bottom row (its high half) low, this lane's upper-half bottom row
+            // from the previous step high.
+            uint32_t const from_above = (uint32_t)shift_down<64>(inject.x, (int32_t)scores[P - 1]);
+            uint32_t const up = __builtin_amdgcn_alignbit(scores[P - 1], from_above, 16);
+            uint32_t vertical_in = 0;
+            if constexpr (affine_k) {
+                uint32_t const gaps_above = (uint32_t)shift_down<64>(inject.y, (int32_t)vertical);
+                vertical_in = __builtin_amdgcn_alignbit(vertical, gaps_above, 16);
+            }
+            // Next step's upper column is the lane above's current lower column: fetch its class and costs now.
+            uint32_t const symbol_next = (uint32_t)shift_down<64>(next.z, (int32_t)symbol_low);
+            uint32_t costs_next[RP / 4];
+            read_costs(symbol_next, costs_next);
+            int32_t const c = first_step + step + 1 - 2 * lane; // upper column; the lower half is at c - 1
+            if (steady_ || (uint32_t)(c - 1) <= (uint32_t)m) {
+                update_rows(up, vertical_in, kappa);
+                if (!steady_) {
+                    if (c == 1) { // the lower half stays on the column-0 border
+#pragma unroll
+                        for (int p = 0; p < P; ++p) {
+                            scores[p] = (scores[p] & 0xFFFFu) | border_high;
+                            if constexpr (affine_k) horizontal[p] = (horizontal[p] & 0xFFFFu) | border_gap_high;
+                        }
+                    }
+                    // The upper half moves on to column m + 1 next step: keep the last row's score at column m.
+                    if (result_upper && c == m) captured = from_u16(register_at(result_register));
+                }
+            }
+            diagonal = up;
+            symbol_low = symbol_now;
+            symbol_now = symbol_next;
+#pragma unroll
+            for (int w = 0; w < RP / 4; ++w) costs_low[w] = costs_now[w], costs_now[w] = costs_next[w];
+            inject = next;
+            // Lane 63's bottom row (high halves: score low, vertical gap high) goes to the spare word of ring entry
+            // `step`, which was read during the previous step: one LDS store under a one-lane mask instead of shift
+            // registers through every lane. Every ring access is a 32-bit integer one, so they all alias.
+            if (collect_ && lane == 63)
+                *(uint32_t *)(ring + step * 16 + 12) =
+                    affine_k ? __builtin_amdgcn_perm(vertical, scores[P - 1], 0x07060302u) : scores[P - 1] >> 16;
+        }
+        // Entry l holds step l's bottom row: column first_step + l - 126.
+        if constexpr (collect_) {
+            int32_t const j_out = first_step + lane - 126;
+            uint32_t const bottom = *(uint32_t const *)(ring + lane * 16 + 12);
+            if (j_out >= 1 && j_out <= m) {
+                out_scores[j_out] = from_u16(bottom);
+                if constexpr (affine_k) out_gaps[j_out] = from_u16(bottom >> 16);
+            }
+        }
+    }
+
+    /** Writes the pair's score from the lane holding its last row, on the last stripe. */
+    __device__ __forceinline__ void end_stripe(int32_t base_row, bool last_stripe) {
+        if (!last_stripe) return;
+        int32_t const last_row = n - 1 - base_row;
+        if (last_row / R != lane) return;
+        int32_t value = result_upper ? captured : from_u16(register_at(result_register) >> 16);
+        value -= domain(n, m);
+        results[output] = (int64_t)value;
+        if (mirror != no_mirror_k) results[mirror] = (int64_t)value;
+    }
+
+    __device__ __forceinline__ int32_t partial() const { return 0; }
+    __device__ __forceinline__ void write_result(int32_t) const {}
+};
+
+} // namespace szs_amd

@@ -225,6 +225,42 @@ static sz_status_t densify(engine_t *engine, uint8_t const *byte_to_class, int8_
     return ok(error_message);
 }
 
+/**
+ *  The packed 16-bit sweep (packed_sweep.hpp) stores cells as u16 around a wave-uniform base re-chosen every 64
+ *  columns, so it needs every value a wave holds during a chunk within 32767 of that base. For a maximizing global
+ *  family with o, e <= 0 (the Gotoh recurrences of serial.hpp:854-964; linear is o = e):
+ *  - C(i,j) - C(i-1,j) >= o, since V(i,j) >= C(i-1,j) + o;
+ *  - C(i,j) - C(i-1,j) <= Dp = max(0, s_max - min(o, e)) + max(0, o - e): the diagonal term gains at most
+ *    s_max - min(o, e) over C(i-1,j) (which is at least C(i-1,j-1) + min(o, e)), V(i,j) <= C(i-1,j), and H(i,j)
+ *    gains at most Dp over H(i-1,j) by induction along the row (the border row adds o - e when o > e);
+ *  - the same holds along rows, so adjacent cells of Ĉ = C - e(i+j) differ by at most
+ *    delta = max(|o|, Dp) + |e|;
+ *  - gap tracks stay within max(|o|, Dp) + |e| + |o| of their cell, and every candidate sum within
+ *    2 delta + |s_min| + |o| + |e| + |o - e|.
+ *  The cells held during a chunk (stripe rows plus the row above, columns from lane 63's lower half to the end of
+ *  the injection ring) lie within a Manhattan distance of 64R + 193 of each other; 64R + 320 is used. Profile bytes
+ *  `cost - (o + e)` must also lie in [0, 127] to read as unsigned bytes.
+ */
+int packed_rows_limit(engine_params_t const &params) {
+    if (params.family != family_nw_linear_k && params.family != family_nw_affine_k) return 0;
+    int64_t const o = params.open, e = params.extend;
+    if (o > 0 || e > 0 || params.shift != o + e || params.dense_count == 0) return 0;
+    int64_t s_max = INT64_MIN, s_min = INT64_MAX;
+    for (int x = 0; x < params.dense_count; ++x)
+        for (int y = 0; y < params.dense_count; ++y) {
+            int64_t const cost = params.dense_costs[x * max_dense_classes_k + y];
+            if (cost - params.shift < 0 || cost - params.shift > 127) return 0;
+            s_max = std::max(s_max, cost), s_min = std::min(s_min, cost);
+        }
+    int64_t const rise = std::max<int64_t>(0, s_max - std::min(o, e)) + std::max<int64_t>(0, o - e);
+    int64_t const step = std::max<int64_t>(-o, rise);
+    int64_t const delta = step + (-e);
+    int64_t const slack = 2 * delta + std::abs(s_min) + (-o) + (-e) + std::abs(o - e) + 64;
+    for (int rows_per_lane = 8; rows_per_lane >= 2; rows_per_lane -= 2)
+        if (delta * (64 * rows_per_lane + 320) + slack <= 32767) return rows_per_lane;
+    return 0;
+}
+
 sz_status_t make_engine(int kind, int match, int mismatch, int open, int extend, uint8_t const *byte_to_class,
                         int8_t const *class_costs, void **handle, char const **error_message) {
     if (!handle) return fail(sz_unexpected_dimensions_k, error_message, "Engine output pointer is null");
@@ -272,6 +308,7 @@ sz_status_t make_engine(int kind, int match, int mismatch, int open, int extend,
             int const biased = engine->params.dense_costs[x * max_dense_classes_k + y] - engine->params.shift;
             if (biased < -128 || biased > 127) engine->params.shift = 0;
         }
+    engine->packed_rows = packed_rows_limit(engine->params);
     *handle = engine;
     return ok(error_message);
 }

@@ -85,21 +85,41 @@ struct shape_t {
     int32_t rows_per_lane;
     int32_t cooperative;
     int32_t bit_parallel; /**< unit-cost bytes: one 64-row Myers word per lane instead of R DP rows */
+    int32_t packed;       /**< global class costs: two 16-bit cells per register (two half-stripes per lane) */
 };
 
 /** Waves per workgroup of the scoring kernels. */
 constexpr int waves_per_group_k = 4;
 
-/** The shapes compiled; index = shape id. Bit-parallel shapes exist for the unit-cost byte Levenshtein only. */
-constexpr int shapes_count_k = 15;
+/**
+ *  The shapes compiled; index = shape id. Bit-parallel shapes exist for the unit-cost byte Levenshtein only, packed
+ *  16-bit shapes for the global class-cost families of engines whose scores provably fit (else their int32 twin
+ *  runs).
+ */
+constexpr int shapes_count_k = 19;
 constexpr shape_t shapes_k[shapes_count_k] = {
-    {16, 1, 0, 0}, {16, 2, 0, 0}, {16, 4, 0, 0},    // up to 16 / 32 / 64 rows, four pairs per wave
-    {64, 2, 0, 0}, {64, 3, 0, 0}, {64, 4, 0, 0},    // up to 128 / 192 / 256 rows
-    {64, 6, 0, 0}, {64, 8, 0, 0}, {64, 16, 0, 0},   // up to 384 / 512 / 1024 rows per stripe
-    {64, 8, 1, 0}, {64, 16, 1, 0},                  // cooperative: 512- or 1024-row stripes, whole workgroup
-    {4, 64, 0, 1}, {16, 64, 0, 1},                  // Myers: up to 256 rows (16 pairs/wave), 1024 rows
-    {64, 64, 0, 1}, {64, 64, 1, 1},                 // Myers: 4096-row stripes on one wave, or cooperative
+    {16, 1, 0, 0, 0}, {16, 2, 0, 0, 0}, {16, 4, 0, 0, 0},  // up to 16 / 32 / 64 rows, four pairs per wave
+    {64, 2, 0, 0, 0}, {64, 3, 0, 0, 0}, {64, 4, 0, 0, 0},  // up to 128 / 192 / 256 rows
+    {64, 6, 0, 0, 0}, {64, 8, 0, 0, 0}, {64, 16, 0, 0, 0}, // up to 384 / 512 / 1024 rows per stripe
+    {64, 8, 1, 0, 0}, {64, 16, 1, 0, 0},                   // cooperative: 512- or 1024-row stripes, whole workgroup
+    {4, 64, 0, 1, 0}, {16, 64, 0, 1, 0},                   // Myers: up to 256 rows (16 pairs/wave), 1024 rows
+    {64, 64, 0, 1, 0}, {64, 64, 1, 1, 0},                  // Myers: 4096-row stripes on one wave, or cooperative
+    {64, 4, 0, 0, 1}, {64, 6, 0, 0, 1}, {64, 8, 0, 0, 1},  // packed 16-bit: 256 / 384 / 512-row stripes
+    {64, 8, 1, 0, 1},                                      // packed 16-bit, cooperative 512-row stripes
 };
+
+/** The int32 shape with the same geometry as a packed one (itself for every other shape). */
+__host__ __device__ constexpr int unpacked_twin(int shape) {
+    if (!shapes_k[shape].packed) return shape;
+    for (int s = 0; s < shapes_count_k; ++s)
+        if (!shapes_k[s].packed && !shapes_k[s].bit_parallel && shapes_k[s].group == shapes_k[shape].group &&
+            shapes_k[s].rows_per_lane == shapes_k[shape].rows_per_lane &&
+            shapes_k[s].cooperative == shapes_k[shape].cooperative)
+            return s;
+    return shape;
+}
+static_assert(unpacked_twin(15) == 5 && unpacked_twin(16) == 6 && unpacked_twin(17) == 7 && unpacked_twin(18) == 9,
+              "every packed shape has an int32 twin");
 
 /** Shape-table slot of the unit-cost byte Levenshtein (after the family x symbols slots). */
 constexpr int unit_bytes_slot_k = families_count_k * 2;
@@ -159,6 +179,8 @@ struct materialize_args_t {
     int32_t unit_costs;                  /**< match 0, mismatch 1, gap 1: bit-parallel shapes apply to bytes */
     int32_t runes;                       /**< lengths come from the rune counts (UTF-8) */
     int32_t reach_magnitude;             /**< max |cost| of the engine, for the int32 reach bound */
+    int32_t packed_rows;                 /**< most rows per lane of a packed 16-bit shape the engine admits (0: none) */
+    int32_t reserved;
 };
 
 /** Bytes of the per-wave LDS ring that stages lane 0's inputs of a 64-step chunk (64 entries of 16 bytes); it

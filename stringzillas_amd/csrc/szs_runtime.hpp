@@ -97,8 +97,12 @@ struct engine_t {
     engine_params_t params;
     int substitution_magnitude = 0;
     int gap_magnitude = 0;
+    int packed_rows = 0; /**< most rows per lane of a packed 16-bit sweep whose range bound holds (0: none) */
     std::mutex mutex; /**< one caller at a time, as the reference's engines are not re-entrant */
 };
+
+/** Most rows per lane (8, 6, 4, 2) a packed 16-bit sweep admits for these parameters, 0 for none. */
+int packed_rows_limit(engine_params_t const &params);
 
 sz_status_t make_engine(int kind, int match, int mismatch, int open, int extend, uint8_t const *byte_to_class,
                         int8_t const *class_costs, void **handle, char const **error_message);

@@ -197,3 +197,29 @@ def test_fingerprints_argument_validation_mirrors_the_reference():
             engine(szs.Strs(["hello"]))
     with pytest.raises(RuntimeError, match="CPU"):
         engine(szs.Strs(["hello"]), device=szs.DeviceScope(cpu_cores=2))
+
+
+def test_packed_rows_follow_the_range_bound():
+    """Rows per lane of the packed 16-bit sweep an engine admits (no GPU needed: engines are host objects).
+
+    Worked values for the seeded protein matrix (costs in [-4, 11]): adjacent cells of the anti-diagonal domain
+    differ by at most delta = max(|o|, s_max - min(o, e) + max(0, o - e)) + |e|, and 8 rows per lane need
+    delta * 832 plus a small slack to stay below 32768.
+    """
+    from stringzillas_amd import _native
+    from tests.helpers import dna_tables, protein_tables
+
+    b2c, costs = protein_tables(np.random.default_rng(3))
+    rows = lambda engine: _native.lib.szs_tuning_packed_rows(engine.handle)  # noqa: E731
+    assert rows(szs.NeedlemanWunschScores(b2c, costs, -11, -1)) == 8  # delta = 23: 23 * 832 = 19,136
+    assert rows(szs.NeedlemanWunschScores(b2c, costs, -2, -2)) == 8  # linear, delta = 15
+    assert rows(szs.NeedlemanWunschScores(b2c, costs, -40, -1)) == 4  # delta = 52: only 52 * 576 fits
+    big = costs.copy()
+    np.fill_diagonal(big, 100)
+    assert rows(szs.NeedlemanWunschScores(b2c, big, -11, -1)) == 0  # delta = 112: no packed shape fits
+    assert rows(szs.NeedlemanWunschScores(b2c, costs, 3, 1)) == 0  # odd signs: the bound does not apply
+    assert rows(szs.SmithWatermanScores(b2c, costs, -11, -1)) == 0  # local families stay int32
+    assert rows(szs.LevenshteinDistances()) == 0
+    dna_b2c, dna_costs = dna_tables()
+    assert rows(szs.NeedlemanWunschScores(dna_b2c, dna_costs, -1, -1)) == 0  # -3 - (-2) < 0: no unsigned profile
+    assert rows(szs.NeedlemanWunschScores(dna_b2c, dna_costs, -2, -2)) == 8  # -3 - (-4) = 1: fits
