@@ -171,10 +171,12 @@ def tune(slots: List[int] = None, target_cells: float = 4e9, verbose: bool = Tru
         for shape, high in chosen:
             if table[shape][2]:
                 break  # cooperative from here on
-            if ladder and (ladder[-1][0] == shape or shape in [s for s, _ in ladder]):
-                ladder[-1] = (ladder[-1][0], high)
-            else:
-                ladder.append((shape, high))
+            earlier = [s for s, _ in ladder]
+            if shape in earlier:
+                # A shape that wins again after others takes over the bands in between (a ladder lists each
+                # shape once, in increasing row limits).
+                ladder = ladder[: earlier.index(shape)]
+            ladder.append((shape, high))
         # Sub-wave shapes hold one stripe: clamp their limits to capacity.
         fixed = []
         for shape, high in ladder:
@@ -212,11 +214,22 @@ def apply(path: str = CACHE) -> bool:
 
 
 if __name__ == "__main__":
-    import sys
+    import argparse
 
+    parser = argparse.ArgumentParser(description="Measure and cache the fastest shape ladders.")
+    parser.add_argument("slots", type=int, nargs="*", help="slots to tune (default: the benchmark's kinds)")
+    parser.add_argument("--out", default=CACHE, help="where to write the merged tuning (default: the package cache)")
+    arguments = parser.parse_args()
     start = time.time()
-    chosen_slots = [int(x) for x in sys.argv[1:]] or None
-    tuned = tune(chosen_slots)
-    with open(CACHE, "w") as f:
+    tuned = tune(arguments.slots or None)
+    # Slots not tuned in this run keep their cached ladders when the cache was measured on the same architecture.
+    if os.path.exists(CACHE):
+        with open(CACHE) as f:
+            cached = json.load(f)
+        if cached.get("arch") == tuned["arch"]:
+            tuned["slots"] = {**cached["slots"], **tuned["slots"]}
+            if "note" in cached:
+                tuned["note"] = cached["note"]
+    with open(arguments.out, "w") as f:
         json.dump(tuned, f, indent=1)
-    print(f"wrote {CACHE} in {time.time() - start:.0f}s")
+    print(f"wrote {arguments.out} in {time.time() - start:.0f}s")
