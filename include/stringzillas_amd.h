@@ -439,7 +439,8 @@ void szs_tape_free(szs_device_tape_t tape);
 /**
  *  Tuning (no reference counterpart): the kernel shape chosen for a pair depends on its rows (the shorter side)
  *  through a per-slot ladder of (shape, most rows) steps plus a shape for anything taller. Slots are
- *  `family * 2 + symbols` for the six recurrence families x {bytes, runes}, plus the unit-cost byte slot last.
+ *  `family * 2 + symbols` for the six recurrence families x {bytes, runes}, then the unit-cost Levenshtein
+ *  over bytes (12) and over runes (13), the only slots whose ladders may use bit-parallel shapes.
  *  An autotuner measures alternatives and installs the fastest ladder; changes apply to batches prepared later.
  */
 int szs_tuning_slots(void);

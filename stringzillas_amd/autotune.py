@@ -26,6 +26,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuned_shapes.json")
 
 FAMILIES = ["lev_linear", "lev_affine", "nw_linear", "nw_affine", "sw_linear", "sw_affine"]
+# Slots of the unit-cost Levenshtein over bytes and over runes follow the family x {bytes, runes} slots; only their
+# ladders may use bit-parallel shapes.
+UNIT_BYTES_SLOT = 2 * len(FAMILIES)
+UNIT_RUNES_SLOT = 2 * len(FAMILIES) + 1
+UNIT_SLOTS = (UNIT_BYTES_SLOT, UNIT_RUNES_SLOT)
 BANDS = [16, 32, 64, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192, 16384]
 
 
@@ -58,15 +63,16 @@ def set_ladder(slot: int, ladder, beyond: int) -> None:
 
 def slot_of(family: str, runes: bool = False, unit: bool = False) -> int:
     if unit:
-        return _native.lib.szs_tuning_slots() - 1
+        return UNIT_RUNES_SLOT if runes else UNIT_BYTES_SLOT
     return FAMILIES.index(family) * 2 + (1 if runes else 0)
 
 
 def _engine_for(slot: int):
-    unit_slot = _native.lib.szs_tuning_slots() - 1
     b2c, costs = W.protein_matrix(np.random.default_rng(W.SEED_BASE + 1))
-    if slot == unit_slot:
+    if slot == UNIT_BYTES_SLOT:
         return szs.LevenshteinDistances(), W.DNA
+    if slot == UNIT_RUNES_SLOT:
+        return szs.LevenshteinDistancesUTF8(), None  # codepoints of the mixed batch's four ranges
     family, runes = FAMILIES[slot // 2], slot % 2 == 1
     if family == "lev_linear":
         engine = (szs.LevenshteinDistancesUTF8 if runes else szs.LevenshteinDistances)(0, 1, 2, 2)
@@ -89,6 +95,12 @@ def _batch_for(rng, alphabet, low: int, high: int, target_cells: float, max_pair
     # The longer side log-uniform from the rows up to 16,384 (4x the rows at least), like the mixed batch's pairs.
     top = np.maximum(rows * 4, 16384)
     columns = np.exp(rng.uniform(np.log(rows), np.log(top))).astype(np.int64)
+    if alphabet is None:  # UTF-8 of the mixed batch: about 2.5 bytes per rune, rows counted in runes
+        import torch
+
+        generator = torch.Generator().manual_seed(int(rng.integers(1 << 62)))
+        tapes = [W._torch_utf8_tape(generator, torch.from_numpy(lengths * 5 // 2)) for lengths in (rows, columns)]
+        return tuple(szs.Strs.from_tape(data[:-1].numpy(), offsets.numpy()) for data, offsets in tapes)
     a = W._random_tape(rng, alphabet, rows)
     b = W._random_tape(rng, alphabet, columns)
     return szs.Strs.from_tape(*a), szs.Strs.from_tape(*b)
@@ -105,14 +117,13 @@ def _time(engine, firsts, seconds, repeats: int = 3) -> float:
 
 
 def candidates_for(slot: int, low: int, high: int, table) -> List[int]:
-    unit_slot = _native.lib.szs_tuning_slots() - 1
-    family = "lev_linear" if slot == unit_slot else FAMILIES[slot // 2]
+    family = "lev_linear" if slot in UNIT_SLOTS else FAMILIES[slot // 2]
     uniform = family.startswith("lev")
     out = []
     for index, (group, rows_per_lane, cooperative, bit_parallel, packed) in enumerate(table):
-        if bit_parallel and slot != unit_slot:
+        if bit_parallel and slot not in UNIT_SLOTS:
             continue
-        if packed and (slot == unit_slot or family not in ("nw_linear", "nw_affine") or slot % 2 == 1):
+        if packed and (slot in UNIT_SLOTS or family not in ("nw_linear", "nw_affine") or slot % 2 == 1):
             continue
         if not uniform and rows_per_lane > 8:
             continue
@@ -136,9 +147,9 @@ def tune(slots: List[int] = None, target_cells: float = 4e9, verbose: bool = Tru
     import torch
 
     table = shapes()
-    unit_slot = _native.lib.szs_tuning_slots() - 1
     if slots is None:
-        slots = [unit_slot, slot_of("lev_linear", runes=True), slot_of("nw_affine"), slot_of("sw_linear")]
+        slots = [UNIT_BYTES_SLOT, UNIT_RUNES_SLOT, slot_of("lev_linear", runes=True), slot_of("nw_affine"),
+                 slot_of("sw_linear")]
     result = {"arch": torch.cuda.get_device_properties(0).gcnArchName.split(":")[0], "slots": {}}
     for slot in slots:
         default_ladder, default_beyond = get_ladder(slot)

@@ -66,7 +66,7 @@ static shape_table_t default_shape_table() {
         std::vector<std::pair<int, int>> ladder; // (shape, most rows)
         int multi = 10;
         int const family = slot / 2;
-        if (slot == unit_bytes_slot_k) {
+        if (slot == unit_bytes_slot_k || slot == unit_runes_slot_k) {
             ladder = {{0, 16}, {1, 32}, {2, 64}, {11, 256}, {12, 1024}, {13, 16384}};
             multi = 14;
         }
@@ -676,14 +676,15 @@ sz_status_t szs_tuning_set_ladder(int slot, int const *shapes, int const *max_ro
     using namespace szs_amd;
     if (slot < 0 || slot >= shape_slots_k || count < 0 || count > shapes_count_k || (count && (!shapes || !max_rows)))
         return fail(sz_unexpected_dimensions_k, error_message, "Invalid tuning slot or ladder size");
-    int const family = slot == unit_bytes_slot_k ? family_lev_linear_k : slot / 2;
+    bool const unit = slot == unit_bytes_slot_k || slot == unit_runes_slot_k;
+    int const family = unit ? family_lev_linear_k : slot / 2;
     bool const uniform = family == family_lev_linear_k || family == family_lev_affine_k;
-    bool const runes = slot != unit_bytes_slot_k && slot % 2 == 1;
+    bool const runes = unit ? slot == unit_runes_slot_k : slot % 2 == 1;
     if (!uniform && runes) return fail(sz_unexpected_dimensions_k, error_message, "Class costs never score runes");
     auto allowed = [&](int shape) {
         if (shape < 0 || shape >= shapes_count_k) return false;
         shape_t const s = shapes_k[shape];
-        if (s.bit_parallel) return slot == unit_bytes_slot_k;
+        if (s.bit_parallel) return unit;
         if (s.packed) return family == family_nw_linear_k || family == family_nw_affine_k;
         if (!uniform && s.rows_per_lane > 8) return false; // LDS profiles are sized for 8 rows per lane
         return true;

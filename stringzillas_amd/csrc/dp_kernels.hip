@@ -147,10 +147,12 @@ __device__ __forceinline__ void run_shape(item_args_t const &a) {
     constexpr shape_t shape = shapes_k[shape_];
     constexpr bool uniform = family_traits<family_>::uniform_k;
     if constexpr (shape.bit_parallel) {
-        // Myers shapes exist for the unit-cost byte Levenshtein only (the binning never picks them otherwise).
-        if constexpr (family_ == family_lev_linear_k && sizeof(symbol_type_) == 1) {
-            if constexpr (shape.cooperative) run_cooperative<myers_sweep_t<64>>(a);
-            else run_independent<myers_sweep_t<shape.group>>(a);
+        // Myers shapes exist for the unit-cost Levenshtein over bytes and runes only (the binning never picks them
+        // otherwise).
+        if constexpr (family_ == family_lev_linear_k) {
+            constexpr bool runes = sizeof(symbol_type_) == 4;
+            if constexpr (shape.cooperative) run_cooperative<myers_sweep_t<64, runes>>(a);
+            else run_independent<myers_sweep_t<shape.group, runes>>(a);
         }
     }
     else if constexpr (shape.packed) {
@@ -581,8 +583,9 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
         else {
             uint32_t const rows = task.rows_length;
             int const family = a.family, symbols = a.symbols;
-            int const slot = (a.unit_costs && family == family_lev_linear_k && symbols == symbols_u8_k)
-                                 ? unit_bytes_slot_k : family * 2 + symbols;
+            int const slot = (a.unit_costs && family == family_lev_linear_k)
+                                 ? (symbols == symbols_u8_k ? unit_bytes_slot_k : unit_runes_slot_k)
+                                 : family * 2 + symbols;
             int shape = table->multi_pass_shape[slot];
             for (int s = 0; s < table->used[slot]; ++s) {
                 int const candidate_shape = table->order[slot][s];
@@ -600,7 +603,7 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
             if (sh.cooperative) passes = (passes + waves_per_group_k - 1) / waves_per_group_k + 1;
             // ~DP rows of VALU work per step; a packed step costs about 3/4 of an int32 one and its stripes are
             // swept in columns + 2 * group steps.
-            uint64_t const per_step = sh.bit_parallel ? 8u
+            uint64_t const per_step = sh.bit_parallel ? (symbols == symbols_u8_k ? 8u : 20u) // ~17 rune planes
                                       : sh.packed     ? (uint64_t)(sh.rows_per_lane * 3 + 3) / 4
                                                       : (uint64_t)sh.rows_per_lane;
             uint64_t const ramp = (uint64_t)sh.group * (sh.packed ? 2u : 1u);

@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "dp_sweep.hpp"
 #include "szs_device.hpp"
 
@@ -64,19 +66,26 @@ struct myers_tables_t {
 };
 static_assert(sizeof(myers_tables_t) <= myers_tables_bytes_k, "the match table follows the renumbering tables");
 
-template <int G>
+/**
+ *  `runes_` scores UTF-8 runes (decoded to u32): each rune is its own code, so nothing is renumbered; the planes
+ *  cover the bits of the largest rune in the wave's rows (at most 21), and a column rune with a higher bit set
+ *  matches no row.
+ */
+template <int G, bool runes_ = false>
 struct myers_sweep_t {
+    using symbol_t = typename std::conditional<runes_, uint32_t, uint8_t>::type;
     static constexpr int group_k = G;
     static constexpr int rows_per_lane = 64;
     static constexpr int skew_k = 1; // lane k + 1 runs one column behind lane k
     static constexpr bool local_k = false;
-    static constexpr bool renumber_k = G == 64; /**< one pair per wave: a full byte -> code table */
+    static constexpr bool renumber_k = !runes_ && G == 64; /**< one pair per wave: a full byte -> code table */
+    static constexpr int planes_k = runes_ ? 21 : 8;        /**< bits of a code: a rune has at most 21 */
     static constexpr reduce_kind_t reduce_k = reduce_sum_k;
 
     int lane, k;
     bool active;
     int32_t n, m;
-    uint8_t const *rows, *columns;
+    symbol_t const *rows, *columns;
     int64_t *results;
     uint64_t output, mirror;
     myers_tables_t *tables;
@@ -88,13 +97,14 @@ struct myers_sweep_t {
     bool table_mode;     /**< wave-uniform: at most 15 distinct symbols, match words come from `eq_table` */
 
     int32_t lane_top;
-    uint64_t planes[8];
+    uint64_t planes[planes_k];
     uint64_t pv, mv;
     uint32_t column_code, bottom_code;
     int32_t vertical_sum;
 
     __device__ __forceinline__ uint32_t code_of(uint32_t byte) const {
-        if constexpr (renumber_k) return tables->codes[byte];
+        if constexpr (runes_) return byte;
+        else if constexpr (renumber_k) return tables->codes[byte];
         else {
             if (distinct == 256) return byte; // no absent code left: bytes stay as they are
             uint32_t const word = group_tables->presence[byte >> 5], bit = byte & 31u;
@@ -112,8 +122,8 @@ struct myers_sweep_t {
         segment_view_t const segment = a.segments[task.segment];
         n = active ? (int32_t)task.rows_length : 0;
         m = active ? (int32_t)task.columns_length : 0;
-        uint8_t const *query = (uint8_t const *)segment.query_symbols;
-        uint8_t const *candidate = (uint8_t const *)segment.candidate_symbols;
+        symbol_t const *query = (symbol_t const *)segment.query_symbols;
+        symbol_t const *candidate = (symbol_t const *)segment.candidate_symbols;
         // Uniform costs are symmetric in the operands, so the transposition needs no compensation here.
         rows = (task.transposed ? candidate : query) + task.rows_offset;
         columns = (task.transposed ? query : candidate) + task.columns_offset;
@@ -129,7 +139,16 @@ struct myers_sweep_t {
         vertical_sum = 0;
         planes_used = 8;
         table_mode = false;
-        if constexpr (renumber_k) {
+        if constexpr (runes_) {
+            // Runes are their own codes: plane b holds bit b, up to the highest bit of the wave's largest row rune.
+            uint32_t largest = 0;
+            for (int32_t i = k; i < n; i += G) largest = max(largest, (uint32_t)rows[i]);
+            for (int offset = 1; offset < 64; offset <<= 1)
+                largest = max(largest, (uint32_t)__shfl_xor((int)largest, offset, 64));
+            planes_used = __builtin_amdgcn_readfirstlane(largest ? 32 - __builtin_clz(largest) : 1);
+            distinct = 0;
+        }
+        else if constexpr (renumber_k) {
             // One pair per wave: collect the byte values of the rows, then number them 1..D in byte order.
             if (lane < 8) tables->presence[lane] = 0;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -190,12 +209,13 @@ struct myers_sweep_t {
     __device__ __forceinline__ void begin_stripe(int32_t base) {
         lane_top = base + k * 64;
 #pragma unroll
-        for (int b = 0; b < 8; ++b) planes[b] = 0;
+        for (int b = 0; b < planes_k; ++b) planes[b] = 0;
         for (int32_t r = 0; r < 64; ++r) {
             int32_t const i = lane_top + r; // 0-based row
             uint32_t const code = i < n ? code_of(rows[i]) : 0u;
 #pragma unroll
-            for (int b = 0; b < 8; ++b) planes[b] |= (uint64_t)((code >> b) & 1u) << r;
+            for (int b = 0; b < planes_k; ++b)
+                if (b < planes_used) planes[b] |= (uint64_t)((code >> b) & 1u) << r;
         }
         pv = ~0ull; // column 0: D[i][0] = i, every vertical delta is +1
         mv = 0;
@@ -203,23 +223,31 @@ struct myers_sweep_t {
         column_code = 0;
         if (table_mode) { // the match word of every code for this lane's 64 rows
             for (uint32_t code = 0; code < (uint32_t)myers_eq_codes_k; ++code)
-                eq_table[code * G + k] = planes_eq(code);
+                eq_table[code * G + k] = planes_eq<4>(code); // codes below 16 use planes 0..3
         }
     }
 
-    /** Eq of `code` over this lane's 64 rows: AND over the used planes of (plane XNOR code bit). */
+    /**
+     *  Eq of `code` over this lane's 64 rows: AND over planes 0 .. PB - 1 of (plane XNOR code bit). Any PB of at
+     *  least `planes_used` is exact, because the rows' bits from `planes_used` up are zero: codes of bytes never
+     *  reach them, and a rune with a bit at PB or above matches no row.
+     */
+    template <int PB>
     __device__ __forceinline__ uint64_t planes_eq(uint32_t code) const {
+        static_assert(PB <= planes_k, "planes beyond the array");
         // One 32-bit half at a time; the all-ones or all-zeros mask is formed with a subtraction: a sign-extended
         // 1-bit field widened to 64 bits was compiled as zero-extended (the upper half of Eq came out wrong).
-        uint32_t const inverted = ~code;
         uint32_t eq_low = ~0u, eq_high = ~0u;
+        if constexpr (runes_ && PB < 32) { // a rune wider than every plane matches no row
+            uint32_t const fits = 0u - (uint32_t)((code >> PB) == 0u);
+            eq_low = fits, eq_high = fits;
+        }
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            if (b < planes_used) {
-                uint32_t const mask = 0u - ((inverted >> b) & 1u);
-                eq_low &= (uint32_t)planes[b] ^ mask;
-                eq_high &= (uint32_t)(planes[b] >> 32) ^ mask;
-            }
+        for (int b = 0; b < PB; ++b) {
+            uint32_t const bit = 0u - ((code >> b) & 1u); // all ones where the code has bit b
+            // eq & ~(plane ^ bit), one v_bitop3 per half (truth table 0x90 over the operands' bits 4a + 2b + c)
+            eq_low = __builtin_amdgcn_bitop3_b32(eq_low, (uint32_t)planes[b], bit, 0x90);
+            eq_high = __builtin_amdgcn_bitop3_b32(eq_high, (uint32_t)(planes[b] >> 32), bit, 0x90);
         }
         return (uint64_t)eq_high << 32 | eq_low;
     }
@@ -246,15 +274,26 @@ struct myers_sweep_t {
     template <bool steady_, bool collect_>
     __device__ __forceinline__ void run_chunk(int32_t chunk, int32_t const *in_codes, int32_t const *,
                                               int32_t *out_codes, int32_t *) {
-        if constexpr (G == 64) {
-            if (table_mode) chunk_64<steady_, collect_, true>(chunk, in_codes, out_codes);
-            else chunk_64<steady_, collect_, false>(chunk, in_codes, out_codes);
+        // The plane count is a compile-time constant of the step loop (a guard per plane would cost selects):
+        // the smallest compiled count covering the wave's `planes_used`.
+        if constexpr (runes_) {
+            if (planes_used <= 8) chunk_of<steady_, collect_, false, 8>(chunk, in_codes, out_codes);
+            else if (planes_used <= 12) chunk_of<steady_, collect_, false, 12>(chunk, in_codes, out_codes);
+            else if (planes_used <= 17) chunk_of<steady_, collect_, false, 17>(chunk, in_codes, out_codes);
+            else chunk_of<steady_, collect_, false, 21>(chunk, in_codes, out_codes);
         }
-        else if (table_mode) chunk_small<true>(chunk, in_codes);
-        else chunk_small<false>(chunk, in_codes);
+        else if (table_mode) chunk_of<steady_, collect_, true, 4>(chunk, in_codes, out_codes);
+        else if (planes_used <= 4) chunk_of<steady_, collect_, false, 4>(chunk, in_codes, out_codes);
+        else chunk_of<steady_, collect_, false, 8>(chunk, in_codes, out_codes);
     }
 
-    template <bool steady_, bool collect_, bool table_>
+    template <bool steady_, bool collect_, bool table_, int PB>
+    __device__ __forceinline__ void chunk_of(int32_t chunk, int32_t const *in_codes, int32_t *out_codes) {
+        if constexpr (G == 64) chunk_64<steady_, collect_, table_, PB>(chunk, in_codes, out_codes);
+        else chunk_small<table_, PB>(chunk, in_codes);
+    }
+
+    template <bool steady_, bool collect_, bool table_, int PB>
     __device__ __forceinline__ void chunk_64(int32_t chunk, int32_t const *in_codes, int32_t *out_codes) {
         int32_t const first_step = chunk * 64;
         int32_t const j_lane = first_step + lane + 1;
@@ -275,7 +314,7 @@ struct myers_sweep_t {
             if constexpr (table_) eq_next = eq_table[code_after * 64 + lane];
             column_code = code;
             int32_t const j = first_step + step + 1 - lane;
-            if (steady_ || (uint32_t)(j - 1) < (uint32_t)m) step_word(table_ ? eq_now : planes_eq(code), hin);
+            if (steady_ || (uint32_t)(j - 1) < (uint32_t)m) step_word(table_ ? eq_now : planes_eq<PB>(code), hin);
             // Lane 63's bottom code goes to ring entry `step`, read two steps earlier: an LDS store under a
             // one-lane mask instead of a shift register through every lane.
             if (collect_ && lane == 63) *(uint32_t *)(ring + step * 8) = bottom_code;
@@ -290,7 +329,7 @@ struct myers_sweep_t {
         }
     }
 
-    template <bool table_>
+    template <bool table_, int PB>
     __device__ __forceinline__ void chunk_small(int32_t chunk, int32_t const *in_codes) {
         int32_t const first_step = chunk * G;
         int32_t const j_lane = first_step + k + 1;
@@ -305,7 +344,7 @@ struct myers_sweep_t {
             chunk_symbols = group_rotate_up<G>(chunk_symbols);
             int32_t const j = first_step + step + 1 - k;
             if ((uint32_t)(j - 1) < (uint32_t)m)
-                step_word(table_ ? eq_table[column_code * G + k] : planes_eq(column_code), hin);
+                step_word(table_ ? eq_table[column_code * G + k] : planes_eq<PB>(column_code), hin);
         }
     }
 

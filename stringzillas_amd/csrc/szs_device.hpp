@@ -121,9 +121,11 @@ __host__ __device__ constexpr int unpacked_twin(int shape) {
 static_assert(unpacked_twin(15) == 5 && unpacked_twin(16) == 6 && unpacked_twin(17) == 7 && unpacked_twin(18) == 9,
               "every packed shape has an int32 twin");
 
-/** Shape-table slot of the unit-cost byte Levenshtein (after the family x symbols slots). */
+/** Shape-table slots of the unit-cost Levenshtein over bytes and over runes (after the family x symbols slots):
+ *  the ones whose ladders may use bit-parallel shapes. */
 constexpr int unit_bytes_slot_k = families_count_k * 2;
-constexpr int shape_slots_k = families_count_k * 2 + 1;
+constexpr int unit_runes_slot_k = families_count_k * 2 + 1;
+constexpr int shape_slots_k = families_count_k * 2 + 2;
 
 /** Pairs one work item holds: one per wave slot of a non-cooperative shape, one pair for a cooperative one. */
 __host__ __device__ inline uint32_t pairs_per_item(shape_t shape) {

@@ -1,0 +1,96 @@
+"""GPU parity of the bit-parallel unit-cost Levenshtein over UTF-8 runes (Myers over raw rune bit-planes).
+
+Every bit-parallel shape (4-, 16- and 64-lane groups, several stripes on one wave, the cooperative workgroup) is
+forced through the tuning ladder of the unit-cost rune slot and compared bit-exactly with the CPU oracle. The
+alphabets set the number of planes a wave needs (ASCII 7 bits, Cyrillic 11, CJK 16, emoji 17, up to the 21 bits
+a malformed 4-byte sequence decodes to), and pairs whose columns hold runes wider than every row rune check that
+those match nothing.
+"""
+
+import zlib
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+szs = pytest.importorskip("stringzillas_amd")
+from stringzillas_amd import autotune  # noqa: E402
+
+TABLE = autotune.shapes()
+MYERS = {(s[0], s[2]): i for i, s in enumerate(TABLE) if s[3]}  # (lanes per pair, cooperative) -> shape id
+LADDERS = {
+    "small_groups": ([(MYERS[(4, 0)], 256), (MYERS[(16, 0)], 1024)], MYERS[(64, 1)]),
+    "wave": ([(MYERS[(64, 0)], 1 << 20)], MYERS[(64, 1)]),
+    "cooperative": ([], MYERS[(64, 1)]),
+}
+ALPHABETS = {
+    "ascii": [chr(c) for c in range(0x20, 0x7F)],
+    "cyrillic": [chr(c) for c in range(0x400, 0x500)] + list("abc"),
+    "cjk": [chr(c) for c in range(0x4E00, 0x4E00 + 300)] + [chr(0x9FFF)],
+    "mixed": [chr(c) for c in range(0x20, 0x7F)] + [chr(c) for c in range(0x400, 0x500)]
+    + [chr(c) for c in range(0x4E00, 0x4E00 + 500)] + [chr(c) for c in range(0x1F300, 0x1F3FF)],
+    "nul_and_astral": ["\x00", "a", "\U0010FFFF", "\U00010000", "z", "߿", "ࠀ"],
+}
+EDGES = [1, 2, 63, 64, 65, 255, 256, 257, 1023, 1025, 4095, 4097]
+
+
+def _text(rng, alphabet, length):
+    return "".join(rng.choice(alphabet, size=length)).encode("utf-8")
+
+
+@pytest.fixture
+def rune_ladder():
+    slot = autotune.slot_of("lev_linear", runes=True, unit=True)
+    saved = autotune.get_ladder(slot)
+
+    def install(name):
+        ladder, beyond = LADDERS[name]
+        autotune.set_ladder(slot, ladder, beyond)
+
+    yield install
+    autotune.set_ladder(slot, *saved)
+
+
+def _check(a, b, label):
+    engine = szs.LevenshteinDistancesUTF8()
+    expected = O.pairs(O.Params("levenshtein_utf8"), O.tape(a), O.tape(b), threads=16)
+    got = engine.pairs(szs.Strs(a), szs.Strs(b))
+    np.testing.assert_array_equal(got.astype(np.int64), expected, err_msg=label)
+
+
+@pytest.mark.parametrize("ladder", list(LADDERS))
+@pytest.mark.parametrize("alphabet", list(ALPHABETS))
+def test_rune_myers_matches_oracle(alphabet, ladder, rune_ladder):
+    rng = np.random.default_rng(zlib.crc32(f"{alphabet}:{ladder}".encode()))
+    rune_ladder(ladder)
+    letters = ALPHABETS[alphabet]
+    edges = [e for e in EDGES if ladder != "small_groups" or e <= 1025]
+    a = [_text(rng, letters, n) for n in edges for _ in edges]
+    b = [_text(rng, letters, m) for _ in edges for m in edges]
+    a += [_text(rng, letters, int(rng.integers(0, 3000))) for _ in range(30)]
+    b += [_text(rng, letters, int(rng.integers(0, 3000))) for _ in range(30)]
+    _check(a, b, f"{alphabet} {ladder}")
+
+
+@pytest.mark.parametrize("ladder", list(LADDERS))
+def test_rune_myers_wider_columns_and_malformed_bytes(ladder, rune_ladder):
+    """Rows of narrow runes against columns of wider ones (no match across widths), and raw malformed bytes."""
+    rng = np.random.default_rng(zlib.crc32(f"wide:{ladder}".encode()))
+    rune_ladder(ladder)
+    ascii_, cjk, mixed = ALPHABETS["ascii"], ALPHABETS["cjk"], ALPHABETS["mixed"]
+    a, b = [], []
+    for n in (5, 70, 300, 1100, 2100):
+        narrow = _text(rng, ascii_, n)
+        a += [narrow, narrow, _text(rng, cjk, n)]
+        # Columns: the same ASCII text with every other rune replaced by a wide one whose low bits equal it.
+        wide = "".join(chr(0x1F300 | (ord(c) & 0xFF)) if i % 2 else c for i, c in enumerate(narrow.decode()))
+        b += [wide.encode(), _text(rng, mixed, n + 7), _text(rng, ascii_, n // 2 + 1)]
+    # Malformed input: stray continuation bytes, truncated sequences and 4-byte leads that decode past U+10FFFF.
+    for n in (40, 500, 1500):
+        raw = bytes(rng.choice([0x41, 0x80, 0xBF, 0xC3, 0xE2, 0xF4, 0xF7, 0x9F, 0xA4], size=n).tolist())
+        a.append(raw)
+        b.append(bytes(rng.choice([0x41, 0x80, 0xF7, 0xBF, 0x9F], size=n + 3).tolist()))
+    _check(a, b, f"wide columns and malformed bytes, {ladder}")
