@@ -446,12 +446,13 @@ void szs_tape_free(szs_device_tape_t tape);
 int szs_tuning_slots(void);
 int szs_tuning_shapes(void);
 /**
- *  Shape `index` as (lanes per pair, rows per lane, cooperative, bit-parallel, packed). Packed shapes keep two
- *  16-bit cells per register; they score the global class-cost families (slots of `nw_linear`/`nw_affine` bytes)
- *  when the engine's range bound admits their rows per lane, and their int32 twin otherwise.
+ *  Shape `index` as (lanes per pair, rows per lane, cooperative, bit-parallel, packed, chained). Packed shapes keep
+ *  two 16-bit cells per register; they score the global class-cost families (slots of `nw_linear`/`nw_affine`
+ *  bytes) when the engine's range bound admits their rows per lane, and their int32 twin otherwise. Chained shapes
+ *  split a tall pair's stripes over 16 workgroups that pass bottom rows through HBM (global families only).
  */
 sz_status_t szs_tuning_shape(int index, int *group, int *rows_per_lane, int *cooperative, int *bit_parallel,
-                             int *packed);
+                             int *packed, int *chained);
 /** Most rows per lane of a packed 16-bit shape an engine handle admits: 8, 6, 4, 2, or 0 when scores could leave
  *  the 16-bit window (the engine then runs int32 cells everywhere). */
 int szs_tuning_packed_rows(void const *engine);

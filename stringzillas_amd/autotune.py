@@ -34,11 +34,11 @@ UNIT_SLOTS = (UNIT_BYTES_SLOT, UNIT_RUNES_SLOT)
 BANDS = [16, 32, 64, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192, 16384]
 
 
-def shapes() -> List[Tuple[int, int, int, int, int]]:
-    """Every compiled shape as (lanes per pair, rows per lane, cooperative, bit-parallel, packed 16-bit)."""
+def shapes() -> List[Tuple[int, int, int, int, int, int]]:
+    """Every compiled shape as (lanes per pair, rows per lane, cooperative, bit-parallel, packed 16-bit, chained)."""
     out = []
     for index in range(_native.lib.szs_tuning_shapes()):
-        values = [ctypes.c_int() for _ in range(5)]
+        values = [ctypes.c_int() for _ in range(6)]
         _native.lib.szs_tuning_shape(index, *[ctypes.byref(v) for v in values])
         out.append(tuple(v.value for v in values))
     return out
@@ -120,8 +120,10 @@ def candidates_for(slot: int, low: int, high: int, table) -> List[int]:
     family = "lev_linear" if slot in UNIT_SLOTS else FAMILIES[slot // 2]
     uniform = family.startswith("lev")
     out = []
-    for index, (group, rows_per_lane, cooperative, bit_parallel, packed) in enumerate(table):
+    for index, (group, rows_per_lane, cooperative, bit_parallel, packed, chained) in enumerate(table):
         if bit_parallel and slot not in UNIT_SLOTS:
+            continue
+        if chained:  # for pairs far taller than the bands measured here
             continue
         if packed and (slot in UNIT_SLOTS or family not in ("nw_linear", "nw_affine") or slot % 2 == 1):
             continue

@@ -34,10 +34,12 @@ hipError_t szs_amd_launch_scatter_bins(szs_amd::bins_view_t const *bins, uint32_
                                        uint32_t const *position_of_bin, uint32_t tasks_count, hipStream_t stream);
 hipError_t szs_amd_launch_persistent(szs_amd::task_t const *tasks, szs_amd::segment_view_t const *segments,
                                      szs_amd::bins_view_t const *bins, int32_t *scratch, uint32_t scratch_capacity,
-                                     uint32_t waves, uint32_t lds_bytes, hipStream_t stream);
+                                     uint32_t waves, uint32_t lds_bytes, szs_amd::chain_view_t const *chain,
+                                     hipStream_t stream);
 hipError_t szs_amd_launch_single(szs_amd::task_t const *tasks, szs_amd::segment_view_t const *segments,
                                  uint32_t const *identity, uint32_t task_index, int variant, int32_t *scratch,
-                                 uint32_t scratch_capacity, uint32_t lds_bytes, hipStream_t stream);
+                                 uint32_t scratch_capacity, uint32_t lds_bytes, szs_amd::chain_view_t const *chain,
+                                 hipStream_t stream);
 hipError_t szs_amd_persistent_occupancy(int *blocks_per_cu, uint32_t lds_bytes);
 }
 
@@ -53,11 +55,16 @@ namespace szs_amd {
 
 /**
  *  Default shape choice. Uniform-cost (Levenshtein) families: the smallest single-stripe shape that holds all
- *  rows, 1024-row stripes on one wave up to 4096 rows, then the cooperative 1024-row shape. Class-cost families
- *  keep LDS profiles small: at most 8 rows per lane, 512-row stripes on one wave up to 2048 rows, then the
- *  cooperative 512-row shape. Unit-cost bytes: DP up to 64 rows, then bit-parallel words: 4 lanes up to 256 rows,
+ *  rows, 1024-row stripes on one wave up to 4096 rows, then the cooperative 1024-row shape, chained 512-row
+ *  groups beyond 24,576 rows. Class-cost families keep LDS profiles small: at most 8 rows per lane, 512-row
+ *  stripes on one wave up to 2048 rows, then the cooperative 512-row shape (chained beyond 24,576 rows for the
+ *  global families). Unit-cost bytes: DP up to 64 rows, then bit-parallel words: 4 lanes up to 256 rows,
  *  16 lanes up to 1024, 4096-row stripes on one wave up to 16384, then cooperative.
  */
+/** Pairs taller than this go to the chained shape (global families): one workgroup each would set the batch's
+ *  critical path, 16 share the pair instead. */
+constexpr int chained_rows_k = 24576;
+
 static shape_table_t default_shape_table() {
     shape_table_t table;
     std::memset(&table, 0, sizeof(table));
@@ -74,13 +81,19 @@ static shape_table_t default_shape_table() {
             for (int shape = 0; shape <= 8; ++shape)
                 ladder.push_back({shape, shapes_k[shape].group * shapes_k[shape].rows_per_lane});
             ladder.back().second = 4096; // up to 4 stripes on one wave
-            multi = 10;
+            ladder.push_back({10, chained_rows_k});
+            multi = 20;
         }
         else {
             for (int shape = 0; shape <= 7; ++shape)
                 ladder.push_back({shape, shapes_k[shape].group * shapes_k[shape].rows_per_lane});
             ladder.back().second = 2048;
-            multi = 9;
+            bool const local = family == family_sw_linear_k || family == family_sw_affine_k;
+            if (local) multi = 9; // local scores need every stripe's maximum: no chained groups
+            else {
+                ladder.push_back({9, chained_rows_k});
+                multi = 20;
+            }
         }
         for (size_t i = 0; i < ladder.size(); ++i) {
             table.order[slot][i] = ladder[i].first;
@@ -104,7 +117,9 @@ static double bin_item_cost(uint32_t bin) {
     int const family = variant / shapes_count_k / 2;
     double const per_cell = (family == family_lev_affine_k || family == family_nw_affine_k ||
                              family == family_sw_affine_k) ? 1.8 : 1.0;
-    return std::pow(2.0, bucket / 2.0) * per_cell;
+    // Chained pairs go first: their groups form a pipeline whose fill is the batch's critical path.
+    double const chained = shapes_k[variant % shapes_count_k].chained ? 1e30 : 1.0;
+    return std::pow(2.0, bucket / 2.0) * per_cell * chained;
 }
 
 static std::vector<uint32_t> const &bin_priority() {
@@ -267,22 +282,36 @@ static sz_status_t enqueue_front(batch_t &b, bool bin, char const **error_messag
     return ok(error_message);
 }
 
+static chain_view_t chain_view(batch_t &b) {
+    chain_view_t chain;
+    chain.rows = b.chain_rows.as<int32_t>();
+    chain.progress = b.chain_progress.as<uint32_t>();
+    chain.pitch = b.chain_pitch;
+    chain.timeout = &b.stats.as<prepare_stats_t>()->chain_timeout;
+    return chain;
+}
+
 /** Scoring launches; HIP events bracket them unless they are being captured into a graph. */
 static sz_status_t enqueue_scoring(batch_t &b, bool timed, char const **error_message) {
     hipStream_t const stream = b.stream;
+    chain_view_t const chain = chain_view(b);
+    if (b.chained_pairs) // every group starts with nothing published by its predecessor
+        SZS_TRY_HIP(szs_amd_launch_fill(chain.progress, b.chained_pairs * (uint32_t)chain_groups_k, 0, stream),
+                    "fill");
     if (timed) SZS_TRY_HIP(hipEventRecord(b.kernel_begin, stream), "event");
     if (b.mode == szs_batch_per_pair_k) {
         for (size_t i = 0; i < b.per_pair_tasks.size(); ++i)
             SZS_TRY_HIP(szs_amd_launch_single(b.tasks.as<task_t>(), b.segments.as<segment_view_t>(),
                                               b.identity.as<uint32_t>(), b.per_pair_tasks[i], b.per_pair_variants[i],
-                                              b.scratch.as<int32_t>(), b.scratch_capacity, b.lds_bytes, stream),
+                                              b.scratch.as<int32_t>(), b.scratch_capacity, b.lds_bytes, &chain,
+                                              stream),
                         "per-pair launch");
     }
     else {
         bins_view_t const bins = bins_view(b);
         SZS_TRY_HIP(szs_amd_launch_persistent(b.tasks.as<task_t>(), b.segments.as<segment_view_t>(), &bins,
                                               b.scratch.as<int32_t>(), b.scratch_capacity, b.waves, b.lds_bytes,
-                                              stream),
+                                              &chain, stream),
                     "persistent scoring launch");
     }
     if (timed) SZS_TRY_HIP(hipEventRecord(b.kernel_end, stream), "event");
@@ -570,6 +599,15 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
     uint64_t const scratch_groups = widest_multi_pass ? b.waves : 1;
     SZS_TRY_HIP(b.scratch.reserve(scratch_groups * waves_per_group_k * 2ull * b.scratch_capacity * 4 + 256),
                 "Failed to allocate scratch");
+    // Chained pairs: one bottom row (scores and gaps) between consecutive groups, and a progress word per group.
+    b.chained_pairs = stats.chained_pairs;
+    b.chain_pitch = (stats.chained_columns + 1 + 63) / 64 * 64;
+    if (b.chained_pairs) {
+        SZS_TRY_HIP(b.chain_rows.reserve((size_t)b.chained_pairs * (chain_groups_k - 1) * 2 * b.chain_pitch * 4),
+                    "Failed to allocate the chain rows");
+        SZS_TRY_HIP(b.chain_progress.reserve((size_t)b.chained_pairs * chain_groups_k * 4),
+                    "Failed to allocate the chain progress words");
+    }
     if (mode == szs_batch_per_pair_k) {
         std::vector<uint32_t> identity(slots);
         std::iota(identity.begin(), identity.end(), 0u);
@@ -617,6 +655,14 @@ sz_status_t batch_wait(batch_t *batch, char const **error_message) {
     batch_t &b = *batch;
     SZS_TRY_HIP(hipSetDevice(b.device), "hipSetDevice failed");
     SZS_TRY_HIP(hipStreamSynchronize(b.stream), "scoring failed on the device");
+    if (b.chained_pairs) { // a chained group that gave up waiting leaves its pair's score undefined
+        uint32_t timeout = 0;
+        SZS_TRY_HIP(hipMemcpy(&timeout, &b.stats.as<prepare_stats_t>()->chain_timeout, 4, hipMemcpyDeviceToHost),
+                    "readback");
+        if (timeout)
+            return fail(sz_status_unknown_k, error_message,
+                        "A chained pair's workgroup stopped waiting for its predecessor; its score is undefined");
+    }
     for (size_t s = 0; s < b.inputs.size(); ++s)
         if (b.pinned_offsets[s] != SIZE_MAX)
             std::memcpy(b.host_results[s], b.pinned_results + b.pinned_offsets[s], b.results_elements[s] * 8);
@@ -643,13 +689,14 @@ int szs_tuning_slots(void) { return szs_amd::shape_slots_k; }
 int szs_tuning_shapes(void) { return szs_amd::shapes_count_k; }
 
 sz_status_t szs_tuning_shape(int index, int *group, int *rows_per_lane, int *cooperative, int *bit_parallel,
-                             int *packed) {
+                             int *packed, int *chained) {
     if (index < 0 || index >= szs_amd::shapes_count_k || !group || !rows_per_lane || !cooperative || !bit_parallel ||
-        !packed)
+        !packed || !chained)
         return sz_unexpected_dimensions_k;
     szs_amd::shape_t const shape = szs_amd::shapes_k[index];
     *group = shape.group, *rows_per_lane = shape.rows_per_lane;
     *cooperative = shape.cooperative, *bit_parallel = shape.bit_parallel, *packed = shape.packed;
+    *chained = shape.chained;
     return sz_success_k;
 }
 
@@ -685,6 +732,8 @@ sz_status_t szs_tuning_set_ladder(int slot, int const *shapes, int const *max_ro
         if (shape < 0 || shape >= shapes_count_k) return false;
         shape_t const s = shapes_k[shape];
         if (s.bit_parallel) return unit;
+        // Chained groups hand over bottom rows only: global families, whose score is the last cell.
+        if (s.chained && (unit || family == family_sw_linear_k || family == family_sw_affine_k)) return false;
         if (s.packed) return family == family_nw_linear_k || family == family_nw_affine_k;
         if (!uniform && s.rows_per_lane > 8) return false; // LDS profiles are sized for 8 rows per lane
         return true;

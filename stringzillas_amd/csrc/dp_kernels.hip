@@ -81,11 +81,42 @@ __device__ void run_independent(item_args_t const &a) {
     if constexpr (sweep_type_::reduce_k != reduce_none_k) sweep.write_result(sweep.partial());
 }
 
-/** All waves of the workgroup pipeline the stripes of one pair (see the file comment for the schedule). */
-template <typename sweep_type_>
+/** Upper bound on polls of a predecessor's progress (each a device-scope load plus a ~128-cycle sleep): seconds,
+ *  far beyond any real wait, so a kernel always ends even if a predecessor never published. */
+constexpr uint32_t chain_poll_limit_k = 1u << 22;
+
+/** Waits until the group before this one has published column `needed` of its bottom row. Returns false, and
+ *  raises the batch's timeout word, if the wait was given up: the caller then waits no more for this pair. */
+__device__ __forceinline__ bool wait_for_columns(uint32_t const *progress, uint32_t needed, uint32_t *timeout) {
+    for (uint32_t polls = 0;; ++polls) {
+        uint32_t const published = __hip_atomic_load(progress, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (published >= needed) return true;
+        if (polls == chain_poll_limit_k) {
+            if (__lane_id() == 0) atomicOr(timeout, 1u);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+/** Makes this wave's bottom-row stores visible device-wide, then publishes how many columns are complete. */
+__device__ __forceinline__ void publish_columns(uint32_t *progress, uint32_t columns) {
+    __threadfence();
+    if (__lane_id() == 0) __hip_atomic_store(progress, columns, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/**
+ *  All waves of the workgroup pipeline the stripes of one pair (see the file comment for the schedule). With
+ *  `chained_`, the pair's stripes are split into `chain_groups_k` groups of consecutive stripes, one work item
+ *  (one workgroup) each: a group's first stripe reads the previous group's bottom row from the chain buffer as
+ *  that group publishes it, and its last stripe publishes its own. Items are taken in queue order and a group
+ *  only waits for earlier groups, which are already running, so every wait ends.
+ */
+template <typename sweep_type_, bool chained_ = false>
 __device__ void run_cooperative(item_args_t const &a) {
     constexpr int32_t W = waves_per_group_k;
     static_assert(sweep_type_::group_k == 64, "cooperative items sweep 64-lane stripes");
+    static_assert(!chained_ || sweep_type_::reduce_k == reduce_none_k, "chained groups combine no partial results");
     int32_t const wave = (int32_t)(threadIdx.x / 64);
     sweep_type_ sweep;
     sweep.load(a, a.order[a.first], true, (uint32_t)wave);
@@ -95,37 +126,83 @@ __device__ void run_cooperative(item_args_t const &a) {
     int32_t const columns = __builtin_amdgcn_readfirstlane(sweep.m);
     // Stripe s - 1 has written the bottom-row columns that stripe s's chunk x reads by the end of its own chunk
     // x + skew: a delay of skew + 1 phases keeps the reader behind.
-    int32_t const delay = max(sweep_type_::skew_k + 1, (chunks + W - 1) / W);
-    int32_t const phases = (stripes - 1) * delay + chunks;
+    // A wave that sweeps several stripes of the item (more than W) runs them one after another: the stagger then
+    // spreads them over the waves, ceil(chunks / W) phases apart.
+    int32_t const stagger = max(sweep_type_::skew_k + 1, (chunks + W - 1) / W);
     size_t const row_pitch = 2 * (size_t)a.scratch_capacity;
     if constexpr (sweep_type_::reduce_k != reduce_none_k) {
         if (threadIdx.x == 0) *a.shared_result = 0;
     }
-    for (int32_t phase = 0; phase < phases; ++phase) {
-        int32_t stripe = -1, chunk = 0;
-        if (phase >= wave * delay) {
-            int32_t const since = phase - wave * delay;
-            int32_t const candidate = wave + (since / (W * delay)) * W;
-            int32_t const at = since % (W * delay);
-            if (candidate < stripes && at < chunks) stripe = candidate, chunk = at;
+    int32_t groups = 1, per_group = stripes, first_group = 0, last_group = 0;
+    uint32_t slot = 0;
+    bool waiting = chained_; // cleared if a wait is given up
+    if constexpr (chained_) {
+        groups = chain_groups_k;
+        per_group = max(W, (stripes + groups - 1) / groups);
+        first_group = a.group >= 0 ? a.group : 0;
+        last_group = a.group >= 0 ? a.group : groups - 1;
+        slot = task_chain_slot(a.tasks[a.order[a.first]]);
+    }
+    for (int32_t group = first_group; group <= last_group; ++group) {
+        int32_t const first_stripe = group * per_group;
+        int32_t const end_stripe = min(stripes, first_stripe + per_group);
+        if (first_stripe >= end_stripe) break; // trailing groups of a shorter pair hold no stripes
+        int32_t const local_stripes = end_stripe - first_stripe;
+        // At most one stripe per wave: the minimal delay that keeps each stripe behind the one above.
+        int32_t const delay = local_stripes <= W ? sweep_type_::skew_k + 1 : stagger;
+        // Chain buffers: the bottom row entering this group and the one leaving it, with their column counts.
+        int32_t *chain_in = nullptr, *chain_out = nullptr;
+        uint32_t *progress_in = nullptr, *progress_out = nullptr;
+        if constexpr (chained_) {
+            size_t const pitch = 2 * (size_t)a.chain_pitch;
+            int32_t *base = a.chain_rows + (size_t)slot * (chain_groups_k - 1) * pitch;
+            if (group > 0) chain_in = base + (size_t)(group - 1) * pitch, progress_in = a.chain_progress + slot * chain_groups_k + group - 1;
+            if (end_stripe < stripes) chain_out = base + (size_t)group * pitch, progress_out = a.chain_progress + slot * chain_groups_k + group;
         }
-        if (stripe >= 0) { // wave-uniform
-            int32_t const base = stripe * stripe_rows;
-            if (chunk == 0) sweep.begin_stripe(base);
-            int32_t *in_row = stripe ? a.scratch + ((stripe - 1) % W) * row_pitch : nullptr;
-            int32_t *out_row = stripe + 1 < stripes ? a.scratch + (stripe % W) * row_pitch : nullptr;
-            int32_t const *in_gaps = in_row ? in_row + a.scratch_capacity : nullptr;
-            bool const steady = is_steady<sweep_type_>(chunk, columns);
-            if (out_row) {
-                int32_t *out_gaps = out_row + a.scratch_capacity;
-                if (steady) sweep.template run_chunk<true, true>(chunk, in_row, in_gaps, out_row, out_gaps);
-                else sweep.template run_chunk<false, true>(chunk, in_row, in_gaps, out_row, out_gaps);
+        int32_t const phases = (local_stripes - 1) * delay + chunks;
+        for (int32_t phase = 0; phase < phases; ++phase) {
+            int32_t local = -1, chunk = 0;
+            if (phase >= wave * delay) {
+                int32_t const since = phase - wave * delay;
+                if (local_stripes <= W) { // wave w sweeps stripe w alone, `delay` phases behind wave w - 1
+                    if (wave < local_stripes && since < chunks) local = wave, chunk = since;
+                }
+                else { // wave w sweeps stripes w, w + W, ... one after another (W * delay >= chunks)
+                    int32_t const candidate = wave + (since / (W * delay)) * W;
+                    int32_t const at = since % (W * delay);
+                    if (candidate < local_stripes && at < chunks) local = candidate, chunk = at;
+                }
             }
-            else if (steady) sweep.template run_chunk<true, false>(chunk, in_row, in_gaps, nullptr, nullptr);
-            else sweep.template run_chunk<false, false>(chunk, in_row, in_gaps, nullptr, nullptr);
-            if (chunk == chunks - 1) sweep.end_stripe(base, stripe == stripes - 1);
+            if (local >= 0) { // wave-uniform
+                int32_t const stripe = first_stripe + local;
+                int32_t const base = stripe * stripe_rows;
+                if (chunk == 0) sweep.begin_stripe(base);
+                int32_t *in_row = local ? a.scratch + ((local - 1) % W) * row_pitch : chain_in;
+                int32_t *out_row = local + 1 < local_stripes ? a.scratch + (local % W) * row_pitch : chain_out;
+                size_t const in_gap_offset = local ? a.scratch_capacity : a.chain_pitch;
+                size_t const out_gap_offset = local + 1 < local_stripes ? a.scratch_capacity : a.chain_pitch;
+                int32_t const *in_gaps = in_row ? in_row + in_gap_offset : nullptr;
+                if (chained_ && !local && chain_in && waiting)
+                    waiting = wait_for_columns(progress_in, (uint32_t)min(chunk * 64 + 64, columns), a.chain_timeout);
+                bool const steady = is_steady<sweep_type_>(chunk, columns);
+                if (out_row) {
+                    int32_t *out_gaps = out_row + out_gap_offset;
+                    if (steady) sweep.template run_chunk<true, true>(chunk, in_row, in_gaps, out_row, out_gaps);
+                    else sweep.template run_chunk<false, true>(chunk, in_row, in_gaps, out_row, out_gaps);
+                    if (chained_ && out_row == chain_out) {
+                        // Columns up to lane 63's last lower-half column of this chunk are final; all of them after
+                        // the last chunk.
+                        int32_t const done = chunk == chunks - 1 ? columns
+                                                                 : chunk * 64 + 1 - 64 * (sweep_type_::skew_k - 1);
+                        publish_columns(progress_out, (uint32_t)max(0, min(done, columns)));
+                    }
+                }
+                else if (steady) sweep.template run_chunk<true, false>(chunk, in_row, in_gaps, nullptr, nullptr);
+                else sweep.template run_chunk<false, false>(chunk, in_row, in_gaps, nullptr, nullptr);
+                if (chunk == chunks - 1) sweep.end_stripe(base, stripe == stripes - 1);
+            }
+            __syncthreads(); // publishes this phase's boundary rows to the next stripe's wave
         }
-        __syncthreads(); // publishes this phase's boundary rows to the next stripe's wave
     }
     if constexpr (sweep_type_::reduce_k != reduce_none_k) {
         int32_t const mine = sweep.partial();
@@ -160,12 +237,18 @@ __device__ __forceinline__ void run_shape(item_args_t const &a) {
         // binning never picks them otherwise).
         if constexpr ((family_ == family_nw_linear_k || family_ == family_nw_affine_k) && sizeof(symbol_type_) == 1 &&
                       shifted_) {
-            if constexpr (shape.cooperative) run_cooperative<packed_sweep_t<family_, shape.rows_per_lane>>(a);
+            if constexpr (shape.chained) run_cooperative<packed_sweep_t<family_, shape.rows_per_lane>, true>(a);
+            else if constexpr (shape.cooperative) run_cooperative<packed_sweep_t<family_, shape.rows_per_lane>>(a);
             else run_independent<packed_sweep_t<family_, shape.rows_per_lane>>(a);
         }
     }
     // Class-cost families keep their LDS profile at <= 8 rows per lane; larger shapes are never binned for them.
     else if constexpr (!uniform && shape.rows_per_lane > 8) return;
+    else if constexpr (shape.chained) {
+        // Chained groups pass only bottom rows along: global families, whose result is the last cell.
+        if constexpr (!family_traits<family_>::local_k)
+            run_cooperative<sweep_t<family_, symbol_type_, 64, shape.rows_per_lane, shifted_>, true>(a);
+    }
     else if constexpr (shape.cooperative)
         run_cooperative<sweep_t<family_, symbol_type_, 64, shape.rows_per_lane, shifted_>>(a);
     else run_independent<sweep_t<family_, symbol_type_, shape.group, shape.rows_per_lane, shifted_>>(a);
@@ -193,10 +276,12 @@ __device__ void run_family(int shape, item_args_t const &a) {
     case 16: run_shape<family_, symbol_type_, 16, shifted_>(a); break;
     case 17: run_shape<family_, symbol_type_, 17, shifted_>(a); break;
     case 18: run_shape<family_, symbol_type_, 18, shifted_>(a); break;
+    case 19: run_shape<family_, symbol_type_, 19, shifted_>(a); break;
+    case 20: run_shape<family_, symbol_type_, 20, shifted_>(a); break;
     default: break;
     }
 }
-static_assert(shapes_count_k == 19, "dispatch switch lists every shape");
+static_assert(shapes_count_k == 21, "dispatch switch lists every shape");
 
 __device__ void run_variant(int variant, item_args_t const &a) {
     int const shape = variant % shapes_count_k;
@@ -231,6 +316,7 @@ struct persistent_args_t {
     int32_t *scratch;
     uint32_t scratch_capacity;
     uint32_t wave_lds_bytes;
+    chain_view_t chain;
 };
 
 /**
@@ -249,6 +335,10 @@ __global__ __launch_bounds__(256, 4) void persistent_score_kernel(persistent_arg
     item.scratch_capacity = args.scratch_capacity;
     item.wave_lds_bytes = args.wave_lds_bytes;
     item.shared_result = &result_shared;
+    item.chain_rows = args.chain.rows;
+    item.chain_progress = args.chain.progress;
+    item.chain_pitch = args.chain.pitch;
+    item.chain_timeout = args.chain.timeout;
     for (;;) {
         if (threadIdx.x == 0) taken_shared = atomicAdd(args.bins.queue_head, 1u);
         __syncthreads();
@@ -264,11 +354,26 @@ __global__ __launch_bounds__(256, 4) void persistent_score_kernel(persistent_arg
         }
         uint32_t const bin = args.bins.priority[low];
         int const variant = (int)(bin / column_buckets_k);
-        uint32_t const per_item = pairs_per_item(shapes_k[variant % shapes_count_k]);
+        // Items far longer than the rest (the batch's tallest pairs, and the groups of chained ones) set its
+        // critical path: their waves take precedence in the SIMD arbiter, and the other waves on those SIMDs fill
+        // the cycles they leave.
+        if ((int)(bin % column_buckets_k) >= critical_bucket_k || shapes_k[variant % shapes_count_k].chained)
+            __builtin_amdgcn_s_setprio(3);
+        else __builtin_amdgcn_s_setprio(0);
+        shape_t const shape = shapes_k[variant % shapes_count_k];
         uint32_t const item_in_bin = taken - args.bins.item_start[low];
-        uint32_t const in_bin = args.bins.histogram[bin];
-        item.first = args.bins.task_start[low] + item_in_bin * per_item;
-        item.count = min(per_item, in_bin - item_in_bin * per_item);
+        if (shape.chained) { // consecutive items are the groups of one pair, in order
+            item.first = args.bins.task_start[low] + item_in_bin / chain_groups_k;
+            item.count = 1;
+            item.group = (int32_t)(item_in_bin % chain_groups_k);
+        }
+        else {
+            uint32_t const per_item = pairs_per_item(shape);
+            uint32_t const in_bin = args.bins.histogram[bin];
+            item.first = args.bins.task_start[low] + item_in_bin * per_item;
+            item.count = min(per_item, in_bin - item_in_bin * per_item);
+            item.group = 0;
+        }
         run_variant(variant, item);
     }
 }
@@ -277,7 +382,7 @@ __global__ __launch_bounds__(256, 4) void persistent_score_kernel(persistent_arg
 __global__ __launch_bounds__(256, 4) void single_task_kernel(task_t const *tasks, segment_view_t const *segments,
                                                           uint32_t const *identity, uint32_t task_index, int variant,
                                                           int32_t *scratch, uint32_t scratch_capacity,
-                                                          uint32_t wave_lds_bytes) {
+                                                          uint32_t wave_lds_bytes, chain_view_t chain) {
     __shared__ int32_t result_shared;
     item_args_t item;
     item.tasks = tasks;
@@ -289,6 +394,11 @@ __global__ __launch_bounds__(256, 4) void single_task_kernel(task_t const *tasks
     item.scratch_capacity = scratch_capacity;
     item.wave_lds_bytes = wave_lds_bytes;
     item.shared_result = &result_shared;
+    item.group = -1; // a chained pair runs all its groups in turn here
+    item.chain_rows = chain.rows;
+    item.chain_progress = chain.progress;
+    item.chain_pitch = chain.pitch;
+    item.chain_timeout = chain.timeout;
     run_variant(variant, item);
 }
 
@@ -552,7 +662,7 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
             task.mirror = (a.mode == 2 && q != c) ? c * a.results_row_stride + q : no_mirror_k;
         }
         bool const transposed = c_length < q_length;
-        task.transposed = transposed;
+        task.flags = transposed ? task_transposed_k : 0u;
         task.rows_offset = transposed ? c_begin : q_begin;
         task.columns_offset = transposed ? q_begin : c_begin;
         task.rows_length = (uint32_t)(transposed ? c_length : q_length);
@@ -600,7 +710,10 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
             // Latency of the item in lane-row-steps, the key for heaviest-first ordering.
             uint64_t const stripe = (uint64_t)sh.group * sh.rows_per_lane;
             uint64_t passes = (rows + stripe - 1) / stripe;
-            if (sh.cooperative) passes = (passes + waves_per_group_k - 1) / waves_per_group_k + 1;
+            if (sh.cooperative) {
+                if (sh.chained) passes = max<uint64_t>(waves_per_group_k, (passes + chain_groups_k - 1) / chain_groups_k);
+                passes = (passes + waves_per_group_k - 1) / waves_per_group_k + 1;
+            }
             // ~DP rows of VALU work per step; a packed step costs about 3/4 of an int32 one and its stripes are
             // swept in columns + 2 * group steps.
             uint64_t const per_step = sh.bit_parallel ? (symbols == symbols_u8_k ? 8u : 20u) // ~17 rune planes
@@ -608,6 +721,11 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
                                                       : (uint64_t)sh.rows_per_lane;
             uint64_t const ramp = (uint64_t)sh.group * (sh.packed ? 2u : 1u);
             uint64_t const work = passes * ((uint64_t)task.columns_length + ramp) * per_step;
+            if (sh.chained) { // a slot in the chain buffers, sized by the host from these stats
+                uint32_t const chain_slot = atomicAdd(&stats->chained_pairs, 1u);
+                atomicMax(&stats->chained_columns, task.columns_length);
+                task.flags |= (chain_slot + 1u) << 8;
+            }
             bin = (uint32_t)variant_of(family, symbols, shape) * column_buckets_k + half_octave(work);
             atomicAdd(&histogram[bin], 1u);
             cells += (unsigned long long)rows * task.columns_length;
@@ -641,9 +759,8 @@ __global__ __launch_bounds__(1024) void scan_bins_kernel(bins_view_t bins) {
         uint32_t tasks_here = 0, items_here = 0;
         if (p < bins_count_k) {
             uint32_t const bin = bins.priority[p];
-            uint32_t const per_item = pairs_per_item(shapes_k[(bin / column_buckets_k) % shapes_count_k]);
             tasks_here = bins.histogram[bin];
-            items_here = (tasks_here + per_item - 1) / per_item;
+            items_here = items_for(shapes_k[(bin / column_buckets_k) % shapes_count_k], tasks_here);
         }
         partial_tasks[threadIdx.x] = tasks_here;
         partial_items[threadIdx.x] = items_here;
@@ -741,7 +858,8 @@ hipError_t szs_amd_launch_scatter_bins(szs_amd::bins_view_t const *bins, uint32_
 
 hipError_t szs_amd_launch_persistent(szs_amd::task_t const *tasks, szs_amd::segment_view_t const *segments,
                                      szs_amd::bins_view_t const *bins, int32_t *scratch, uint32_t scratch_capacity,
-                                     uint32_t workgroups, uint32_t profile_bytes, hipStream_t stream) {
+                                     uint32_t workgroups, uint32_t profile_bytes, szs_amd::chain_view_t const *chain,
+                                     hipStream_t stream) {
     szs_amd::persistent_args_t args;
     args.tasks = tasks;
     args.segments = segments;
@@ -749,6 +867,7 @@ hipError_t szs_amd_launch_persistent(szs_amd::task_t const *tasks, szs_amd::segm
     args.scratch = scratch;
     args.scratch_capacity = scratch_capacity;
     args.wave_lds_bytes = profile_bytes;
+    args.chain = *chain;
     hipLaunchKernelGGL(szs_amd::persistent_score_kernel, dim3(workgroups), dim3(64 * szs_amd::waves_per_group_k),
                        profile_bytes * szs_amd::waves_per_group_k, stream, args);
     return hipGetLastError();
@@ -756,10 +875,11 @@ hipError_t szs_amd_launch_persistent(szs_amd::task_t const *tasks, szs_amd::segm
 
 hipError_t szs_amd_launch_single(szs_amd::task_t const *tasks, szs_amd::segment_view_t const *segments,
                                  uint32_t const *identity, uint32_t task_index, int variant, int32_t *scratch,
-                                 uint32_t scratch_capacity, uint32_t profile_bytes, hipStream_t stream) {
+                                 uint32_t scratch_capacity, uint32_t profile_bytes, szs_amd::chain_view_t const *chain,
+                                 hipStream_t stream) {
     hipLaunchKernelGGL(szs_amd::single_task_kernel, dim3(1), dim3(64 * szs_amd::waves_per_group_k),
                        profile_bytes * szs_amd::waves_per_group_k, stream, tasks, segments, identity, task_index,
-                       variant, scratch, scratch_capacity, profile_bytes);
+                       variant, scratch, scratch_capacity, profile_bytes, *chain);
     return hipGetLastError();
 }
 

@@ -109,6 +109,14 @@ struct item_args_t {
     uint32_t scratch_capacity;
     uint32_t wave_lds_bytes; /**< dynamic LDS per wave: profile or Myers tables, then the injection ring */
     int32_t *shared_result; /**< LDS word for cooperative reductions */
+    // Chained shapes: group `group` of the pair (-1: every group in turn, in one workgroup), the bottom rows
+    // between groups ([slot][group][scores, gaps][chain_pitch] int32) and their published column counts
+    // ([slot][group] u32), and the word that records a wait given up.
+    int32_t group;
+    int32_t *chain_rows;
+    uint32_t *chain_progress;
+    uint32_t chain_pitch;
+    uint32_t *chain_timeout;
 };
 
 /**
@@ -186,7 +194,7 @@ struct sweep_t {
         shift = shifted_ ? params->shift : 0;
         match_shifted = match - shift, mismatch_shifted = mismatch - shift;
         open_minus_extend = open - extend;
-        transposed = task.transposed != 0;
+        transposed = task_transposed(task);
         symbol_type_ const *query = (symbol_type_ const *)segment.query_symbols;
         symbol_type_ const *candidate = (symbol_type_ const *)segment.candidate_symbols;
         rows = (transposed ? candidate : query) + task.rows_offset;

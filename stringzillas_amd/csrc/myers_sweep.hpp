@@ -125,8 +125,8 @@ struct myers_sweep_t {
         symbol_t const *query = (symbol_t const *)segment.query_symbols;
         symbol_t const *candidate = (symbol_t const *)segment.candidate_symbols;
         // Uniform costs are symmetric in the operands, so the transposition needs no compensation here.
-        rows = (task.transposed ? candidate : query) + task.rows_offset;
-        columns = (task.transposed ? query : candidate) + task.columns_offset;
+        rows = (task_transposed(task) ? candidate : query) + task.rows_offset;
+        columns = (task_transposed(task) ? query : candidate) + task.columns_offset;
         results = segment.results;
         output = task.output;
         mirror = task.mirror;

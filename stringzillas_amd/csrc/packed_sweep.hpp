@@ -100,7 +100,7 @@ struct packed_sweep_t {
         open = params->open, extend = params->extend;
         dense_count = params->dense_count;
         shift = params->shift;
-        transposed = task.transposed != 0;
+        transposed = task_transposed(task);
         uint8_t const *query = (uint8_t const *)segment.query_symbols;
         uint8_t const *candidate = (uint8_t const *)segment.candidate_symbols;
         rows = (transposed ? candidate : query) + task.rows_offset;

@@ -59,20 +59,27 @@ struct segment_view_t {
     int32_t family;
 };
 
-/** One scored pair. `rows` is the shorter side; `transposed` says the rows are the candidate. */
+/** One scored pair. `rows` is the shorter side. */
 struct task_t {
     uint64_t rows_offset;
     uint64_t columns_offset;
     uint32_t rows_length;
     uint32_t columns_length;
     uint32_t segment;
-    uint32_t transposed;
+    /** Bit 0: the rows are the candidate (transposed); bits 8-31: 1 + the pair's slot in the chain buffers when
+     *  a chained shape scores it (0 otherwise). */
+    uint32_t flags;
     uint64_t output;
     uint64_t mirror; /**< second output slot of a symmetric cross-product, or ~0 */
 };
 static_assert(sizeof(task_t) == 48, "task layout");
 
 constexpr uint64_t no_mirror_k = ~0ull;
+constexpr uint32_t task_transposed_k = 1u;
+
+__host__ __device__ inline bool task_transposed(task_t const &task) { return (task.flags & task_transposed_k) != 0; }
+/** Slot of a chained pair in the chain buffers. */
+__host__ __device__ inline uint32_t task_chain_slot(task_t const &task) { return (task.flags >> 8) - 1u; }
 
 /**
  *  A kernel shape: `group` lanes share one pair (64 / group pairs per wave), each lane holds `rows_per_lane`
@@ -86,6 +93,9 @@ struct shape_t {
     int32_t cooperative;
     int32_t bit_parallel; /**< unit-cost bytes: one 64-row Myers word per lane instead of R DP rows */
     int32_t packed;       /**< global class costs: two 16-bit cells per register (two half-stripes per lane) */
+    /** Cooperative shapes only: the pair's stripes are split into `chain_groups_k` consecutive work items, each
+     *  a workgroup; a group reads the previous group's bottom row from HBM as it is published. */
+    int32_t chained;
 };
 
 /** Waves per workgroup of the scoring kernels. */
@@ -96,16 +106,17 @@ constexpr int waves_per_group_k = 4;
  *  16-bit shapes for the global class-cost families of engines whose scores provably fit (else their int32 twin
  *  runs).
  */
-constexpr int shapes_count_k = 19;
+constexpr int shapes_count_k = 21;
 constexpr shape_t shapes_k[shapes_count_k] = {
-    {16, 1, 0, 0, 0}, {16, 2, 0, 0, 0}, {16, 4, 0, 0, 0},  // up to 16 / 32 / 64 rows, four pairs per wave
-    {64, 2, 0, 0, 0}, {64, 3, 0, 0, 0}, {64, 4, 0, 0, 0},  // up to 128 / 192 / 256 rows
-    {64, 6, 0, 0, 0}, {64, 8, 0, 0, 0}, {64, 16, 0, 0, 0}, // up to 384 / 512 / 1024 rows per stripe
-    {64, 8, 1, 0, 0}, {64, 16, 1, 0, 0},                   // cooperative: 512- or 1024-row stripes, whole workgroup
-    {4, 64, 0, 1, 0}, {16, 64, 0, 1, 0},                   // Myers: up to 256 rows (16 pairs/wave), 1024 rows
-    {64, 64, 0, 1, 0}, {64, 64, 1, 1, 0},                  // Myers: 4096-row stripes on one wave, or cooperative
-    {64, 4, 0, 0, 1}, {64, 6, 0, 0, 1}, {64, 8, 0, 0, 1},  // packed 16-bit: 256 / 384 / 512-row stripes
-    {64, 8, 1, 0, 1},                                      // packed 16-bit, cooperative 512-row stripes
+    {16, 1, 0, 0, 0, 0}, {16, 2, 0, 0, 0, 0}, {16, 4, 0, 0, 0, 0},  // up to 16 / 32 / 64 rows, four pairs per wave
+    {64, 2, 0, 0, 0, 0}, {64, 3, 0, 0, 0, 0}, {64, 4, 0, 0, 0, 0},  // up to 128 / 192 / 256 rows
+    {64, 6, 0, 0, 0, 0}, {64, 8, 0, 0, 0, 0}, {64, 16, 0, 0, 0, 0}, // up to 384 / 512 / 1024 rows per stripe
+    {64, 8, 1, 0, 0, 0}, {64, 16, 1, 0, 0, 0},     // cooperative: 512- or 1024-row stripes, whole workgroup
+    {4, 64, 0, 1, 0, 0}, {16, 64, 0, 1, 0, 0},     // Myers: up to 256 rows (16 pairs/wave), 1024 rows
+    {64, 64, 0, 1, 0, 0}, {64, 64, 1, 1, 0, 0},    // Myers: 4096-row stripes on one wave, or cooperative
+    {64, 4, 0, 0, 1, 0}, {64, 6, 0, 0, 1, 0}, {64, 8, 0, 0, 1, 0}, // packed 16-bit: 256 / 384 / 512-row stripes
+    {64, 8, 1, 0, 1, 0},                           // packed 16-bit, cooperative 512-row stripes
+    {64, 8, 1, 0, 1, 1}, {64, 8, 1, 0, 0, 1},      // chained cooperative 512-row stripes, packed or int32
 };
 
 /** The int32 shape with the same geometry as a packed one (itself for every other shape). */
@@ -114,12 +125,30 @@ __host__ __device__ constexpr int unpacked_twin(int shape) {
     for (int s = 0; s < shapes_count_k; ++s)
         if (!shapes_k[s].packed && !shapes_k[s].bit_parallel && shapes_k[s].group == shapes_k[shape].group &&
             shapes_k[s].rows_per_lane == shapes_k[shape].rows_per_lane &&
-            shapes_k[s].cooperative == shapes_k[shape].cooperative)
+            shapes_k[s].cooperative == shapes_k[shape].cooperative && shapes_k[s].chained == shapes_k[shape].chained)
             return s;
     return shape;
 }
-static_assert(unpacked_twin(15) == 5 && unpacked_twin(16) == 6 && unpacked_twin(17) == 7 && unpacked_twin(18) == 9,
+static_assert(unpacked_twin(15) == 5 && unpacked_twin(16) == 6 && unpacked_twin(17) == 7 && unpacked_twin(18) == 9 &&
+                  unpacked_twin(19) == 20,
               "every packed shape has an int32 twin");
+
+/**
+ *  Work items (workgroups) a chained pair is split into: groups of at most 4 stripes, one per wave, so that a
+ *  group's last stripe starts a few phases after its first and the next group can follow right behind; pairs of
+ *  more than 4 x 64 stripes put more stripes in each group. Trailing groups of shorter pairs are empty.
+ */
+constexpr int chain_groups_k = 64;
+
+/** Chain buffers of a batch: the bottom rows between the groups of each chained pair
+ *  ([slot][group < chain_groups_k - 1][scores, gaps][pitch] int32), the columns each has published
+ *  ([slot][group] u32, zeroed before every launch), and a word raised when a group gives up waiting. */
+struct chain_view_t {
+    int32_t *rows;
+    uint32_t *progress;
+    uint32_t pitch; /**< >= the longest chained pair's columns + 1 */
+    uint32_t *timeout;
+};
 
 /** Shape-table slots of the unit-cost Levenshtein over bytes and over runes (after the family x symbols slots):
  *  the ones whose ladders may use bit-parallel shapes. */
@@ -132,8 +161,18 @@ __host__ __device__ inline uint32_t pairs_per_item(shape_t shape) {
     return shape.cooperative ? 1u : (uint32_t)(waves_per_group_k * (64 / shape.group));
 }
 
+/** Work items of `tasks` pairs of one shape: a chained pair is `chain_groups_k` items, one per group. */
+__host__ __device__ inline uint32_t items_for(shape_t shape, uint32_t tasks) {
+    if (shape.chained) return tasks * (uint32_t)chain_groups_k;
+    uint32_t const per_item = pairs_per_item(shape);
+    return (tasks + per_item - 1) / per_item;
+}
+
 /** Cost buckets per shape for binning: half-octaves of (stripes x (columns + group)). */
 constexpr int column_buckets_k = 64;
+/** Items of this bucket and above (about 2.1 M lane-row-steps, e.g. a 65,536 x 65,536 pair) run at raised wave
+ *  priority: their latency, not the batch's total work, would otherwise decide when the launch ends. */
+constexpr int critical_bucket_k = 42;
 /** Variant = (family, symbols, shape); bins = variants x column buckets. */
 constexpr int variants_count_k = families_count_k * 2 * shapes_count_k;
 constexpr int bins_count_k = variants_count_k * column_buckets_k;
@@ -212,6 +251,9 @@ struct prepare_stats_t {
     uint32_t widest_multi_pass;          /**< most columns of a pair that needs several stripes */
     uint32_t overflow;                   /**< nonzero if some pair's scores could leave int32 */
     uint32_t bad_tape;                   /**< nonzero if a UTF-8 tape's offsets run past its stated byte count */
+    uint32_t chained_pairs;              /**< pairs scored by a chained shape: slots of the chain buffers */
+    uint32_t chained_columns;            /**< most columns of such a pair */
+    uint32_t chain_timeout;              /**< nonzero if a chained group gave up waiting for its predecessor */
     uint32_t reserved;
 };
 

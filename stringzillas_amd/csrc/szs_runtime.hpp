@@ -166,6 +166,9 @@ struct batch_t {
     device_buffer_t params, segments, shape_table, tasks, bin_of_task, order, shape_of_task, identity;
     device_buffer_t histogram, cursor, task_start, item_start, priority, position_of_bin, queue_head, scratch, stats;
     device_buffer_t materialize_device, segment_overflow, results_block, decode_tapes;
+    device_buffer_t chain_rows, chain_progress;  /**< chained pairs' bottom rows between groups, and progress */
+    uint32_t chained_pairs = 0;                /**< pairs scored by a chained shape (slots of the chain buffers) */
+    uint32_t chain_pitch = 0;
     uint32_t decode_tapes_count = 0;           /**< UTF-8 tapes transcoded by each run */
     uint64_t decode_strings = 0;               /**< strings across those tapes */
     std::vector<char> materialize_args;        /**< one `materialize_args_t` per segment (opaque here) */

@@ -21,7 +21,7 @@ szs = pytest.importorskip("stringzillas_amd")
 from stringzillas_amd import _native, autotune  # noqa: E402
 
 TABLE = autotune.shapes()
-PACKED = {(s[1], s[2]): i for i, s in enumerate(TABLE) if s[4]}  # (rows per lane, cooperative) -> shape id
+PACKED = {(s[1], s[2]): i for i, s in enumerate(TABLE) if s[4] and not s[5]}  # (rows per lane, cooperative) -> id
 LADDERS = {
     # Every pair on 512-row packed stripes swept by one wave (taller pairs: several stripes through HBM).
     "wave8": ([(PACKED[(8, 0)], 1 << 20)], PACKED[(8, 1)]),
@@ -30,6 +30,10 @@ LADDERS = {
     # Every pair on the whole workgroup, 512-row stripes pipelined over its four waves.
     "cooperative": ([], PACKED[(8, 1)]),
 }
+CHAINED = {s[4]: i for i, s in enumerate(TABLE) if s[5]}  # packed -> chained cooperative shape id
+# Every pair split over 16 workgroups that hand bottom rows over through HBM (engines outside the packed range
+# bound run the int32 twin of the packed shape).
+CHAINED_LADDERS = {"chained": ([], CHAINED[1]), "chained_int32": ([], CHAINED[0])}
 # Edge lengths: tiny, around the 64-column chunk and 128-step ramp, around the 256/384/512-row stripes.
 EDGES = [1, 2, 3, 5, 63, 64, 65, 127, 128, 129, 191, 255, 256, 257, 383, 384, 385, 511, 512, 513, 1023, 1025]
 
@@ -66,7 +70,7 @@ def packed_ladders():
     saved = {slot: autotune.get_ladder(slot) for slot in slots}
 
     def install(name):
-        ladder, beyond = LADDERS[name]
+        ladder, beyond = {**LADDERS, **CHAINED_LADDERS}[name]
         for slot in slots:
             autotune.set_ladder(slot, ladder, beyond)
 
@@ -145,3 +149,43 @@ def test_packed_batches_and_graph_replays(packed_ladders):
                 out.fill(777)
             for got, want in zip(batch.run(), expected):
                 np.testing.assert_array_equal(got.astype(np.int64), want, err_msg=f"mode {mode} replay {replay}")
+
+
+@pytest.mark.parametrize("ladder", list(CHAINED_LADDERS))
+@pytest.mark.parametrize("name", ["nw_affine", "nw_linear", "nw_far_gap", "nw_big_diagonal"])
+def test_chained_groups_match_oracle(name, ladder, packed_ladders):
+    """One-stripe pairs (15 empty groups), one stripe per group, and pairs spanning all 16 groups."""
+    rng = np.random.default_rng(zlib.crc32(f"chained:{name}:{ladder}".encode()))
+    engine, params = _engine(name, rng)
+    packed_ladders(ladder)
+    sizes = [(1, 1), (5, 700), (511, 513), (513, 2000), (700, 700), (3000, 3100), (4100, 4096), (8193, 8300),
+             (9100, 9000), (2, 9500)]
+    a = [random_bytes(rng, PROTEIN, n, n) for n, _ in sizes]
+    b = [random_bytes(rng, PROTEIN, m, m) for _, m in sizes]
+    text = random_bytes(rng, PROTEIN, 8800, 8800)  # similar long strings: scores far beyond 16 bits
+    a += [text, text]
+    b += [text, _mutated(rng, text, 0.05, 0.01)]
+    _check(engine, params, a, b, f"{name} {ladder}")
+
+
+def test_chained_levenshtein_dp_and_batch_modes():
+    """Chained int32 stripes for a non-unit Levenshtein, in a persistent launch, per pair and graph replays."""
+    rng = np.random.default_rng(31)
+    slot = autotune.slot_of("lev_affine")
+    saved = autotune.get_ladder(slot)
+    try:
+        autotune.set_ladder(slot, [], CHAINED[0])
+        engine = szs.LevenshteinDistances(0, 4, 3, 2)
+        params = O.Params("levenshtein", 0, 4, 3, 2)
+        a = [random_bytes(rng, b"ACGT", n, n) for n in (1, 600, 2500, 8400, 9000)]
+        b = [random_bytes(rng, b"ACGT", m, m) for m in (700, 1, 2600, 8500, 8999)]
+        expected = O.pairs(params, O.tape(a), O.tape(b), threads=16)
+        segments = [(engine, szs.Strs(a), szs.Strs(b))]
+        for mode in (szs.BATCH_PERSISTENT, szs.BATCH_PER_PAIR, szs.BATCH_GRAPH):
+            batch = szs.Batch(segments, mode=mode)
+            for replay in range(3 if mode == szs.BATCH_GRAPH else 1):
+                batch.outputs[0].fill(99)
+                got = batch.run()[0]
+                np.testing.assert_array_equal(got.astype(np.int64), expected, err_msg=f"mode {mode} replay {replay}")
+    finally:
+        autotune.set_ladder(slot, *saved)

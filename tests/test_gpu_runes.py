@@ -20,7 +20,7 @@ szs = pytest.importorskip("stringzillas_amd")
 from stringzillas_amd import autotune  # noqa: E402
 
 TABLE = autotune.shapes()
-MYERS = {(s[0], s[2]): i for i, s in enumerate(TABLE) if s[3]}  # (lanes per pair, cooperative) -> shape id
+MYERS = {(s[0], s[2]): i for i, s in enumerate(TABLE) if s[3] and not s[5]}  # (lanes per pair, cooperative) -> id
 LADDERS = {
     "small_groups": ([(MYERS[(4, 0)], 256), (MYERS[(16, 0)], 1024)], MYERS[(64, 1)]),
     "wave": ([(MYERS[(64, 0)], 1 << 20)], MYERS[(64, 1)]),
