@@ -31,6 +31,8 @@ FAMILIES = ["lev_linear", "lev_affine", "nw_linear", "nw_affine", "sw_linear", "
 UNIT_BYTES_SLOT = 2 * len(FAMILIES)
 UNIT_RUNES_SLOT = 2 * len(FAMILIES) + 1
 UNIT_SLOTS = (UNIT_BYTES_SLOT, UNIT_RUNES_SLOT)
+# Rows beyond which global families use the chained cooperative shape (the library's default ladders agree).
+CHAINED_ROWS = 24576
 BANDS = [16, 32, 64, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192, 16384]
 
 
@@ -88,9 +90,10 @@ def _engine_for(slot: int):
 def _batch_for(rng, alphabet, low: int, high: int, target_cells: float, max_pairs: int = 200_000):
     mean_rows = (low + high) / 2
     mean_cells = mean_rows * max(mean_rows * 2.5, (16384 - mean_rows) / np.log(16384 / mean_rows))
-    # Enough pairs to fill every wave slot of the device (256 CUs x 4 workgroups x 4 waves) twice, as inside a
-    # large batch, so shapes are compared at full occupancy.
-    pairs = int(min(max_pairs, max(8192, target_cells / mean_cells)))
+    # Enough pairs to give every workgroup of the device (256 CUs x 4) two work items, as inside a large batch,
+    # so shapes are compared at full occupancy: a 4-lane shape takes 64 pairs per item.
+    floor = 131_072 if high <= 256 else 8192
+    pairs = int(min(max_pairs, max(floor, target_cells / mean_cells)))
     rows = rng.integers(low, high + 1, size=pairs)
     # The longer side log-uniform from the rows up to 16,384 (4x the rows at least), like the mixed batch's pairs.
     top = np.maximum(rows * 4, 16384)
@@ -181,9 +184,14 @@ def tune(slots: List[int] = None, target_cells: float = 4e9, verbose: bool = Tru
                       f"{timings[best]:.0f} GCUPS of {timings}", flush=True)
             low = high + 1
         ladder: List[Tuple[int, int]] = []
+        # The last band's winner, if cooperative, becomes the shape for every taller pair; the ladder lists the
+        # winners of the bands below it, cooperative ones included.
+        if chosen and table[chosen[-1][0]][2]:
+            tail = chosen[-1][0]
+            chosen = chosen[:-1]
+        else:
+            tail = None
         for shape, high in chosen:
-            if table[shape][2]:
-                break  # cooperative from here on
             earlier = [s for s, _ in ladder]
             if shape in earlier:
                 # A shape that wins again after others takes over the bands in between (a ladder lists each
@@ -198,10 +206,17 @@ def tune(slots: List[int] = None, target_cells: float = 4e9, verbose: bool = Tru
                 high = min(high, group * rows_per_lane)
             if not fixed or high > fixed[-1][1]:
                 fixed.append((shape, high))
-        beyond = default_beyond
-        coop = [s for s, _ in chosen if table[s][2]]
-        if coop:
-            beyond = coop[-1]
+        beyond = tail if tail is not None else default_beyond
+        if beyond in [s for s, _ in fixed]:  # it also won a lower band: it takes over from there
+            fixed = fixed[: [s for s, _ in fixed].index(beyond)]
+        # Global DP families split pairs beyond the measured bands over chained workgroups (the tallest pairs
+        # would otherwise set a batch's critical path on one workgroup each).
+        family = "lev_linear" if slot in UNIT_SLOTS else FAMILIES[slot // 2]
+        if slot not in UNIT_SLOTS and not family.startswith("sw"):
+            chained = [i for i, s in enumerate(table) if s[5] and s[4] == table[beyond][4]]
+            if chained:
+                fixed.append((beyond, CHAINED_ROWS))
+                beyond = chained[0]
         set_ladder(slot, fixed, beyond)
         result["slots"][str(slot)] = {"ladder": fixed, "beyond": beyond, "measurements": measurements}
     return result
