@@ -24,10 +24,10 @@ INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # CUs x lanes x clock: 39.3 T la
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the scoring kernel on the full mixed batch: FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
 # (factors from a plain 1 GiB device copy), recorded in profiles/persistent_score_kernel_mixed_1M_pairs.md.
-MEASURED_TRAFFIC_BYTES = {"mixed_1000000_pairs_log_uniform_16_16384_plus_32x65536": 56.91e9}
+MEASURED_TRAFFIC_BYTES = {"mixed_1000000_pairs_log_uniform_16_16384_plus_32x65536": 52.36e9}
 # VALU issue rate of the same launch from SQ_INSTS_VALU (same file): wave64 instructions per second over the
 # gfx950 peak of one per cycle per CU.
-MEASURED_VALU_ISSUE = {"mixed_1000000_pairs_log_uniform_16_16384_plus_32x65536": 0.985}
+MEASURED_VALU_ISSUE = {"mixed_1000000_pairs_log_uniform_16_16384_plus_32x65536": 0.898}
 
 
 def parse():
@@ -283,10 +283,10 @@ def main():
             "traffic": MEASURED_TRAFFIC_BYTES.get(workload.name),
             "ops_per_cell": round(ops_per_cell, 3),
             "kernel_ms": round(kernel_ms_mean, 3),
-            "note": "fixed DP ops per cell (linear 5, affine 9); unit-cost byte Levenshtein runs the bit-parallel "
-                    "tier (64 cells per word op) and the global DP families run 5 (affine) or 2 (linear) VALU per "
-                    "cell, which is why the fraction exceeds 1; valu_issue_frac is the measured share of the VALU "
-                    "issue rate the kernel uses",
+            "note": "fixed DP ops per cell (linear 5, affine 9); unit-cost Levenshtein over bytes and runes runs "
+                    "the bit-parallel tier (64 cells per word op) and affine NW runs packed 16-bit cells (about 4.7 "
+                    "VALU per cell), which is why the fraction exceeds 1; valu_issue_frac is the measured share of "
+                    "the VALU issue rate the kernel uses",
             "valu_issue_frac": MEASURED_VALU_ISSUE.get(workload.name),
         },
         "cpu_baseline": cpu,
