@@ -228,8 +228,8 @@ __device__ __forceinline__ void run_shape(item_args_t const &a) {
         // otherwise).
         if constexpr (family_ == family_lev_linear_k) {
             constexpr bool runes = sizeof(symbol_type_) == 4;
-            if constexpr (shape.cooperative) run_cooperative<myers_sweep_t<64, runes>>(a);
-            else run_independent<myers_sweep_t<shape.group, runes>>(a);
+            if constexpr (shape.cooperative) run_cooperative<myers_sweep_t<64, runes, shape.rows_per_lane>>(a);
+            else run_independent<myers_sweep_t<shape.group, runes, shape.rows_per_lane>>(a);
         }
     }
     else if constexpr (shape.packed) {
@@ -278,10 +278,12 @@ __device__ void run_family(int shape, item_args_t const &a) {
     case 18: run_shape<family_, symbol_type_, 18, shifted_>(a); break;
     case 19: run_shape<family_, symbol_type_, 19, shifted_>(a); break;
     case 20: run_shape<family_, symbol_type_, 20, shifted_>(a); break;
+    case 21: run_shape<family_, symbol_type_, 21, shifted_>(a); break;
+    case 22: run_shape<family_, symbol_type_, 22, shifted_>(a); break;
     default: break;
     }
 }
-static_assert(shapes_count_k == 21, "dispatch switch lists every shape");
+static_assert(shapes_count_k == 23, "dispatch switch lists every shape");
 
 __device__ void run_variant(int variant, item_args_t const &a) {
     int const shape = variant % shapes_count_k;
@@ -716,7 +718,7 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
             }
             // ~DP rows of VALU work per step; a packed step costs about 3/4 of an int32 one and its stripes are
             // swept in columns + 2 * group steps.
-            uint64_t const per_step = sh.bit_parallel ? (symbols == symbols_u8_k ? 8u : 20u) // ~17 rune planes
+            uint64_t const per_step = sh.bit_parallel ? (symbols == symbols_u8_k ? 8u : 20u) * sh.rows_per_lane / 64 // ~17 rune planes
                                       : sh.packed     ? (uint64_t)(sh.rows_per_lane * 3 + 3) / 4
                                                       : (uint64_t)sh.rows_per_lane;
             uint64_t const ramp = (uint64_t)sh.group * (sh.packed ? 2u : 1u);

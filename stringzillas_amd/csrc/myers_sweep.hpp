@@ -6,7 +6,8 @@
  *  The reference routes unit costs to Myers as well (include/stringzillas/similarities/serial.hpp:1657-1865,
  *  2126-2140); results equal the unit-cost DP, which is what the tests pin against the oracle.
  *
- *  Layout: lane k of a group owns one 64-row word of the stripe (rows 64k .. 64k+63, bit 0 = top row) as the
+ *  Layout: lane k of a group owns one 64-row word of the stripe (rows 64k .. 64k+63, bit 0 = top row; 32-row
+ *  words for the half-height shapes) as the
  *  vertical-delta vectors Pv/Mv. At step t it processes column j = t + 1 - k and passes the horizontal delta of
  *  its bottom row (+1, 0 or -1, coded in 2 bits) to lane k+1 by DPP; lane 0 receives +1 from the matrix border
  *  or the previous stripe's bottom deltas from HBM scratch. The match vector of a column symbol is built from
@@ -71,11 +72,15 @@ static_assert(sizeof(myers_tables_t) <= myers_tables_bytes_k, "the match table f
  *  cover the bits of the largest rune in the wave's rows (at most 21), and a column rune with a higher bit set
  *  matches no row.
  */
-template <int G, bool runes_ = false>
+template <int G, bool runes_ = false, int word_bits_ = 64>
 struct myers_sweep_t {
+    static_assert(word_bits_ == 64 || word_bits_ == 32, "a lane's word holds 64 or 32 rows");
     using symbol_t = typename std::conditional<runes_, uint32_t, uint8_t>::type;
+    /** One lane's rows as bits: 64-row words, or 32-row words (half-height stripes for pairs that would leave most
+     *  of a 64-row stripe empty; a step then costs about half). */
+    using word_t = typename std::conditional<word_bits_ == 64, uint64_t, uint32_t>::type;
     static constexpr int group_k = G;
-    static constexpr int rows_per_lane = 64;
+    static constexpr int rows_per_lane = word_bits_;
     static constexpr int skew_k = 1; // lane k + 1 runs one column behind lane k
     static constexpr bool local_k = false;
     static constexpr bool renumber_k = !runes_ && G == 64; /**< one pair per wave: a full byte -> code table */
@@ -91,14 +96,14 @@ struct myers_sweep_t {
     myers_tables_t *tables;
     myers_group_tables_t *group_tables; /**< this lane's group (G < 64) */
     int32_t distinct;    /**< distinct byte values in the rows */
-    uint64_t *eq_table;  /**< [16 codes][G lanes] match words of this group's current stripe (table mode) */
+    word_t *eq_table;    /**< [16 codes][G lanes] match words of this group's current stripe (table mode) */
     int8_t *ring;        /**< lane-0 inputs of a 64-step chunk (G = 64) */
     int32_t planes_used;
     bool table_mode;     /**< wave-uniform: at most 15 distinct symbols, match words come from `eq_table` */
 
     int32_t lane_top;
-    uint64_t planes[planes_k];
-    uint64_t pv, mv;
+    word_t planes[planes_k];
+    word_t pv, mv;
     uint32_t column_code, bottom_code;
     int32_t vertical_sum;
 
@@ -133,7 +138,7 @@ struct myers_sweep_t {
         tables = wave_tables;
         uint32_t const group = (uint32_t)lane / G;
         group_tables = reinterpret_cast<myers_group_tables_t *>(wave_lds(wave, a.wave_lds_bytes)) + group;
-        eq_table = reinterpret_cast<uint64_t *>(wave_lds(wave, a.wave_lds_bytes) + myers_tables_bytes_k) +
+        eq_table = reinterpret_cast<word_t *>(wave_lds(wave, a.wave_lds_bytes) + myers_tables_bytes_k) +
                    group * (myers_eq_codes_k * G);
         ring = wave_lds(wave, a.wave_lds_bytes) + (a.wave_lds_bytes - inject_ring_bytes_k);
         vertical_sum = 0;
@@ -207,17 +212,17 @@ struct myers_sweep_t {
     }
 
     __device__ __forceinline__ void begin_stripe(int32_t base) {
-        lane_top = base + k * 64;
+        lane_top = base + k * word_bits_;
 #pragma unroll
         for (int b = 0; b < planes_k; ++b) planes[b] = 0;
-        for (int32_t r = 0; r < 64; ++r) {
+        for (int32_t r = 0; r < word_bits_; ++r) {
             int32_t const i = lane_top + r; // 0-based row
             uint32_t const code = i < n ? code_of(rows[i]) : 0u;
 #pragma unroll
             for (int b = 0; b < planes_k; ++b)
-                if (b < planes_used) planes[b] |= (uint64_t)((code >> b) & 1u) << r;
+                if (b < planes_used) planes[b] |= (word_t)((code >> b) & 1u) << r;
         }
-        pv = ~0ull; // column 0: D[i][0] = i, every vertical delta is +1
+        pv = ~(word_t)0; // column 0: D[i][0] = i, every vertical delta is +1
         mv = 0;
         bottom_code = 1;
         column_code = 0;
@@ -233,7 +238,7 @@ struct myers_sweep_t {
      *  reach them, and a rune with a bit at PB or above matches no row.
      */
     template <int PB>
-    __device__ __forceinline__ uint64_t planes_eq(uint32_t code) const {
+    __device__ __forceinline__ word_t planes_eq(uint32_t code) const {
         static_assert(PB <= planes_k, "planes beyond the array");
         // One 32-bit half at a time; the all-ones or all-zeros mask is formed with a subtraction: a sign-extended
         // 1-bit field widened to 64 bits was compiled as zero-extended (the upper half of Eq came out wrong).
@@ -247,20 +252,23 @@ struct myers_sweep_t {
             uint32_t const bit = 0u - ((code >> b) & 1u); // all ones where the code has bit b
             // eq & ~(plane ^ bit), one v_bitop3 per half (truth table 0x90 over the operands' bits 4a + 2b + c)
             eq_low = __builtin_amdgcn_bitop3_b32(eq_low, (uint32_t)planes[b], bit, 0x90);
-            eq_high = __builtin_amdgcn_bitop3_b32(eq_high, (uint32_t)(planes[b] >> 32), bit, 0x90);
+            if constexpr (word_bits_ == 64)
+                eq_high = __builtin_amdgcn_bitop3_b32(eq_high, (uint32_t)((uint64_t)planes[b] >> 32), bit, 0x90);
         }
-        return (uint64_t)eq_high << 32 | eq_low;
+        if constexpr (word_bits_ == 64) return (word_t)((uint64_t)eq_high << 32 | eq_low);
+        else return (word_t)eq_low;
     }
 
     /** One column of this lane's word (Hyyro's step); `hin` is the horizontal delta entering its top row. */
-    __device__ __forceinline__ void step_word(uint64_t eq, uint32_t hin) {
-        uint64_t const hin_positive = hin & 1u, hin_negative = (hin >> 1) & 1u;
-        uint64_t const xv = eq | mv;
+    __device__ __forceinline__ void step_word(word_t eq, uint32_t hin) {
+        constexpr int top_bit = word_bits_ - 1;
+        word_t const hin_positive = hin & 1u, hin_negative = (hin >> 1) & 1u;
+        word_t const xv = eq | mv;
         eq |= hin_negative;
-        uint64_t const xh = (((eq & pv) + pv) ^ pv) | eq;
-        uint64_t ph = mv | ~(xh | pv);
-        uint64_t mh = pv & xh;
-        bottom_code = (uint32_t)(ph >> 63) | (uint32_t)(mh >> 63) << 1;
+        word_t const xh = (((eq & pv) + pv) ^ pv) | eq;
+        word_t ph = mv | ~(xh | pv);
+        word_t mh = pv & xh;
+        bottom_code = (uint32_t)(ph >> top_bit) | (uint32_t)(mh >> top_bit) << 1;
         ph = (ph << 1) | hin_positive;
         mh = (mh << 1) | hin_negative;
         pv = mh | ~(xv | ph);
@@ -303,14 +311,14 @@ struct myers_sweep_t {
         // Column codes do not depend on the scores, so step s + 1's code and match word are fetched during step s.
         uint2 now = *(uint2 const *)ring, after = *(uint2 const *)(ring + 8);
         uint32_t code = (uint32_t)shift_down<64>((int32_t)now.y, (int32_t)column_code);
-        uint64_t eq_next = 0;
+        word_t eq_next = 0;
         if constexpr (table_) eq_next = eq_table[code * 64 + lane];
 #pragma unroll 2
         for (int32_t step = 0; step < 64; ++step) {
             uint2 const later = *(uint2 const *)(ring + ((step + 2) & 63) * 8);
             uint32_t const hin = (uint32_t)shift_down<64>((int32_t)now.x, (int32_t)bottom_code);
             uint32_t const code_after = (uint32_t)shift_down<64>((int32_t)after.y, (int32_t)code);
-            uint64_t const eq_now = eq_next;
+            word_t const eq_now = eq_next;
             if constexpr (table_) eq_next = eq_table[code_after * 64 + lane];
             column_code = code;
             int32_t const j = first_step + step + 1 - lane;
@@ -349,8 +357,8 @@ struct myers_sweep_t {
     }
 
     __device__ __forceinline__ void end_stripe(int32_t, bool) {
-        int32_t const rows_here = min(max(n - lane_top, 0), 64);
-        uint64_t const mask = rows_here == 64 ? ~0ull : ((1ull << rows_here) - 1ull);
+        int32_t const rows_here = min(max(n - lane_top, 0), word_bits_);
+        word_t const mask = rows_here == word_bits_ ? ~(word_t)0 : (word_t)(((word_t)1 << rows_here) - 1u);
         vertical_sum += __builtin_popcountll(pv & mask) - __builtin_popcountll(mv & mask);
     }
 

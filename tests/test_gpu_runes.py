@@ -20,11 +20,14 @@ szs = pytest.importorskip("stringzillas_amd")
 from stringzillas_amd import autotune  # noqa: E402
 
 TABLE = autotune.shapes()
-MYERS = {(s[0], s[2]): i for i, s in enumerate(TABLE) if s[3] and not s[5]}  # (lanes per pair, cooperative) -> id
+# (lanes per pair, rows per lane, cooperative) -> shape id of the bit-parallel shapes
+MYERS = {(s[0], s[1], s[2]): i for i, s in enumerate(TABLE) if s[3] and not s[5]}
 LADDERS = {
-    "small_groups": ([(MYERS[(4, 0)], 256), (MYERS[(16, 0)], 1024)], MYERS[(64, 1)]),
-    "wave": ([(MYERS[(64, 0)], 1 << 20)], MYERS[(64, 1)]),
-    "cooperative": ([], MYERS[(64, 1)]),
+    "small_groups": ([(MYERS[(4, 64, 0)], 256), (MYERS[(16, 64, 0)], 1024)], MYERS[(64, 64, 1)]),
+    "wave": ([(MYERS[(64, 64, 0)], 1 << 20)], MYERS[(64, 64, 1)]),
+    "cooperative": ([], MYERS[(64, 64, 1)]),
+    # 32-row words: 16-lane groups up to 512 rows, then 2048-row stripes on one wave for every taller pair.
+    "half_words": ([(MYERS[(16, 32, 0)], 512), (MYERS[(64, 32, 0)], 1 << 20)], MYERS[(64, 64, 1)]),
 }
 ALPHABETS = {
     "ascii": [chr(c) for c in range(0x20, 0x7F)],
@@ -54,6 +57,19 @@ def rune_ladder():
     autotune.set_ladder(slot, *saved)
 
 
+@pytest.fixture
+def byte_ladder():
+    slot = autotune.slot_of("lev_linear", unit=True)
+    saved = autotune.get_ladder(slot)
+
+    def install(name):
+        ladder, beyond = LADDERS[name]
+        autotune.set_ladder(slot, ladder, beyond)
+
+    yield install
+    autotune.set_ladder(slot, *saved)
+
+
 def _check(a, b, label):
     engine = szs.LevenshteinDistancesUTF8()
     expected = O.pairs(O.Params("levenshtein_utf8"), O.tape(a), O.tape(b), threads=16)
@@ -68,6 +84,8 @@ def test_rune_myers_matches_oracle(alphabet, ladder, rune_ladder):
     rune_ladder(ladder)
     letters = ALPHABETS[alphabet]
     edges = [e for e in EDGES if ladder != "small_groups" or e <= 1025]
+    if ladder == "half_words":
+        edges += [511, 512, 513, 2047, 2048, 2049]
     a = [_text(rng, letters, n) for n in edges for _ in edges]
     b = [_text(rng, letters, m) for _ in edges for m in edges]
     a += [_text(rng, letters, int(rng.integers(0, 3000))) for _ in range(30)]
@@ -94,3 +112,18 @@ def test_rune_myers_wider_columns_and_malformed_bytes(ladder, rune_ladder):
         a.append(raw)
         b.append(bytes(rng.choice([0x41, 0x80, 0xF7, 0xBF, 0x9F], size=n + 3).tolist()))
     _check(a, b, f"wide columns and malformed bytes, {ladder}")
+
+
+@pytest.mark.parametrize("alphabet", [b"AC", b"ACGT", bytes(range(32, 127)), bytes(range(256))])
+def test_half_word_myers_over_bytes(alphabet, byte_ladder):
+    """32-row words over bytes: table mode (few symbols) and bit-planes (up to 256), one and several stripes."""
+    rng = np.random.default_rng(zlib.crc32(alphabet))
+    byte_ladder("half_words")
+    table = np.frombuffer(alphabet, dtype=np.uint8)
+    lengths = [1, 31, 32, 33, 300, 511, 512, 513, 2047, 2048, 2049, 5000]
+    a = [bytes(rng.choice(table, size=n)) for n in lengths for _ in lengths]
+    b = [bytes(rng.choice(table, size=m)) for _ in lengths for m in lengths]
+    engine = szs.LevenshteinDistances()
+    expected = O.pairs(O.Params("levenshtein"), O.tape(a), O.tape(b), threads=16)
+    got = engine.pairs(szs.Strs(a), szs.Strs(b))
+    np.testing.assert_array_equal(got.astype(np.int64), expected)

@@ -176,3 +176,26 @@ def test_understated_tape_bytes_fail_only_their_request():
         assert (server.wait(good) == 0).all()  # every string against itself
     finally:
         server.close()
+
+
+def test_server_rounds_with_chained_pairs():
+    """Pairs taller than the chained threshold split over workgroups inside server rounds (chain buffers come from
+    the server's pool); their scores equal the oracle's and the same request scored alone."""
+    rng = np.random.default_rng(17)
+    b2c, costs = protein_tables(rng)
+    nw = szs.NeedlemanWunschScores(b2c, costs, -11, -1)
+    params = O.Params("needleman_wunsch", open=-11, extend=-1, byte_to_class=b2c, class_costs=costs)
+    server = szs.Server(hbm_budget_bytes=256 << 20, max_inflight_rounds=2)
+    try:
+        requests = []
+        for length in (25_000, 26_000, 300):
+            text = random_bytes(rng, PROTEIN, length, length)
+            firsts = szs.Strs([text, random_bytes(rng, PROTEIN, length, length)])
+            seconds = szs.Strs([random_bytes(rng, PROTEIN, length, length), text])
+            requests.append((server.submit(nw, firsts, seconds), firsts, seconds))
+        for ticket, firsts, seconds in requests:
+            out = server.wait(ticket)
+            np.testing.assert_array_equal(out, _expected(params, firsts, seconds))
+            np.testing.assert_array_equal(out, nw.pairs(firsts, seconds))
+    finally:
+        server.close()
