@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "dp_sweep.hpp"
 #include "myers_sweep.hpp"
 #include "packed_sweep.hpp"
@@ -38,7 +40,59 @@ __device__ __forceinline__ bool is_steady(int32_t chunk, int32_t columns) {
     return chunk >= skew && chunk * 64 + 64 + (skew - 1) <= columns;
 }
 
-/** Each wave scores 64/G pairs of the item on its own, stripe after stripe for tall pairs (G = 64). */
+/**
+ *  Shorter sweeps for the last stripe of a tall pair whose remaining rows fit one: the same lanes, skew and
+ *  boundary-row format (int32 rows of the anti-diagonal form, or Myers delta codes), fewer rows per lane, so the
+ *  stripe costs fewer instructions per step. `void` where there is none.
+ */
+template <typename sweep_type_>
+struct tail_sweeps_t {
+    using shorter = void;
+    using middle = void;
+};
+template <int family_>
+struct tail_sweeps_t<packed_sweep_t<family_, 8>> { // 512-row stripes: 256- or 384-row tails
+    using shorter = packed_sweep_t<family_, 4>;
+    using middle = packed_sweep_t<family_, 6>;
+};
+template <bool runes_>
+struct tail_sweeps_t<myers_sweep_t<64, runes_, 64>> { // 4096-row stripes: 2048-row tails of 32-row words
+    using shorter = myers_sweep_t<64, runes_, 32>;
+    using middle = void;
+};
+
+template <typename tail_type_>
+__device__ constexpr int32_t tail_rows() {
+    if constexpr (std::is_void<tail_type_>::value) return 0;
+    else return 64 * tail_type_::rows_per_lane;
+}
+
+/**
+ *  Sweeps the last stripe (rows from `base`) of one pair with `tail_type_`, reading the row above from `in_row`
+ *  and `gaps_row` (the matrix border when `in_row` is null). Returns the stripe's share of a summed result.
+ */
+template <typename tail_type_>
+__device__ __forceinline__ int32_t run_tail(item_args_t const &a, uint32_t task_index, uint32_t wave, int32_t base,
+                                            int32_t const *in_row, int32_t const *gaps_row) {
+    static_assert(tail_type_::group_k == 64, "tails are whole-wave stripes");
+    tail_type_ tail;
+    tail.load(a, task_index, true, wave);
+    int32_t const columns = __builtin_amdgcn_readfirstlane(tail.m);
+    int32_t const chunks = chunks_for<tail_type_>(columns);
+    tail.begin_stripe(base);
+    for (int32_t chunk = 0; chunk < chunks; ++chunk) {
+        if (is_steady<tail_type_>(chunk, columns))
+            tail.template run_chunk<true, false>(chunk, in_row, gaps_row, nullptr, nullptr);
+        else tail.template run_chunk<false, false>(chunk, in_row, gaps_row, nullptr, nullptr);
+    }
+    tail.end_stripe(base, true);
+    return tail.partial();
+}
+
+/**
+ *  Each wave scores 64/G pairs of the item on its own, stripe after stripe for tall pairs (G = 64); a tall pair's
+ *  last stripe runs with a shorter sweep when one covers its rows.
+ */
 template <typename sweep_type_>
 __device__ void run_independent(item_args_t const &a) {
     constexpr int G = sweep_type_::group_k;
@@ -61,7 +115,18 @@ __device__ void run_independent(item_args_t const &a) {
     int32_t *gaps_row = scores_row + a.scratch_capacity;
     // With one pair per wave, chunks whose 64 columns are valid for every lane skip the per-step column check.
     int32_t const columns = G == 64 ? __builtin_amdgcn_readfirstlane(sweep.m) : 0;
-    for (int32_t pass = 0; pass < passes; ++pass) {
+    // The last stripe's rows pick its sweep (wave-uniform: one pair per wave): 1 = shorter, 2 = middle tail.
+    using shorter_t = typename tail_sweeps_t<sweep_type_>::shorter;
+    using middle_t = typename tail_sweeps_t<sweep_type_>::middle;
+    int32_t const tail_base = (passes - 1) * stripe_rows;
+    int32_t tail = 0;
+    if constexpr (tail_rows<shorter_t>() > 0) {
+        int32_t const last_rows = __builtin_amdgcn_readfirstlane(sweep.n) - tail_base;
+        if (passes > 0 && last_rows <= tail_rows<shorter_t>()) tail = 1;
+        else if (passes > 0 && last_rows <= tail_rows<middle_t>()) tail = 2;
+    }
+    int32_t const full_passes = passes - (tail ? 1 : 0);
+    for (int32_t pass = 0; pass < full_passes; ++pass) {
         int32_t const base = pass * stripe_rows;
         bool const more = pass + 1 < passes;
         int32_t const *in_row = pass ? scores_row : nullptr;
@@ -78,7 +143,16 @@ __device__ void run_independent(item_args_t const &a) {
         if (more) __threadfence_block(); // lane 63's stores before lane 0's loads in the next pass
         sweep.end_stripe(base, !more);
     }
-    if constexpr (sweep_type_::reduce_k != reduce_none_k) sweep.write_result(sweep.partial());
+    int32_t tail_sum = 0;
+    if constexpr (tail_rows<shorter_t>() > 0) {
+        int32_t const *in_row = full_passes ? scores_row : nullptr;
+        uint32_t const task_index = a.order[a.first + wave];
+        if (tail == 1) tail_sum = run_tail<shorter_t>(a, task_index, wave, tail_base, in_row, gaps_row);
+        if constexpr (tail_rows<middle_t>() > 0) {
+            if (tail == 2) tail_sum = run_tail<middle_t>(a, task_index, wave, tail_base, in_row, gaps_row);
+        }
+    }
+    if constexpr (sweep_type_::reduce_k != reduce_none_k) sweep.write_result(sweep.partial() + tail_sum);
 }
 
 /** Upper bound on polls of a predecessor's progress (each a device-scope load plus a ~128-cycle sleep): seconds,
@@ -722,7 +796,16 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
                                       : sh.packed     ? (uint64_t)(sh.rows_per_lane * 3 + 3) / 4
                                                       : (uint64_t)sh.rows_per_lane;
             uint64_t const ramp = (uint64_t)sh.group * (sh.packed ? 2u : 1u);
-            uint64_t const work = passes * ((uint64_t)task.columns_length + ramp) * per_step;
+            // A tall pair's last stripe may run a shorter sweep (`tail_sweeps_t`): half the Myers word, or 4 or 6
+            // packed rows per lane.
+            uint64_t last_step = per_step;
+            if (!sh.cooperative && sh.group == 64 && passes > 0) {
+                uint64_t const last_rows = rows - (passes - 1) * stripe;
+                if (sh.bit_parallel && sh.rows_per_lane == 64 && last_rows <= 64 * 32) last_step = per_step / 2;
+                else if (sh.packed && sh.rows_per_lane == 8 && last_rows <= 64 * 6) last_step = last_rows <= 64 * 4 ? 3 : 5;
+            }
+            uint64_t const work = (passes ? (passes - 1) * per_step + last_step : 0) *
+                                  ((uint64_t)task.columns_length + ramp);
             if (sh.chained) { // a slot in the chain buffers, sized by the host from these stats
                 uint32_t const chain_slot = atomicAdd(&stats->chained_pairs, 1u);
                 atomicMax(&stats->chained_columns, task.columns_length);

@@ -216,15 +216,19 @@ struct packed_sweep_t {
             uint32_t const cost = __builtin_amdgcn_perm(costs_low[(P + p) / 4], costs_now[p / 4], selector);
             uint32_t const left = scores[p];
             uint32_t const if_substitution = pk_add_u16(diag, cost);
+            // The maxima over the terms that do not depend on the row above come first: the chain from one row
+            // to the next is then 3 dependent packed ops (affine) or 1 (linear) instead of 4 or 2, which lets the
+            // scheduler interleave the rows instead of padding back-to-back dependent packed ops with wait states.
             uint32_t stored;
             if constexpr (affine_k) {
                 uint32_t const h = pk_max_u16(left, horizontal[p]);
+                uint32_t const off_chain = pk_max_u16(h, if_substitution);
                 uint32_t const v = pk_max_u16(up, vert);
                 horizontal[p] = h;
                 vert = v;
-                stored = pk_add_u16(pk_max_u16(pk_max_u16(h, v), if_substitution), kappa);
+                stored = pk_add_u16(pk_max_u16(off_chain, v), kappa);
             }
-            else stored = pk_max_u16(pk_max_u16(up, left), if_substitution);
+            else stored = pk_max_u16(up, pk_max_u16(left, if_substitution));
             diag = left;
             up = stored;
             scores[p] = stored;

@@ -129,6 +129,29 @@ def test_packed_shapes_keep_exact_scores_far_beyond_16_bits(name, ladder, packed
     assert int(expected[0]) > 32767  # a 7,000-residue self-alignment scores far beyond int16
 
 
+# Rows whose last 512-row stripe keeps at most 256 rows, 257-384 or more: it runs 4, 6 or 8 rows per lane.
+TAIL_ROWS = [1, 255, 256, 257, 383, 384, 385, 512, 513, 768, 769, 896, 897, 1024, 1280, 1281, 1408, 1409, 2000]
+
+
+@pytest.mark.parametrize("name", ["nw_affine", "nw_linear", "nw_asymmetric", "nw_far_gap"])
+def test_packed_tail_stripes(name, packed_ladders):
+    """Every pair on 512-row stripes: the last stripe alone or after full ones, on both sides of each tail size."""
+    rng = np.random.default_rng(zlib.crc32(f"tails:{name}".encode()))
+    engine, params = _engine(name, rng)
+    packed_ladders("wave8")
+    a, b = [], []
+    for n in TAIL_ROWS:
+        for m in (n, n + 1, n + 130, 3000):
+            rows, columns = random_bytes(rng, PROTEIN, n, n), random_bytes(rng, PROTEIN, m, m)
+            # Either side first: the shorter side is the rows either way.
+            a.append(rows if m % 2 else columns)
+            b.append(columns if m % 2 else rows)
+    text = random_bytes(rng, PROTEIN, 4400, 4400)  # 8 full stripes and a 304-row tail, scores beyond 16 bits
+    a += [text, text]
+    b += [text, _mutated(rng, text, 0.05, 0.01)]
+    _check(engine, params, a, b, f"{name} tail stripes")
+
+
 def test_packed_batches_and_graph_replays(packed_ladders):
     """The packed sweep inside a mixed batch, as a persistent launch, one launch per pair and graph replays."""
     rng = np.random.default_rng(29)

@@ -127,3 +127,31 @@ def test_half_word_myers_over_bytes(alphabet, byte_ladder):
     expected = O.pairs(O.Params("levenshtein"), O.tape(a), O.tape(b), threads=16)
     got = engine.pairs(szs.Strs(a), szs.Strs(b))
     np.testing.assert_array_equal(got.astype(np.int64), expected)
+
+
+# Rows whose last 4096-row stripe keeps at most 2048 rows (32-row words) or more (64-row words).
+MYERS_TAIL_ROWS = [1, 2047, 2048, 2049, 4096, 4097, 6144, 6145, 8192, 8193, 11000]
+
+
+@pytest.mark.parametrize("alphabet", [b"ACGT", bytes(range(256))])
+def test_myers_tail_stripes_over_bytes(alphabet, byte_ladder):
+    """Every pair on 4096-row stripes: the last stripe runs 32-row words when its rows fit (table and planes)."""
+    rng = np.random.default_rng(zlib.crc32(b"tails:" + alphabet))
+    byte_ladder("wave")
+    table = np.frombuffer(alphabet, dtype=np.uint8)
+    a = [bytes(rng.choice(table, size=n)) for n in MYERS_TAIL_ROWS for _ in range(2)]
+    b = [bytes(rng.choice(table, size=n + extra)) for n in MYERS_TAIL_ROWS for extra in (0, 70)]
+    engine = szs.LevenshteinDistances()
+    expected = O.pairs(O.Params("levenshtein"), O.tape(a), O.tape(b), threads=16)
+    got = engine.pairs(szs.Strs(a), szs.Strs(b))
+    np.testing.assert_array_equal(got.astype(np.int64), expected)
+
+
+@pytest.mark.parametrize("alphabet", ["cjk", "mixed"])
+def test_myers_tail_stripes_over_runes(alphabet, rune_ladder):
+    rng = np.random.default_rng(zlib.crc32(f"tails:{alphabet}".encode()))
+    rune_ladder("wave")
+    letters = ALPHABETS[alphabet]
+    a = [_text(rng, letters, n) for n in MYERS_TAIL_ROWS for _ in range(2)]
+    b = [_text(rng, letters, n + extra) for n in MYERS_TAIL_ROWS for extra in (1, 90)]
+    _check(a, b, f"{alphabet} tail stripes")
