@@ -222,6 +222,139 @@ def tune(slots: List[int] = None, target_cells: float = 4e9, verbose: bool = Tru
     return result
 
 
+class _DeviceTape:
+    """A tape held by device tensors (8-byte offsets), handed to the C ABI in place."""
+
+    def __init__(self, data, offsets):
+        self.data, self.offsets = data.contiguous(), offsets.contiguous()
+
+    def __len__(self):
+        return self.offsets.numel() - 1
+
+    def tape(self):
+        return _native.U64Tape(self.data.data_ptr(), self.offsets.data_ptr(), len(self))
+
+
+def _segment_ms(items, repeats: int = 3) -> float:
+    """Best scoring-kernel time of `items` (a list of (engine, firsts, seconds)) under the installed ladders."""
+    batch = szs.Batch(items, mode=szs.BATCH_PERSISTENT)
+    batch.run()
+    best = float("inf")
+    for _ in range(repeats):
+        batch.run()
+        best = min(best, batch.last_kernel_ms)
+    return best
+
+
+def _moves(slot: int, ladder, beyond: int, table) -> List[Tuple[str, list, int]]:
+    """Ladders one step away from `ladder`: another admissible shape for one entry or beyond it, one limit moved to
+    the next band edge either way, or one entry dropped (the next one covers its rows)."""
+    out = []
+    used = {s for s, _ in ladder} | {beyond}
+    edges = sorted(set(BANDS) | {g * r for g, r, *_ in table})
+    lows = [1] + [limit + 1 for _, limit in ladder]
+    for i, (shape, limit) in enumerate(ladder):
+        for other in candidates_for(slot, lows[i], limit, table):
+            group, rows_per_lane = table[other][0], table[other][1]
+            if other in used or table[other][2] or (group < 64 and limit > group * rows_per_lane):
+                continue
+            # Bit-parallel words and DP rows are not close alternatives, nor int32 cells for packed ones.
+            if table[other][3] != table[shape][3] or table[other][4] < table[shape][4]:
+                continue
+            out.append((f"entry {i}: shape {shape} -> {other}", ladder[:i] + [(other, limit)] + ladder[i + 1:], beyond))
+        floor = ladder[i - 1][1] if i else 0
+        ceiling = ladder[i + 1][1] if i + 1 < len(ladder) else None
+        below = [e for e in edges if floor < e < limit]
+        above = [e for e in edges if e > limit and (ceiling is None or e < ceiling)]
+        group, rows_per_lane = table[shape][0], table[shape][1]
+        if group < 64:
+            above = [e for e in above if e <= group * rows_per_lane]
+        for edge in ([below[-1]] if below else []) + ([above[0]] if above and ceiling is not None else []):
+            out.append((f"entry {i}: limit {limit} -> {edge}", ladder[:i] + [(shape, edge)] + ladder[i + 1:], beyond))
+        if i + 1 < len(ladder):
+            out.append((f"entry {i}: drop shape {shape}", ladder[:i] + ladder[i + 1:], beyond))
+        # An unused shape for the lower part of this entry's rows, up to a band edge inside them.
+        for edge in below:
+            for other in candidates_for(slot, lows[i], edge, table):
+                group, rows_per_lane = table[other][0], table[other][1]
+                if other in used or table[other][2] or table[other][3] != table[shape][3]:
+                    continue
+                if table[other][4] < table[shape][4]:
+                    continue
+                if group < 64 and edge > group * rows_per_lane:
+                    continue
+                out.append((f"entry {i}: shape {other} up to {edge}", ladder[:i] + [(other, edge)] + ladder[i:],
+                            beyond))
+    top = ladder[-1][1] if ladder else 0
+    for other in candidates_for(slot, top + 1, max(top + 1, 16384), table):
+        if table[other][2] and other not in used and table[other][3] == table[beyond][3]:
+            out.append((f"beyond: shape {beyond} -> {other}", list(ladder), other))
+    return out
+
+
+def refine(slot: int, items, rounds: int = 2, gain: float = 0.01, verbose: bool = True) -> Dict:
+    """Local search over one slot's ladder on a real workload rather than on bands in isolation.
+
+    `items` are the (engine, firsts, seconds) lists whose pairs use `slot`. Every ladder one move away (see
+    `_moves`) is installed and the whole workload timed; a move is kept when it shortens the scoring kernel by more
+    than `gain`, and the search repeats from the new ladder for up to `rounds` passes. Shapes then compete inside a
+    full, mixed-length launch, as they do in the engine's real batches.
+    """
+    table = shapes()
+    ladder, beyond = get_ladder(slot)
+    ladder = [tuple(x) for x in ladder]
+    best_ms = _segment_ms(items)
+    start_ms = best_ms
+    history = []
+    if verbose:
+        print(f"slot {slot}: start {best_ms:.2f} ms with {ladder} beyond {beyond}", flush=True)
+    for _ in range(rounds):
+        improved = False
+        for label, candidate, candidate_beyond in _moves(slot, ladder, beyond, table):
+            try:
+                set_ladder(slot, candidate, candidate_beyond)
+            except Exception:  # an inadmissible ladder: the library rejects it
+                continue
+            ms = _segment_ms(items)
+            if verbose:
+                print(f"slot {slot}:   {label}: {ms:.2f} ms", flush=True)
+            if ms < best_ms * (1.0 - gain):
+                history.append({"move": label, "ms": round(ms, 3)})
+                if verbose:
+                    print(f"slot {slot}: {label}: {best_ms:.2f} -> {ms:.2f} ms", flush=True)
+                ladder, beyond, best_ms, improved = [tuple(x) for x in candidate], candidate_beyond, ms, True
+            set_ladder(slot, ladder, beyond)
+        if not improved:
+            break
+    set_ladder(slot, ladder, beyond)
+    return {"ladder": [list(x) for x in ladder], "beyond": beyond, "start_ms": round(start_ms, 3),
+            "refined_ms": round(best_ms, 3), "moves": history}
+
+
+def refine_on_mixed_batch(pairs: int = 1_000_000, long_pairs: int = 32, verbose: bool = True) -> Dict:
+    """Refines the ladders of the mixed batch's three kinds (unit-cost bytes and runes, affine NW) on its segments,
+    generated on the device from the benchmark's seed."""
+    import torch
+
+    workload = W.mixed_c3(pairs, long_pairs, device="cuda")
+    torch.cuda.synchronize()
+    b2c, costs = W.protein_matrix(np.random.default_rng(W.SEED_BASE + 1))
+    result = {}
+    for segment in workload.segments:
+        if segment.kind == "levenshtein":
+            engine, slot = szs.LevenshteinDistances(), UNIT_BYTES_SLOT
+        elif segment.kind == "levenshtein_utf8":
+            engine, slot = szs.LevenshteinDistancesUTF8(), UNIT_RUNES_SLOT
+        else:
+            engine = szs.NeedlemanWunschScores(segment.byte_to_class, segment.class_costs, segment.open,
+                                               segment.extend)
+            slot = slot_of("nw_affine")
+        items = [(engine, _DeviceTape(segment.firsts_data, segment.firsts_offsets),
+                  _DeviceTape(segment.seconds_data, segment.seconds_offsets))]
+        result[str(slot)] = refine(slot, items, verbose=verbose)
+    return result
+
+
 def apply(path: str = CACHE) -> bool:
     """Installs a cached tuning if it was measured on the architecture the library is compiled for (gfx950).
 
@@ -247,9 +380,27 @@ if __name__ == "__main__":
     parser = argparse.ArgumentParser(description="Measure and cache the fastest shape ladders.")
     parser.add_argument("slots", type=int, nargs="*", help="slots to tune (default: the benchmark's kinds)")
     parser.add_argument("--out", default=CACHE, help="where to write the merged tuning (default: the package cache)")
+    parser.add_argument("--refine", action="store_true",
+                        help="start from the installed ladders and refine those of the mixed batch's kinds on its "
+                             "own segments instead of timing bands in isolation")
     arguments = parser.parse_args()
     start = time.time()
-    tuned = tune(arguments.slots or None)
+    if arguments.refine:
+        import torch
+
+        apply()
+        refined = refine_on_mixed_batch()
+        tuned = {"arch": torch.cuda.get_device_properties(0).gcnArchName.split(":")[0], "slots": {}}
+        cached_slots = {}
+        if os.path.exists(CACHE):
+            with open(CACHE) as f:
+                cached_slots = json.load(f).get("slots", {})
+        for slot, entry in refined.items():
+            # The band measurements stay; the ladder is the refined one, with the search's record beside it.
+            tuned["slots"][slot] = {**cached_slots.get(slot, {}), "ladder": entry["ladder"], "beyond": entry["beyond"],
+                                    "refined_on_mixed_batch": {k: entry[k] for k in ("start_ms", "refined_ms", "moves")}}
+    else:
+        tuned = tune(arguments.slots or None)
     # Slots not tuned in this run keep their cached ladders when the cache was measured on the same architecture.
     if os.path.exists(CACHE):
         with open(CACHE) as f:

@@ -480,6 +480,7 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
                        engine->params.extend == 1;
         a.reach_magnitude = std::max(1, std::max(engine->substitution_magnitude, engine->gap_magnitude));
         a.packed_rows = a.symbols == symbols_u8_k ? engine->packed_rows : 0;
+        a.latency_stripes = options.latency_stripes;
         tasks_total += tasks;
         if (tasks_total >= 0xFFFFFFF0ull)
             return fail(sz_overflow_risk_k, error_message, "A batch holds at most 2^32 - 16 scored cells");

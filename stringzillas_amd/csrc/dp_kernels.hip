@@ -782,6 +782,11 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
             }
             // Packed 16-bit shapes only for engines whose score range provably fits them; else the int32 twin.
             if (shapes_k[shape].packed && shapes_k[shape].rows_per_lane > a.packed_rows) shape = unpacked_twin(shape);
+            // Latency-oriented batches give a pair that one wave would sweep in many stripes the whole workgroup.
+            if (a.latency_stripes > 0 && cooperative_twin(shape) >= 0) {
+                uint64_t const stripe_rows = (uint64_t)shapes_k[shape].group * shapes_k[shape].rows_per_lane;
+                if ((rows + stripe_rows - 1) / stripe_rows > (uint64_t)a.latency_stripes) shape = cooperative_twin(shape);
+            }
             shape_t const sh = shapes_k[shape];
             // Latency of the item in lane-row-steps, the key for heaviest-first ordering.
             uint64_t const stripe = (uint64_t)sh.group * sh.rows_per_lane;

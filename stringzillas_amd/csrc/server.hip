@@ -220,6 +220,9 @@ void server_t::run() {
             batch_options_t options;
             options.stream = streams[free_stream];
             options.defer_overflow = true;
+            // A round's latency is its longest item: pairs taller than 16 stripes of a one-wave shape (8,192 rows of
+            // packed NW, 65,536 of Myers words) take the whole workgroup.
+            options.latency_stripes = 16;
             if (pinned[free_stream].reserve(admitted_results) == hipSuccess) {
                 options.pinned_results = pinned[free_stream].data;
                 options.pinned_capacity = pinned[free_stream].capacity;

@@ -134,6 +134,20 @@ static_assert(unpacked_twin(15) == 5 && unpacked_twin(16) == 6 && unpacked_twin(
                   unpacked_twin(19) == 20,
               "every packed shape has an int32 twin");
 
+/** The cooperative shape with the same geometry as a one-wave shape (-1 when there is none). */
+__host__ __device__ constexpr int cooperative_twin(int shape) {
+    if (shapes_k[shape].cooperative || shapes_k[shape].group != 64) return -1;
+    for (int s = 0; s < shapes_count_k; ++s)
+        if (shapes_k[s].cooperative && !shapes_k[s].chained && shapes_k[s].group == 64 &&
+            shapes_k[s].rows_per_lane == shapes_k[shape].rows_per_lane &&
+            shapes_k[s].bit_parallel == shapes_k[shape].bit_parallel && shapes_k[s].packed == shapes_k[shape].packed)
+            return s;
+    return -1;
+}
+static_assert(cooperative_twin(17) == 18 && cooperative_twin(13) == 14 && cooperative_twin(7) == 9 &&
+                  cooperative_twin(8) == 10 && cooperative_twin(15) == -1 && cooperative_twin(22) == -1,
+              "one-wave shapes and their cooperative twins");
+
 /**
  *  Work items (workgroups) a chained pair is split into: groups of at most 4 stripes, one per wave, so that a
  *  group's last stripe starts a few phases after its first and the next group can follow right behind; pairs of
@@ -222,7 +236,7 @@ struct materialize_args_t {
     int32_t runes;                       /**< lengths come from the rune counts (UTF-8) */
     int32_t reach_magnitude;             /**< max |cost| of the engine, for the int32 reach bound */
     int32_t packed_rows;                 /**< most rows per lane of a packed 16-bit shape the engine admits (0: none) */
-    int32_t reserved;
+    int32_t latency_stripes;             /**< > 0: pairs of more stripes than this on one wave go cooperative */
 };
 
 /** Bytes of the per-wave LDS ring that stages lane 0's inputs of a 64-step chunk (64 entries of 16 bytes); it

@@ -148,6 +148,9 @@ struct batch_options_t {
     bool defer_overflow = false;    /**< flag failing segments (overflow, bad tape) instead of failing the batch */
     char *pinned_results = nullptr; /**< page-locked host staging for pair-list results (null: copy directly) */
     size_t pinned_capacity = 0;     /**< bytes at `pinned_results` */
+    /** Latency over throughput: a pair that a one-wave shape would sweep in more than this many stripes takes the
+     *  cooperative shape of the same geometry, so no single wave sets the end of a short launch (0: off). */
+    int32_t latency_stripes = 0;
 };
 
 /** A batch staged on one device: everything `run` needs, ready to be launched (or replayed from a graph). */

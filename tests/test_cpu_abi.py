@@ -150,6 +150,31 @@ def test_tuning_ladders_validate_and_round_trip():
             autotune.set_ladder(slot, ladder, beyond)
 
 
+def test_refinement_moves_are_admissible_neighbours():
+    """Every ladder the workload-level search would try is one the library accepts, and differs from the start."""
+    from stringzillas_amd import autotune
+
+    table = autotune.shapes()
+    for slot in (autotune.UNIT_BYTES_SLOT, autotune.UNIT_RUNES_SLOT, autotune.slot_of("nw_affine")):
+        saved = autotune.get_ladder(slot)
+        ladder, beyond = [tuple(x) for x in saved[0]], saved[1]
+        moves = autotune._moves(slot, ladder, beyond, table)
+        assert moves, slot
+        try:
+            for label, candidate, candidate_beyond in moves:
+                assert (candidate, candidate_beyond) != (ladder, beyond), label
+                limits = [limit for _, limit in candidate]
+                assert limits == sorted(set(limits)), label
+                assert len({s for s, _ in candidate} | {candidate_beyond}) == len(candidate) + 1, label
+                for shape, limit in candidate:
+                    group, rows_per_lane = table[shape][0], table[shape][1]
+                    assert group == 64 or limit <= group * rows_per_lane, label
+                    assert table[shape][3] == (slot in autotune.UNIT_SLOTS) or limit <= 64, label
+                autotune.set_ladder(slot, candidate, candidate_beyond)  # the library validates it
+        finally:
+            autotune.set_ladder(slot, *saved)
+
+
 def test_tape_files_round_trip_and_reject_damage(tmp_path):
     pa = pytest.importorskip("pyarrow")
     strings = [b"", b"ACGT" * 40, "Ж中😀".encode(), b"x"]
