@@ -259,19 +259,34 @@ struct myers_sweep_t {
         else return (word_t)eq_low;
     }
 
-    /** One column of this lane's word (Hyyro's step); `hin` is the horizontal delta entering its top row. */
+    /** A three-input bitwise function per 32-bit half: bit i of the result is bit 4a + 2b + c of `table_`. */
+    template <uint32_t table_>
+    __device__ __forceinline__ static word_t bitop3(word_t a, word_t b, word_t c) {
+        uint32_t const low = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, (uint32_t)c, table_);
+        if constexpr (word_bits_ == 64) {
+            uint32_t const high = __builtin_amdgcn_bitop3_b32((uint32_t)((uint64_t)a >> 32), (uint32_t)((uint64_t)b >> 32),
+                                                              (uint32_t)((uint64_t)c >> 32), table_);
+            return (word_t)((uint64_t)high << 32 | low);
+        }
+        else return (word_t)low;
+    }
+
+    /**
+     *  One column of this lane's word (Hyyro's step); `hin` is the horizontal delta entering its top row. The
+     *  three-input terms are single `v_bitop3` per half: (x ^ y) | z is table 0xBE, x | ~(y | z) is table 0xF1.
+     */
     __device__ __forceinline__ void step_word(word_t eq, uint32_t hin) {
         constexpr int top_bit = word_bits_ - 1;
         word_t const hin_positive = hin & 1u, hin_negative = (hin >> 1) & 1u;
         word_t const xv = eq | mv;
         eq |= hin_negative;
-        word_t const xh = (((eq & pv) + pv) ^ pv) | eq;
-        word_t ph = mv | ~(xh | pv);
+        word_t const xh = bitop3<0xBE>((eq & pv) + pv, pv, eq); // (((eq & pv) + pv) ^ pv) | eq
+        word_t ph = bitop3<0xF1>(mv, xh, pv);                   // mv | ~(xh | pv)
         word_t mh = pv & xh;
         bottom_code = (uint32_t)(ph >> top_bit) | (uint32_t)(mh >> top_bit) << 1;
         ph = (ph << 1) | hin_positive;
         mh = (mh << 1) | hin_negative;
-        pv = mh | ~(xv | ph);
+        pv = bitop3<0xF1>(mh, xv, ph); // mh | ~(xv | ph)
         mv = ph & xv;
     }
 
