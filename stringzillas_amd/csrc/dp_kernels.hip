@@ -539,31 +539,49 @@ __device__ uint64_t decode_walk(uint8_t const *first, uint8_t const *last, uint3
 }
 
 /**
- *  The whole wave decodes one string (the same arguments in every lane), 256 bytes per pass, 4 per lane. When the
- *  string is well formed (every lead byte followed by exactly its continuation bytes, no stray continuation
+ *  The whole wave decodes one string (the same arguments in every lane), 1 KB per pass, 16 bytes per lane. When
+ *  the string is well formed (every lead byte followed by exactly its continuation bytes, no stray continuation
  *  byte; a sequence cut by the end of the string is allowed), the walk above visits exactly the bytes that are
  *  not continuation bytes, so a rune's index is the count of such bytes before it: a wave prefix sum. Returns
  *  the rune count, or -1 if some byte breaks the rule (the caller then walks the string in one lane). The output
- *  must have room for one rune per byte.
+ *  must have room for one rune per byte. A lane's 16 bytes come in one load when they all belong to the string
+ *  (so no load leaves its pages), byte by byte at the string's end; the next pass's load is issued before this
+ *  pass decodes, so its latency overlaps the work.
  */
 __device__ int64_t decode_wave(uint8_t const *first, uint8_t const *last, uint32_t *out) {
+    constexpr int32_t lane_bytes_k = 16;
+    constexpr int64_t pass_bytes_k = 64 * lane_bytes_k;
     uint32_t const lane = __lane_id();
     int64_t const length = last - first;
     int64_t written = 0;
     uint32_t carry = 0; // the previous pass's last 4 bytes (zero bytes before the string: ASCII, cover nothing)
-    for (int64_t base = 0; base < length; base += 256) {
-        int64_t const mine = base + 4 * (int64_t)lane;
-        int32_t const valid = (int32_t)max((int64_t)0, min((int64_t)4, length - mine));
-        uint32_t word = 0;
-        for (int32_t i = 0; i < valid; ++i) word |= (uint32_t)first[mine + i] << (8 * i);
-        uint32_t const previous_lane = (uint32_t)__shfl_up((int)word, 1, 64);
-        uint32_t const before = lane == 0 ? carry : previous_lane;   // bytes mine-4 .. mine-1
-        uint32_t following = (uint32_t)__shfl_down((int)word, 1, 64); // bytes mine+4 .. mine+7
+    auto whole_chunk = [&](int64_t at) { // 16 bytes at `at`, all inside the string, or zeros
+        uint4 chunk = make_uint4(0, 0, 0, 0);
+        if (at + lane_bytes_k <= length) __builtin_memcpy(&chunk, first + at, lane_bytes_k);
+        return chunk;
+    };
+    uint4 upcoming = whole_chunk(lane_bytes_k * (int64_t)lane);
+    for (int64_t base = 0; base < length; base += pass_bytes_k) {
+        int64_t const mine = base + lane_bytes_k * (int64_t)lane;
+        int32_t const valid = (int32_t)max((int64_t)0, min((int64_t)lane_bytes_k, length - mine));
+        uint4 const chunk = upcoming;
+        upcoming = whole_chunk(mine + pass_bytes_k);
+        // w[0..3]: this lane's bytes; w[4]: the 3 bytes after them (for the last runes' continuation bytes).
+        uint32_t w[5] = {chunk.x, chunk.y, chunk.z, chunk.w, 0};
+        if (valid > 0 && valid < lane_bytes_k) { // the string's last bytes, one at a time
+#pragma unroll
+            for (int32_t i = 0; i < lane_bytes_k; ++i)
+                if (i < valid) w[i / 4] |= (uint32_t)first[mine + i] << (8 * (i % 4));
+        }
+        uint32_t const previous_lane = (uint32_t)__shfl_up((int)w[3], 1, 64);
+        uint32_t const before = lane == 0 ? carry : previous_lane;     // bytes mine-4 .. mine-1
+        uint32_t following = (uint32_t)__shfl_down((int)w[0], 1, 64); // bytes mine+16 .. mine+19
         if (lane == 63) {
             following = 0;
             for (int32_t i = 0; i < 3; ++i)
-                if (mine + 4 + i < length) following |= (uint32_t)first[mine + 4 + i] << (8 * i);
+                if (mine + lane_bytes_k + i < length) following |= (uint32_t)first[mine + lane_bytes_k + i] << (8 * i);
         }
+        w[4] = following;
         // Continuation bytes still owed by the last lead among the 3 bytes before this lane.
         int32_t need = 0;
         for (int32_t d = 3; d >= 1; --d) {
@@ -572,17 +590,20 @@ __device__ int64_t decode_wave(uint8_t const *first, uint8_t const *last, uint32
         }
         bool broken = false;
         uint32_t leads = 0;
-        for (int32_t i = 0; i < valid; ++i) {
-            uint32_t const byte = (word >> (8 * i)) & 0xFFu;
-            bool const continuation = (byte & 0xC0u) == 0x80u;
-            if (need > 0) {
-                broken |= !continuation;
-                --need;
-            }
-            else if (continuation) broken = true;
-            else {
-                leads |= 1u << i;
-                need = (int32_t)utf8_width(byte) - 1;
+#pragma unroll
+        for (int32_t i = 0; i < lane_bytes_k; ++i) {
+            if (i < valid) {
+                uint32_t const byte = (w[i / 4] >> (8 * (i % 4))) & 0xFFu;
+                bool const continuation = (byte & 0xC0u) == 0x80u;
+                if (need > 0) {
+                    broken |= !continuation;
+                    --need;
+                }
+                else if (continuation) broken = true;
+                else {
+                    leads |= 1u << i;
+                    need = (int32_t)utf8_width(byte) - 1;
+                }
             }
         }
         if (__ballot(broken)) return -1;
@@ -592,20 +613,22 @@ __device__ int64_t decode_wave(uint8_t const *first, uint8_t const *last, uint32
             int32_t const other = __shfl_up(inclusive, d, 64);
             if ((int32_t)lane >= d) inclusive += other;
         }
-        int64_t rank = written + inclusive - count;
-        uint64_t const window = (uint64_t)following << 32 | word; // 8 bytes from `mine`
-        for (int32_t i = 0; i < valid; ++i) {
-            if (!((leads >> i) & 1u)) continue;
-            uint32_t const lead = (uint32_t)(window >> (8 * i)) & 0xFFu;
-            uint32_t const width = utf8_width(lead);
-            auto tail = [&](int32_t d) { // continuation byte d of this lead, 0 past the end of the string
-                return mine + i + d < length ? (uint32_t)(window >> (8 * (i + d))) & 0xFFu : 0u;
-            };
-            out[rank++] = rune_of(lead, width, width > 1 ? tail(1) : 0u, width > 2 ? tail(2) : 0u,
-                                  width > 3 ? tail(3) : 0u);
+        uint32_t *to = out + written + inclusive - count;
+#pragma unroll
+        for (int32_t i = 0; i < lane_bytes_k; ++i) {
+            if ((leads >> i) & 1u) {
+                auto byte_at = [&](int32_t j) { return (w[j / 4] >> (8 * (j % 4))) & 0xFFu; };
+                auto tail = [&](int32_t d) { // continuation byte d of this lead, 0 past the end of the string
+                    return mine + i + d < length ? byte_at(i + d) : 0u;
+                };
+                uint32_t const lead = byte_at(i);
+                uint32_t const width = utf8_width(lead);
+                *to++ = rune_of(lead, width, width > 1 ? tail(1) : 0u, width > 2 ? tail(2) : 0u,
+                                width > 3 ? tail(3) : 0u);
+            }
         }
         written += __shfl(inclusive, 63, 64);
-        carry = (uint32_t)__shfl((int)word, 63, 64);
+        carry = (uint32_t)__shfl((int)w[3], 63, 64);
     }
     return written;
 }

@@ -155,3 +155,15 @@ def test_myers_tail_stripes_over_runes(alphabet, rune_ladder):
     a = [_text(rng, letters, n) for n in MYERS_TAIL_ROWS for _ in range(2)]
     b = [_text(rng, letters, n + extra) for n in MYERS_TAIL_ROWS for extra in (1, 90)]
     _check(a, b, f"{alphabet} tail stripes")
+
+
+@pytest.mark.parametrize("alphabet", ["cyrillic", "cjk", "mixed"])
+def test_wave_decode_across_lane_and_pass_boundaries(alphabet):
+    """Long strings decode 16 bytes per lane, 1 KB per pass: sequences straddle lanes and passes, and strings cut
+    at byte lengths around those edges end inside a sequence."""
+    rng = np.random.default_rng(zlib.crc32(f"decode:{alphabet}".encode()))
+    letters = ALPHABETS[alphabet]
+    sizes = [256, 257, 271, 1008, 1023, 1024, 1025, 1039, 2047, 2048, 2049, 3073, 5000]
+    a = [_text(rng, letters, size)[:size] for size in sizes for _ in range(3)]
+    b = [_text(rng, letters, size + 5)[: size + int(rng.integers(0, 40))] for size in sizes for _ in range(3)]
+    _check(a, b, f"{alphabet} decode boundaries")
