@@ -111,7 +111,7 @@ static shape_table_t &active_shape_table() {
     return table;
 }
 
-/** Rough latency of one work item of a bin (its bucket is the half-octave of lane-row-steps), heaviest first. */
+/** Rough latency of one work item of a bin (its bucket is the quarter-octave of lane-row-steps), heaviest first. */
 static double bin_item_cost(uint32_t bin) {
     int const variant = (int)(bin / column_buckets_k);
     int const bucket = (int)(bin % column_buckets_k);
@@ -120,7 +120,7 @@ static double bin_item_cost(uint32_t bin) {
                              family == family_sw_affine_k) ? 1.8 : 1.0;
     // Chained pairs go first: their groups form a pipeline whose fill is the batch's critical path.
     double const chained = shapes_k[variant % shapes_count_k].chained ? 1e30 : 1.0;
-    return std::pow(2.0, bucket / 2.0) * per_cell * chained;
+    return std::pow(2.0, bucket / (double)bucket_steps_per_octave_k) * per_cell * chained;
 }
 
 static std::vector<uint32_t> const &bin_priority() {

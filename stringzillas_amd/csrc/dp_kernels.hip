@@ -700,11 +700,13 @@ __global__ void widen_offsets_kernel(uint32_t const *narrow, uint64_t *wide, uin
         wide[i] = narrow[i];
 }
 
-__device__ __forceinline__ uint32_t half_octave(uint64_t x) {
-    if (x < 2) return 0;
+/** The cost bucket of `x`: its octave and the next two bits below the leading one (quarter-octaves). */
+__device__ __forceinline__ uint32_t cost_bucket(uint64_t x) {
+    static_assert(bucket_steps_per_octave_k == 4, "two bits below the leading one");
+    if (x < 4) return (uint32_t)x;
     uint32_t const lg = 63 - __clzll(x);
-    uint32_t const half = (x >> (lg - 1)) & 1u; // second-highest bit
-    uint32_t const bucket = 2 * lg + half;
+    uint32_t const quarter = (uint32_t)(x >> (lg - 2)) & 3u; // the two bits after the leading one
+    uint32_t const bucket = 4 * lg + quarter;
     return bucket < column_buckets_k ? bucket : column_buckets_k - 1;
 }
 
@@ -839,7 +841,7 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
                 atomicMax(&stats->chained_columns, task.columns_length);
                 task.flags |= (chain_slot + 1u) << 8;
             }
-            bin = (uint32_t)variant_of(family, symbols, shape) * column_buckets_k + half_octave(work);
+            bin = (uint32_t)variant_of(family, symbols, shape) * column_buckets_k + cost_bucket(work);
             atomicAdd(&histogram[bin], 1u);
             cells += (unsigned long long)rows * task.columns_length;
             if (rows > stripe && task.columns_length > widest) widest = task.columns_length;

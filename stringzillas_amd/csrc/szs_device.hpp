@@ -183,11 +183,13 @@ __host__ __device__ inline uint32_t items_for(shape_t shape, uint32_t tasks) {
     return (tasks + per_item - 1) / per_item;
 }
 
-/** Cost buckets per shape for binning: half-octaves of (stripes x (columns + group)). */
-constexpr int column_buckets_k = 64;
+/** Cost buckets per shape for binning: quarter-octaves of (stripes x (columns + group)), so that the pairs sharing
+ *  a wave in a small-group shape (which runs until its longest pair ends) differ by at most 19% in work. */
+constexpr int bucket_steps_per_octave_k = 4;
+constexpr int column_buckets_k = 32 * bucket_steps_per_octave_k;
 /** Items of this bucket and above (about 2.1 M lane-row-steps, e.g. a 65,536 x 65,536 pair) run at raised wave
  *  priority: their latency, not the batch's total work, would otherwise decide when the launch ends. */
-constexpr int critical_bucket_k = 42;
+constexpr int critical_bucket_k = 21 * bucket_steps_per_octave_k;
 /** Variant = (family, symbols, shape); bins = variants x column buckets. */
 constexpr int variants_count_k = families_count_k * 2 * shapes_count_k;
 constexpr int bins_count_k = variants_count_k * column_buckets_k;
