@@ -111,7 +111,7 @@ static shape_table_t &active_shape_table() {
     return table;
 }
 
-/** Rough latency of one work item of a bin (its bucket is the quarter-octave of lane-row-steps), heaviest first. */
+/** Rough latency of one work item of a bin (its bucket is the eighth-octave of lane-row-steps), heaviest first. */
 static double bin_item_cost(uint32_t bin) {
     int const variant = (int)(bin / column_buckets_k);
     int const bucket = (int)(bin % column_buckets_k);

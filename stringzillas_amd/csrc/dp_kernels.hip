@@ -700,13 +700,15 @@ __global__ void widen_offsets_kernel(uint32_t const *narrow, uint64_t *wide, uin
         wide[i] = narrow[i];
 }
 
-/** The cost bucket of `x`: its octave and the next two bits below the leading one (quarter-octaves). */
+/** The cost bucket of `x`: its octave and the bits after the leading one (`bucket_steps_per_octave_k` steps). */
 __device__ __forceinline__ uint32_t cost_bucket(uint64_t x) {
-    static_assert(bucket_steps_per_octave_k == 4, "two bits below the leading one");
-    if (x < 4) return (uint32_t)x;
+    constexpr uint32_t steps = bucket_steps_per_octave_k;
+    constexpr uint32_t bits = steps == 8 ? 3 : steps == 4 ? 2 : steps == 2 ? 1 : 0;
+    static_assert((1u << bits) == steps, "a power of two steps per octave");
+    if (x < steps) return (uint32_t)x;
     uint32_t const lg = 63 - __clzll(x);
-    uint32_t const quarter = (uint32_t)(x >> (lg - 2)) & 3u; // the two bits after the leading one
-    uint32_t const bucket = 4 * lg + quarter;
+    uint32_t const fraction = (uint32_t)(x >> (lg - bits)) & (steps - 1u); // the bits after the leading one
+    uint32_t const bucket = steps * lg + fraction;
     return bucket < column_buckets_k ? bucket : column_buckets_k - 1;
 }
 
