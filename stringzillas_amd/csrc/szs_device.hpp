@@ -184,7 +184,7 @@ __host__ __device__ inline uint32_t items_for(shape_t shape, uint32_t tasks) {
 }
 
 /** Cost buckets per shape for binning: eighth-octaves of (stripes x (columns + group)), so that the pairs sharing
-  *  a wave in a small-group shape (which runs until its longest pair ends) differ by at most 9% in work. */
+ *  a wave in a small-group shape (which runs until its longest pair ends) differ by at most 9% in work. */
 constexpr int bucket_steps_per_octave_k = 8;
 constexpr int column_buckets_k = 32 * bucket_steps_per_octave_k;
 /** Items of this bucket and above (about 2.1 M lane-row-steps, e.g. a 65,536 x 65,536 pair) run at raised wave
