@@ -16,18 +16,24 @@
  *  columns behind lane k (lane k is at column t + 1 - 2k at step t) and a stripe needs m + 127 steps.
  *  Per two cells the affine step costs one `v_perm_b32` (both halves' costs from the int8 LDS profile: this
  *  column's byte for the upper row, the previous column's for the lower row), one `v_pk_add_u16` (diagonal +
- *  cost), four `v_pk_max_u16` and one `v_pk_add_u16` (+ o - e); linear three fewer. Per step add three DPP moves
- *  and two `v_alignbit_b32` that assemble the packed top inputs (the lane above's bottom row in the low half, this
- *  lane's upper-half bottom row in the high half).
+ *  cost), two `v_pk_max_u16` (ĥ, v̂), one `v_pk_maximum3_f16` (the three-way maximum, see below) and one
+ *  `v_pk_add_u16` (+ o - e); linear two fewer. Per step add three DPP moves, two `v_alignbit_b32` that assemble
+ *  the packed top inputs (the lane above's bottom row in the low half, this lane's upper-half bottom row in the
+ *  high half) and two `v_pk_add_u16` that move the lane above's values into this lane's base.
  *
- *  Value range. Cells are stored as u16 = value - base + 32768, where `base` is one wave-uniform int32 value chosen
- *  at the start of every 64-step chunk (the row above the stripe at the chunk's first column) and every register is
- *  moved to the new base with one packed add. The engine admits this form only when every value the wave holds
- *  during a chunk provably stays within 32767 of that base: adjacent cells differ by at most
- *  delta = max(|o|, max(0, s_max - min(o, e)) + max(0, o - e)) + |e| in this domain, the cells a wave holds during
- *  a chunk lie within a Manhattan distance of 64R + 320 of each other, and gap tracks and intermediate sums stay
- *  within a few more deltas of their cell (runtime.hip, `packed_rows_limit`). Otherwise pairs run the int32 sweep.
- *  Profile bytes hold `cost - (o + e)`, which the engine requires in [0, 127], so they read as unsigned bytes.
+ *  Value range. Cells are stored as u16 = value - base + 0x3E00, where `base` is an int32 value of each lane,
+ *  chosen at the start of every 64-step chunk (the lane's own first row at its current column) and every register
+ *  is moved to the new base with one packed add. Values then lie in [0, 0x7BFF]: the bit patterns of the finite,
+ *  non-negative halves, whose float order is their integer order, so the float `maximum3` (gfx950 has no 3-input
+ *  integer packed maximum) is exact on them; kernels run with half denormals preserved. The engine admits this form
+ *  only when every value a lane holds during a chunk provably stays within 15871 of its base: adjacent cells differ
+ *  by at most delta = max(|o|, max(0, s_max - min(o, e)) + max(0, o - e)) + |e| in this domain, the cells a lane
+ *  holds during a chunk (its R rows and the row above, 66 columns) lie within a Manhattan distance of R + 70 of its
+ *  base cell, and gap tracks and intermediate sums stay within a few more deltas of their cell (runtime.hip,
+ *  `packed_rows_limit`). Otherwise pairs run the int32 sweep. A lane whose upper half has passed column m holds no
+ *  cell that is kept and keeps its base; the row above the stripe repeats column m's values beyond it; cells beyond
+ *  column m or row n only feed cells that are discarded, and the two halves of a register never mix. Profile
+ *  bytes hold `cost - (o + e)`, which the engine requires in [0, 127], so they read as unsigned bytes.
  */
 #pragma once
 #include <hip/hip_runtime.h>
@@ -49,8 +55,17 @@ __device__ __forceinline__ uint32_t pk_add_u16(uint32_t a, uint32_t b) {
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2_t, a) + __builtin_bit_cast(u16x2_t, b));
 }
 
-/** The u16 that stands for int32 `value - base`, centred at 32768. */
-constexpr int32_t packed_centre_k = 32768;
+/** The u16 that stands for int32 `value - base`: centred in [0, 0x7BFF], the finite non-negative halves. */
+constexpr int32_t packed_centre_k = 0x3E00;
+
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+
+/** Packed maximum of three, exact on u16 values in [0, 0x7BFF] read as halves (one `v_pk_maximum3_f16`). */
+__device__ __forceinline__ uint32_t pk_max3_ordered(uint32_t a, uint32_t b, uint32_t c) {
+    f16x2_t const x = __builtin_bit_cast(f16x2_t, a), y = __builtin_bit_cast(f16x2_t, b);
+    f16x2_t const z = __builtin_bit_cast(f16x2_t, c);
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_maximum(__builtin_elementwise_maximum(x, y), z));
+}
 
 template <int family_, int R>
 struct packed_sweep_t {
@@ -79,7 +94,7 @@ struct packed_sweep_t {
 
     // The stripe.
     int32_t lane_top;
-    int32_t base;                     /**< wave-uniform: the int32 value that the u16 32768 stands for */
+    int32_t base;                     /**< this lane's: the int32 value that the u16 `packed_centre_k` stands for */
     uint32_t scores[P], horizontal[P];
     uint32_t diagonal;                /**< the packed top input of the previous step: this step's diagonal */
     uint32_t vertical;                /**< vertical gaps below the last register after the previous step */
@@ -179,14 +194,17 @@ struct packed_sweep_t {
     __device__ __forceinline__ void column_inputs(int32_t j, int32_t const *in_scores, int32_t const *in_gaps,
                                                   int32_t &score, int32_t &gap, uint32_t &symbol) const {
         symbol = j <= m ? (uint32_t)params->byte_to_dense[columns[j - 1]] : 0u;
+        // Beyond column m the row above repeats column m's values: near every lane's base, and they only feed
+        // cells that are discarded.
+        int32_t const at = min(j, m);
         if (in_scores) {
-            score = j <= m ? in_scores[j] : 0;
-            gap = (affine_k && j <= m) ? in_gaps[j] : 0;
+            score = in_scores[at];
+            gap = affine_k ? in_gaps[at] : 0;
         }
         else {
-            int32_t const border = border_score<family_>(j, open, extend);
-            score = border + domain(0, j);
-            gap = (open + extend) + border + gap_domain(0, j); // discard value of the vertical track on row 0
+            int32_t const border = border_score<family_>(at, open, extend);
+            score = border + domain(0, at);
+            gap = (open + extend) + border + gap_domain(0, at); // discard value of the vertical track on row 0
         }
     }
 
@@ -216,19 +234,18 @@ struct packed_sweep_t {
             uint32_t const cost = __builtin_amdgcn_perm(costs_low[(P + p) / 4], costs_now[p / 4], selector);
             uint32_t const left = scores[p];
             uint32_t const if_substitution = pk_add_u16(diag, cost);
-            // The maxima over the terms that do not depend on the row above come first: the chain from one row
-            // to the next is then 3 dependent packed ops (affine) or 1 (linear) instead of 4 or 2, which lets the
-            // scheduler interleave the rows instead of padding back-to-back dependent packed ops with wait states.
+            // The chain from one row to the next is v̂, the three-way maximum and + (o - e) (affine) or the maximum
+            // alone (linear), so the scheduler interleaves the rows instead of padding back-to-back dependent
+            // packed ops with wait states.
             uint32_t stored;
             if constexpr (affine_k) {
                 uint32_t const h = pk_max_u16(left, horizontal[p]);
-                uint32_t const off_chain = pk_max_u16(h, if_substitution);
                 uint32_t const v = pk_max_u16(up, vert);
                 horizontal[p] = h;
                 vert = v;
-                stored = pk_add_u16(pk_max_u16(off_chain, v), kappa);
+                stored = pk_add_u16(pk_max3_ordered(h, if_substitution, v), kappa);
             }
-            else stored = pk_max_u16(up, pk_max_u16(left, if_substitution));
+            else stored = pk_max3_ordered(up, left, if_substitution);
             diag = left;
             up = stored;
             scores[p] = stored;
@@ -253,20 +270,24 @@ struct packed_sweep_t {
     __device__ __forceinline__ void run_chunk(int32_t chunk, int32_t const *in_scores, int32_t const *in_gaps,
                                               int32_t *out_scores, int32_t *out_gaps) {
         int32_t const first_step = chunk * 64;
-        // The chunk's base: the row above the stripe at its first column (or the last one), a cell every value
-        // the wave holds during the chunk is close to.
-        int32_t const base_column = min(first_step + 1, m);
-        int32_t base_score, base_gap;
-        uint32_t base_symbol;
-        column_inputs(base_column, in_scores, in_gaps, base_score, base_gap, base_symbol);
-        rebase(__builtin_amdgcn_readfirstlane(base_score));
+        // Each lane's base for the chunk: its first row at its current column, a cell every value the lane holds
+        // during the chunk is close to. A lane past column m keeps its base (it holds no kept cell).
+        int32_t const current = first_step - 2 * lane;
+        rebase(current <= m ? from_u16(scores[0]) : base);
+        // The lane above's values arrive in its base: one packed add of (base above - base, 0) moves them into
+        // this lane's (lane 0 reads the ring, staged in its own base).
+        int32_t const base_above = shift_down<64>(base, base);
+        uint32_t const into_mine = (uint32_t)(base_above - base) & 0xFFFFu;
+        int32_t const ring_base = __builtin_amdgcn_readfirstlane(base);
 
         int32_t const j_lane = first_step + lane + 1;
         int32_t chunk_score, chunk_gap;
         uint32_t chunk_symbol;
         column_inputs(j_lane, in_scores, in_gaps, chunk_score, chunk_gap, chunk_symbol);
-        *(int4 *)(ring + lane * 16) = make_int4((int32_t)(to_u16(chunk_score) << 16),
-                                                (int32_t)(to_u16(chunk_gap) << 16), (int32_t)chunk_symbol, 0);
+        *(int4 *)(ring + lane * 16) =
+            make_int4((int32_t)(((uint32_t)(chunk_score - ring_base + packed_centre_k) & 0xFFFFu) << 16),
+                      (int32_t)(((uint32_t)(chunk_gap - ring_base + packed_centre_k) & 0xFFFFu) << 16),
+                      (int32_t)chunk_symbol, 0);
         int4 inject = *(int4 const *)ring;
         // Lane 0's column symbol comes from the ring; the other lanes fetched theirs during the previous step.
         if (lane == 0) symbol_now = (uint32_t)inject.z;
@@ -280,11 +301,11 @@ struct packed_sweep_t {
             // Packed top inputs: the lane above's bottom row (its high half) low, this lane's upper-half bottom row
             // from the previous step high.
             uint32_t const from_above = (uint32_t)shift_down<64>(inject.x, (int32_t)scores[P - 1]);
-            uint32_t const up = __builtin_amdgcn_alignbit(scores[P - 1], from_above, 16);
+            uint32_t const up = pk_add_u16(__builtin_amdgcn_alignbit(scores[P - 1], from_above, 16), into_mine);
             uint32_t vertical_in = 0;
             if constexpr (affine_k) {
                 uint32_t const gaps_above = (uint32_t)shift_down<64>(inject.y, (int32_t)vertical);
-                vertical_in = __builtin_amdgcn_alignbit(vertical, gaps_above, 16);
+                vertical_in = pk_add_u16(__builtin_amdgcn_alignbit(vertical, gaps_above, 16), into_mine);
             }
             // Next step's upper column is the lane above's current lower column: fetch its class and costs now.
             uint32_t const symbol_next = (uint32_t)shift_down<64>(next.z, (int32_t)symbol_low);
@@ -318,13 +339,14 @@ struct packed_sweep_t {
                 *(uint32_t *)(ring + step * 16 + 12) =
                     affine_k ? __builtin_amdgcn_perm(vertical, scores[P - 1], 0x07060302u) : scores[P - 1] >> 16;
         }
-        // Entry l holds step l's bottom row: column first_step + l - 126.
+        // Entry l holds step l's bottom row (lane 63's, in its base): column first_step + l - 126.
         if constexpr (collect_) {
             int32_t const j_out = first_step + lane - 126;
             uint32_t const bottom = *(uint32_t const *)(ring + lane * 16 + 12);
+            int32_t const bottom_base = __builtin_amdgcn_readlane(base, 63) - packed_centre_k;
             if (j_out >= 1 && j_out <= m) {
-                out_scores[j_out] = from_u16(bottom);
-                if constexpr (affine_k) out_gaps[j_out] = from_u16(bottom >> 16);
+                out_scores[j_out] = (int32_t)(bottom & 0xFFFFu) + bottom_base;
+                if constexpr (affine_k) out_gaps[j_out] = (int32_t)(bottom >> 16) + bottom_base;
             }
         }
     }

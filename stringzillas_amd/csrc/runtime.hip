@@ -226,9 +226,10 @@ static sz_status_t densify(engine_t *engine, uint8_t const *byte_to_class, int8_
 }
 
 /**
- *  The packed 16-bit sweep (packed_sweep.hpp) stores cells as u16 around a wave-uniform base re-chosen every 64
- *  columns, so it needs every value a wave holds during a chunk within 32767 of that base. For a maximizing global
- *  family with o, e <= 0 (the Gotoh recurrences of serial.hpp:854-964; linear is o = e):
+ *  The packed 16-bit sweep (packed_sweep.hpp) stores cells as u16 around a base per lane, re-chosen every 64
+ *  columns, in [0, 0x7BFF] (finite non-negative halves, compared by a float maximum), so it needs every value a
+ *  lane holds during a chunk within 15871 of that base. For a maximizing global family with o, e <= 0 (the Gotoh
+ *  recurrences of serial.hpp:854-964; linear is o = e):
  *  - C(i,j) - C(i-1,j) >= o, since V(i,j) >= C(i-1,j) + o;
  *  - C(i,j) - C(i-1,j) <= Dp = max(0, s_max - min(o, e)) + max(0, o - e): the diagonal term gains at most
  *    s_max - min(o, e) over C(i-1,j) (which is at least C(i-1,j-1) + min(o, e)), V(i,j) <= C(i-1,j), and H(i,j)
@@ -237,9 +238,10 @@ static sz_status_t densify(engine_t *engine, uint8_t const *byte_to_class, int8_
  *    delta = max(|o|, Dp) + |e|;
  *  - gap tracks stay within max(|o|, Dp) + |e| + |o| of their cell, and every candidate sum within
  *    2 delta + |s_min| + |o| + |e| + |o - e|.
- *  The cells held during a chunk (stripe rows plus the row above, columns from lane 63's lower half to the end of
- *  the injection ring) lie within a Manhattan distance of 64R + 193 of each other; 64R + 320 is used. Profile bytes
- *  `cost - (o + e)` must also lie in [0, 127] to read as unsigned bytes.
+ *  The base is the lane's first row at its current column when the chunk starts; the cells the lane holds during
+ *  the chunk (its R rows and the row above, from its lower half's column to 64 columns on) lie within a Manhattan
+ *  distance of R + 63 of it; R + 70 is used. Profile bytes `cost - (o + e)` must also lie in [0, 127] to read as
+ *  unsigned bytes.
  */
 int packed_rows_limit(engine_params_t const &params) {
     if (params.family != family_nw_linear_k && params.family != family_nw_affine_k) return 0;
@@ -257,7 +259,7 @@ int packed_rows_limit(engine_params_t const &params) {
     int64_t const delta = step + (-e);
     int64_t const slack = 2 * delta + std::abs(s_min) + (-o) + (-e) + std::abs(o - e) + 64;
     for (int rows_per_lane = 8; rows_per_lane >= 2; rows_per_lane -= 2)
-        if (delta * (64 * rows_per_lane + 320) + slack <= 32767) return rows_per_lane;
+        if (delta * (rows_per_lane + 70) + slack <= 15871) return rows_per_lane;
     return 0;
 }
 

@@ -228,20 +228,26 @@ def test_packed_rows_follow_the_range_bound():
     """Rows per lane of the packed 16-bit sweep an engine admits (no GPU needed: engines are host objects).
 
     Worked values for the seeded protein matrix (costs in [-4, 11]): adjacent cells of the anti-diagonal domain
-    differ by at most delta = max(|o|, s_max - min(o, e) + max(0, o - e)) + |e|, and 8 rows per lane need
-    delta * 832 plus a small slack to stay below 32768.
+    differ by at most delta = max(|o|, s_max - min(o, e) + max(0, o - e)) + |e|; R rows per lane need
+    delta * (R + 70) plus a slack of 2 delta + |s_min| + |o| + |e| + |o - e| + 64 to stay within 15871 of the
+    lane's base.
     """
     from stringzillas_amd import _native
     from tests.helpers import dna_tables, protein_tables
 
     b2c, costs = protein_tables(np.random.default_rng(3))
     rows = lambda engine: _native.lib.szs_tuning_packed_rows(engine.handle)  # noqa: E731
-    assert rows(szs.NeedlemanWunschScores(b2c, costs, -11, -1)) == 8  # delta = 23: 23 * 832 = 19,136
+    assert rows(szs.NeedlemanWunschScores(b2c, costs, -11, -1)) == 8  # delta = 23: 23 * 78 = 1,794
     assert rows(szs.NeedlemanWunschScores(b2c, costs, -2, -2)) == 8  # linear, delta = 15
-    assert rows(szs.NeedlemanWunschScores(b2c, costs, -40, -1)) == 4  # delta = 52: only 52 * 576 fits
+    assert rows(szs.NeedlemanWunschScores(b2c, costs, -40, -1)) == 8  # delta = 52
     big = costs.copy()
     np.fill_diagonal(big, 100)
-    assert rows(szs.NeedlemanWunschScores(b2c, big, -11, -1)) == 0  # delta = 112: no packed shape fits
+    assert rows(szs.NeedlemanWunschScores(b2c, big, -11, -1)) == 8  # delta = 112: 112 * 78 + 314 = 9,050
+    np.fill_diagonal(big, 120)
+    assert rows(szs.NeedlemanWunschScores(b2c, big, -11, -1)) == 0  # 120 - (-12) = 132: not an unsigned byte
+    # o = -128, e = -72 with every cost 85 lower (in [-89, -74], cost - (o + e) in [111, 126]): delta = 200, slack
+    # 809; 4 rows: 200 * 74 + 809 = 15,609 fits, 6 rows: 200 * 76 + 809 = 16,009 does not.
+    assert rows(szs.NeedlemanWunschScores(b2c, (costs.astype(np.int64) - 85).astype(np.int8), -128, -72)) == 4
     assert rows(szs.NeedlemanWunschScores(b2c, costs, 3, 1)) == 0  # odd signs: the bound does not apply
     assert rows(szs.SmithWatermanScores(b2c, costs, -11, -1)) == 0  # local families stay int32
     assert rows(szs.LevenshteinDistances()) == 0

@@ -47,11 +47,18 @@ def _engine(name, rng):
     elif name == "nw_asymmetric":
         b2c, costs = protein_tables(rng, asymmetric=True)
         open_, extend = -5, -1
-    elif name == "nw_far_gap":  # admits 4 rows per lane only
+    elif name == "nw_wide_gap":  # delta = 52
         open_, extend = -40, -1
-    elif name == "nw_big_diagonal":  # admits no packed shape: everything runs int32
+    elif name == "nw_big_diagonal":  # delta = 112
         costs = costs.copy()
         np.fill_diagonal(costs, 100)
+        open_, extend = -11, -1
+    elif name == "nw_far_gap":  # delta = 200: admits 4 rows per lane only (its 6- and 8-row shapes run int32)
+        costs = (costs.astype(np.int64) - 85).astype(np.int8)
+        open_, extend = -128, -72
+    elif name == "nw_over_byte":  # 120 - (o + e) does not fit a profile byte: everything runs int32
+        costs = costs.copy()
+        np.fill_diagonal(costs, 120)
         open_, extend = -11, -1
     else:
         raise KeyError(name)
@@ -60,7 +67,8 @@ def _engine(name, rng):
     return engine, params
 
 
-EXPECTED_ROWS = {"nw_affine": 8, "nw_linear": 8, "nw_asymmetric": 8, "nw_far_gap": 4, "nw_big_diagonal": 0}
+EXPECTED_ROWS = {"nw_affine": 8, "nw_linear": 8, "nw_asymmetric": 8, "nw_wide_gap": 8, "nw_big_diagonal": 8,
+                 "nw_far_gap": 4, "nw_over_byte": 0}
 
 
 @pytest.fixture
@@ -113,7 +121,7 @@ def test_packed_shapes_match_oracle(name, ladder, packed_ladders):
 
 
 @pytest.mark.parametrize("ladder", ["wave8", "cooperative"])
-@pytest.mark.parametrize("name", ["nw_affine", "nw_linear", "nw_far_gap"])
+@pytest.mark.parametrize("name", ["nw_affine", "nw_linear", "nw_big_diagonal", "nw_far_gap"])
 def test_packed_shapes_keep_exact_scores_far_beyond_16_bits(name, ladder, packed_ladders):
     """Identical and lightly mutated long strings: scores grow by ~10 per row, far past any fixed 16-bit window."""
     rng = np.random.default_rng(zlib.crc32(f"drift:{name}:{ladder}".encode()))
@@ -126,7 +134,7 @@ def test_packed_shapes_keep_exact_scores_far_beyond_16_bits(name, ladder, packed
         b += [text, _mutated(rng, text, 0.05, 0.01), _mutated(rng, text, 0.3, 0.05)]
     _check(engine, params, a, b, f"{name} {ladder} long similar pairs")
     expected = O.pairs(params, O.tape(a[6:7]), O.tape(b[6:7]), threads=1)
-    assert int(expected[0]) > 32767  # a 7,000-residue self-alignment scores far beyond int16
+    assert abs(int(expected[0])) > 32767  # a 7,000-residue self-alignment scores far beyond int16
 
 
 # Rows whose last 512-row stripe keeps at most 256 rows, 257-384 or more: it runs 4, 6 or 8 rows per lane.
