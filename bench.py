@@ -284,7 +284,7 @@ def main():
             "ops_per_cell": round(ops_per_cell, 3),
             "kernel_ms": round(kernel_ms_mean, 3),
             "note": "fixed DP ops per cell (linear 5, affine 9); unit-cost Levenshtein over bytes and runes runs "
-                    "the bit-parallel tier (64 cells per word op) and affine NW runs packed 16-bit cells (about 4.7 "
+                    "the bit-parallel tier (64 cells per word op) and affine NW runs packed 16-bit cells (about 4.3 "
                     "VALU per cell), which is why the fraction exceeds 1; valu_issue_frac is the measured share of "
                     "the VALU issue rate the kernel uses",
             "valu_issue_frac": MEASURED_VALU_ISSUE.get(workload.name),
