@@ -123,15 +123,33 @@ static double bin_item_cost(uint32_t bin) {
     return std::pow(2.0, bucket / (double)bucket_steps_per_octave_k) * per_cell * chained;
 }
 
+/** Whole-workgroup items (chained, then cooperative) come before one-wave items, each part heaviest first. */
+static bool whole_workgroup_bin(uint32_t bin) {
+    return shapes_k[(bin / column_buckets_k) % shapes_count_k].cooperative;
+}
+
 static std::vector<uint32_t> const &bin_priority() {
     static std::vector<uint32_t> priority = [] {
         std::vector<uint32_t> order(bins_count_k);
         std::iota(order.begin(), order.end(), 0u);
-        std::stable_sort(order.begin(), order.end(),
-                         [](uint32_t a, uint32_t b) { return bin_item_cost(a) > bin_item_cost(b); });
+        std::stable_sort(order.begin(), order.end(), [](uint32_t a, uint32_t b) {
+            if (whole_workgroup_bin(a) != whole_workgroup_bin(b)) return whole_workgroup_bin(a);
+            return bin_item_cost(a) > bin_item_cost(b);
+        });
         return order;
     }();
     return priority;
+}
+
+/** Priority position of the first one-wave bin. */
+static uint32_t wave_bins_from() {
+    static uint32_t const position = [] {
+        std::vector<uint32_t> const &order = bin_priority();
+        uint32_t p = 0;
+        while (p < bins_count_k && whole_workgroup_bin(order[p])) ++p;
+        return p;
+    }();
+    return position;
 }
 
 #pragma endregion
@@ -252,6 +270,7 @@ static bins_view_t bins_view(batch_t &b) {
     bins.order = b.order.as<uint32_t>();
     bins.queue_head = b.queue_head.as<uint32_t>();
     bins.shape_of_task = b.shape_of_task.as<uint32_t>();
+    bins.wave_bins_from = wave_bins_from();
     return bins;
 }
 

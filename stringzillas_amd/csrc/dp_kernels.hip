@@ -99,11 +99,12 @@ __device__ void run_independent(item_args_t const &a) {
     constexpr int R = sweep_type_::rows_per_lane;
     constexpr uint32_t per_wave = 64 / G;
     uint32_t const wave = threadIdx.x / 64;
-    if (wave * per_wave >= a.count) return; // wave-uniform: no pairs left for this wave
-    uint32_t const slot = wave * per_wave + (uint32_t)__lane_id() / G;
+    uint32_t const first_slot = a.wave_item ? 0u : wave * per_wave; // this wave's pairs within the item
+    if (first_slot >= a.count) return;                              // wave-uniform: no pairs left for this wave
+    uint32_t const slot = first_slot + (uint32_t)__lane_id() / G;
     bool const active = slot < a.count;
     sweep_type_ sweep;
-    sweep.load(a, a.order[a.first + (active ? slot : wave * per_wave)], active, wave);
+    sweep.load(a, a.order[a.first + (active ? slot : first_slot)], active, wave);
     int32_t longest = sweep.m;
     if constexpr (G < 64) {
         for (int offset = G; offset < 64; offset <<= 1) longest = max(longest, __shfl_xor(longest, offset, 64));
@@ -146,7 +147,7 @@ __device__ void run_independent(item_args_t const &a) {
     int32_t tail_sum = 0;
     if constexpr (tail_rows<shorter_t>() > 0) {
         int32_t const *in_row = full_passes ? scores_row : nullptr;
-        uint32_t const task_index = a.order[a.first + wave];
+        uint32_t const task_index = a.order[a.first + first_slot];
         if (tail == 1) tail_sum = run_tail<shorter_t>(a, task_index, wave, tail_base, in_row, gaps_row);
         if constexpr (tail_rows<middle_t>() > 0) {
             if (tail == 2) tail_sum = run_tail<middle_t>(a, task_index, wave, tail_base, in_row, gaps_row);
@@ -395,14 +396,39 @@ struct persistent_args_t {
     chain_view_t chain;
 };
 
+/** Priority position p of work item `taken`: item_start[p] <= taken < item_start[p + 1]. */
+__device__ __forceinline__ uint32_t position_of_item(bins_view_t const &bins, uint32_t taken) {
+    uint32_t low = 0, high = bins_count_k;
+    while (high - low > 1) {
+        uint32_t const mid = (low + high) / 2;
+        if (bins.item_start[mid] <= taken) low = mid;
+        else high = mid;
+    }
+    return low;
+}
+
+/** Items far longer than the rest (the batch's tallest pairs, and the groups of chained ones) set its critical
+ *  path: their waves take precedence in the SIMD arbiter, and the other waves on those SIMDs fill the cycles they
+ *  leave. */
+__device__ __forceinline__ void set_item_priority(uint32_t bin) {
+    if ((int)(bin % column_buckets_k) >= critical_bucket_k ||
+        shapes_k[(bin / column_buckets_k) % shapes_count_k].chained)
+        __builtin_amdgcn_s_setprio(3);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 /**
- *  Persistent scoring kernel: each 4-wave workgroup pops work items from a device counter, heaviest bins first,
- *  until the queue (`item_start[bins]` items) is drained; every workgroup exits once the counter passes it.
+ *  Persistent scoring kernel: work items are popped from one device counter in priority order until the queue
+ *  (`item_start[bins]` items) is drained. The queue holds whole-workgroup items first (chained groups, then
+ *  cooperative pairs, heaviest first), taken by a workgroup at a time, then one-wave items (heaviest first): once
+ *  a workgroup draws past the whole-workgroup part, its waves each take items on their own, with no barrier, so
+ *  a wave never idles behind a sibling's longer item.
  */
 __global__ __launch_bounds__(256, 4) void persistent_score_kernel(persistent_args_t args) {
     __shared__ uint32_t taken_shared;
     __shared__ int32_t result_shared;
     uint32_t const total_items = args.bins.item_start[bins_count_k];
+    uint32_t const wave_items_from = args.bins.item_start[args.bins.wave_bins_from];
     item_args_t item;
     item.tasks = args.tasks;
     item.segments = args.segments;
@@ -415,42 +441,54 @@ __global__ __launch_bounds__(256, 4) void persistent_score_kernel(persistent_arg
     item.chain_progress = args.chain.progress;
     item.chain_pitch = args.chain.pitch;
     item.chain_timeout = args.chain.timeout;
-    for (;;) {
+    uint32_t taken;
+    for (;;) { // whole-workgroup items
         if (threadIdx.x == 0) taken_shared = atomicAdd(args.bins.queue_head, 1u);
         __syncthreads();
-        uint32_t const taken = taken_shared;
+        taken = taken_shared;
         __syncthreads();
-        if (taken >= total_items) break;
-        // Priority position p with item_start[p] <= taken < item_start[p + 1].
-        uint32_t low = 0, high = bins_count_k;
-        while (high - low > 1) {
-            uint32_t const mid = (low + high) / 2;
-            if (args.bins.item_start[mid] <= taken) low = mid;
-            else high = mid;
-        }
+        if (taken >= total_items) return;
+        if (taken >= wave_items_from) break;
+        uint32_t const low = position_of_item(args.bins, taken);
         uint32_t const bin = args.bins.priority[low];
         int const variant = (int)(bin / column_buckets_k);
-        // Items far longer than the rest (the batch's tallest pairs, and the groups of chained ones) set its
-        // critical path: their waves take precedence in the SIMD arbiter, and the other waves on those SIMDs fill
-        // the cycles they leave.
-        if ((int)(bin % column_buckets_k) >= critical_bucket_k || shapes_k[variant % shapes_count_k].chained)
-            __builtin_amdgcn_s_setprio(3);
-        else __builtin_amdgcn_s_setprio(0);
-        shape_t const shape = shapes_k[variant % shapes_count_k];
+        set_item_priority(bin);
         uint32_t const item_in_bin = taken - args.bins.item_start[low];
-        if (shape.chained) { // consecutive items are the groups of one pair, in order
+        if (shapes_k[variant % shapes_count_k].chained) { // consecutive items are the groups of one pair, in order
             item.first = args.bins.task_start[low] + item_in_bin / chain_groups_k;
-            item.count = 1;
             item.group = (int32_t)(item_in_bin % chain_groups_k);
         }
         else {
-            uint32_t const per_item = pairs_per_item(shape);
-            uint32_t const in_bin = args.bins.histogram[bin];
-            item.first = args.bins.task_start[low] + item_in_bin * per_item;
-            item.count = min(per_item, in_bin - item_in_bin * per_item);
+            item.first = args.bins.task_start[low] + item_in_bin;
             item.group = 0;
         }
+        item.count = 1;
+        item.wave_item = false;
         run_variant(variant, item);
+    }
+    // One-wave items: wave 0 sweeps the item the workgroup drew, every wave then draws its own.
+    uint32_t const wave = threadIdx.x / 64;
+    item.wave_item = true;
+    item.group = 0;
+    if (wave != 0) {
+        uint32_t drawn = 0;
+        if (__lane_id() == 0) drawn = atomicAdd(args.bins.queue_head, 1u);
+        taken = (uint32_t)__builtin_amdgcn_readfirstlane((int)drawn);
+    }
+    while (taken < total_items) {
+        uint32_t const low = position_of_item(args.bins, taken);
+        uint32_t const bin = args.bins.priority[low];
+        int const variant = (int)(bin / column_buckets_k);
+        set_item_priority(bin);
+        uint32_t const per_item = pairs_per_item(shapes_k[variant % shapes_count_k]);
+        uint32_t const item_in_bin = taken - args.bins.item_start[low];
+        uint32_t const in_bin = args.bins.histogram[bin];
+        item.first = args.bins.task_start[low] + item_in_bin * per_item;
+        item.count = min(per_item, in_bin - item_in_bin * per_item);
+        run_variant(variant, item);
+        uint32_t drawn = 0;
+        if (__lane_id() == 0) drawn = atomicAdd(args.bins.queue_head, 1u);
+        taken = (uint32_t)__builtin_amdgcn_readfirstlane((int)drawn);
     }
 }
 
@@ -466,6 +504,7 @@ __global__ __launch_bounds__(256, 4) void single_task_kernel(task_t const *tasks
     item.order = identity;
     item.first = task_index;
     item.count = 1;
+    item.wave_item = false;
     item.scratch = scratch;
     item.scratch_capacity = scratch_capacity;
     item.wave_lds_bytes = wave_lds_bytes;

@@ -105,6 +105,7 @@ struct item_args_t {
     uint32_t const *order; /**< task ids; `first .. first + count` belong to this item */
     uint32_t first;
     uint32_t count;
+    bool wave_item;        /**< the item is this wave's alone (else the workgroup's: wave w takes its w-th share) */
     int32_t *scratch;      /**< this workgroup's 4 x 2 x `scratch_capacity` int32 rows */
     uint32_t scratch_capacity;
     uint32_t wave_lds_bytes; /**< dynamic LDS per wave: profile or Myers tables, then the injection ring */

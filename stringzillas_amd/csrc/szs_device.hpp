@@ -172,8 +172,10 @@ constexpr int unit_runes_slot_k = families_count_k * 2 + 1;
 constexpr int shape_slots_k = families_count_k * 2 + 2;
 
 /** Pairs one work item holds: one per wave slot of a non-cooperative shape, one pair for a cooperative one. */
+/** Pairs of one work item: a cooperative item is one pair for the whole workgroup, any other item is the pairs of
+ *  one wave (64 / lanes per pair), which each wave of a workgroup takes and sweeps on its own. */
 __host__ __device__ inline uint32_t pairs_per_item(shape_t shape) {
-    return shape.cooperative ? 1u : (uint32_t)(waves_per_group_k * (64 / shape.group));
+    return shape.cooperative ? 1u : (uint32_t)(64 / shape.group);
 }
 
 /** Work items of `tasks` pairs of one shape: a chained pair is `chain_groups_k` items, one per group. */
@@ -208,6 +210,9 @@ struct bins_view_t {
     uint32_t *order;         /**< [tasks] task ids grouped by bin */
     uint32_t *queue_head;    /**< persistent-kernel work counter */
     uint32_t *shape_of_task; /**< [tasks] chosen shape id (kept for the per-pair mode) */
+    /** Priority position of the first bin of one-wave items: the bins before it (chained, then cooperative) hold
+     *  whole-workgroup items, so `item_start[wave_bins_from]` is where the waves start taking items alone. */
+    uint32_t wave_bins_from;
 };
 
 /** Shape choice per (family, symbols): single-pass shapes by ascending capacity, then the multi-pass shape. */
