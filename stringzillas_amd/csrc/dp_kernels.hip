@@ -4,8 +4,9 @@
  *         work items over the sweep of dp_sweep.hpp), the one-pair baseline kernel, UTF-8 transcoding, pair
  *         materialization and the device-side length binning.
  *
- *  Workgroups have 4 waves. A work item of a non-cooperative shape gives each wave 64/G pairs of the same bin;
- *  a cooperative item gives the whole workgroup one tall pair, whose stripes the waves sweep in a lock-step
+ *  Workgroups have 4 waves. A work item of a non-cooperative shape is one wave's 64/G pairs of one bin, which a
+ *  wave takes and sweeps on its own; a cooperative item gives the whole workgroup one tall pair, whose stripes the
+ *  waves sweep in a lock-step
  *  pipeline: stripe s runs on wave s mod 4 during phases [s*D, s*D + C) (C = 64-column chunks per stripe,
  *  D = max(skew + 1, ceil(C / 4)), skew = 1, or 2 for the packed 16-bit sweep), reading stripe s-1's bottom row
  *  from an HBM ring of 4 rows that stripe s-1 wrote at least one phase (one workgroup barrier) earlier.
