@@ -591,6 +591,9 @@ __device__ uint64_t decode_walk(uint8_t const *first, uint8_t const *last, uint3
 __device__ int64_t decode_wave(uint8_t const *first, uint8_t const *last, uint32_t *out) {
     constexpr int32_t lane_bytes_k = 16;
     constexpr int64_t pass_bytes_k = 64 * lane_bytes_k;
+    // One pass's runes (at most one per byte) per wave of the 256-thread decode workgroup.
+    __shared__ uint32_t pass_runes_lds[4][pass_bytes_k];
+    uint32_t *staging = pass_runes_lds[threadIdx.x / 64];
     uint32_t const lane = __lane_id();
     int64_t const length = last - first;
     int64_t written = 0;
@@ -653,7 +656,9 @@ __device__ int64_t decode_wave(uint8_t const *first, uint8_t const *last, uint32
             int32_t const other = __shfl_up(inclusive, d, 64);
             if ((int32_t)lane >= d) inclusive += other;
         }
-        uint32_t *to = out + written + inclusive - count;
+        // Runes go to their ranks in the wave's LDS staging (scattered LDS stores are cheap), then out to HBM in
+        // contiguous dword stores (one scattered global store per byte position would touch ~24 lines each).
+        uint32_t at = (uint32_t)(inclusive - count);
 #pragma unroll
         for (int32_t i = 0; i < lane_bytes_k; ++i) {
             if ((leads >> i) & 1u) {
@@ -663,11 +668,14 @@ __device__ int64_t decode_wave(uint8_t const *first, uint8_t const *last, uint32
                 };
                 uint32_t const lead = byte_at(i);
                 uint32_t const width = utf8_width(lead);
-                *to++ = rune_of(lead, width, width > 1 ? tail(1) : 0u, width > 2 ? tail(2) : 0u,
-                                width > 3 ? tail(3) : 0u);
+                staging[at++] = rune_of(lead, width, width > 1 ? tail(1) : 0u, width > 2 ? tail(2) : 0u,
+                                        width > 3 ? tail(3) : 0u);
             }
         }
-        written += __shfl(inclusive, 63, 64);
+        int32_t const pass_runes = __shfl(inclusive, 63, 64);
+        // The wave's LDS operations complete in order, so every lane's staging stores precede these loads.
+        for (int32_t t = (int32_t)lane; t < pass_runes; t += 64) out[written + t] = staging[t];
+        written += pass_runes;
         carry = (uint32_t)__shfl((int)w[3], 63, 64);
     }
     return written;
