@@ -51,10 +51,10 @@ struct tail_sweeps_t {
     using shorter = void;
     using middle = void;
 };
-template <int family_>
-struct tail_sweeps_t<packed_sweep_t<family_, 8>> { // 512-row stripes: 256- or 384-row tails
-    using shorter = packed_sweep_t<family_, 4>;
-    using middle = packed_sweep_t<family_, 6>;
+template <int family_, bool uniform_base_>
+struct tail_sweeps_t<packed_sweep_t<family_, 8, uniform_base_>> { // 512-row stripes: 256- or 384-row tails
+    using shorter = packed_sweep_t<family_, 4, uniform_base_>;
+    using middle = packed_sweep_t<family_, 6, uniform_base_>;
 };
 template <bool runes_>
 struct tail_sweeps_t<myers_sweep_t<64, runes_, 64>> { // 4096-row stripes: 2048-row tails of 32-row words
@@ -313,9 +313,19 @@ __device__ __forceinline__ void run_shape(item_args_t const &a) {
         // binning never picks them otherwise).
         if constexpr ((family_ == family_nw_linear_k || family_ == family_nw_affine_k) && sizeof(symbol_type_) == 1 &&
                       shifted_) {
-            if constexpr (shape.chained) run_cooperative<packed_sweep_t<family_, shape.rows_per_lane>, true>(a);
-            else if constexpr (shape.cooperative) run_cooperative<packed_sweep_t<family_, shape.rows_per_lane>>(a);
-            else run_independent<packed_sweep_t<family_, shape.rows_per_lane>>(a);
+            // One base for the whole wave when the pair's engine admits it at these rows (no per-step moves of
+            // values between lane bases), else one base per lane. Every pair of a packed item has one wave to
+            // itself or the whole workgroup, so the item's first pair decides.
+            engine_params_t const *params = a.segments[a.tasks[a.order[a.first]].segment].params;
+            constexpr int rows = shape.rows_per_lane;
+            if (params->packed_uniform_rows >= rows) {
+                if constexpr (shape.chained) run_cooperative<packed_sweep_t<family_, rows, true>, true>(a);
+                else if constexpr (shape.cooperative) run_cooperative<packed_sweep_t<family_, rows, true>>(a);
+                else run_independent<packed_sweep_t<family_, rows, true>>(a);
+            }
+            else if constexpr (shape.chained) run_cooperative<packed_sweep_t<family_, rows>, true>(a);
+            else if constexpr (shape.cooperative) run_cooperative<packed_sweep_t<family_, rows>>(a);
+            else run_independent<packed_sweep_t<family_, rows>>(a);
         }
     }
     // Class-cost families keep their LDS profile at <= 8 rows per lane; larger shapes are never binned for them.

@@ -67,7 +67,7 @@ __device__ __forceinline__ uint32_t pk_max3_ordered(uint32_t a, uint32_t b, uint
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_maximum(__builtin_elementwise_maximum(x, y), z));
 }
 
-template <int family_, int R>
+template <int family_, int R, bool uniform_base_ = false>
 struct packed_sweep_t {
     static_assert(family_ == family_nw_linear_k || family_ == family_nw_affine_k, "global class-cost families only");
     static_assert(R % 2 == 0 && R >= 2 && R <= 8, "two half-stripes of R/2 rows; profiles hold at most 8 rows");
@@ -270,14 +270,21 @@ struct packed_sweep_t {
     __device__ __forceinline__ void run_chunk(int32_t chunk, int32_t const *in_scores, int32_t const *in_gaps,
                                               int32_t *out_scores, int32_t *out_gaps) {
         int32_t const first_step = chunk * 64;
-        // Each lane's base for the chunk: its first row at its current column, a cell every value the lane holds
-        // during the chunk is close to. A lane past column m keeps its base (it holds no kept cell).
-        int32_t const current = first_step - 2 * lane;
-        rebase(current <= m ? from_u16(scores[0]) : base);
-        // The lane above's values arrive in its base: one packed add of (base above - base, 0) moves them into
-        // this lane's (lane 0 reads the ring, staged in its own base).
-        int32_t const base_above = shift_down<64>(base, base);
-        uint32_t const into_mine = (uint32_t)(base_above - base) & 0xFFFFu;
+        uint32_t into_mine = 0;
+        if constexpr (uniform_base_) {
+            // One base for the wave: lane 32's first row at its current column, the middle of what the wave holds.
+            rebase(__builtin_amdgcn_readlane(from_u16(scores[0]), 32));
+        }
+        else {
+            // Each lane's base for the chunk: its first row at its current column, a cell every value the lane
+            // holds during the chunk is close to. A lane past column m keeps its base (it holds no kept cell).
+            int32_t const current = first_step - 2 * lane;
+            rebase(current <= m ? from_u16(scores[0]) : base);
+            // The lane above's values arrive in its base: one packed add of (base above - base, 0) moves them into
+            // this lane's (lane 0 reads the ring, staged in its own base).
+            int32_t const base_above = shift_down<64>(base, base);
+            into_mine = (uint32_t)(base_above - base) & 0xFFFFu;
+        }
         int32_t const ring_base = __builtin_amdgcn_readfirstlane(base);
 
         int32_t const j_lane = first_step + lane + 1;
@@ -301,11 +308,13 @@ struct packed_sweep_t {
             // Packed top inputs: the lane above's bottom row (its high half) low, this lane's upper-half bottom row
             // from the previous step high.
             uint32_t const from_above = (uint32_t)shift_down<64>(inject.x, (int32_t)scores[P - 1]);
-            uint32_t const up = pk_add_u16(__builtin_amdgcn_alignbit(scores[P - 1], from_above, 16), into_mine);
+            uint32_t up = __builtin_amdgcn_alignbit(scores[P - 1], from_above, 16);
+            if constexpr (!uniform_base_) up = pk_add_u16(up, into_mine);
             uint32_t vertical_in = 0;
             if constexpr (affine_k) {
                 uint32_t const gaps_above = (uint32_t)shift_down<64>(inject.y, (int32_t)vertical);
-                vertical_in = pk_add_u16(__builtin_amdgcn_alignbit(vertical, gaps_above, 16), into_mine);
+                vertical_in = __builtin_amdgcn_alignbit(vertical, gaps_above, 16);
+                if constexpr (!uniform_base_) vertical_in = pk_add_u16(vertical_in, into_mine);
             }
             // Next step's upper column is the lane above's current lower column: fetch its class and costs now.
             uint32_t const symbol_next = (uint32_t)shift_down<64>(next.z, (int32_t)symbol_low);

@@ -263,6 +263,29 @@ int packed_rows_limit(engine_params_t const &params) {
     return 0;
 }
 
+/**
+ *  The same bound for one base per wave, taken at lane 32 (the middle of the stripe) when a chunk starts: the
+ *  cells any lane holds during the chunk lie within 32R + 1 rows and 129 columns of that cell (lanes run two
+ *  columns apart, plus the chunk's 64 steps), so R rows per lane need delta * (32R + 140) + slack <= 15871. The
+ *  sweep then moves no value between lane bases.
+ */
+int packed_uniform_rows_limit(engine_params_t const &params, int packed_rows) {
+    if (packed_rows == 0) return 0;
+    int64_t const o = params.open, e = params.extend;
+    int64_t s_max = INT64_MIN, s_min = INT64_MAX;
+    for (int x = 0; x < params.dense_count; ++x)
+        for (int y = 0; y < params.dense_count; ++y) {
+            int64_t const cost = params.dense_costs[x * max_dense_classes_k + y];
+            s_max = std::max(s_max, cost), s_min = std::min(s_min, cost);
+        }
+    int64_t const rise = std::max<int64_t>(0, s_max - std::min(o, e)) + std::max<int64_t>(0, o - e);
+    int64_t const delta = std::max<int64_t>(-o, rise) + (-e);
+    int64_t const slack = 2 * delta + std::abs(s_min) + (-o) + (-e) + std::abs(o - e) + 64;
+    for (int rows_per_lane = packed_rows; rows_per_lane >= 2; rows_per_lane -= 2)
+        if (delta * (32 * rows_per_lane + 140) + slack <= 15871) return rows_per_lane;
+    return 0;
+}
+
 sz_status_t make_engine(int kind, int match, int mismatch, int open, int extend, uint8_t const *byte_to_class,
                         int8_t const *class_costs, void **handle, char const **error_message) {
     if (!handle) return fail(sz_unexpected_dimensions_k, error_message, "Engine output pointer is null");
@@ -311,6 +334,7 @@ sz_status_t make_engine(int kind, int match, int mismatch, int open, int extend,
             if (biased < -128 || biased > 127) engine->params.shift = 0;
         }
     engine->packed_rows = packed_rows_limit(engine->params);
+    engine->params.packed_uniform_rows = packed_uniform_rows_limit(engine->params, engine->packed_rows);
     *handle = engine;
     return ok(error_message);
 }

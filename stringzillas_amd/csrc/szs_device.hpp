@@ -45,6 +45,9 @@ struct engine_params_t {
      *  the anti-diagonal), `open` for local ones (scores kept as H + open). The LDS profile holds
      *  `dense_costs - shift` as int8; 0 when some used cost would not fit (the plain recurrences then run). */
     int32_t shift;
+    /** Most rows per lane (8, 6, 4, 2; 0 = none) for which the packed 16-bit sweep may hold one base for the whole
+     *  wave instead of one per lane (runtime.hip, `packed_uniform_rows_limit`). */
+    int32_t packed_uniform_rows;
     uint8_t byte_to_dense[256];
     int8_t dense_costs[max_dense_classes_k * max_dense_classes_k]; /**< [dense(query)][dense(candidate)] */
 };
