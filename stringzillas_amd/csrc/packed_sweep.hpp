@@ -302,7 +302,7 @@ struct packed_sweep_t {
         uint32_t const kappa = ((uint32_t)(open - extend) & 0xFFFFu) * 0x10001u;
         uint32_t const border_high = to_u16(border_cell()) << 16, border_gap_high = to_u16(border_gap()) << 16;
 
-#pragma unroll 4
+#pragma unroll 8
         for (int32_t step = 0; step < 64; ++step) {
             int4 const next = *(int4 const *)(ring + ((step + 1) & 63) * 16);
             // Packed top inputs: the lane above's bottom row (its high half) low, this lane's upper-half bottom row
