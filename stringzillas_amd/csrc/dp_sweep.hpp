@@ -349,7 +349,7 @@ struct sweep_t {
         if constexpr (G == 64) {
             *(int4 *)(ring + lane * 16) = make_int4(chunk_scores, chunk_gaps, (int32_t)chunk_symbols, 0);
             int4 inject = *(int4 const *)ring;
-#pragma unroll 4
+#pragma unroll 8
             for (int32_t step = 0; step < 64; ++step) {
                 int4 const next = *(int4 const *)(ring + ((step + 1) & 63) * 16); // prefetch the next entry
                 int32_t const up_score = shift_down<64>(inject.x, bottom_score);
@@ -380,7 +380,7 @@ struct sweep_t {
             }
         }
         else {
-#pragma unroll 4
+#pragma unroll 8
             for (int32_t step = 0; step < G; ++step) {
                 int32_t const up_score = shift_down<G>(chunk_scores, bottom_score);
                 int32_t up_gap = 0;

@@ -359,7 +359,7 @@ struct myers_sweep_t {
         uint32_t chunk_symbols = j_lane <= m ? code_of(columns[j_lane - 1]) : 0u;
         uint32_t chunk_codes = in_codes ? (j_lane <= m ? (uint32_t)in_codes[j_lane] : 0u) : 1u;
         bool const first_in_group = k == 0;
-#pragma unroll 2
+#pragma unroll 4
         for (int32_t step = 0; step < G; ++step) {
             uint32_t const hin = group_shift_down<G>(chunk_codes, bottom_code, first_in_group);
             column_code = group_shift_down<G>(chunk_symbols, column_code, first_in_group);
