@@ -366,10 +366,11 @@ __device__ void run_family(int shape, item_args_t const &a) {
     case 20: run_shape<family_, symbol_type_, 20, shifted_>(a); break;
     case 21: run_shape<family_, symbol_type_, 21, shifted_>(a); break;
     case 22: run_shape<family_, symbol_type_, 22, shifted_>(a); break;
+    case 23: run_shape<family_, symbol_type_, 23, shifted_>(a); break;
     default: break;
     }
 }
-static_assert(shapes_count_k == 23, "dispatch switch lists every shape");
+static_assert(shapes_count_k == 24, "dispatch switch lists every shape");
 
 __device__ void run_variant(int variant, item_args_t const &a) {
     int const shape = variant % shapes_count_k;

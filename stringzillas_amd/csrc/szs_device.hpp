@@ -109,7 +109,7 @@ constexpr int waves_per_group_k = 4;
  *  16-bit shapes for the global class-cost families of engines whose scores provably fit (else their int32 twin
  *  runs).
  */
-constexpr int shapes_count_k = 23;
+constexpr int shapes_count_k = 24;
 constexpr shape_t shapes_k[shapes_count_k] = {
     {16, 1, 0, 0, 0, 0}, {16, 2, 0, 0, 0, 0}, {16, 4, 0, 0, 0, 0},  // up to 16 / 32 / 64 rows, four pairs per wave
     {64, 2, 0, 0, 0, 0}, {64, 3, 0, 0, 0, 0}, {64, 4, 0, 0, 0, 0},  // up to 128 / 192 / 256 rows
@@ -121,6 +121,7 @@ constexpr shape_t shapes_k[shapes_count_k] = {
     {64, 8, 1, 0, 1, 0},                           // packed 16-bit, cooperative 512-row stripes
     {64, 8, 1, 0, 1, 1}, {64, 8, 1, 0, 0, 1},      // chained cooperative 512-row stripes, packed or int32
     {16, 32, 0, 1, 0, 0}, {64, 32, 0, 1, 0, 0},    // Myers over 32-row words: up to 512 rows, 2048-row stripes
+    {16, 8, 0, 0, 0, 0},                           // up to 128 rows, four pairs per wave
 };
 
 /** The int32 shape with the same geometry as a packed one (itself for every other shape). */

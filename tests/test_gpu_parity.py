@@ -212,3 +212,36 @@ def test_utf8_malformed_and_truncated_input(low, high):
         expected = O.pairs(params, (a.data, a.offsets), (b.data, b.offsets), threads=8)
         np.testing.assert_array_equal(engine.pairs(a, b), expected)
         np.testing.assert_array_equal(engine.pairs(szs.to_device(a), szs.to_device(b)), expected)
+
+
+@pytest.mark.parametrize("family", ["nw_affine", "lev_affine"])
+def test_sixteen_lane_eight_row_groups(family):
+    """Four pairs per wave on 16 lanes of 8 rows (up to 128 rows), forced through the tuning ladder."""
+    from stringzillas_amd import autotune
+
+    table = autotune.shapes()
+    shape = next(i for i, s in enumerate(table) if tuple(s) == (16, 8, 0, 0, 0, 0))
+    slot = autotune.slot_of(family)
+    saved = autotune.get_ladder(slot)
+    rng = np.random.default_rng(zlib.crc32(f"g16r8:{family}".encode()))
+    if family == "nw_affine":
+        b2c, costs = protein_tables(rng)
+        engine = szs.NeedlemanWunschScores(b2c, costs, -11, -1)
+        params = O.Params("needleman_wunsch", open=-11, extend=-1, byte_to_class=b2c, class_costs=costs)
+        alphabet = PROTEIN
+    else:
+        engine = szs.LevenshteinDistances(0, 4, 3, 2)
+        params = O.Params("levenshtein", 0, 4, 3, 2)
+        alphabet = DNA
+    edges = [1, 2, 15, 16, 17, 63, 64, 65, 127, 128]
+    a = [random_bytes(rng, alphabet, n, n) for n in edges for _ in range(3)]
+    b = [random_bytes(rng, alphabet, 1, 3000) for _ in edges for _ in range(3)]
+    a += [random_bytes(rng, alphabet, 1, 128) for _ in range(60)]
+    b += [random_bytes(rng, alphabet, 1, 2000) for _ in range(60)]
+    try:
+        autotune.set_ladder(slot, [(shape, 128)], saved[1])
+        expected = O.pairs(params, O.tape(a), O.tape(b), threads=16)
+        got = engine.pairs(szs.Strs(a), szs.Strs(b))
+        np.testing.assert_array_equal(got.astype(np.int64), expected)
+    finally:
+        autotune.set_ladder(slot, *saved)
