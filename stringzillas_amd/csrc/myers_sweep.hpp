@@ -328,7 +328,7 @@ struct myers_sweep_t {
         uint32_t code = (uint32_t)shift_down<64>((int32_t)now.y, (int32_t)column_code);
         word_t eq_next = 0;
         if constexpr (table_) eq_next = eq_table[code * 64 + lane];
-#pragma unroll 4
+#pragma unroll 8
         for (int32_t step = 0; step < 64; ++step) {
             uint2 const later = *(uint2 const *)(ring + ((step + 2) & 63) * 8);
             uint32_t const hin = (uint32_t)shift_down<64>((int32_t)now.x, (int32_t)bottom_code);
