@@ -24,10 +24,10 @@ INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # CUs x lanes x clock: 39.3 T la
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the scoring kernel on the full mixed batch: FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
 # (factors from a plain 1 GiB device copy), recorded in profiles/persistent_score_kernel_mixed_1M_pairs.md.
-MEASURED_TRAFFIC_BYTES = {"mixed_1000000_pairs_log_uniform_16_16384_plus_32x65536": 63.09e9}
+MEASURED_TRAFFIC_BYTES = {"mixed_1000000_pairs_log_uniform_16_16384_plus_32x65536": 62.27e9}
 # VALU issue rate of the same launch from SQ_INSTS_VALU (same file): wave64 instructions per second over the
 # gfx950 peak of one per cycle per CU.
-MEASURED_VALU_ISSUE = {"mixed_1000000_pairs_log_uniform_16_16384_plus_32x65536": 0.952}
+MEASURED_VALU_ISSUE = {"mixed_1000000_pairs_log_uniform_16_16384_plus_32x65536": 0.953}
 
 
 def parse():
