@@ -106,7 +106,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}): the pairs are independent, so a "
-              "multi-GPU run is a sharded copy of this one; see DESIGN.md for the sharding plan.", file=sys.stderr)
+              "multi-GPU run is a sharded copy of this one (pairs dealt to GPUs by estimated cost, each GPU writing its "
+              "own slice of the results, no collective beyond a final gather).", file=sys.stderr)
         sys.exit(2)
     import torch
 
