@@ -277,6 +277,31 @@ def _int8(value, name: str) -> int:
     return value
 
 
+def _pair_output(out, count: int, dtype) -> int:
+    """Checks a caller's pair-list output and returns its address: the library writes ``count`` 8-byte results
+    there, so it must be a 1-D, C-contiguous, writable array (or device tensor) of the engine's dtype holding at
+    least ``count`` elements."""
+    if hasattr(out, "data_ptr"):  # a torch tensor, typically in HBM
+        if out.dim() != 1 or out.shape[0] < count:
+            raise ValueError(f"out must be a 1-D tensor of at least {count} elements")
+        if out.element_size() != 8 or out.is_floating_point() or out.is_complex():
+            raise TypeError(f"out must hold 64-bit integers ({np.dtype(dtype).name})")
+        if not out.is_contiguous():
+            raise ValueError("out must be contiguous")
+        return out.data_ptr()
+    if not isinstance(out, np.ndarray):
+        raise TypeError("out must be a NumPy array or a device tensor")
+    if out.ndim != 1 or out.shape[0] < count:
+        raise ValueError(f"out must be a 1-D array of at least {count} elements")
+    if out.dtype != np.dtype(dtype):
+        raise TypeError(f"out array must have {np.dtype(dtype).name} dtype")
+    if not out.flags.c_contiguous:
+        raise ValueError("out array must be C-contiguous")
+    if not out.flags.writeable:
+        raise ValueError("out array must be writable")
+    return out.ctypes.data
+
+
 class _Engine:
     _kind = 0
     _dtype = np.uint64
@@ -336,7 +361,7 @@ class _Engine:
         scope = _scope(device)
         if out is None:
             out = np.empty(len(firsts), dtype=self._dtype)
-        pointer = out.data_ptr() if hasattr(out, "data_ptr") else out.ctypes.data
+        pointer = _pair_output(out, len(firsts), self._dtype)
         a, b = firsts.tape(), seconds.tape()
         error = ctypes.c_char_p()
         status = getattr(_lib, f"{self._prefix}_pairs_u64tape")(
@@ -536,11 +561,13 @@ class Batch:
         self.outputs = []
         for i, (engine, firsts, seconds) in enumerate(segments):
             firsts, seconds = _as_tape(firsts, "firsts"), _as_tape(seconds, "seconds")
+            if len(firsts) != len(seconds):
+                raise ValueError("firsts and seconds must hold the same number of strings")
             if outputs is not None:
                 out = outputs[i]
             else:
                 out = np.zeros(len(firsts), dtype=engine._dtype)
-            pointer = out.data_ptr() if hasattr(out, "data_ptr") else out.ctypes.data
+            pointer = _pair_output(out, len(firsts), engine._dtype)
             a, b = firsts.tape(), seconds.tape()
             self._keep += [firsts, seconds, a, b, out]
             array[i].kind = engine._kind

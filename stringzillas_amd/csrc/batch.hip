@@ -194,7 +194,7 @@ static sz_status_t stage_tape(sz_cptr_t data, offset_type_ const *offsets, uint6
     }
     else {
         std::vector<uint64_t> wide(count + 1);
-        for (uint64_t i = 0; i <= count; ++i) wide[i] = (uint64_t)offsets[i];
+        for (uint64_t i = 0; i <= count; ++i) wide[i] = (uint64_t)offsets[i]; // checked by `check_host_offsets`
         SZS_TRY_HIP(tape.offsets_copy.reserve((count + 1) * 8), "Failed to allocate tape offsets");
         SZS_TRY_HIP(hipMemcpyAsync(tape.offsets_copy.data, wide.data(), (count + 1) * 8, hipMemcpyHostToDevice,
                                    stream), "tape offsets upload");
@@ -221,6 +221,23 @@ static sz_status_t stage_tape(sz_cptr_t data, offset_type_ const *offsets, uint6
         tape.rune_counts = tape.rune_counts_buffer.as<uint32_t>();
     }
     return ok(error_message);
+}
+
+template <typename offset_type_>
+static bool offsets_never_decrease(offset_type_ const *offsets, uint64_t count) {
+    for (uint64_t i = 0; i < count; ++i)
+        if (offsets[i + 1] < offsets[i]) return false;
+    return true;
+}
+
+sz_status_t check_host_offsets(segment_input_t const &input, char const **error_message) {
+    auto check = [&](auto const *tape) {
+        if (!tape || !tape->count || !tape->offsets || is_device_pointer(tape->offsets)) return true;
+        return offsets_never_decrease(tape->offsets, tape->count);
+    };
+    if (check(input.queries) && check(input.queries32) && check(input.candidates) && check(input.candidates32))
+        return ok(error_message);
+    return fail(sz_unexpected_dimensions_k, error_message, "Tape offsets must not decrease");
 }
 
 sz_status_t stage_bytes_tape(sz_sequence_u64tape_t const *input, staged_tape_t &tape, hipStream_t stream,
@@ -361,6 +378,12 @@ static sz_status_t enqueue_results(batch_t &b, char const **error_message) {
 
 sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> const &inputs, szs_batch_mode_t mode,
                           batch_t **out, char const **error_message, batch_options_t const &options) {
+    // Host offsets are checked before any device work: a decreasing offset would send the sweeps outside the tape
+    // (offsets in HBM are checked by the materialization kernel instead).
+    for (segment_input_t const &input : inputs) {
+        sz_status_t const checked = check_host_offsets(input, error_message);
+        if (checked != sz_success_k) return checked;
+    }
     int device = 0;
     hipStream_t stream = nullptr;
     sz_status_t status = resolve_gpu(scope, &device, &stream, error_message);
@@ -480,6 +503,8 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
         a.candidate_rune_counts = input.mode == segment_symmetric_k ? nullptr : candidates->rune_counts;
         a.queries_count = q;
         a.candidates_count = c;
+        a.query_bytes = queries->bytes;
+        a.candidate_bytes = candidates->bytes;
         a.results_row_stride = input.results_row_stride;
         a.tasks_count = tasks;
         a.task_base = (uint32_t)tasks_total;
@@ -595,7 +620,8 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
     SZS_TRY_HIP(hipStreamSynchronize(stream), "materialization");
     if (stats.overflow || stats.bad_tape) {
         if (!options.defer_overflow && stats.bad_tape)
-            return fail(sz_unexpected_dimensions_k, error_message, "A tape's offsets run past its stated byte count");
+            return fail(sz_unexpected_dimensions_k, error_message,
+                        "A tape's offsets decrease or run past its byte count");
         if (!options.defer_overflow)
             return fail(sz_overflow_risk_k, error_message, "Scores of this pair would overflow 32-bit cells");
         if (segments_count) {

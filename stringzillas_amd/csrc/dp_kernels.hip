@@ -720,7 +720,11 @@ __global__ void utf8_decode_kernel(utf8_tape_t const *tapes, uint32_t tapes_coun
         }
         utf8_tape_t const &tape = tapes[low];
         uint64_t const s = mine ? global - tape.string_base : 0;
-        uint64_t const begin = mine ? tape.offsets[s] : 0, end = mine ? tape.offsets[s + 1] : 0;
+        uint64_t const begin_read = mine ? tape.offsets[s] : 0, end_read = mine ? tape.offsets[s + 1] : 0;
+        // A string whose offsets decrease or run past the tape's bytes is decoded as empty (nothing outside the
+        // tape is read) and fails its segment.
+        bool const out_of_tape = end_read < begin_read || end_read > tape.capacity;
+        uint64_t const begin = out_of_tape ? 0 : begin_read, end = out_of_tape ? 0 : end_read;
         uint64_t const room = begin < tape.capacity ? tape.capacity - begin : 0;
         bool const is_long = mine && (int64_t)(end - begin) >= long_string_k && room >= end - begin;
         uint64_t written = 0;
@@ -743,8 +747,8 @@ __global__ void utf8_decode_kernel(utf8_tape_t const *tapes, uint32_t tapes_coun
                 written = decoded >= 0 ? (uint64_t)decoded : decode_walk(first, last, out, room);
         }
         if (mine) {
-            if (written > room) { // the offsets disagree with the byte count the buffer was sized for
-                written = room;
+            if (written > room || out_of_tape) { // the offsets disagree with the byte count the buffer was sized for
+                written = out_of_tape ? 0 : room;
                 atomicOr(&stats->bad_tape, 1u);
                 atomicOr(&segment_flags[tape.segment], (uint32_t)segment_bad_tape_k);
             }
@@ -812,9 +816,14 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
         uint64_t const *c_offsets = a.candidate_offsets ? a.candidate_offsets : a.query_offsets;
         uint32_t const *c_runes = a.candidate_offsets ? a.candidate_rune_counts : a.query_rune_counts;
         uint64_t const q_begin = a.query_offsets[q], c_begin = c_offsets[c];
-        uint64_t q_length = a.query_offsets[q + 1] - q_begin;
-        uint64_t c_length = c_offsets[c + 1] - c_begin;
-        if (a.runes) q_length = a.query_rune_counts[q], c_length = c_runes[c];
+        uint64_t const q_end = a.query_offsets[q + 1], c_end = c_offsets[c + 1];
+        uint64_t const c_bytes = a.candidate_offsets ? a.candidate_bytes : a.query_bytes;
+        // Offsets that decrease or run past the tape's data would make the sweeps read outside it: such a pair
+        // is never scored and its segment fails.
+        bool const bad_tape = q_end < q_begin || q_end > a.query_bytes || c_end < c_begin || c_end > c_bytes;
+        uint64_t q_length = bad_tape ? 0 : q_end - q_begin;
+        uint64_t c_length = bad_tape ? 0 : c_end - c_begin;
+        if (a.runes && !bad_tape) q_length = a.query_rune_counts[q], c_length = c_runes[c];
 
         task_t task;
         task.segment = a.segment;
@@ -839,7 +848,11 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
         // carry in their cells.
         uint64_t const carried = local ? 0 : q_length + c_length + 2;
         bool const overflows = (span + (affine ? 3 : 1) + carried) * (uint64_t)a.reach_magnitude >= 0x7FFFFFFFull;
-        if (q_length == 0 || c_length == 0) {
+        if (bad_tape) {
+            atomicOr(&stats->bad_tape, 1u);
+            atomicOr(&segment_overflow[a.segment], (uint32_t)segment_bad_tape_k);
+        }
+        else if (q_length == 0 || c_length == 0) {
             int64_t value = 0;
             if (!local)
                 value = affine ? (longest ? a.open + (int64_t)a.extend * ((int64_t)longest - 1) : 0)

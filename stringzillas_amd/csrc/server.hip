@@ -151,7 +151,7 @@ void server_t::retire(round_t &round, sz_status_t status, char const *error) {
             if (status != sz_success_k) ticket.status = status, ticket.error = error;
             else if (flags & segment_bad_tape_k)
                 ticket.status = sz_unexpected_dimensions_k,
-                ticket.error = "A tape's offsets run past its stated byte count";
+                ticket.error = "A tape's offsets decrease or run past its byte count";
             else ticket.status = sz_overflow_risk_k, ticket.error = "Scores of a pair would overflow 32-bit cells";
         }
         else ticket.state = ticket_done_k;
@@ -317,6 +317,10 @@ sz_status_t szs_server_submit(szs_server_t handle, szs_pairs_segment_t const *re
     request_t r;
     r.firsts = *request->firsts;
     r.seconds = *request->seconds;
+    r.input.queries = &r.firsts;
+    r.input.candidates = &r.seconds;
+    status = check_host_offsets(r.input, error_message); // rejected now rather than failing its round later
+    if (status != sz_success_k) return status;
     r.input.engine = engine;
     r.input.mode = segment_pairs_k;
     r.input.results = request->results;

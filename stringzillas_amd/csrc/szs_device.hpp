@@ -235,6 +235,8 @@ struct materialize_args_t {
     uint32_t const *candidate_rune_counts;
     uint64_t queries_count;
     uint64_t candidates_count;
+    uint64_t query_bytes;                /**< bytes of the query tape's data: every string must end within it */
+    uint64_t candidate_bytes;            /**< the same for the candidates */
     uint64_t results_row_stride;
     uint64_t tasks_count;                /**< pairs, or cells of the (triangular) cross-product */
     uint32_t task_base;
@@ -276,7 +278,7 @@ struct prepare_stats_t {
     unsigned long long cells;            /**< sum of n*m over non-empty tasks */
     uint32_t widest_multi_pass;          /**< most columns of a pair that needs several stripes */
     uint32_t overflow;                   /**< nonzero if some pair's scores could leave int32 */
-    uint32_t bad_tape;                   /**< nonzero if a UTF-8 tape's offsets run past its stated byte count */
+    uint32_t bad_tape;                   /**< nonzero if some tape's offsets decrease or run past its byte count */
     uint32_t chained_pairs;              /**< pairs scored by a chained shape: slots of the chain buffers */
     uint32_t chained_columns;            /**< most columns of such a pair */
     uint32_t chain_timeout;              /**< nonzero if a chained group gave up waiting for its predecessor */

@@ -194,6 +194,9 @@ struct batch_t {
     ~batch_t();
 };
 
+/** `sz_unexpected_dimensions_k` if a host-resident offsets array of the input decreases somewhere. */
+sz_status_t check_host_offsets(segment_input_t const &input, char const **error_message);
+
 /** Stages and bins a batch on the scope's device (or on `options.stream`). */
 sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> const &inputs, szs_batch_mode_t mode,
                           batch_t **out, char const **error_message, batch_options_t const &options = {});

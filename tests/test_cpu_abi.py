@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from tests.conftest import gpu_available
+from tests.helpers import dna_tables
 
 szs = pytest.importorskip("stringzillas_amd")
 from stringzillas_amd import _native  # noqa: E402
@@ -76,6 +77,52 @@ def test_out_argument_validation():
         engine(q, c, out=np.zeros((2, 1), dtype=np.int64))
     with pytest.raises(ValueError):
         engine(q, c, out=np.zeros((2, 4), dtype=np.uint64)[:, ::2])
+
+
+def test_pair_outputs_are_checked_before_the_library_writes():
+    engine = szs.LevenshteinDistances()
+    a, b = szs.Strs(["ab", "c", "d"]), szs.Strs(["a", "b", "e"])
+    with pytest.raises(TypeError):
+        engine.pairs(a, b, out=np.zeros(3, dtype=np.int32))  # 4-byte elements would overrun
+    with pytest.raises(TypeError):
+        engine.pairs(a, b, out=np.zeros(3, dtype=np.int64))  # Levenshtein results are uint64
+    with pytest.raises(ValueError):
+        engine.pairs(a, b, out=np.zeros(2, dtype=np.uint64))
+    with pytest.raises(ValueError):
+        engine.pairs(a, b, out=np.zeros((3, 1), dtype=np.uint64))
+    with pytest.raises(ValueError):
+        engine.pairs(a, b, out=np.zeros(6, dtype=np.uint64)[::2])
+    frozen = np.zeros(3, dtype=np.uint64)
+    frozen.flags.writeable = False
+    with pytest.raises(ValueError):
+        engine.pairs(a, b, out=frozen)
+    with pytest.raises(TypeError):
+        engine.pairs(a, b, out=[0, 0, 0])
+    with pytest.raises(TypeError):
+        szs.Batch([(engine, a, b)], outputs=[np.zeros(3, dtype=np.float64)])
+    with pytest.raises(ValueError):
+        szs.Batch([(engine, a, b)], outputs=[np.zeros(1, dtype=np.uint64)])
+    with pytest.raises(ValueError):
+        szs.Batch([(engine, a, szs.Strs(["x"]))])
+
+
+def test_decreasing_offsets_are_rejected_before_any_device_work():
+    # An intermediate offset past the last one would have the kernels read outside the tape.
+    data = np.frombuffer(b"ACGTACGTACGTACGTACGT", dtype=np.uint8)
+    bad = szs.Strs.from_tape(data, np.array([0, 15, 5], dtype=np.uint64))
+    good = szs.Strs(["ACG", "TT"])
+    nw = szs.NeedlemanWunschScores(*dna_tables(), -2, -1)
+    for engine in (szs.LevenshteinDistances(), szs.LevenshteinDistancesUTF8(), nw):
+        with pytest.raises(ValueError, match="decrease"):
+            engine.pairs(bad, good)
+        with pytest.raises(ValueError, match="decrease"):
+            engine.pairs(good, bad)
+        with pytest.raises(ValueError, match="decrease"):
+            engine(bad, good)
+        with pytest.raises(ValueError, match="decrease"):
+            engine(bad)  # symmetric
+        with pytest.raises(ValueError, match="decrease"):
+            szs.Batch([(engine, good, bad)])
 
 
 def test_device_scopes_without_scoring():
