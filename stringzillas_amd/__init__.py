@@ -21,7 +21,7 @@ from typing import Iterable, List, Optional, Sequence, Tuple, Union
 import numpy as np
 
 from . import _native
-from ._native import SzsError, check as _check, lib as _lib, U64Tape as _U64Tape
+from ._native import SzsError, check as _check, lib as _lib, U32Tape as _U32Tape, U64Tape as _U64Tape
 
 __version__ = "%d.%d.%d" % (_lib.szs_version_major(), _lib.szs_version_minor(), _lib.szs_version_patch())
 
@@ -52,10 +52,23 @@ def reset_capabilities(capabilities: Sequence[str]) -> None:
 #  Strings -----------------------------------------------------------------------------------------------------
 
 
+def _offsets_array(offsets) -> np.ndarray:
+    """Tape offsets as ``uint32`` (from 32-bit input) or ``uint64`` (anything wider), C-contiguous."""
+    offsets = np.ascontiguousarray(offsets)
+    if offsets.dtype in (np.int32, np.uint32):
+        return offsets.view(np.uint32)
+    if offsets.dtype in (np.int64, np.uint64):
+        return offsets.view(np.uint64)
+    return offsets.astype(np.uint64)
+
+
 class Strs:
-    """An Arrow-style tape of byte strings (``data`` + ``count + 1`` u64 offsets), the input type the engines take.
+    """An Arrow-style tape of byte strings (``data`` + ``count + 1`` offsets), the input type the engines take.
 
     Plays the role of ``stringzilla.Strs`` (python/stringzilla/strs.c), which the reference's engines require.
+    Offsets are ``uint64`` (built from a list, or an Arrow ``large_string``/``large_binary`` array) or ``uint32``
+    (32-bit offsets given to ``from_tape``, or an Arrow ``string``/``binary`` array): the engines then call the
+    ``*_u64tape`` or ``*_u32tape`` entry point, as python/stringzillas/similarities.c:243-270 does.
     """
 
     __slots__ = ("data", "offsets", "_keep")
@@ -74,26 +87,45 @@ class Strs:
 
     @classmethod
     def from_tape(cls, data: np.ndarray, offsets: np.ndarray) -> "Strs":
+        """A tape over existing buffers: 32-bit offsets stay 32-bit (a u32 tape), wider ones become u64."""
         strs = cls.__new__(cls)
         strs.data = np.ascontiguousarray(data, dtype=np.uint8)
-        strs.offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        strs.offsets = _offsets_array(offsets)
         strs._keep = None
         return strs
 
     @classmethod
     def from_arrow(cls, array) -> "Strs":
-        """Zero-copy view of a pyarrow ``string``/``large_string``/``binary`` array without nulls."""
+        """Zero-copy view of a pyarrow ``string``/``binary`` (u32 tape) or ``large_string``/``large_binary``
+        (u64 tape) array without nulls."""
         import pyarrow as pa
 
         if array.null_count:
             raise ValueError("Strs cannot hold null strings")
         if pa.types.is_string(array.type) or pa.types.is_binary(array.type):
-            array = array.cast(pa.large_binary())
+            width = np.int32
+        elif pa.types.is_large_string(array.type) or pa.types.is_large_binary(array.type):
+            width = np.int64
+        else:
+            raise TypeError(f"Strs views Arrow string or binary arrays, not {array.type}")
         buffers = array.buffers()
-        offsets = np.frombuffer(buffers[1], dtype=np.int64)[array.offset: array.offset + len(array) + 1]
+        offsets = np.frombuffer(buffers[1], dtype=width)[array.offset: array.offset + len(array) + 1]
         data = np.frombuffer(buffers[2], dtype=np.uint8) if buffers[2] is not None else np.zeros(1, np.uint8)
-        strs = cls.from_tape(data, offsets.astype(np.uint64))
+        strs = cls.from_tape(data, offsets)
         strs._keep = array
+        return strs
+
+    @property
+    def u32(self) -> bool:
+        """Whether the offsets are 32-bit (the engines then call their ``*_u32tape`` entry points)."""
+        return self.offsets.dtype == np.uint32
+
+    def as_u64(self) -> "Strs":
+        """The same strings with 64-bit offsets (the data is shared, not copied)."""
+        if not self.u32:
+            return self
+        strs = Strs.from_tape(self.data, self.offsets.astype(np.uint64))
+        strs._keep = self
         return strs
 
     def __len__(self) -> int:
@@ -108,22 +140,24 @@ class Strs:
         begin, end = int(self.offsets[index]), int(self.offsets[index + 1])
         return self.data[begin:end].tobytes()
 
-    def tape(self) -> _U64Tape:
-        return _U64Tape(self.data.ctypes.data, self.offsets.ctypes.data, len(self))
+    def tape(self):
+        """``sz_sequence_u32tape_t`` or ``sz_sequence_u64tape_t`` over these buffers."""
+        kind = _U32Tape if self.u32 else _U64Tape
+        return kind(self.data.ctypes.data, self.offsets.ctypes.data, len(self))
 
 
 class DeviceStrs:
     """A tape resident in HBM (torch device tensors), used in place by every call.
 
-    ``to_device`` (python/stringzillas/stringzillas.c:148-162) returns one of these.
+    ``to_device`` (python/stringzillas/stringzillas.c:148-162) returns one of these; 32-bit offsets stay 32-bit.
     """
 
     def __init__(self, strs: Strs, device: int = 0):
         import torch
 
         self.device = torch.device("cuda", device)
-        self._data = torch.from_numpy(np.ascontiguousarray(strs.data)).to(self.device)
-        self._offsets = torch.from_numpy(strs.offsets.astype(np.int64)).to(self.device)
+        self._data = torch.from_numpy(np.array(strs.data, dtype=np.uint8)).to(self.device)
+        self._offsets = torch.from_numpy(strs.offsets.astype(np.int32 if strs.u32 else np.int64)).to(self.device)
         self.count = len(strs)
         self.host = strs
 
@@ -131,11 +165,28 @@ class DeviceStrs:
         return self.count
 
     @property
+    def u32(self) -> bool:
+        return self.host.u32
+
+    @property
     def nbytes(self) -> int:
         return self.host.nbytes
 
-    def tape(self) -> _U64Tape:
-        return _U64Tape(self._data.data_ptr(), self._offsets.data_ptr(), self.count)
+    def as_u64(self) -> "DeviceStrs":
+        """The same strings with 64-bit offsets (widened in HBM; the data is shared)."""
+        if not self.u32:
+            return self
+        wide = DeviceStrs.__new__(DeviceStrs)
+        wide.device, wide._data, wide.count = self.device, self._data, self.count
+        import torch
+
+        wide._offsets = self._offsets.to(torch.int64)
+        wide.host = self.host.as_u64()
+        return wide
+
+    def tape(self):
+        kind = _U32Tape if self.u32 else _U64Tape
+        return kind(self._data.data_ptr(), self._offsets.data_ptr(), self.count)
 
 
 def to_device(strs: Strs, device: int = 0) -> DeviceStrs:
@@ -151,6 +202,7 @@ def save_tape(strs, path: str) -> None:
     if not isinstance(tape, Strs):
         raise TypeError("save_tape takes host strings (Strs, a pyarrow array or a list)")
     error = ctypes.c_char_p()
+    tape = tape.as_u64()  # held until the call returns: the view points into it
     view = tape.tape()
     _check(_lib.szs_tape_save(path.encode(), ctypes.byref(view), ctypes.byref(error)), error, "Saving a tape")
 
@@ -202,6 +254,15 @@ class DeviceTape:
 def load_tape(path: str, device: Optional[DeviceScope] = None) -> DeviceTape:
     """Streams a tape file into HBM without reading it whole into host memory."""
     return DeviceTape(path, device)
+
+
+def _is_u32(tape) -> bool:
+    return bool(getattr(tape, "u32", False))
+
+
+def _u64(tape):
+    """The pair-list, batch and serving entry points take u64 tapes: 32-bit offsets are widened for them."""
+    return tape.as_u64() if _is_u32(tape) else tape
 
 
 def _as_tape(obj, name: str):
@@ -343,10 +404,16 @@ class _Engine:
                 raise ValueError("out array rows must be contiguous (unit stride along columns)")
             results = out
         stride = results.strides[0] // 8 if rows else columns
+        narrow = _is_u32(queries)
+        if candidates is not None and _is_u32(candidates) != narrow:
+            if len(queries) and len(candidates):  # python/stringzillas/similarities.c:283-303
+                raise TypeError(f"Expected Strs objects of one offset width, got {type(queries).__name__} and "
+                                f"{type(candidates).__name__} with 32- and 64-bit offsets")
+            queries, candidates, narrow = _u64(queries), _u64(candidates), False
         q_tape = queries.tape()
         c_tape = candidates.tape() if candidates is not None else None
         error = ctypes.c_char_p()
-        status = getattr(_lib, f"{self._prefix}_u64tape")(
+        status = getattr(_lib, f"{self._prefix}_u32tape" if narrow else f"{self._prefix}_u64tape")(
             self.handle, scope.handle, ctypes.byref(q_tape), ctypes.byref(c_tape) if c_tape is not None else None,
             ctypes.c_void_p(results.ctypes.data), ctypes.c_size_t(stride), ctypes.byref(error))
         _check(status, error, type(self).__name__)
@@ -362,6 +429,7 @@ class _Engine:
         if out is None:
             out = np.empty(len(firsts), dtype=self._dtype)
         pointer = _pair_output(out, len(firsts), self._dtype)
+        firsts, seconds = _u64(firsts), _u64(seconds)  # held until the call returns: the tapes point into them
         a, b = firsts.tape(), seconds.tape()
         error = ctypes.c_char_p()
         status = getattr(_lib, f"{self._prefix}_pairs_u64tape")(
@@ -533,7 +601,7 @@ class Fingerprints:
         counts_pointer, counts_stride = self._output(counts, rows, self.ndim, "counts")
         tape = texts.tape()
         error = ctypes.c_char_p()
-        status = _lib.szs_fingerprints_u64tape(
+        status = (_lib.szs_fingerprints_u32tape if _is_u32(texts) else _lib.szs_fingerprints_u64tape)(
             self.handle, scope.handle, ctypes.byref(tape), ctypes.c_void_p(hashes_pointer),
             ctypes.c_size_t(max(hashes_stride, self.ndim * 4)), ctypes.c_void_p(counts_pointer),
             ctypes.c_size_t(max(counts_stride, self.ndim * 4)), ctypes.byref(error))
@@ -560,7 +628,7 @@ class Batch:
         array = (_native.Segment * max(len(segments), 1))()
         self.outputs = []
         for i, (engine, firsts, seconds) in enumerate(segments):
-            firsts, seconds = _as_tape(firsts, "firsts"), _as_tape(seconds, "seconds")
+            firsts, seconds = _u64(_as_tape(firsts, "firsts")), _u64(_as_tape(seconds, "seconds"))
             if len(firsts) != len(seconds):
                 raise ValueError("firsts and seconds must hold the same number of strings")
             if outputs is not None:
@@ -626,7 +694,7 @@ class Server:
         self._requests = {}
 
     def submit(self, engine: _Engine, firsts, seconds) -> int:
-        firsts, seconds = _as_tape(firsts, "firsts"), _as_tape(seconds, "seconds")
+        firsts, seconds = _u64(_as_tape(firsts, "firsts")), _u64(_as_tape(seconds, "seconds"))
         out = np.zeros(len(firsts), dtype=engine._dtype)
         a, b = firsts.tape(), seconds.tape()
         segment = _native.Segment()

@@ -38,6 +38,23 @@ class U64Tape(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("offsets", ctypes.c_void_p), ("count", ctypes.c_size_t)]
 
 
+class U32Tape(ctypes.Structure):
+    """``sz_sequence_u32tape_t`` (include/stringzillas/stringzillas.h:68-72)."""
+
+    _fields_ = [("data", ctypes.c_void_p), ("offsets", ctypes.c_void_p), ("count", ctypes.c_size_t)]
+
+
+SequenceStart = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64)
+SequenceLength = ctypes.CFUNCTYPE(ctypes.c_size_t, ctypes.c_void_p, ctypes.c_uint64)
+
+
+class Sequence(ctypes.Structure):
+    """``sz_sequence_t`` (include/stringzilla/types.h:1177-1182): strings behind two callbacks."""
+
+    _fields_ = [("handle", ctypes.c_void_p), ("count", ctypes.c_size_t), ("get_start", SequenceStart),
+                ("get_length", SequenceLength)]
+
+
 class Segment(ctypes.Structure):
     """``szs_pairs_segment_t`` (include/stringzillas_amd.h)."""
 

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -57,15 +58,19 @@ struct gathered_tape_t {
     }
 };
 
+/** One call on the scope's workspace: its device buffers persist between calls and only grow (the reference keeps
+ *  such scratch per engine, include/stringzillas/similarities/serial.hpp:3030-3040). */
 sz_status_t run_segments(szs_device_scope_t scope, std::vector<segment_input_t> const &inputs,
                          char const **error_message) {
+    device_scope_t *s = as_scope(scope);
     return guarded(error_message, [&]() {
-        batch_t *batch = nullptr;
-        sz_status_t status = batch_prepare(as_scope(scope), inputs, szs_batch_persistent_k, &batch, error_message);
-        if (status != sz_success_k) return status;
-        status = batch_launch(batch, error_message);
-        if (status == sz_success_k) status = batch_wait(batch, error_message);
-        delete batch;
+        std::lock_guard<std::mutex> lock(s->call_mutex);
+        if (!s->workspace) s->workspace = new batch_t();
+        batch_t &batch = *s->workspace;
+        sz_status_t status = batch_stage(batch, s, inputs, szs_batch_persistent_k, error_message);
+        if (status == sz_success_k) status = batch_launch(&batch, error_message);
+        if (status == sz_success_k) status = batch_wait(&batch, error_message);
+        if (status != sz_success_k) (void)hipStreamSynchronize(batch.stream); // nothing of this call is in flight
         return status;
     });
 }

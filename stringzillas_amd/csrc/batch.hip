@@ -268,13 +268,48 @@ static sz_status_t stage_input(segment_input_t const &input, bool queries, bool 
 
 #pragma endregion
 
-batch_t::~batch_t() {
+void batch_t::reset() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr, graph = nullptr;
+    for (auto *tape : tapes) {
+        tape->reset_view();
+        spare_tapes.push_back(tape);
+    }
+    tapes.clear();
+    for (auto *buffer : result_buffers) spare_results.push_back(buffer);
+    result_buffers.clear();
+    inputs.clear(), host_results.clear(), results_elements.clear(), result_rows.clear(), result_columns.clear();
+    device_results.clear(), materialize_args.clear(), overflowed.clear(), pinned_offsets.clear();
+    per_pair_tasks.clear(), per_pair_variants.clear();
+    chained_pairs = 0, chain_pitch = 0, decode_tapes_count = 0, decode_strings = 0;
+    pinned_results = nullptr, pinned_bytes = 0, tasks_count = 0, cells = 0;
+    scratch_capacity = 0, waves = 0, lds_bytes = 0, last_kernel_ms = 0;
+}
+
+batch_t::~batch_t() {
+    reset();
     if (kernel_begin) (void)hipEventDestroy(kernel_begin);
     if (kernel_end) (void)hipEventDestroy(kernel_end);
-    for (auto *tape : tapes) delete tape;
-    for (auto *buffer : result_buffers) delete buffer;
+    for (auto *tape : spare_tapes) delete tape;
+    for (auto *buffer : spare_results) delete buffer;
+}
+
+/** A staged tape of the batch: an earlier call's (buffers kept) when there is one, else a new one. */
+static staged_tape_t *next_tape(batch_t &b) {
+    staged_tape_t *tape = nullptr;
+    if (!b.spare_tapes.empty()) tape = b.spare_tapes.back(), b.spare_tapes.pop_back();
+    else tape = new (std::nothrow) staged_tape_t();
+    if (tape) b.tapes.push_back(tape);
+    return tape;
+}
+
+static device_buffer_t *next_result_buffer(batch_t &b) {
+    device_buffer_t *buffer = nullptr;
+    if (!b.spare_results.empty()) buffer = b.spare_results.back(), b.spare_results.pop_back();
+    else buffer = new (std::nothrow) device_buffer_t();
+    if (buffer) b.result_buffers.push_back(buffer);
+    return buffer;
 }
 
 static bins_view_t bins_view(batch_t &b) {
@@ -378,6 +413,20 @@ static sz_status_t enqueue_results(batch_t &b, char const **error_message) {
 
 sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> const &inputs, szs_batch_mode_t mode,
                           batch_t **out, char const **error_message, batch_options_t const &options) {
+    batch_t *batch = new (std::nothrow) batch_t();
+    if (!batch) return fail(sz_bad_alloc_k, error_message, "Failed to allocate a batch");
+    sz_status_t const status = batch_stage(*batch, scope, inputs, mode, error_message, options);
+    if (status != sz_success_k) {
+        delete batch;
+        return status;
+    }
+    *out = batch;
+    return ok(error_message);
+}
+
+sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_input_t> const &inputs,
+                        szs_batch_mode_t mode, char const **error_message, batch_options_t const &options) {
+    b.reset();
     // Host offsets are checked before any device work: a decreasing offset would send the sweeps outside the tape
     // (offsets in HBM are checked by the materialization kernel instead).
     for (segment_input_t const &input : inputs) {
@@ -390,13 +439,6 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
     if (status != sz_success_k) return status;
     if (options.stream) stream = options.stream;
     SZS_TRY_HIP(order_after_default_stream(stream), "Ordering after the default stream");
-    batch_t *batch = new (std::nothrow) batch_t();
-    if (!batch) return fail(sz_bad_alloc_k, error_message, "Failed to allocate a batch");
-    struct guard_t {
-        batch_t *&b;
-        ~guard_t() { delete b; }
-    } guard {batch};
-    batch_t &b = *batch;
     b.device = device;
     b.stream = stream;
     b.mode = mode;
@@ -438,17 +480,15 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
         segment_input_t const &input = inputs[s];
         engine_t const *engine = input.engine;
         bool const runes = engine->kind == szs_engine_levenshtein_utf8_k;
-        auto *queries = new (std::nothrow) staged_tape_t();
+        staged_tape_t *queries = next_tape(b);
         if (!queries) return fail(sz_bad_alloc_k, error_message, "Failed to allocate a tape");
-        b.tapes.push_back(queries);
         queries->segment = (uint32_t)s;
         status = stage_input(input, true, runes, *queries, stream, error_message);
         if (status != sz_success_k) return status;
         staged_tape_t *candidates = queries;
         if (input.mode != segment_symmetric_k) {
-            candidates = new (std::nothrow) staged_tape_t();
+            candidates = next_tape(b);
             if (!candidates) return fail(sz_bad_alloc_k, error_message, "Failed to allocate a tape");
-            b.tapes.push_back(candidates);
             candidates->segment = (uint32_t)s;
             status = stage_input(input, false, runes, *candidates, stream, error_message);
             if (status != sz_success_k) return status;
@@ -474,9 +514,8 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
             b.device_results[s] = reinterpret_cast<int64_t *>(b.results_block.as<char>() + b.pinned_offsets[s]);
         }
         else if (elements && !is_device_pointer(input.results)) {
-            auto *buffer = new (std::nothrow) device_buffer_t();
+            device_buffer_t *buffer = next_result_buffer(b);
             if (!buffer) return fail(sz_bad_alloc_k, error_message, "Failed to allocate results");
-            b.result_buffers.push_back(buffer);
             SZS_TRY_HIP(buffer->reserve(elements * 8), "Failed to allocate results in HBM");
             b.host_results[s] = input.results;
             b.device_results[s] = buffer->as<int64_t>();
@@ -576,15 +615,21 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
     SZS_TRY_HIP(b.shape_table.reserve(sizeof(shape_table_t)), "allocation");
     SZS_TRY_HIP(hipMemcpyAsync(b.shape_table.data, &shape_table, sizeof(shape_table_t), hipMemcpyHostToDevice, stream),
                 "upload");
-    std::vector<uint32_t> const &priority = bin_priority();
-    std::vector<uint32_t> position(bins_count_k);
-    for (uint32_t p = 0; p < bins_count_k; ++p) position[priority[p]] = p;
-    SZS_TRY_HIP(b.priority.reserve(bins_count_k * 4), "allocation");
-    SZS_TRY_HIP(hipMemcpyAsync(b.priority.data, priority.data(), bins_count_k * 4, hipMemcpyHostToDevice, stream),
-                "upload");
-    SZS_TRY_HIP(b.position_of_bin.reserve(bins_count_k * 4), "allocation");
-    SZS_TRY_HIP(hipMemcpyAsync(b.position_of_bin.data, position.data(), bins_count_k * 4, hipMemcpyHostToDevice,
-                               stream), "upload");
+    if (!b.static_tables_ready) { // constant tables: once per batch object
+        std::vector<uint32_t> const &priority = bin_priority();
+        static std::vector<uint32_t> const position = [&] {
+            std::vector<uint32_t> inverse(bins_count_k);
+            for (uint32_t p = 0; p < bins_count_k; ++p) inverse[priority[p]] = p;
+            return inverse;
+        }();
+        SZS_TRY_HIP(b.priority.reserve(bins_count_k * 4), "allocation");
+        SZS_TRY_HIP(hipMemcpyAsync(b.priority.data, priority.data(), bins_count_k * 4, hipMemcpyHostToDevice, stream),
+                    "upload");
+        SZS_TRY_HIP(b.position_of_bin.reserve(bins_count_k * 4), "allocation");
+        SZS_TRY_HIP(hipMemcpyAsync(b.position_of_bin.data, position.data(), bins_count_k * 4, hipMemcpyHostToDevice,
+                                   stream), "upload");
+        b.static_tables_ready = true;
+    }
     uint64_t const slots = std::max<uint64_t>(tasks_total, 1);
     SZS_TRY_HIP(b.tasks.reserve(slots * sizeof(task_t)), "Failed to allocate tasks");
     SZS_TRY_HIP(b.bin_of_task.reserve(slots * 4), "Failed to allocate tasks");
@@ -597,8 +642,8 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
     SZS_TRY_HIP(b.queue_head.reserve(64), "allocation");
     SZS_TRY_HIP(b.stats.reserve(sizeof(prepare_stats_t)), "allocation");
     SZS_TRY_HIP(b.segment_overflow.reserve(table_count * 4), "allocation");
-    SZS_TRY_HIP(hipEventCreate(&b.kernel_begin), "event");
-    SZS_TRY_HIP(hipEventCreate(&b.kernel_end), "event");
+    if (!b.kernel_begin) SZS_TRY_HIP(hipEventCreate(&b.kernel_begin), "event");
+    if (!b.kernel_end) SZS_TRY_HIP(hipEventCreate(&b.kernel_end), "event");
 
     // Materialize once: the stats size the scratch rows, carry the reach check and count the cells.
     status = enqueue_front(b, false, error_message);
@@ -675,8 +720,6 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
         b.graph = graph;
         SZS_TRY_HIP(hipGraphInstantiate(&b.graph_exec, graph, nullptr, nullptr, 0), "graph instantiation");
     }
-    *out = batch;
-    batch = nullptr; // ownership moves to the caller
     return ok(error_message);
 }
 

@@ -70,6 +70,14 @@ hipError_t arena_t::init(size_t bytes) {
     return hipSuccess;
 }
 
+void arena_t::adopt(void *memory, size_t bytes) {
+    base = static_cast<char *>(memory);
+    capacity = bytes & ~size_t(4095);
+    owned = false;
+    free_blocks.clear();
+    if (capacity) free_blocks[0] = capacity;
+}
+
 void *arena_t::allocate(size_t bytes) {
     std::lock_guard<std::mutex> lock(mutex);
     for (auto block = free_blocks.begin(); block != free_blocks.end(); ++block) {
@@ -101,7 +109,7 @@ void arena_t::deallocate(void *pointer, size_t bytes) {
 }
 
 arena_t::~arena_t() {
-    if (base) (void)hipFree(base);
+    if (base && owned) (void)hipFree(base);
 }
 
 arena_t *&thread_arena() {
@@ -128,6 +136,7 @@ hipError_t device_buffer_t::reserve(size_t bytes) {
             capacity = rounded;
             return hipSuccess;
         }
+        if (pool->exclusive) return hipErrorOutOfMemory;
     }
     hipError_t const error = hipMalloc(&data, rounded);
     if (error != hipSuccess) {
@@ -172,6 +181,12 @@ bool is_device_pointer(void const *pointer) {
 #pragma endregion
 
 #pragma region Scopes
+
+device_scope_t::~device_scope_t() {
+    if (!workspace) return;
+    (void)hipSetDevice(gpu_device);
+    delete workspace;
+}
 
 sz_status_t resolve_gpu(device_scope_t *scope, int *device, hipStream_t *stream, char const **error_message) {
     if (!scope) return fail(sz_device_code_mismatch_k, error_message, "Device scope must be initialized");

@@ -52,7 +52,11 @@ struct arena_t {
     size_t capacity = 0;
     std::map<size_t, size_t> free_blocks; /**< offset -> bytes */
     std::mutex mutex;
+    bool owned = true;     /**< `base` came from `init` (freed with the arena), not from `adopt` */
+    bool exclusive = false; /**< buffers that do not fit fail instead of falling back to `hipMalloc` */
     hipError_t init(size_t bytes);
+    /** Carves the pool from memory the caller keeps (e.g. a static device array); never freed here. */
+    void adopt(void *memory, size_t bytes);
     void *allocate(size_t bytes);
     void deallocate(void *pointer, size_t bytes);
     ~arena_t();
@@ -80,6 +84,8 @@ struct device_buffer_t {
 
 enum scope_kind_t { scope_default_k = 0, scope_cpu_k = 1, scope_gpu_k = 2 };
 
+struct batch_t;
+
 /** `szs_device_scope_t` target (c/stringzillas/stringzillas.cuh:241-321 in the reference). */
 struct device_scope_t {
     scope_kind_t kind = scope_default_k;
@@ -87,6 +93,12 @@ struct device_scope_t {
     size_t cpu_cores = 0;
     hipStream_t stream = nullptr;
     std::mutex mutex;
+    /** Reference-shaped and pair-list calls on this scope run one at a time (as the reference's executor admits one
+     *  driver) and restage into `workspace`, whose device buffers only grow: after the first calls no call
+     *  allocates device memory (the reference keeps such scratch per engine, serial.hpp:3030-3040). */
+    std::mutex call_mutex;
+    batch_t *workspace = nullptr;
+    ~device_scope_t();
 };
 
 sz_status_t resolve_gpu(device_scope_t *scope, int *device, hipStream_t *stream, char const **error_message);
@@ -117,6 +129,10 @@ struct staged_tape_t {
     uint32_t *runes = nullptr;      /**< UTF-8 only: u32[bytes] */
     uint32_t *rune_counts = nullptr;
     device_buffer_t data_copy, offsets_copy, runes_buffer, rune_counts_buffer;
+    /** Forgets the staged view (keeping the buffers' capacity) so the tape can be staged again. */
+    void reset_view() {
+        data = nullptr, offsets = nullptr, count = 0, bytes = 0, segment = 0, runes = nullptr, rune_counts = nullptr;
+    }
 };
 
 /** Stages a host or device tape of raw bytes on `stream` (device data and u64 offsets; copies only what is needed). */
@@ -160,6 +176,9 @@ struct batch_t {
     szs_batch_mode_t mode = szs_batch_persistent_k;
     std::vector<segment_input_t> inputs;
     std::vector<staged_tape_t *> tapes;        /**< owned; queries then candidates per segment */
+    std::vector<staged_tape_t *> spare_tapes;  /**< owned; staged by an earlier call, reused by the next one */
+    std::vector<device_buffer_t *> spare_results; /**< owned; result buffers of an earlier call */
+    bool static_tables_ready = false;          /**< the bin priority tables are already in HBM */
     std::vector<void *> host_results;          /**< per segment: the caller's host pointer, or null if device */
     std::vector<uint64_t> results_elements;    /**< per segment: elements spanned by the result matrix */
     std::vector<uint64_t> result_rows, result_columns;
@@ -191,6 +210,9 @@ struct batch_t {
     hipEvent_t kernel_begin = nullptr, kernel_end = nullptr;
     double last_kernel_ms = 0;
 
+    /** Drops everything one staging set up (tapes and result buffers become spares, the graph is destroyed)
+     *  while every device buffer keeps its capacity, so the next `batch_stage` allocates only what grows. */
+    void reset();
     ~batch_t();
 };
 
@@ -200,6 +222,9 @@ sz_status_t check_host_offsets(segment_input_t const &input, char const **error_
 /** Stages and bins a batch on the scope's device (or on `options.stream`). */
 sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> const &inputs, szs_batch_mode_t mode,
                           batch_t **out, char const **error_message, batch_options_t const &options = {});
+/** The same into an existing batch, reusing its device buffers (grown where the new inputs need more). */
+sz_status_t batch_stage(batch_t &batch, device_scope_t *scope, std::vector<segment_input_t> const &inputs,
+                        szs_batch_mode_t mode, char const **error_message, batch_options_t const &options = {});
 sz_status_t batch_launch(batch_t *batch, char const **error_message);
 sz_status_t batch_wait(batch_t *batch, char const **error_message);
 

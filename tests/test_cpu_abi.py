@@ -4,6 +4,8 @@ fails loudly (never silently computes on the CPU) when no GPU is visible. No ker
 """
 
 import ctypes
+import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -167,6 +169,28 @@ def test_arrow_inputs_are_viewed_as_tapes():
         tape = szs._as_tape(array, "queries")
         expected = strings[1:4] if len(array) == 3 else strings
         assert [tape[i].decode() for i in range(len(tape))] == expected
+        # Arrow's 32-bit offsets are viewed, not widened: the engines then take their u32-tape entry points.
+        assert tape.u32 == (array.type == pa.string())
+        assert np.shares_memory(tape.offsets, np.frombuffer(array.buffers()[1], dtype=tape.offsets.dtype))
+        assert isinstance(tape.tape(), _native.U32Tape if tape.u32 else _native.U64Tape)
+        wide = tape.as_u64()
+        assert wide.offsets.dtype == np.uint64 and [wide[i] for i in range(len(wide))] == [tape[i] for i in range(len(tape))]
+
+
+def test_reference_import_name_is_this_package():
+    import stringzillas
+
+    assert stringzillas is szs
+    assert stringzillas.LevenshteinDistances is szs.LevenshteinDistances
+
+
+def test_u32_and_u64_tapes_do_not_mix_in_one_call():
+    """python/stringzillas/similarities.c:283-303 rejects a cross-product of a 32- and a 64-bit offset tape."""
+    narrow = szs.Strs.from_tape(np.frombuffer(b"ACGTT", dtype=np.uint8), np.array([0, 2, 5], dtype=np.uint32))
+    wide = szs.Strs(["AC", "GTT"])
+    assert narrow.u32 and not wide.u32
+    with pytest.raises(TypeError):
+        szs.LevenshteinDistances()(narrow, wide)
 
 
 def test_tuning_ladders_validate_and_round_trip():
@@ -301,3 +325,73 @@ def test_packed_rows_follow_the_range_bound():
     dna_b2c, dna_costs = dna_tables()
     assert rows(szs.NeedlemanWunschScores(dna_b2c, dna_costs, -1, -1)) == 0  # -3 - (-2) < 0: no unsigned profile
     assert rows(szs.NeedlemanWunschScores(dna_b2c, dna_costs, -2, -2)) == 8  # -3 - (-4) = 1: fits
+
+
+REFERENCE_INCLUDE = "/root/reference/include"
+
+CALLER = r"""
+#include <stdio.h>
+#include <stringzillas/stringzillas.h>
+static sz_cptr_t start(void const *handle, sz_sorted_idx_t i) { return ((char const *const *)handle)[i]; }
+static sz_size_t length(void const *handle, sz_sorted_idx_t i) {
+    sz_size_t n = 0;
+    while (((char const *const *)handle)[i][n]) ++n;
+    return n;
+}
+int main(void) {
+    char const *error = NULL;
+    szs_device_scope_t scope = NULL;
+    if (szs_device_scope_init_default(&scope, &error) != sz_success_k) return 2;
+    szs_levenshtein_distances_t lev = NULL;
+    if (szs_levenshtein_distances_init(0, 1, 1, 1, NULL, sz_cap_serial_k, &lev, &error) != sz_success_k) return 3;
+    char const data[] = "kittensitting";
+    sz_u64_t wide[3] = {0, 6, 13};
+    sz_u32_t narrow[3] = {0, 6, 13};
+    sz_sequence_u64tape_t tape64 = {data, wide, 2};
+    sz_sequence_u32tape_t tape32 = {data, narrow, 2};
+    char const *words[2] = {"kitten", "sitting"};
+    sz_sequence_t sequence = {(void const *)words, 2, start, length};
+    sz_size_t results[3][4] = {{0}};
+    sz_status_t statuses[3];
+    statuses[0] = szs_levenshtein_distances_u64tape(lev, scope, &tape64, NULL, results[0], 2, &error);
+    statuses[1] = szs_levenshtein_distances_u32tape(lev, scope, &tape32, &tape32, results[1], 2, &error);
+    statuses[2] = szs_levenshtein_distances(lev, scope, &sequence, NULL, results[2], 2, &error);
+    sz_u8_t classes[256] = {0};
+    sz_error_cost_t costs[32 * 32] = {0};
+    szs_needleman_wunsch_scores_t nw = NULL;
+    if (szs_needleman_wunsch_scores_init(classes, costs, -1, -1, NULL, sz_cap_serial_k, &nw, &error) != sz_success_k)
+        return 4;
+    sz_ssize_t scores[4] = {0};
+    sz_status_t const nw_status = szs_needleman_wunsch_scores_u64tape(nw, scope, &tape64, NULL, scores, 2, &error);
+    for (int k = 0; k < 3; ++k)
+        printf("%d %zu %zu %zu %zu\n", (int)statuses[k], results[k][0], results[k][1], results[k][2], results[k][3]);
+    printf("%d %td %td %td %td\n", (int)nw_status, scores[0], scores[1], scores[2], scores[3]);
+    szs_needleman_wunsch_scores_free(nw);
+    szs_levenshtein_distances_free(lev);
+    szs_device_scope_free(scope);
+    return 0;
+}
+"""
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REFERENCE_INCLUDE, "stringzillas")),
+                    reason="needs the reference's own include/stringzillas/stringzillas.h")
+def test_a_caller_built_against_the_reference_header_links_unchanged(tmp_path):
+    """A C program written against include/stringzillas/stringzillas.h (with the dynamic-dispatch declarations its
+    shared-library users compile with) links against this library unchanged and gets a score, or
+    sz_missing_gpu_k where no GPU is visible."""
+    source = tmp_path / "caller.c"
+    source.write_text(CALLER)
+    binary = tmp_path / "caller"
+    library_dir = os.path.dirname(_native.LIBRARY_PATH)
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-Wno-unknown-pragmas", "-DSZ_DYNAMIC_DISPATCH=1", f"-I{REFERENCE_INCLUDE}",
+                    str(source), f"-L{library_dir}", "-lstringzillas_amd", f"-Wl,-rpath,{library_dir}", "-o",
+                    str(binary)], check=True, capture_output=True, text=True)
+    lines = subprocess.run([str(binary)], check=True, capture_output=True, text=True).stdout.split("\n")
+    rows = [[int(x) for x in line.split()] for line in lines if line.strip()]
+    if gpu_available():
+        for row in rows[:3]:
+            assert row == [0, 0, 3, 3, 0]  # Lev(kitten, sitting) = 3
+        assert rows[3] == [0, 0, -3, -3, 0]  # NW with 0 / -1 costs is minus Levenshtein
+    else:
+        assert all(row[0] == -16 for row in rows)  # sz_missing_gpu_k
