@@ -372,6 +372,20 @@ typedef enum szs_batch_mode_t {
 
 sz_status_t szs_batch_init(szs_device_scope_t device, szs_pairs_segment_t const *segments, sz_size_t segments_count,
                            szs_batch_mode_t mode, szs_batch_t *batch, char const **error_message);
+/**
+ *  No reference counterpart; for callers that cannot allocate device memory (tight serving loops, harnesses that
+ *  forbid `hipMalloc`). Scores `segments` as `szs_batch_init` (persistent mode) + `szs_batch_run` + `szs_batch_free`
+ *  would, with the same kernels and results, but calls no `hipMalloc`/`hipFree`: every device buffer comes from a
+ *  static array of `SZS_NOALLOC_SCRATCH_BYTES` per GPU, held from the library's load on that GPU until the process
+ *  ends. Tapes and results must be in device memory (`sz_device_memory_mismatch_k` otherwise); an input needing
+ *  more than the array holds fails with `sz_bad_alloc_k` (wide multi-stripe pairs run fewer workgroups first).
+ *  Calls on one GPU share the array, so they would race: the library serializes them per GPU. The first call on a
+ *  device looks the array up there (and, like any call, the first on a scope creates the scope's stream).
+ */
+#define SZS_NOALLOC_SCRATCH_BYTES (256ull << 20)
+sz_status_t szs_batch_run_noalloc(szs_device_scope_t device, szs_pairs_segment_t const *segments,
+                                  sz_size_t segments_count, char const **error_message);
+
 /** Runs the prepared batch once and waits for it; results land where the segments point. */
 sz_status_t szs_batch_run(szs_batch_t batch, char const **error_message);
 /** Queues the batch on its stream without waiting (`szs_batch_wait` joins it). */

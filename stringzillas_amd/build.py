@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBRARY = os.path.join(HERE, "libstringzillas_amd.so")
-SOURCES = ["dp_kernels.hip", "runtime.hip", "batch.hip", "abi.hip", "server.hip", "tape_file.hip", "fingerprints.hip"]
+SOURCES = ["dp_kernels.hip", "runtime.hip", "batch.hip", "abi.hip", "server.hip", "tape_file.hip", "fingerprints.hip",
+           "noalloc.hip"]
 ARCH = os.environ.get("SZS_AMD_ARCH", "gfx950")
 
 

@@ -287,6 +287,21 @@ void batch_t::reset() {
     scratch_capacity = 0, waves = 0, lds_bytes = 0, last_kernel_ms = 0;
 }
 
+void batch_t::release_device_memory() {
+    reset();
+    for (device_buffer_t *buffer : {&params, &segments, &shape_table, &tasks, &bin_of_task, &order, &shape_of_task,
+                                    &identity, &histogram, &cursor, &task_start, &item_start, &priority,
+                                    &position_of_bin, &queue_head, &scratch, &stats, &materialize_device,
+                                    &segment_overflow, &results_block, &decode_tapes, &chain_rows, &chain_progress})
+        buffer->release();
+    for (staged_tape_t *tape : spare_tapes)
+        for (device_buffer_t *buffer : {&tape->data_copy, &tape->offsets_copy, &tape->runes_buffer,
+                                        &tape->rune_counts_buffer})
+            buffer->release();
+    for (device_buffer_t *buffer : spare_results) buffer->release();
+    static_tables_ready = false;
+}
+
 batch_t::~batch_t() {
     reset();
     if (kernel_begin) (void)hipEventDestroy(kernel_begin);
@@ -688,6 +703,9 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
     SZS_TRY_HIP(szs_amd_persistent_occupancy(&blocks_per_cu, b.lds_bytes), "occupancy");
     blocks_per_cu = std::max(1, std::min(blocks_per_cu, 8));
     b.waves = mode == szs_batch_per_pair_k ? 1u : (uint32_t)(compute_units * blocks_per_cu); // workgroups
+    uint64_t const group_scratch_bytes = waves_per_group_k * 2ull * b.scratch_capacity * 4;
+    if (widest_multi_pass && options.max_scratch_bytes && b.waves * group_scratch_bytes > options.max_scratch_bytes)
+        b.waves = (uint32_t)std::max<uint64_t>(1, options.max_scratch_bytes / group_scratch_bytes);
     uint64_t const scratch_groups = widest_multi_pass ? b.waves : 1;
     SZS_TRY_HIP(b.scratch.reserve(scratch_groups * waves_per_group_k * 2ull * b.scratch_capacity * 4 + 256),
                 "Failed to allocate scratch");

@@ -167,6 +167,9 @@ struct batch_options_t {
     /** Latency over throughput: a pair that a one-wave shape would sweep in more than this many stripes takes the
      *  cooperative shape of the same geometry, so no single wave sets the end of a short launch (0: off). */
     int32_t latency_stripes = 0;
+    /** Most bytes of stripe-boundary scratch the launch may use (0: no limit): with wide multi-stripe pairs the
+     *  persistent launch then runs fewer workgroups, each holding its rows. */
+    size_t max_scratch_bytes = 0;
 };
 
 /** A batch staged on one device: everything `run` needs, ready to be launched (or replayed from a graph). */
@@ -213,6 +216,8 @@ struct batch_t {
     /** Drops everything one staging set up (tapes and result buffers become spares, the graph is destroyed)
      *  while every device buffer keeps its capacity, so the next `batch_stage` allocates only what grows. */
     void reset();
+    /** `reset`, then returns every device buffer (spares included) to where it came from. */
+    void release_device_memory();
     ~batch_t();
 };
 
