@@ -1,14 +1,21 @@
 """Benchmark: GCUPS of the batched persistent similarity engine on one MI355X, against the one-launch-per-pair
-form and the reference's CPU path, on the uneven mixed batch (1,000,000 pairs with lengths log-uniform in
-[16, 16384] plus 32 pairs of 65,536: 50% byte Levenshtein, 25% UTF-8 Levenshtein, 25% affine Needleman-Wunsch).
+form and the reference's CPU path.
 
-One step = one replay of the HIP graph that transcodes, materializes, bins and scores the whole batch, with the
-tapes already resident in HBM. Prints one JSON line.
+Headline (``value``): the uneven mixed batch, 1,000,000 pairs with lengths log-uniform in [16, 16384] plus 32 pairs
+of 65,536 (50% byte Levenshtein, 25% UTF-8 Levenshtein, 25% affine Needleman-Wunsch). One step = one replay of the
+HIP graph that transcodes, materializes, bins and scores the whole batch, tapes already resident in HBM.
+
+Beside it, in the same JSON line: the roofline of the scoring kernel (core recurrence ops over its HIP-event
+duration), each kind alone, the per-call form on a shared sample, the reference's CPU scorers on all cores of the
+share and on one core, the Levenshtein (1,024 pairs), Needleman-Wunsch (16,384) and Smith-Waterman (100,000)
+configurations, the reference's published all-pairs shapes (equal-length ACGT, 100 and 1,000 bytes), the serving
+loop (60 s of Poisson arrivals per rate, every result verified) and the fingerprint engine.
 """
 
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -21,13 +28,21 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GCUPS on 1 MI355X (batched vs per-call) + p50/p99 latency in serving loop"
 INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # CUs x lanes x clock: 39.3 T lane-ops/s
-HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the scoring kernel on the full mixed batch: FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
-# (factors from a plain 1 GiB device copy), recorded in profiles/persistent_score_kernel_mixed_1M_pairs.md.
-MEASURED_TRAFFIC_BYTES = {"mixed_1000000_pairs_log_uniform_16_16384_plus_32x65536": 62.27e9}
-# VALU issue rate of the same launch from SQ_INSTS_VALU (same file): wave64 instructions per second over the
-# gfx950 peak of one per cycle per CU.
-MEASURED_VALU_ISSUE = {"mixed_1000000_pairs_log_uniform_16_16384_plus_32x65536": 0.953}
+VALU_ISSUE_PEAK = 256 * 2.4e9  # wave64 VALU instructions per second: one per cycle per CU
+MIXED = "mixed_1000000_pairs_log_uniform_16_16384_plus_32x65536"
+# From profiles/persistent_score_kernel_mixed_1M_pairs.md (rocprofv3 --pmc on this workload): HBM bytes per launch
+# (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors a plain 1 GiB device copy reads) and SQ_INSTS_VALU per
+# launch (wave64 instructions), whose rate over the kernel's time is the measured VALU issue fraction.
+MEASURED_TRAFFIC_BYTES = {MIXED: 62.27e9}
+MEASURED_VALU_INSTRUCTIONS = {MIXED: 142.05e9}
+# The reference's published all-pairs GCUPS on equal-length ACGT (include/stringzillas/similarities/README.md:8-26):
+# Ice Lake AVX-512 on a 16-core Xeon4, and the H100 kernels (no UTF-8 kernel there).
+PUBLISHED = {
+    100: {"levenshtein": (114.6, 1499.8), "levenshtein_utf8": (6.7, None), "needleman_wunsch": (16.0, 187.3),
+          "smith_waterman": (15.8, 164.8)},
+    1000: {"levenshtein": (339.4, 5910.8), "levenshtein_utf8": (36.6, None), "needleman_wunsch": (97.1, 711.3),
+           "smith_waterman": (85.0, 615.8)},
+}
 
 
 def parse():
@@ -38,24 +53,47 @@ def parse():
     parser.add_argument("--pairs", type=int, default=1_000_000, help="pairs of the mixed batch (1,000,000)")
     parser.add_argument("--long-pairs", type=int, default=32)
     parser.add_argument("--per-call-sample", type=int, default=2000, help="pairs timed with one launch each")
-    parser.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
+    parser.add_argument("--cpu-seconds", type=float, default=10.0, help="target duration of an all-cores CPU sample")
     parser.add_argument("--check-pairs", type=int, default=48, help="pairs per segment checked against the oracle")
     parser.add_argument("--mode", choices=["graph", "persistent"], default="graph")
     parser.add_argument("--skip-cpu", action="store_true")
     parser.add_argument("--skip-per-call", action="store_true")
-    parser.add_argument("--breakdown", action="store_true", help="also time each kind alone")
+    parser.add_argument("--skip-breakdown", action="store_true", help="skip timing each kind alone")
+    parser.add_argument("--skip-configs", action="store_true", help="skip the 1,024 / 16,384 / 100,000-pair lines")
+    parser.add_argument("--skip-cross", action="store_true", help="skip the published all-pairs shapes")
     parser.add_argument("--serve-qps", type=float, nargs="*", default=[200.0, 1000.0, 5000.0],
-                        help="offered rates of the short serving run (none to skip)")
-    parser.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per serving rate")
+                        help="offered rates of the serving run (none to skip)")
+    parser.add_argument("--serve-seconds", type=float, default=60.0, help="arrival window per serving rate")
+    parser.add_argument("--baseline-seconds", type=float, default=20.0,
+                        help="arrival window of the one-launch-per-request baseline")
     parser.add_argument("--skip-fingerprints", action="store_true", help="skip the fingerprint engine's measurement")
     parser.add_argument("--fingerprint-mb", type=int, default=200, help="corpus size per fingerprint shape, MB")
     parser.add_argument("--only-fingerprints", action="store_true",
                         help="measure only the fingerprint engine and print its report (for profiling)")
+    parser.add_argument("--only-headline", action="store_true",
+                        help="measure the mixed batch alone, no side measurements (for profiling)")
     return parser.parse_args()
 
 
 def log(message: str) -> None:
     print(f"[bench] {message}", file=sys.stderr, flush=True)
+
+
+def cpu_share() -> int:
+    """Threads for CPU-side work: the cores this process may use, at most 16 (one GPU's share of the host)."""
+    try:
+        available = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        available = os.cpu_count() or 1
+    return max(1, min(16, available))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next((line.split(":", 1)[1].strip() for line in f if line.startswith("model name")), "")
+    except OSError:
+        return ""
 
 
 class TensorTape:
@@ -67,26 +105,31 @@ class TensorTape:
     def __len__(self):
         return self.offsets.numel() - 1
 
+    @property
+    def nbytes(self):
+        return int(self.offsets[-1].item()) if len(self) else 0
+
     def tape(self):
         from stringzillas_amd import _native
 
         return _native.U64Tape(self.data.data_ptr(), self.offsets.data_ptr(), len(self))
 
 
-def engines_for(segments, szs):
-    engines = []
-    for segment in segments:
-        if segment.kind == "levenshtein":
-            engines.append(szs.LevenshteinDistances(segment.match, segment.mismatch, segment.open, segment.extend))
-        elif segment.kind == "levenshtein_utf8":
-            engines.append(szs.LevenshteinDistancesUTF8(segment.match, segment.mismatch, segment.open, segment.extend))
-        elif segment.kind == "needleman_wunsch":
-            engines.append(szs.NeedlemanWunschScores(segment.byte_to_class, segment.class_costs, segment.open,
-                                                     segment.extend))
-        else:
-            engines.append(szs.SmithWatermanScores(segment.byte_to_class, segment.class_costs, segment.open,
-                                                   segment.extend))
-    return engines
+def to_device_tape(data: np.ndarray, offsets: np.ndarray) -> TensorTape:
+    import torch
+
+    return TensorTape(torch.from_numpy(np.array(data, dtype=np.uint8)).cuda(),
+                      torch.from_numpy(np.asarray(offsets).astype(np.int64)).cuda())
+
+
+def engine_for(segment, szs):
+    if segment.kind == "levenshtein":
+        return szs.LevenshteinDistances(segment.match, segment.mismatch, segment.open, segment.extend)
+    if segment.kind == "levenshtein_utf8":
+        return szs.LevenshteinDistancesUTF8(segment.match, segment.mismatch, segment.open, segment.extend)
+    if segment.kind == "needleman_wunsch":
+        return szs.NeedlemanWunschScores(segment.byte_to_class, segment.class_costs, segment.open, segment.extend)
+    return szs.SmithWatermanScores(segment.byte_to_class, segment.class_costs, segment.open, segment.extend)
 
 
 def oracle_params(segment):
@@ -100,6 +143,293 @@ def oracle_params(segment):
 
 def sample_indices(rng, count: int, size: int) -> np.ndarray:
     return np.sort(rng.choice(count, size=min(size, count), replace=False))
+
+
+def nominal_ops(segment) -> int:
+    """The fixed per-cell VALU count of the textbook recurrence: linear 5, affine global 9, affine local 10."""
+    if segment.open == segment.extend:
+        return 5
+    return 10 if segment.kind == "smith_waterman" else 9
+
+
+# --------------------------------------------------------------------------------------------------------------
+# Timing helpers
+
+
+def time_batch(batch, steps: int, warmup: int):
+    """Runs a prepared batch `warmup` + `steps` times; returns (wall ms per step, HIP-event ms per step)."""
+    import torch
+
+    for _ in range(warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    event_ms = []
+    start = time.perf_counter()
+    for _ in range(steps):
+        batch.launch()
+        batch.wait()
+        event_ms.append(batch.last_kernel_ms)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - start) / steps * 1e3, float(np.mean(event_ms))
+
+
+def roofline_of(core_ops: int, cells: int, kernel_ms: float, nominal_ops_per_cell: float):
+    """Algorithmic VALU work rate of the scoring kernel: cells weighted by the core op count of the tier that
+    scores each (a lower bound on what the kernel issues), over its HIP-event duration."""
+    achieved = core_ops / (kernel_ms / 1e3) / 1e12
+    return {
+        "bound": "valu",
+        "achieved": round(achieved, 3),
+        "peak": round(INT32_VALU_PEAK_TOPS, 2),
+        "unit": "T INT32 VALU lane-ops/s (core recurrence ops per cell of each pair's tier)",
+        "frac": round(achieved / INT32_VALU_PEAK_TOPS, 4),
+        "core_ops_per_cell": round(core_ops / max(cells, 1), 4),
+        "kernel_ms": round(kernel_ms, 3),
+        # The old fixed-op figure (linear 5, affine 9 per cell), which the bit-parallel and packed tiers undercut.
+        "nominal_dp_ratio": round(cells * nominal_ops_per_cell / (kernel_ms / 1e3) / 1e12 / INT32_VALU_PEAK_TOPS, 3),
+    }
+
+
+# --------------------------------------------------------------------------------------------------------------
+# CPU baseline: the reference's own scorers (Ice Lake AVX-512 where the host has it), driven over threads
+
+
+def cpu_baseline(host_segments, target_seconds: float, threads: int, seed: int = 99):
+    from oracle import oracle as O
+
+    variant = "icelake" if O.reference_available("icelake") else "serial" if O.reference_available("serial") else None
+    rng = np.random.default_rng(seed)
+    total = sum(s.count for s in host_segments)
+
+    def run(sample_pairs):
+        cells = 0
+        start = time.perf_counter()
+        for segment in host_segments:
+            share = max(1, int(round(sample_pairs * segment.count / total)))
+            sub = segment.subset(sample_indices(rng, segment.count, share))
+            tapes = ((sub.firsts_data, sub.firsts_offsets), (sub.seconds_data, sub.seconds_offsets))
+            if variant:
+                O.reference_pairs(oracle_params(sub), *tapes, threads=threads, variant=variant)
+            else:
+                O.pairs(oracle_params(sub), *tapes, threads=threads)
+            cells += sub.cells()
+        return cells, time.perf_counter() - start
+
+    sample_pairs = 16 * threads
+    cells, seconds = run(sample_pairs)
+    while seconds < target_seconds / 3 and sample_pairs < 400_000:
+        sample_pairs = int(min(sample_pairs * max(2.0, 0.9 * target_seconds / max(seconds, 1e-3)), 400_000))
+        cells, seconds = run(sample_pairs)
+    return {
+        "value": round(cells / seconds / 1e9, 3),
+        "unit": "GCUPS",
+        "cores": threads,
+        "kind": "reference" if variant else "port",
+        "sample": f"{sample_pairs} pairs drawn uniformly from the same pairs ({cells / 1e9:.1f} G cells, "
+                  f"{seconds:.1f}s), scorer={'reference ' + variant if variant else 'oracle'}",
+    }
+
+
+def cpu_baselines(host_segments, target_seconds: float):
+    """All cores of the share, then one core on a smaller sample."""
+    threads = cpu_share()
+    report = cpu_baseline(host_segments, target_seconds, threads)
+    single = cpu_baseline(host_segments, max(2.0, target_seconds / 3), 1, seed=98)
+    report["single_core_value"] = single["value"]
+    report["single_core_sample"] = single["sample"]
+    report["nproc"] = os.cpu_count()
+    report["cpu"] = cpu_model()
+    return report
+
+
+# --------------------------------------------------------------------------------------------------------------
+# Workload lines
+
+
+def parity_check(host_segments, outputs, check_pairs: int, seed: int = 2024):
+    """Oracle equality on a sample of every segment that includes one of its longest pairs."""
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(seed)
+    checked = mismatches = 0
+    for host, output in zip(host_segments, outputs):
+        got = output.cpu().numpy() if hasattr(output, "cpu") else np.asarray(output)
+        picks = sample_indices(rng, host.count, check_pairs)
+        longest = int(np.argmax(host.lengths("firsts") * host.lengths("seconds")))
+        picks = np.unique(np.concatenate([picks, [longest]]))
+        sub = host.subset(picks)
+        expected = O.pairs(oracle_params(host), (sub.firsts_data, sub.firsts_offsets),
+                           (sub.seconds_data, sub.seconds_offsets), threads=cpu_share())
+        mismatches += int((got[picks].astype(np.int64) != expected).sum())
+        checked += len(picks)
+    return {"checked_pairs": checked, "mismatches": mismatches}
+
+
+def per_call_line(szs, engines, host_segments, sample_pairs: int, seed: int = 7):
+    """One launch per pair (the plain per-call form) and the persistent launch, on the same sample."""
+    rng = np.random.default_rng(seed)
+    total = sum(s.count for s in host_segments)
+    subs = []
+    for host in host_segments:
+        share = max(1, int(round(sample_pairs * host.count / total)))
+        subs.append(host.subset(sample_indices(rng, host.count, share)))
+    cells = sum(s.cells() for s in subs)
+    rates, outputs = {}, {}
+    for name, mode in (("per_call", szs.BATCH_PER_PAIR), ("batched", szs.BATCH_PERSISTENT)):
+        segments = [(e, to_device_tape(s.firsts_data, s.firsts_offsets), to_device_tape(s.seconds_data,
+                                                                                        s.seconds_offsets))
+                    for e, s in zip(engines, subs)]
+        batch = szs.Batch(segments, mode=mode)
+        batch.run()
+        start = time.perf_counter()
+        batch.run()
+        rates[name] = cells / (time.perf_counter() - start) / 1e9
+        outputs[name] = [o.copy() for o in batch.outputs]
+        del batch
+    for a, b in zip(outputs["per_call"], outputs["batched"]):
+        if not np.array_equal(a, b):
+            raise SystemExit("per-call and batched scores differ")
+    return {
+        "gcups": round(rates["per_call"], 2),
+        "batched_same_sample_gcups": round(rates["batched"], 2),
+        "batched_over_per_call": round(rates["batched"] / rates["per_call"], 1),
+        "pairs": int(sum(s.count for s in subs)),
+        "cells": int(cells),
+        "launch": "one 256-thread workgroup per pair, one kernel launch per pair, same scorers; tapes in HBM",
+    }
+
+
+def config_line(szs, workload, args):
+    """One configuration end to end: graph replay on HBM tapes, scoring-kernel roofline, per-call form, parity and
+    the reference's CPU scorers."""
+    import torch
+
+    segments = workload.segments
+    engines = [engine_for(s, szs) for s in segments]
+    tapes = [(to_device_tape(s.firsts_data, s.firsts_offsets), to_device_tape(s.seconds_data, s.seconds_offsets))
+             for s in segments]
+    outputs = [torch.zeros(s.count, dtype=torch.int64, device="cuda") for s in segments]
+    cells = workload.cells()
+    batch = szs.Batch([(e, a, b) for e, (a, b) in zip(engines, tapes)], mode=szs.BATCH_GRAPH, outputs=outputs)
+    steps = max(args.steps, 10)
+    ms, _ = time_batch(batch, steps, max(args.warmup, 2))
+    scoring = szs.Batch([(e, a, b) for e, (a, b) in zip(engines, tapes)], mode=szs.BATCH_PERSISTENT, outputs=outputs)
+    _, kernel_ms = time_batch(scoring, steps, 1)
+    nominal = sum(s.cells() * nominal_ops(s) for s in segments) / max(cells, 1)
+    line = {
+        "workload": workload.name,
+        "pairs": workload.pairs,
+        "cells": cells,
+        "gcups": round(cells / (ms / 1e3) / 1e9, 2),
+        "ms_per_step": round(ms, 3),
+        "roofline": roofline_of(scoring.core_ops, cells, kernel_ms, nominal),
+        "parity": parity_check([s for s in segments], outputs, 256),
+    }
+    del batch, scoring
+    if not args.skip_per_call:
+        line["per_call"] = per_call_line(szs, engines, segments, min(workload.pairs, args.per_call_sample))
+    if not args.skip_cpu:
+        line["cpu_baseline"] = cpu_baselines(segments, max(3.0, args.cpu_seconds / 3))
+        line["over_cpu"] = round(line["gcups"] / line["cpu_baseline"]["value"], 1)
+    return line
+
+
+def cross_shapes(szs, args):
+    """The reference's published shape: all-pairs cross-products of equal-length ACGT strings through the
+    reference-shaped call (`szs_*_u64tape` with tapes and the result matrix in HBM), under unit costs
+    (match/mismatch/gap 0/1/1, negated for the maximizing engines), at 100 and 1,000 bytes. Tiles: the
+    reference's per-call budget rule (256 pairs per parallel unit: 256 x 256 CUs, a 256 x 256 tile here) and a
+    large tile where the call is bound by the kernels."""
+    import torch
+    from oracle import oracle as O
+    from stringzillas_amd import _native
+
+    class_costs = np.full((32, 32), -1, dtype=np.int8)
+    np.fill_diagonal(class_costs, 0)
+    byte_to_class = np.zeros(256, dtype=np.uint8)
+    byte_to_class[np.frombuffer(b"acgt", dtype=np.uint8)] = np.arange(1, 5, dtype=np.uint8)
+    kinds = {
+        "levenshtein": (szs.LevenshteinDistances(), O.Params("levenshtein")),
+        "levenshtein_utf8": (szs.LevenshteinDistancesUTF8(), O.Params("levenshtein_utf8")),
+        "needleman_wunsch": (szs.NeedlemanWunschScores(byte_to_class, class_costs, -1, -1),
+                             O.Params("needleman_wunsch", open=-1, extend=-1, byte_to_class=byte_to_class,
+                                      class_costs=class_costs)),
+        "smith_waterman": (szs.SmithWatermanScores(byte_to_class, class_costs, -1, -1),
+                           O.Params("smith_waterman", open=-1, extend=-1, byte_to_class=byte_to_class,
+                                    class_costs=class_costs)),
+    }
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    reference_tile = int(np.sqrt(256 * cus))
+    scope = szs._scope(None)
+    report = {"strings": "equal-length random 'acgt', all-pairs cross-product, unit costs",
+              "reference_tile": f"{reference_tile}x{reference_tile} (256 pairs per CU, as the reference sizes its "
+                                "H100 tile: 256 x 132 SMs -> 183x184)", "shapes": []}
+    rng = np.random.default_rng(0x535A0010)
+    letters = np.frombuffer(b"acgt", dtype=np.uint8)
+    for length, large in ((100, 4096), (1000, 1024)):
+        for tile in (reference_tile, large):
+            data = letters[rng.integers(0, 4, size=tile * length)]
+            offsets = np.arange(tile + 1, dtype=np.uint64) * length
+            tape = to_device_tape(data, offsets)
+            results = torch.empty((tile, tile), dtype=torch.int64, device="cuda")
+            view = tape.tape()
+            for name, (engine, params) in kinds.items():
+                if tile == large and name == "levenshtein_utf8":
+                    continue  # the large tiles time the four engines' kernels; UTF-8 only at the reference tile
+                call = getattr(_native.lib, engine._prefix + "_u64tape")
+                error = ctypes.c_char_p()
+
+                def once():
+                    status = call(engine.handle, scope.handle, ctypes.byref(view), ctypes.byref(view),
+                                  ctypes.c_void_p(results.data_ptr()), ctypes.c_size_t(tile), ctypes.byref(error))
+                    _native.check(status, error, name)
+
+                once()
+                torch.cuda.synchronize()
+                calls, start = 0, time.perf_counter()
+                while calls < 3 or time.perf_counter() - start < 0.3:
+                    once()
+                    calls += 1
+                seconds = (time.perf_counter() - start) / calls
+                cells = tile * tile * length * length
+                picks = rng.integers(0, tile, size=(6, 2))
+                got = results.cpu().numpy()
+                strings = [data[i * length:(i + 1) * length].tobytes() for i in range(tile)]
+                for q, c in picks:
+                    if int(got[q, c]) != O.score(params, strings[q], strings[c]):
+                        raise SystemExit(f"cross-product {name} {length} B differs from the oracle at ({q}, {c})")
+                icelake, h100 = PUBLISHED[length][name]
+                entry = {"kind": name, "bytes": length, "tile": f"{tile}x{tile}", "call_ms": round(seconds * 1e3, 3),
+                         "gcups": round(cells / seconds / 1e9, 1), "reference_icelake_16core_gcups": icelake,
+                         "reference_h100_gcups": h100, "checked_cells": len(picks)}
+                if h100:
+                    entry["over_h100"] = round(entry["gcups"] / h100, 2)
+                log(f"cross {entry}")
+                report["shapes"].append(entry)
+            del tape, results
+    if not args.skip_cpu:  # the reference's Ice Lake scorer on pairs of the same strings, this host's share
+        threads = cpu_share()
+        variant = "icelake" if O.reference_available("icelake") else "serial" if O.reference_available("serial") else None
+        cpu = []
+        for length, count in ((100, 1_000_000), (1000, 16_000)):
+            data = letters[rng.integers(0, 4, size=2 * count * length)]
+            a = (np.ascontiguousarray(data[:count * length]), np.arange(count + 1, dtype=np.uint64) * length)
+            b = (np.ascontiguousarray(data[count * length:]), np.arange(count + 1, dtype=np.uint64) * length)
+            start = time.perf_counter()
+            if variant:
+                O.reference_pairs(O.Params("levenshtein"), a, b, threads=threads, variant=variant)
+            else:
+                O.pairs(O.Params("levenshtein"), a, b, threads=threads)
+            seconds = time.perf_counter() - start
+            cpu.append({"kind": "levenshtein", "bytes": length, "pairs": count, "cores": threads,
+                        "gcups": round(count * length * length / seconds / 1e9, 2),
+                        "scorer": f"reference {variant}" if variant else "oracle"})
+        report["cpu_baseline"] = cpu
+        log(f"cross cpu {cpu}")
+    return report
+
+
+# --------------------------------------------------------------------------------------------------------------
 
 
 def main():
@@ -124,15 +454,13 @@ def main():
     torch.cuda.synchronize()
     log(f"generated {workload.name}: {workload.pairs} pairs in {time.time() - t0:.1f}s")
     cells = workload.cells()
-    ops_per_cell = workload.weighted_ops_per_cell()
-    log(f"{cells / 1e12:.3f} T cells, {ops_per_cell:.2f} VALU ops per cell (cell-weighted)")
+    nominal = workload.weighted_ops_per_cell()
 
-    engines = engines_for(workload.segments, szs)
+    engines = [engine_for(s, szs) for s in workload.segments]
     tapes, outputs = [], []
     for segment in workload.segments:
-        a = TensorTape(segment.firsts_data, segment.firsts_offsets)
-        b = TensorTape(segment.seconds_data, segment.seconds_offsets)
-        tapes.append((a, b))
+        tapes.append((TensorTape(segment.firsts_data, segment.firsts_offsets),
+                      TensorTape(segment.seconds_data, segment.seconds_offsets)))
         outputs.append(torch.zeros(segment.count, dtype=torch.int64, device=device))
     mode = szs.BATCH_GRAPH if args.mode == "graph" else szs.BATCH_PERSISTENT
     t0 = time.time()
@@ -140,121 +468,91 @@ def main():
     log(f"prepared the batch in {time.time() - t0:.1f}s ({batch.cells / 1e12:.3f} T cells)")
     assert batch.cells == cells, (batch.cells, cells)
 
-    for _ in range(args.warmup):
-        batch.run()
-    torch.cuda.synchronize()
-    kernel_ms = []
-    start = time.perf_counter()
-    for _ in range(args.steps):
-        batch.launch()
-        batch.wait()
-        kernel_ms.append(batch.last_kernel_ms)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - start
-    ms_per_step = elapsed / args.steps * 1e3
-    gcups = cells / (elapsed / args.steps) / 1e9
-    replay_ms_mean = float(np.mean(kernel_ms))
+    ms_per_step, replay_ms = time_batch(batch, args.steps, args.warmup)
+    gcups = cells / (ms_per_step / 1e3) / 1e9
     # The scoring kernel alone, from HIP events on its stream, in the non-graph form of the same batch.
     scoring = szs.Batch([(e, a, b) for e, (a, b) in zip(engines, tapes)], mode=szs.BATCH_PERSISTENT, outputs=outputs)
     scoring.run()
-    scoring_ms = []
-    for _ in range(max(1, min(args.steps, 3))):
-        scoring.run()
-        scoring_ms.append(scoring.last_kernel_ms)
-    kernel_ms_mean = float(np.mean(scoring_ms))
+    _, kernel_ms = time_batch(scoring, max(1, min(args.steps, 5)), 0)
+    roofline = roofline_of(scoring.core_ops, cells, kernel_ms, nominal)
+    instructions = MEASURED_VALU_INSTRUCTIONS.get(workload.name)
+    roofline["traffic"] = MEASURED_TRAFFIC_BYTES.get(workload.name)
+    if instructions:  # from the committed rocprofv3 profile of this kernel on this workload
+        roofline["valu_issue_frac"] = round(instructions / (kernel_ms / 1e3) / VALU_ISSUE_PEAK, 4)
+        roofline["valu_per_cell"] = round(instructions * 64 / cells, 3)
+    roofline["note"] = ("frac = core recurrence lane-ops (each cell weighted by the op count of its tier's update: "
+                        "Myers words, packed 16-bit or int32 DP) over the scoring kernel's HIP-event time and the "
+                        "INT32 VALU peak; valu_issue_frac = all issued VALU instructions (rocprofv3 SQ_INSTS_VALU) "
+                        "over the same time and the issue peak, so frac <= valu_issue_frac <= 1")
     del scoring
-    log(f"batched: {ms_per_step:.1f} ms/step ({replay_ms_mean:.1f} ms replay), {gcups:.1f} GCUPS; "
-        f"scoring kernel {kernel_ms_mean:.1f} ms")
+    log(f"batched: {ms_per_step:.1f} ms/step ({replay_ms:.1f} ms replay), {gcups:.1f} GCUPS; scoring kernel "
+        f"{kernel_ms:.1f} ms; roofline {roofline['frac']}")
+    if args.only_headline:
+        print(json.dumps({"value": round(gcups, 2), "ms_per_step": round(ms_per_step, 2), "roofline": roofline}),
+              flush=True)
+        return
 
-    # Each kind alone (persistent form): where the scoring time goes.
-    breakdown = {}
-    if args.breakdown:
+    # Each kind alone (persistent form): where the scoring time goes, against each tier's ceiling.
+    per_kind = {}
+    if not args.skip_breakdown:
         for engine, (a, b), segment, output in zip(engines, tapes, workload.segments, outputs):
             alone = szs.Batch([(engine, a, b)], mode=szs.BATCH_PERSISTENT, outputs=[output])
             alone.run()
-            alone.run()
-            seg_cells = alone.cells
-            breakdown[segment.kind] = {
-                "pairs": segment.count,
-                "cells": seg_cells,
-                "kernel_ms": round(alone.last_kernel_ms, 2),
-                "gcups": round(seg_cells / (alone.last_kernel_ms / 1e3) / 1e9, 1),
-                "ops_per_cell": segment.ops_per_cell(),
+            _, alone_ms = time_batch(alone, 2, 0)
+            seg_cells, core = alone.cells, alone.core_ops
+            rate = seg_cells / (alone_ms / 1e3)
+            ceiling = INT32_VALU_PEAK_TOPS * 1e12 / (core / seg_cells)  # cells/s if every lane-op were core work
+            entry = {
+                "pairs": segment.count, "cells": seg_cells, "kernel_ms": round(alone_ms, 2),
+                "gcups": round(rate / 1e9, 1),
+                "core_ops_per_cell": round(core / seg_cells, 4),
+                "tier_ceiling_gcups": round(ceiling / 1e9, 1),
+                "frac": round(rate / ceiling, 4),
+                "nominal_ops_per_cell": nominal_ops(segment),
+                "nominal_int32_frac": round(rate * nominal_ops(segment) / 1e12 / INT32_VALU_PEAK_TOPS, 3),
             }
+            if segment.kind == "needleman_wunsch":  # packed 16-bit cells: two per lane-op at the nominal count
+                entry["int16x2_frac"] = round(rate * nominal_ops(segment) / 2 / 1e12 / INT32_VALU_PEAK_TOPS, 3)
+            per_kind[segment.kind] = entry
             del alone
-        log(f"per kind: {breakdown}")
+        log(f"per kind: {per_kind}")
 
-    # Parity spot check against the oracle, including one of the longest pairs of each kind.
-    from oracle import oracle as O
-
-    rng = np.random.default_rng(2024)
-    checked = mismatches = 0
-    host_segments = []
-    for segment, output in zip(workload.segments, outputs):
-        host = segment.host()
-        host_segments.append(host)
-        got = output.cpu().numpy()
-        picks = sample_indices(rng, host.count, args.check_pairs)
-        longest = int(np.argmax(host.lengths("firsts")))
-        picks = np.unique(np.concatenate([picks, [longest]]))
-        sub = host.subset(picks)
-        expected = O.pairs(oracle_params(host), (sub.firsts_data, sub.firsts_offsets),
-                           (sub.seconds_data, sub.seconds_offsets), threads=min(16, os.cpu_count() or 1))
-        mismatches += int((got[picks] != expected).sum())
-        checked += len(picks)
-    log(f"parity: {checked} pairs checked against the oracle, {mismatches} mismatches")
-    if mismatches:
+    host_segments = [s.host() for s in workload.segments]
+    parity = parity_check(host_segments, outputs, args.check_pairs)
+    log(f"parity: {parity}")
+    if parity["mismatches"]:
         raise SystemExit("batched scores differ from the oracle")
 
-    # One launch per pair on a uniform sample, and the persistent form on the same sample.
     per_call = None
     if not args.skip_per_call:
-        rng = np.random.default_rng(7)
-        total = workload.pairs
-        sample_segments, sample_engines = [], []
-        for segment, host, engine in zip(workload.segments, host_segments, engines):
-            share = max(1, int(round(args.per_call_sample * segment.count / total)))
-            sub = host.subset(sample_indices(rng, host.count, share))
-            sample_segments.append(sub)
-            sample_engines.append(engine)
-        sample_cells = sum(s.cells() for s in sample_segments)
-        results = {}
-        for name, sample_mode in (("per_call", szs.BATCH_PER_PAIR), ("batched", szs.BATCH_PERSISTENT)):
-            strs = [(e, szs.Strs.from_tape(s.firsts_data, s.firsts_offsets),
-                     szs.Strs.from_tape(s.seconds_data, s.seconds_offsets))
-                    for e, s in zip(sample_engines, sample_segments)]
-            sample_batch = szs.Batch(strs, mode=sample_mode)
-            sample_batch.run()
-            t = time.perf_counter()
-            sample_batch.run()
-            results[name] = sample_cells / (time.perf_counter() - t) / 1e9
-            results[name + "_outputs"] = [o.copy() for o in sample_batch.outputs]
-        for a, b in zip(results["per_call_outputs"], results["batched_outputs"]):
-            if not np.array_equal(a, b):
-                raise SystemExit("per-call and batched scores differ")
-        per_call = {
-            "gcups": round(results["per_call"], 2),
-            "batched_same_sample_gcups": round(results["batched"], 2),
-            "pairs": int(sum(s.count for s in sample_segments)),
-            "cells": int(sample_cells),
-            "launch": "one 64-lane workgroup per pair, one launch per pair, same scorer",
-        }
-        log(f"per-call: {per_call['gcups']} GCUPS; batched on the same sample: {per_call['batched_same_sample_gcups']}")
-
-    serving = None
-    if args.serve_qps:
-        serving = serving_run(args.serve_qps, args.serve_seconds)
+        per_call = per_call_line(szs, engines, host_segments, args.per_call_sample)
+        log(f"per-call: {per_call}")
 
     cpu = None
     if not args.skip_cpu:
-        cpu = cpu_baseline(host_segments, args.cpu_seconds)
+        cpu = cpu_baselines(host_segments, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
+    del batch, tapes, workload, host_segments
+
+    configs = None
+    if not args.skip_configs:
+        configs = {}
+        for key, make in (("levenshtein_1024", W.levenshtein_c0), ("needleman_wunsch_16384", W.needleman_wunsch_c1),
+                          ("smith_waterman_100000", W.smith_waterman_c2)):
+            t0 = time.time()
+            configs[key] = config_line(szs, make(), args)
+            log(f"config {key}: {configs[key]} ({time.time() - t0:.1f}s)")
+            if configs[key]["parity"]["mismatches"]:
+                raise SystemExit(f"{key}: scores differ from the oracle")
+
+    cross = None if args.skip_cross else cross_shapes(szs, args)
+
+    serving = serving_run(args.serve_qps, args.serve_seconds, args.baseline_seconds) if args.serve_qps else None
 
     fingerprints = None
     if not args.skip_fingerprints:
         fingerprints = fingerprints_run(args.fingerprint_mb * 1_000_000, args.steps, not args.skip_cpu)
 
-    achieved = cells * ops_per_cell / (kernel_ms_mean / 1e3) / 1e12  # T lane-ops/s over the scoring kernel
     line = {
         "metric": METRIC,
         "value": round(gcups, 2),
@@ -269,38 +567,70 @@ def main():
         "dtype": "int32",
         "data": "synthetic (seeded, generated on the device)",
         "config": {
-            "workload": workload.name,
-            "pairs": workload.pairs,
+            "workload": MIXED if args.pairs == 1_000_000 and args.long_pairs == 32 else f"mixed_{args.pairs}",
+            "pairs": args.pairs + args.long_pairs,
             "cells": cells,
             "mix": "50% byte Levenshtein ACGT, 25% UTF-8 Levenshtein, 25% NW affine -11/-1 (21-class matrix)",
             "launch": args.mode,
         },
-        "roofline": {
-            "bound": "valu",
-            "achieved": round(achieved, 3),
-            "peak": round(INT32_VALU_PEAK_TOPS, 2),
-            "unit": "TOP/s (INT32 VALU lane-ops)",
-            "frac": round(achieved / INT32_VALU_PEAK_TOPS, 4),
-            "traffic": MEASURED_TRAFFIC_BYTES.get(workload.name),
-            "ops_per_cell": round(ops_per_cell, 3),
-            "kernel_ms": round(kernel_ms_mean, 3),
-            "note": "fixed DP ops per cell (linear 5, affine 9); unit-cost Levenshtein over bytes and runes runs "
-                    "the bit-parallel tier (64 cells per word op) and affine NW runs packed 16-bit cells (about 4.3 "
-                    "VALU per cell), which is why the fraction exceeds 1; valu_issue_frac is the measured share of "
-                    "the VALU issue rate the kernel uses",
-            "valu_issue_frac": MEASURED_VALU_ISSUE.get(workload.name),
-        },
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "per_call": per_call,
-        "batched_over_per_call": round(gcups / per_call["gcups"], 1) if per_call else None,
-        "parity": {"checked_pairs": checked, "mismatches": mismatches},
+        # Same-sample ratio: the persistent launch over one launch per pair on identical pairs.
+        "batched_over_per_call": per_call["batched_over_per_call"] if per_call else None,
+        "full_batch_over_per_call": round(gcups / per_call["gcups"], 1) if per_call else None,
+        "parity": parity,
+        "per_kind": per_kind or None,
+        "configs": configs,
+        "cross_product": cross,
         "serving": serving,
         "fingerprints": fingerprints,
     }
-    if breakdown:
-        line["per_kind"] = breakdown
     print(json.dumps(line), flush=True)
 
+
+# --------------------------------------------------------------------------------------------------------------
+# Serving
+
+
+def serving_run(rates, seconds: float, baseline_seconds: float):
+    """Poisson arrivals into the continuous-batching server at each rate, every completed request checked against
+    the same request scored alone; then one plain call per request at the first rate.
+
+    Requests follow the mixed batch's kind and length mix with 1..4096 pairs each, drawn from 512 HBM-resident
+    templates; latency runs from a request's arrival to the completion of its round.
+    """
+    from stringzillas_amd import serving as S
+
+    t0 = time.time()
+    templates = S.make_templates(512)
+    engines = S.engines_by_kind()
+    log(f"serving: {len(templates)} templates in {time.time() - t0:.1f}s")
+    S.run_server(templates[:4], engines, qps=50.0, duration=0.2, hbm_budget=16 << 30)  # warm-up
+    report = {"requests": "log-uniform 1..4096 pairs of the mixed kinds, lengths log-uniform 16..16384",
+              "templates": len(templates),
+              "mean_cells_per_request": round(float(np.mean([t.cells for t in templates]))),
+              "seconds_per_rate": seconds, "server": [], "per_request": None}
+    for qps in rates:
+        summary, results = S.run_server(templates, engines, qps, seconds, hbm_budget=16 << 30, drain_timeout=20.0)
+        check = S.verify(templates, engines, results)
+        if check["equal_to_alone"] != check["verified"]:
+            raise SystemExit(f"served results differ from the same requests scored alone at {qps} req/s: {check}")
+        summary.update(check)
+        log(f"serving at {qps:g} req/s: {summary}")
+        report["server"].append({k: summary[k] for k in ("offered_qps", "achieved_qps", "gcups", "p50_ms", "p99_ms",
+                                                         "completed", "rounds", "verified", "equal_to_alone",
+                                                         "distinct_requests")})
+    baseline = S.run_baseline(templates, engines, rates[0], baseline_seconds, drain_timeout=5.0)
+    log(f"one call per request at {rates[0]:g} req/s: {baseline}")
+    report["per_request"] = {k: baseline[k] for k in ("offered_qps", "achieved_qps", "gcups", "p50_ms", "p99_ms",
+                                                      "completed", "dropped_after_drain_timeout")}
+    report["per_request"]["seconds"] = baseline_seconds
+    return report
+
+
+# --------------------------------------------------------------------------------------------------------------
+# Fingerprints
 
 # The reference's published fingerprint throughput on an H100 (include/stringzillas/fingerprints/README.md:15-42):
 # equal-length ACGT documents, 64 dimensions, default window widths; MB there is 2^20 bytes.
@@ -311,7 +641,6 @@ FINGERPRINT_H100_MIB_S = {100: 438.7, 1000: 3400.0 * 1.024, 10000: 11200.0 * 1.0
 FINGERPRINT_VALU_PER_UPDATE = {(64, 100): 13.6, (64, 1000): 11.3, (64, 10000): 11.6, (64, 100000): 11.4,
                                (1024, 10000): 10.4}
 FINGERPRINT_TRAFFIC_BYTES = {(64, 10000): 673.7e6, (1024, 10000): 571.1e6}
-VALU_ISSUE_PEAK = 256 * 2.4e9  # wave64 VALU instructions per second: one per cycle per CU
 
 
 def fingerprints_run(corpus_bytes: int, steps: int, with_cpu: bool):
@@ -392,7 +721,7 @@ def fingerprint_cpu_baseline(target_seconds: float = 6.0):
     `basic_rolling_hashers` engine (serial code), whichever SIMD build the shim comes from."""
     from oracle import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_share()
     variant = next((v for v in ("skylake", "serial") if O.reference_fingerprints_available(v)), None)
     rng = np.random.default_rng(5)
     documents = 2 * threads
@@ -413,76 +742,6 @@ def fingerprint_cpu_baseline(target_seconds: float = 6.0):
         "kind": "reference" if variant else "port",
         "sample": f"{documents} ACGT documents of 10,000 bytes at 64 dimensions ({seconds:.1f}s), engine="
                   f"{'reference basic_rolling_hashers (' + variant + ' build)' if variant else 'oracle'}",
-    }
-
-
-def serving_run(rates, seconds: float):
-    """Poisson arrivals into the continuous-batching server, and one plain launch per request at the first rate.
-
-    Requests follow the mixed batch's kind and length mix with 1..4096 pairs each, from HBM-resident templates;
-    latency runs from a request's arrival to the completion of its round.
-    """
-    from stringzillas_amd import serving as S
-
-    templates = S.make_templates(96)
-    engines = S.engines_by_kind()
-    S.run_server(templates[:4], engines, qps=50.0, duration=0.2, hbm_budget=16 << 30)  # warm-up
-    report = {"requests": "log-uniform 1..4096 pairs of the mixed kinds, lengths log-uniform 16..16384",
-              "mean_cells_per_request": round(float(np.mean([t.cells for t in templates]))),
-              "seconds_per_rate": seconds, "server": [], "per_request": None}
-    for qps in rates:
-        summary, _ = S.run_server(templates, engines, qps, seconds, hbm_budget=16 << 30, drain_timeout=10.0)
-        log(f"serving at {qps:g} req/s: {summary}")
-        report["server"].append({k: summary[k] for k in ("offered_qps", "achieved_qps", "gcups", "p50_ms", "p99_ms",
-                                                         "completed", "rounds")})
-    baseline = S.run_baseline(templates, engines, rates[0], seconds, drain_timeout=4.0)
-    log(f"one launch per request at {rates[0]:g} req/s: {baseline}")
-    report["per_request"] = {k: baseline[k] for k in ("offered_qps", "achieved_qps", "gcups", "p50_ms", "p99_ms",
-                                                      "completed", "dropped_after_drain_timeout")}
-    return report
-
-
-def cpu_baseline(host_segments, target_seconds: float):
-    """The reference's own CPU scorers (Ice Lake AVX-512 when the host has it) on a bounded uniform sample."""
-    from oracle import oracle as O
-
-    threads = min(16, os.cpu_count() or 1)
-    variant = "icelake" if O.reference_available("icelake") else "serial" if O.reference_available("serial") else None
-    rng = np.random.default_rng(99)
-    total = sum(s.count for s in host_segments)
-
-    def run(sample_pairs):
-        cells = 0
-        start = time.perf_counter()
-        for segment in host_segments:
-            share = max(1, int(round(sample_pairs * segment.count / total)))
-            sub = segment.subset(sample_indices(rng, segment.count, share))
-            tapes = ((sub.firsts_data, sub.firsts_offsets), (sub.seconds_data, sub.seconds_offsets))
-            if variant:
-                O.reference_pairs(oracle_params(sub), *tapes, threads=threads, variant=variant)
-            else:
-                O.pairs(oracle_params(sub), *tapes, threads=threads)
-            cells += sub.cells()
-        return cells, time.perf_counter() - start
-
-    sample_pairs = 64
-    cells, seconds = run(sample_pairs)
-    while seconds < target_seconds / 3 and sample_pairs < 200_000:
-        sample_pairs = int(min(sample_pairs * max(2.0, 0.9 * target_seconds / max(seconds, 1e-3)), 200_000))
-        cells, seconds = run(sample_pairs)
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((line.split(":", 1)[1].strip() for line in f if line.startswith("model name")), "")
-    except OSError:
-        pass
-    return {
-        "value": round(cells / seconds / 1e9, 3),
-        "unit": "GCUPS",
-        "cores": threads,
-        "kind": "reference" if variant else "port",
-        "sample": f"{sample_pairs} pairs drawn uniformly from the same batch ({cells / 1e9:.1f} G cells, "
-                  f"{seconds:.1f}s), scorer={'reference ' + variant if variant else 'oracle'}, cpu={model}",
     }
 
 

@@ -393,6 +393,10 @@ sz_status_t szs_batch_launch(szs_batch_t batch, char const **error_message);
 sz_status_t szs_batch_wait(szs_batch_t batch, char const **error_message);
 /** Total DP cells (sum of n*m, runes for UTF-8) the batch scores per run. */
 sz_size_t szs_batch_cells(szs_batch_t batch);
+/** Core recurrence work per run in 32-bit VALU lane-ops: each cell weighted by the op count of the cell update in
+ *  the tier that scores it (bit-parallel words, packed 16-bit or int32 DP), data movement excluded; a launch
+ *  issues at least this many lane-ops, so this over its duration is a lower bound on the VALU work rate. */
+sz_size_t szs_batch_core_ops(szs_batch_t batch);
 /** Device time of the last run in milliseconds from HIP events on the batch stream: the scoring kernel alone for the
  *  persistent and per-pair modes, the whole replay for the graph mode. */
 double szs_batch_last_kernel_ms(szs_batch_t batch);

@@ -658,6 +658,11 @@ class Batch:
         return int(_lib.szs_batch_cells(self.handle))
 
     @property
+    def core_ops(self) -> int:
+        """Core recurrence lane-ops per run (each cell weighted by its tier's cell-update op count)."""
+        return int(_lib.szs_batch_core_ops(self.handle))
+
+    @property
     def last_kernel_ms(self) -> float:
         return float(_lib.szs_batch_last_kernel_ms(self.handle))
 

@@ -451,6 +451,8 @@ sz_status_t szs_batch_run(szs_batch_t batch, char const **error_message) {
 
 sz_size_t szs_batch_cells(szs_batch_t batch) { return batch ? reinterpret_cast<batch_t *>(batch)->cells : 0; }
 
+sz_size_t szs_batch_core_ops(szs_batch_t batch) { return batch ? reinterpret_cast<batch_t *>(batch)->core_ops : 0; }
+
 double szs_batch_last_kernel_ms(szs_batch_t batch) {
     return batch ? reinterpret_cast<batch_t *>(batch)->last_kernel_ms : 0;
 }

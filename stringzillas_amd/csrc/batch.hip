@@ -283,8 +283,8 @@ void batch_t::reset() {
     device_results.clear(), materialize_args.clear(), overflowed.clear(), pinned_offsets.clear();
     per_pair_tasks.clear(), per_pair_variants.clear();
     chained_pairs = 0, chain_pitch = 0, decode_tapes_count = 0, decode_strings = 0;
-    pinned_results = nullptr, pinned_bytes = 0, tasks_count = 0, cells = 0;
-    scratch_capacity = 0, waves = 0, lds_bytes = 0, last_kernel_ms = 0;
+    pinned_results = nullptr, pinned_bytes = 0, tasks_count = 0, cells = 0, core_ops = 0;
+    scratch_capacity = 0, waves = 0, lds_bytes = 0, last_kernel_ms = 0, materialized = false;
 }
 
 void batch_t::release_device_memory() {
@@ -341,24 +341,28 @@ static bins_view_t bins_view(batch_t &b) {
     return bins;
 }
 
-/** Device work up to (excluding) scoring: clear counters, transcode UTF-8, materialize, bin. */
-static sz_status_t enqueue_front(batch_t &b, bool bin, char const **error_message) {
+/** Device work up to (excluding) scoring: clear counters, transcode UTF-8, materialize, bin. With `reuse`, the
+ *  runes, tasks and bin histogram that the staging's own pass left on the device are used as they are. */
+static sz_status_t enqueue_front(batch_t &b, bool bin, char const **error_message, bool reuse = false) {
     hipStream_t const stream = b.stream;
-    SZS_TRY_HIP(szs_amd_launch_fill(b.histogram.as<uint32_t>(), bins_count_k, 0, stream), "fill");
     SZS_TRY_HIP(szs_amd_launch_fill(b.cursor.as<uint32_t>(), bins_count_k, 0, stream), "fill");
     SZS_TRY_HIP(szs_amd_launch_fill(b.queue_head.as<uint32_t>(), 1, 0, stream), "fill");
-    uint32_t const segments_count = (uint32_t)b.inputs.size();
-    SZS_TRY_HIP(szs_amd_launch_fill(b.stats.as<uint32_t>(), sizeof(prepare_stats_t) / 4, 0, stream), "fill");
-    SZS_TRY_HIP(szs_amd_launch_fill(b.segment_overflow.as<uint32_t>(), segments_count, 0, stream), "fill");
-    SZS_TRY_HIP(szs_amd_launch_utf8_decode(b.decode_tapes.as<utf8_tape_t>(), b.decode_tapes_count, b.decode_strings,
-                                           b.stats.as<prepare_stats_t>(), b.segment_overflow.as<uint32_t>(), stream),
-                "UTF-8 decoding");
-    SZS_TRY_HIP(szs_amd_launch_materialize(b.materialize_device.as<materialize_args_t>(), segments_count,
-                                           b.tasks_count, b.shape_table.as<shape_table_t>(), b.tasks.as<task_t>(),
-                                           b.bin_of_task.as<uint32_t>(), b.histogram.as<uint32_t>(),
-                                           b.segments.as<segment_view_t>(), b.stats.as<prepare_stats_t>(),
-                                           b.segment_overflow.as<uint32_t>(), stream),
-                "materialization");
+    if (!reuse) {
+        uint32_t const segments_count = (uint32_t)b.inputs.size();
+        SZS_TRY_HIP(szs_amd_launch_fill(b.histogram.as<uint32_t>(), bins_count_k, 0, stream), "fill");
+        SZS_TRY_HIP(szs_amd_launch_fill(b.stats.as<uint32_t>(), sizeof(prepare_stats_t) / 4, 0, stream), "fill");
+        SZS_TRY_HIP(szs_amd_launch_fill(b.segment_overflow.as<uint32_t>(), segments_count, 0, stream), "fill");
+        SZS_TRY_HIP(szs_amd_launch_utf8_decode(b.decode_tapes.as<utf8_tape_t>(), b.decode_tapes_count,
+                                               b.decode_strings, b.stats.as<prepare_stats_t>(),
+                                               b.segment_overflow.as<uint32_t>(), stream),
+                    "UTF-8 decoding");
+        SZS_TRY_HIP(szs_amd_launch_materialize(b.materialize_device.as<materialize_args_t>(), segments_count,
+                                               b.tasks_count, b.shape_table.as<shape_table_t>(), b.tasks.as<task_t>(),
+                                               b.bin_of_task.as<uint32_t>(), b.histogram.as<uint32_t>(),
+                                               b.segments.as<segment_view_t>(), b.stats.as<prepare_stats_t>(),
+                                               b.segment_overflow.as<uint32_t>(), stream),
+                    "materialization");
+    }
     if (bin) {
         bins_view_t const bins = bins_view(b);
         SZS_TRY_HIP(szs_amd_launch_scan_bins(&bins, stream), "bin scan");
@@ -691,6 +695,7 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
         }
     }
     b.cells = stats.cells;
+    b.core_ops = stats.core_ops_64 / 64;
     for (uint64_t t = 0; t < tasks.size(); ++t) {
         if (bins[t] == ~0u) continue;
         b.per_pair_tasks.push_back((uint32_t)t);
@@ -738,6 +743,7 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
         b.graph = graph;
         SZS_TRY_HIP(hipGraphInstantiate(&b.graph_exec, graph, nullptr, nullptr, 0), "graph instantiation");
     }
+    b.materialized = mode != szs_batch_graph_k;
     return ok(error_message);
 }
 
@@ -751,7 +757,9 @@ sz_status_t batch_launch(batch_t *batch, char const **error_message) {
         SZS_TRY_HIP(hipEventRecord(b.kernel_end, b.stream), "event");
     }
     else {
-        sz_status_t status = enqueue_front(b, b.mode != szs_batch_per_pair_k, error_message);
+        // The first launch after staging scores what the staging pass materialized; later ones redo it all.
+        sz_status_t status = enqueue_front(b, b.mode != szs_batch_per_pair_k, error_message, b.materialized);
+        b.materialized = false;
         if (status != sz_success_k) return status;
         status = enqueue_scoring(b, true, error_message);
         if (status != sz_success_k) return status;

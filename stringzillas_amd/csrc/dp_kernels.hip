@@ -157,17 +157,23 @@ __device__ void run_independent(item_args_t const &a) {
     if constexpr (sweep_type_::reduce_k != reduce_none_k) sweep.write_result(sweep.partial() + tail_sum);
 }
 
-/** Upper bound on polls of a predecessor's progress (each a device-scope load plus a ~128-cycle sleep): seconds,
- *  far beyond any real wait, so a kernel always ends even if a predecessor never published. */
-constexpr uint32_t chain_poll_limit_k = 1u << 22;
+/** Polls of a predecessor's progress (each a device-scope load plus a ~128-cycle sleep) a group may make on top of
+ *  the work it waits for: seconds, far beyond any real wait, so a kernel always ends even if a predecessor never
+ *  published. */
+constexpr uint64_t chain_poll_floor_k = 1ull << 22;
+/** Polls allowed per 64-column chunk of one stripe of predecessor work: ~14 us of sleeping, several times what a
+ *  chunk takes on a fully loaded CU. */
+constexpr uint64_t chain_polls_per_chunk_k = 256;
 
 /** Waits until the group before this one has published column `needed` of its bottom row. Returns false, and
- *  raises the batch's timeout word, if the wait was given up: the caller then waits no more for this pair. */
-__device__ __forceinline__ bool wait_for_columns(uint32_t const *progress, uint32_t needed, uint32_t *timeout) {
-    for (uint32_t polls = 0;; ++polls) {
+ *  raises the batch's timeout word, if the wait was given up (after `limit` polls): the caller then waits no more
+ *  for this pair. */
+__device__ __forceinline__ bool wait_for_columns(uint32_t const *progress, uint32_t needed, uint64_t limit,
+                                                 uint32_t *timeout) {
+    for (uint64_t polls = 0;; ++polls) {
         uint32_t const published = __hip_atomic_load(progress, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         if (published >= needed) return true;
-        if (polls == chain_poll_limit_k) {
+        if (polls == limit) {
             if (__lane_id() == 0) atomicOr(timeout, 1u);
             return false;
         }
@@ -236,6 +242,10 @@ __device__ void run_cooperative(item_args_t const &a) {
             if (end_stripe < stripes) chain_out = base + (size_t)group * pitch, progress_out = a.chain_progress + slot * chain_groups_k + group;
         }
         int32_t const phases = (local_stripes - 1) * delay + chunks;
+        // A group may wait, at worst, for all the stripes of the groups before it: a pair of more than 4 x 64
+        // stripes puts more stripes in each group, which runs them W at a time, so the bound grows with them.
+        uint64_t const poll_limit = chain_poll_floor_k + chain_polls_per_chunk_k * (uint64_t)group *
+                                                             (uint64_t)per_group * (uint64_t)chunks;
         for (int32_t phase = 0; phase < phases; ++phase) {
             int32_t local = -1, chunk = 0;
             if (phase >= wave * delay) {
@@ -259,7 +269,8 @@ __device__ void run_cooperative(item_args_t const &a) {
                 size_t const out_gap_offset = local + 1 < local_stripes ? a.scratch_capacity : a.chain_pitch;
                 int32_t const *in_gaps = in_row ? in_row + in_gap_offset : nullptr;
                 if (chained_ && !local && chain_in && waiting)
-                    waiting = wait_for_columns(progress_in, (uint32_t)min(chunk * 64 + 64, columns), a.chain_timeout);
+                    waiting = wait_for_columns(progress_in, (uint32_t)min(chunk * 64 + 64, columns), poll_limit,
+                                               a.chain_timeout);
                 bool const steady = is_steady<sweep_type_>(chunk, columns);
                 if (out_row) {
                     int32_t *out_gaps = out_row + out_gap_offset;
@@ -763,6 +774,42 @@ __global__ void widen_offsets_kernel(uint32_t const *narrow, uint64_t *wide, uin
         wide[i] = narrow[i];
 }
 
+/**
+ *  Adds one to `counters[key]` for every active lane with `valid`, issuing one atomic per distinct key of the wave
+ *  (the pairs of a batch often share a few bins, and one atomic per pair on one counter serializes in L2). Returns
+ *  the value the counter had before this lane's increment, as a plain per-lane atomicAdd would (in lane order
+ *  within the wave). Any subset of lanes may call it, as long as every active lane does.
+ */
+__device__ __forceinline__ uint32_t wave_counted_add(uint32_t *counters, uint32_t key, bool valid) {
+    uint32_t const lane = __lane_id();
+    uint64_t pending = __ballot(valid);
+    uint32_t mine = 0;
+    while (pending) { // one pass per distinct key, uniform over the active lanes
+        int const leader = __builtin_ctzll(pending);
+        uint32_t const leader_key = (uint32_t)__shfl((int)key, leader, 64);
+        uint64_t const same = __ballot(((pending >> lane) & 1ull) && key == leader_key);
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&counters[leader_key], (uint32_t)__popcll(same));
+        base = (uint32_t)__shfl((int)base, leader, 64);
+        if ((same >> lane) & 1ull)
+            mine = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0));
+        pending &= ~same;
+    }
+    return mine;
+}
+
+template <typename value_type_>
+__device__ __forceinline__ value_type_ wave_sum(value_type_ x) {
+    for (int offset = 32; offset; offset >>= 1) x += __shfl_xor(x, offset, 64);
+    return x;
+}
+
+template <typename value_type_>
+__device__ __forceinline__ value_type_ wave_max(value_type_ x) {
+    for (int offset = 32; offset; offset >>= 1) x = max(x, __shfl_xor(x, offset, 64));
+    return x;
+}
+
 /** The cost bucket of `x`: its octave and the bits after the leading one (`bucket_steps_per_octave_k` steps). */
 __device__ __forceinline__ uint32_t cost_bucket(uint64_t x) {
     constexpr uint32_t steps = bucket_steps_per_octave_k;
@@ -786,11 +833,11 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
                                    shape_table_t const *table, task_t *tasks, uint32_t *bin_of_task,
                                    uint32_t *histogram, segment_view_t const *segments, prepare_stats_t *stats,
                                    uint32_t *segment_overflow) {
-    __shared__ unsigned long long block_cells;
+    __shared__ unsigned long long block_cells, block_core_ops;
     __shared__ uint32_t block_widest, block_overflow;
-    if (threadIdx.x == 0) block_cells = 0, block_widest = 0, block_overflow = 0;
+    if (threadIdx.x == 0) block_cells = 0, block_core_ops = 0, block_widest = 0, block_overflow = 0;
     __syncthreads();
-    unsigned long long cells = 0;
+    unsigned long long cells = 0, core_ops = 0;
     uint32_t widest = 0, overflow = 0;
     for (uint64_t global = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; global < tasks_total;
          global += (uint64_t)gridDim.x * blockDim.x) {
@@ -916,19 +963,27 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
                 task.flags |= (chain_slot + 1u) << 8;
             }
             bin = (uint32_t)variant_of(family, symbols, shape) * column_buckets_k + cost_bucket(work);
-            atomicAdd(&histogram[bin], 1u);
             cells += (unsigned long long)rows * task.columns_length;
+            core_ops += (unsigned long long)rows * task.columns_length * core_ops_per_64_cells(family, symbols, sh);
             if (rows > stripe && task.columns_length > widest) widest = task.columns_length;
         }
+        (void)wave_counted_add(histogram, bin, bin != ~0u);
         tasks[global] = task;
         bin_of_task[global] = bin;
     }
-    if (cells) atomicAdd(&block_cells, cells);
-    if (widest) atomicMax(&block_widest, widest);
-    if (overflow) atomicOr(&block_overflow, overflow);
+    // Every lane is active again here: one LDS atomic per wave.
+    cells = wave_sum(cells), core_ops = wave_sum(core_ops);
+    widest = wave_max(widest), overflow = wave_max(overflow);
+    if (__lane_id() == 0) {
+        if (cells) atomicAdd(&block_cells, cells);
+        if (core_ops) atomicAdd(&block_core_ops, core_ops);
+        if (widest) atomicMax(&block_widest, widest);
+        if (overflow) atomicOr(&block_overflow, overflow);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         if (block_cells) atomicAdd(&stats->cells, block_cells);
+        if (block_core_ops) atomicAdd(&stats->core_ops_64, block_core_ops);
         if (block_widest) atomicMax(&stats->widest_multi_pass, block_widest);
         if (block_overflow) atomicOr(&stats->overflow, block_overflow);
     }
@@ -978,9 +1033,10 @@ __global__ void scatter_bins_kernel(bins_view_t bins, uint32_t const *bin_of_tas
                                     uint32_t tasks_count) {
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tasks_count; t += gridDim.x * blockDim.x) {
         uint32_t const bin = bin_of_task[t];
-        if (bin == ~0u) continue;
-        uint32_t const slot = bins.task_start[position_of_bin[bin]] + atomicAdd(&bins.cursor[bin], 1u);
-        bins.order[slot] = t;
+        bool const valid = bin != ~0u;
+        uint32_t const rank = wave_counted_add(bins.cursor, bin, valid); // lanes of the wave take adjacent slots
+        if (!valid) continue;
+        bins.order[bins.task_start[position_of_bin[bin]] + rank] = t;
         bins.shape_of_task[t] = (bin / column_buckets_k) % shapes_count_k;
     }
 }

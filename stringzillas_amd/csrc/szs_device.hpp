@@ -273,9 +273,40 @@ struct utf8_tape_t {
     uint32_t reserved;
 };
 
+/**
+ *  Core recurrence work of one cell in the tier that scores it, in 32-bit VALU lane-ops per 64 cells: the ops of
+ *  the cell update as each tier formulates it, without data movement (DPP hand-offs, LDS reads, addressing, loop
+ *  control), so a launch issues at least this many. The roofline's algorithmic work is the cell count weighted by
+ *  these figures:
+ *  - bit-parallel (Hyyro's step on one word per lane, three-input terms as `v_bitop3`): 25 ops per 64-row word
+ *    step on 32-bit halves, 15 per 32-row word; over runes the match word adds one `v_bfe` and one `v_bitop3` per
+ *    half for each of at least 8 bit-planes (the smallest plane count compiled);
+ *  - packed 16-bit NW: 6 packed ops per two cells (affine: perm, add, two max, three-way maximum, add), 3 for
+ *    linear gaps (perm, add, three-way maximum);
+ *  - int32 DP in the anti-diagonal domain (global families): linear 2 (add, max3), affine 5 (two max, add, max3,
+ *    add); uniform costs add a compare and a select (Levenshtein 4 and 7); local families 4 (linear) and 8 (affine).
+ */
+__host__ __device__ constexpr uint32_t core_ops_per_64_cells(int family, int symbols, shape_t shape) {
+    bool const affine = family == family_lev_affine_k || family == family_nw_affine_k || family == family_sw_affine_k;
+    if (shape.bit_parallel) {
+        bool const runes = symbols == symbols_u32_k;
+        return shape.rows_per_lane == 64 ? (runes ? 25 + 3 * 8 : 25) : (runes ? 2 * (15 + 2 * 8) : 2 * 15);
+    }
+    if (shape.packed) return affine ? 192 : 96;
+    switch (family) {
+    case family_lev_linear_k: return 4 * 64;
+    case family_lev_affine_k: return 7 * 64;
+    case family_nw_linear_k: return 2 * 64;
+    case family_nw_affine_k: return 5 * 64;
+    case family_sw_linear_k: return 4 * 64;
+    default: return 8 * 64;
+    }
+}
+
 /** What transcoding and materialization report back to the host: a few bytes instead of the task list. */
 struct prepare_stats_t {
     unsigned long long cells;            /**< sum of n*m over non-empty tasks */
+    unsigned long long core_ops_64;      /**< sum of n*m x `core_ops_per_64_cells` of each task's tier */
     uint32_t widest_multi_pass;          /**< most columns of a pair that needs several stripes */
     uint32_t overflow;                   /**< nonzero if some pair's scores could leave int32 */
     uint32_t bad_tape;                   /**< nonzero if some tape's offsets decrease or run past its byte count */

@@ -182,6 +182,8 @@ struct batch_t {
     std::vector<staged_tape_t *> spare_tapes;  /**< owned; staged by an earlier call, reused by the next one */
     std::vector<device_buffer_t *> spare_results; /**< owned; result buffers of an earlier call */
     bool static_tables_ready = false;          /**< the bin priority tables are already in HBM */
+    /** The staging pass's runes, tasks and bin histogram are on the device and no launch has used them yet. */
+    bool materialized = false;
     std::vector<void *> host_results;          /**< per segment: the caller's host pointer, or null if device */
     std::vector<uint64_t> results_elements;    /**< per segment: elements spanned by the result matrix */
     std::vector<uint64_t> result_rows, result_columns;
@@ -203,6 +205,7 @@ struct batch_t {
     size_t pinned_bytes = 0;                   /**< bytes of results staged through `pinned_results` */
     uint64_t tasks_count = 0;
     uint64_t cells = 0;                        /**< sum of n*m over non-empty tasks */
+    uint64_t core_ops = 0;                     /**< core recurrence lane-ops of those cells (`core_ops_per_64_cells`) */
     uint32_t scratch_capacity = 0;
     uint32_t waves = 0;
     uint32_t lds_bytes = 0;

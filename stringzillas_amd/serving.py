@@ -106,7 +106,8 @@ def run_server(templates, engines, qps: float, duration: float, hbm_budget: int 
                inflight: int = 2, seed: int = 1, drain_timeout: float = 120.0):
     """Continuous batching: a submitter thread follows the arrival schedule; the main thread joins tickets in
     arrival order, blocking in the library (the interpreter lock is released), and takes each request's latency
-    from the completion time the server stamped on it."""
+    from the completion time the server stamped on it. Returns the summary and, for every completed request, its
+    template index and results."""
     rng = np.random.default_rng(seed)
     arrivals = _arrivals(rng, qps, duration)
     picks = rng.integers(0, len(templates), size=len(arrivals))
@@ -121,29 +122,29 @@ def run_server(templates, engines, qps: float, duration: float, hbm_budget: int 
                 time.sleep(delay)
             template = templates[pick]
             ticket = server.submit(engines[template.kind], template.firsts, template.seconds)
-            submitted.put((ticket, start + at, template))
+            submitted.put((ticket, start + at, int(pick)))
         submitted.put(None)
 
     submitter = threading.Thread(target=submit_all)
     submitter.start()
     latencies, cells, done, late = [], 0, 0, 0
     deadline = start + duration + drain_timeout
-    results = {}
+    results = []
     last_completion = start
     while True:
         item = submitted.get()
         if item is None:
             break
-        ticket, arrived, template = item
+        ticket, arrived, pick = item
         out, completed = server.wait_timed(ticket)
         last_completion = max(last_completion, completed)
         if completed > deadline:
             late += 1
             continue
         latencies.append(completed - arrived)
-        cells += template.cells
+        cells += templates[pick].cells
         done += 1
-        results.setdefault(id(template), out)
+        results.append((pick, out))
     submitter.join()
     stats = server.stats()
     server.close()
@@ -155,6 +156,20 @@ def run_server(templates, engines, qps: float, duration: float, hbm_budget: int 
     summary["prepare_s"] = round(stats["prepare_seconds"], 3)
     summary["device_s"] = round(stats["device_seconds"], 3)
     return summary, results
+
+
+def verify(templates, engines, results) -> dict:
+    """Scores every template used by `results` alone (one ``engine.pairs`` call each) and counts the served
+    requests whose results equal it: each pair is scored independently, so a request's results must not depend on
+    the round it joined or what else that round held."""
+    alone = {}
+    matched = 0
+    for pick, out in results:
+        if pick not in alone:
+            template = templates[pick]
+            alone[pick] = engines[template.kind].pairs(template.firsts, template.seconds)
+        matched += int(np.array_equal(out, alone[pick]))
+    return {"verified": len(results), "equal_to_alone": matched, "distinct_requests": len(alone)}
 
 
 def run_baseline(templates, engines, qps: float, duration: float, seed: int = 1, drain_timeout: float = 120.0):
@@ -210,8 +225,10 @@ def main(argv=None) -> None:
     report = {"templates": len(templates), "mean_cells_per_request": mean_cells,
               "tapes": "host" if args.host_tapes else "device", "runs": []}
     for qps in args.qps:
-        summary, _ = run_server(templates, engines, qps, args.seconds, hbm_budget=int(args.budget_gib * (1 << 30)),
-                                inflight=args.inflight, drain_timeout=args.drain)
+        summary, results = run_server(templates, engines, qps, args.seconds,
+                                      hbm_budget=int(args.budget_gib * (1 << 30)), inflight=args.inflight,
+                                      drain_timeout=args.drain)
+        summary.update(verify(templates, engines, results))
         log(f"server at {qps:g} req/s: {summary}")
         entry = {"offered_qps": qps, "server": summary}
         if not args.skip_baseline:

@@ -220,3 +220,25 @@ def test_chained_levenshtein_dp_and_batch_modes():
                 np.testing.assert_array_equal(got.astype(np.int64), expected, err_msg=f"mode {mode} replay {replay}")
     finally:
         autotune.set_ladder(slot, *saved)
+
+
+def test_chained_pairs_of_more_than_256_stripes(packed_ladders):
+    """A pair over 4 x 64 stripes (131,072 rows) puts more than four stripes in each of its chained groups, which a
+    group then sweeps four at a time, so each group waits about as long as all the groups before it take; the
+    wait bound grows with that work, and the scores equal the closed form and the one-workgroup sweep."""
+    rng = np.random.default_rng(140000)
+    engine, params = _engine("nw_affine", rng)
+    text = random_bytes(rng, PROTEIN, 140_000, 140_000)
+    other = _mutated(rng, text, 0.05, 0.01)
+    assert min(len(text), len(other)) > 4 * 64 * 512
+    # NW(a, a) = sum of C[a_i][a_i] when no off-diagonal cost reaches a diagonal one and gaps cost (as here).
+    costs, classes = params.class_costs.astype(np.int64), params.byte_to_class
+    codes = classes[np.frombuffer(text, dtype=np.uint8)]
+    self_score = int(costs[codes, codes].sum())
+    packed_ladders("chained")
+    chained = engine.pairs(szs.Strs([text, text, other]), szs.Strs([text, other, text])).astype(np.int64)
+    assert chained[0] == self_score
+    assert chained[1] == chained[2]  # a symmetric matrix scores both orders alike
+    packed_ladders("cooperative")  # the same pairs on one workgroup, no chain
+    single = engine.pairs(szs.Strs([text, text]), szs.Strs([text, other])).astype(np.int64)
+    np.testing.assert_array_equal(chained[:2], single)

@@ -199,3 +199,44 @@ def test_server_rounds_with_chained_pairs():
             np.testing.assert_array_equal(out, nw.pairs(firsts, seconds))
     finally:
         server.close()
+
+
+def test_serving_workload_shape_matches_each_request_alone_and_the_oracle():
+    """The serving workload's own shape: Poisson arrivals at 100 req/s for 2.5 s, each request 1..4,096 pairs
+    (log-uniform) of one kind drawn from the mixed batch's mix (50% byte Levenshtein over ACGT, 25% UTF-8
+    Levenshtein, 25% affine NW) with both lengths log-uniform in [16, 16384], tapes resident in HBM. Every
+    completed request must equal the same request scored alone, and a sample holding each kind's longest served
+    pair must equal the oracle."""
+    from stringzillas_amd import serving as S
+    from stringzillas_amd import workloads as W
+
+    templates = S.make_templates(count=250, seed=0x535A0044)
+    engines = S.engines_by_kind()
+    summary, results = S.run_server(templates, engines, qps=100.0, duration=2.5, hbm_budget=16 << 30,
+                                    drain_timeout=60.0)
+    assert summary["dropped_after_drain_timeout"] == 0 and summary["completed"] == len(results) > 150
+    assert summary["largest_round"] > 1  # requests shared rounds
+    check = S.verify(templates, engines, results)
+    assert check["equal_to_alone"] == check["verified"] == len(results)
+    b2c, costs = W.protein_matrix(np.random.default_rng(W.SEED_BASE + 1))
+    params = {"levenshtein": O.Params("levenshtein"), "levenshtein_utf8": O.Params("levenshtein_utf8"),
+              "needleman_wunsch": O.Params("needleman_wunsch", open=-11, extend=-1, byte_to_class=b2c,
+                                           class_costs=costs)}
+    rng = np.random.default_rng(4)
+    served = {}
+    for pick, out in results:
+        served.setdefault(pick, out)
+    for kind in params:
+        picks = [p for p in served if templates[p].kind == kind]
+        assert picks, f"no {kind} request was served"
+        sample = []  # (pick, pair index): the kind's longest pair, then random pairs
+        for p in picks:
+            host = templates[p].firsts.host, templates[p].seconds.host
+            sizes = np.diff(host[0].offsets.astype(np.int64)) * np.diff(host[1].offsets.astype(np.int64))
+            sample.append((int(sizes.max()), p, int(sizes.argmax())))
+        sample = [max(sample)[1:]] + [(p, int(rng.integers(0, len(templates[p].firsts))))
+                                      for p in rng.choice(picks, size=min(8, len(picks)), replace=False)]
+        for p, i in sample:
+            firsts, seconds = templates[p].firsts.host, templates[p].seconds.host
+            want = O.score(params[kind], firsts[i], seconds[i])
+            assert int(served[p][i]) == want, f"{kind} request {p} pair {i}"
