@@ -68,7 +68,7 @@ sz_status_t run_segments(szs_device_scope_t scope, std::vector<segment_input_t> 
         if (!s->workspace) s->workspace = new batch_t();
         batch_t &batch = *s->workspace;
         sz_status_t status = batch_stage(batch, s, inputs, szs_batch_persistent_k, error_message);
-        if (status == sz_success_k) status = batch_launch(&batch, error_message);
+        if (status == sz_success_k) status = batch_launch(&batch, error_message, true);
         if (status == sz_success_k) status = batch_wait(&batch, error_message);
         if (status != sz_success_k) (void)hipStreamSynchronize(batch.stream); // nothing of this call is in flight
         return status;

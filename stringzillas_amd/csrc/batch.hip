@@ -21,6 +21,7 @@
 
 extern "C" {
 hipError_t szs_amd_launch_fill(uint32_t *data, uint32_t count, uint32_t value, hipStream_t stream);
+hipError_t szs_amd_launch_fill_many(szs_amd::fill_list_t const *list, hipStream_t stream);
 hipError_t szs_amd_launch_utf8_decode(szs_amd::utf8_tape_t const *tapes, uint32_t tapes_count, uint64_t strings_total,
                                       szs_amd::prepare_stats_t *stats, uint32_t *segment_flags, hipStream_t stream);
 hipError_t szs_amd_launch_widen_offsets(uint32_t const *narrow, uint64_t *wide, uint64_t count, hipStream_t stream);
@@ -106,6 +107,12 @@ static shape_table_t default_shape_table() {
     return table;
 }
 
+/** Guards the active shape table between tuning calls and the batches that copy it. */
+static std::mutex &tuning_mutex() {
+    static std::mutex mutex;
+    return mutex;
+}
+
 static shape_table_t &active_shape_table() {
     static shape_table_t table = default_shape_table();
     return table;
@@ -156,13 +163,16 @@ static uint32_t wave_bins_from() {
 
 #pragma region Tape staging
 
-/** `bytes_hint` is `offsets[count]` when the caller knows it (0: read it back from device offsets). */
+/** `bytes_hint` is `offsets[count]` when the caller knows it (0: read it back from device offsets, which only a
+ *  UTF-8 tape (sizing its runes) or host data behind device offsets needs; otherwise the kernels read it). */
 template <typename offset_type_>
 static sz_status_t stage_tape(sz_cptr_t data, offset_type_ const *offsets, uint64_t count, bool runes,
                               uint64_t bytes_hint, staged_tape_t &tape, hipStream_t stream,
                               char const **error_message) {
     tape.count = count;
     bool const device_offsets = is_device_pointer(offsets);
+    bool const device_data = is_device_pointer(data);
+    bool const need_bytes = runes || !device_data;
     // Offsets as u64 on the device.
     if (count == 0 || !offsets) {
         SZS_TRY_HIP(tape.offsets_copy.reserve(16), "Failed to allocate tape offsets");
@@ -172,8 +182,8 @@ static sz_status_t stage_tape(sz_cptr_t data, offset_type_ const *offsets, uint6
     }
     else if (sizeof(offset_type_) == 8 && device_offsets) {
         tape.offsets = reinterpret_cast<uint64_t const *>(offsets);
-        uint64_t last = bytes_hint;
-        if (!last) {
+        uint64_t last = bytes_hint ? bytes_hint : unknown_bytes_k;
+        if (!bytes_hint && need_bytes) {
             SZS_TRY_HIP(hipMemcpyAsync(&last, offsets + count, 8, hipMemcpyDeviceToHost, stream), "offset readback");
             SZS_TRY_HIP(hipStreamSynchronize(stream), "offset readback");
         }
@@ -185,25 +195,29 @@ static sz_status_t stage_tape(sz_cptr_t data, offset_type_ const *offsets, uint6
                                                  tape.offsets_copy.as<uint64_t>(), count + 1, stream),
                     "offset widening");
         tape.offsets = tape.offsets_copy.as<uint64_t>();
-        uint32_t last = (uint32_t)bytes_hint;
-        if (!last) {
-            SZS_TRY_HIP(hipMemcpyAsync(&last, offsets + count, 4, hipMemcpyDeviceToHost, stream), "offset readback");
+        uint64_t last = bytes_hint ? bytes_hint : unknown_bytes_k;
+        if (!bytes_hint && need_bytes) {
+            uint32_t narrow = 0;
+            SZS_TRY_HIP(hipMemcpyAsync(&narrow, offsets + count, 4, hipMemcpyDeviceToHost, stream), "offset readback");
             SZS_TRY_HIP(hipStreamSynchronize(stream), "offset readback");
+            last = narrow;
         }
         tape.bytes = last;
     }
     else {
-        std::vector<uint64_t> wide(count + 1);
+        // Widened into the tape's own host vector, which outlives the copy (the tape is reused only after the
+        // batch's work has completed).
+        std::vector<uint64_t> &wide = tape.host_offsets;
+        wide.resize(count + 1);
         for (uint64_t i = 0; i <= count; ++i) wide[i] = (uint64_t)offsets[i]; // checked by `check_host_offsets`
         SZS_TRY_HIP(tape.offsets_copy.reserve((count + 1) * 8), "Failed to allocate tape offsets");
         SZS_TRY_HIP(hipMemcpyAsync(tape.offsets_copy.data, wide.data(), (count + 1) * 8, hipMemcpyHostToDevice,
                                    stream), "tape offsets upload");
-        SZS_TRY_HIP(hipStreamSynchronize(stream), "tape offsets upload"); // `wide` is about to go out of scope
         tape.offsets = tape.offsets_copy.as<uint64_t>();
         tape.bytes = wide[count];
     }
     // Data.
-    if (tape.bytes == 0 || is_device_pointer(data)) tape.data = reinterpret_cast<uint8_t const *>(data);
+    if (tape.bytes == 0 || device_data) tape.data = reinterpret_cast<uint8_t const *>(data);
     else {
         SZS_TRY_HIP(tape.data_copy.reserve(tape.bytes + 16), "Failed to allocate the tape in HBM");
         SZS_TRY_HIP(hipMemcpyAsync(tape.data_copy.data, data, tape.bytes, hipMemcpyHostToDevice, stream),
@@ -285,14 +299,17 @@ void batch_t::reset() {
     chained_pairs = 0, chain_pitch = 0, decode_tapes_count = 0, decode_strings = 0;
     pinned_results = nullptr, pinned_bytes = 0, tasks_count = 0, cells = 0, core_ops = 0;
     scratch_capacity = 0, waves = 0, lds_bytes = 0, last_kernel_ms = 0, materialized = false;
+    params_table = nullptr, segment_table = nullptr, materialize_table = nullptr, decode_table = nullptr;
+    shape_table = nullptr;
 }
 
 void batch_t::release_device_memory() {
     reset();
-    for (device_buffer_t *buffer : {&params, &segments, &shape_table, &tasks, &bin_of_task, &order, &shape_of_task,
-                                    &identity, &histogram, &cursor, &task_start, &item_start, &priority,
-                                    &position_of_bin, &queue_head, &scratch, &stats, &materialize_device,
-                                    &segment_overflow, &results_block, &decode_tapes, &chain_rows, &chain_progress})
+    for (device_buffer_t *buffer : {&tables, &tasks, &bin_of_task, &order, &shape_of_task, &identity, &histogram,
+                                    &cursor, &task_start, &item_start, &priority, &position_of_bin, &queue_head,
+                                    &scan_blocks,
+                                    &scratch, &stats, &segment_overflow, &results_block, &chain_rows,
+                                    &chain_progress})
         buffer->release();
     for (staged_tape_t *tape : spare_tapes)
         for (device_buffer_t *buffer : {&tape->data_copy, &tape->offsets_copy, &tape->runes_buffer,
@@ -337,6 +354,7 @@ static bins_view_t bins_view(batch_t &b) {
     bins.order = b.order.as<uint32_t>();
     bins.queue_head = b.queue_head.as<uint32_t>();
     bins.shape_of_task = b.shape_of_task.as<uint32_t>();
+    bins.scan_blocks = b.scan_blocks.as<uint32_t>();
     bins.wave_bins_from = wave_bins_from();
     return bins;
 }
@@ -345,21 +363,32 @@ static bins_view_t bins_view(batch_t &b) {
  *  runes, tasks and bin histogram that the staging's own pass left on the device are used as they are. */
 static sz_status_t enqueue_front(batch_t &b, bool bin, char const **error_message, bool reuse = false) {
     hipStream_t const stream = b.stream;
-    SZS_TRY_HIP(szs_amd_launch_fill(b.cursor.as<uint32_t>(), bins_count_k, 0, stream), "fill");
-    SZS_TRY_HIP(szs_amd_launch_fill(b.queue_head.as<uint32_t>(), 1, 0, stream), "fill");
+    uint32_t const segments_count = (uint32_t)b.inputs.size();
+    // Every counter the run starts from zero, in one launch: bin cursors, the work queue, chained pairs' progress
+    // words, and (when materializing) the histogram, the stats and the segment flags.
+    fill_list_t fills {};
+    auto fill = [&](device_buffer_t &buffer, uint32_t count) {
+        fills.data[fills.entries] = buffer.as<uint32_t>(), fills.count[fills.entries] = count;
+        fills.value[fills.entries++] = 0;
+    };
+    fill(b.cursor, bins_count_k);
+    fill(b.queue_head, 1);
+    if (b.chained_pairs) fill(b.chain_progress, b.chained_pairs * (uint32_t)chain_groups_k);
     if (!reuse) {
-        uint32_t const segments_count = (uint32_t)b.inputs.size();
-        SZS_TRY_HIP(szs_amd_launch_fill(b.histogram.as<uint32_t>(), bins_count_k, 0, stream), "fill");
-        SZS_TRY_HIP(szs_amd_launch_fill(b.stats.as<uint32_t>(), sizeof(prepare_stats_t) / 4, 0, stream), "fill");
-        SZS_TRY_HIP(szs_amd_launch_fill(b.segment_overflow.as<uint32_t>(), segments_count, 0, stream), "fill");
-        SZS_TRY_HIP(szs_amd_launch_utf8_decode(b.decode_tapes.as<utf8_tape_t>(), b.decode_tapes_count,
+        fill(b.histogram, bins_count_k);
+        fill(b.stats, sizeof(prepare_stats_t) / 4);
+        fill(b.segment_overflow, segments_count);
+    }
+    SZS_TRY_HIP(szs_amd_launch_fill_many(&fills, stream), "fill");
+    if (!reuse) {
+        SZS_TRY_HIP(szs_amd_launch_utf8_decode(b.decode_table, b.decode_tapes_count,
                                                b.decode_strings, b.stats.as<prepare_stats_t>(),
                                                b.segment_overflow.as<uint32_t>(), stream),
                     "UTF-8 decoding");
-        SZS_TRY_HIP(szs_amd_launch_materialize(b.materialize_device.as<materialize_args_t>(), segments_count,
-                                               b.tasks_count, b.shape_table.as<shape_table_t>(), b.tasks.as<task_t>(),
+        SZS_TRY_HIP(szs_amd_launch_materialize(b.materialize_table, segments_count,
+                                               b.tasks_count, b.shape_table, b.tasks.as<task_t>(),
                                                b.bin_of_task.as<uint32_t>(), b.histogram.as<uint32_t>(),
-                                               b.segments.as<segment_view_t>(), b.stats.as<prepare_stats_t>(),
+                                               b.segment_table, b.stats.as<prepare_stats_t>(),
                                                b.segment_overflow.as<uint32_t>(), stream),
                     "materialization");
     }
@@ -385,14 +414,11 @@ static chain_view_t chain_view(batch_t &b) {
 /** Scoring launches; HIP events bracket them unless they are being captured into a graph. */
 static sz_status_t enqueue_scoring(batch_t &b, bool timed, char const **error_message) {
     hipStream_t const stream = b.stream;
-    chain_view_t const chain = chain_view(b);
-    if (b.chained_pairs) // every group starts with nothing published by its predecessor
-        SZS_TRY_HIP(szs_amd_launch_fill(chain.progress, b.chained_pairs * (uint32_t)chain_groups_k, 0, stream),
-                    "fill");
+    chain_view_t const chain = chain_view(b); // progress words were cleared with the front's counters
     if (timed) SZS_TRY_HIP(hipEventRecord(b.kernel_begin, stream), "event");
     if (b.mode == szs_batch_per_pair_k) {
         for (size_t i = 0; i < b.per_pair_tasks.size(); ++i)
-            SZS_TRY_HIP(szs_amd_launch_single(b.tasks.as<task_t>(), b.segments.as<segment_view_t>(),
+            SZS_TRY_HIP(szs_amd_launch_single(b.tasks.as<task_t>(), b.segment_table,
                                               b.identity.as<uint32_t>(), b.per_pair_tasks[i], b.per_pair_variants[i],
                                               b.scratch.as<int32_t>(), b.scratch_capacity, b.lds_bytes, &chain,
                                               stream),
@@ -400,7 +426,7 @@ static sz_status_t enqueue_scoring(batch_t &b, bool timed, char const **error_me
     }
     else {
         bins_view_t const bins = bins_view(b);
-        SZS_TRY_HIP(szs_amd_launch_persistent(b.tasks.as<task_t>(), b.segments.as<segment_view_t>(), &bins,
+        SZS_TRY_HIP(szs_amd_launch_persistent(b.tasks.as<task_t>(), b.segment_table, &bins,
                                               b.scratch.as<int32_t>(), b.scratch_capacity, b.waves, b.lds_bytes,
                                               &chain, stream),
                     "persistent scoring launch");
@@ -596,18 +622,6 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
     lds_bytes = std::max<uint32_t>(lds_bytes, 512u);
     b.lds_bytes = ((lds_bytes + 15u) & ~15u) + inject_ring_bytes_k;
 
-    // Device-side tables, uploaded on the batch stream (the host vectors outlive the synchronization below).
-    size_t const table_count = std::max<size_t>(segments_count, 1);
-    SZS_TRY_HIP(b.params.reserve(sizeof(engine_params_t) * table_count), "allocation");
-    SZS_TRY_HIP(hipMemcpyAsync(b.params.data, params.data(), sizeof(engine_params_t) * segments_count,
-                               hipMemcpyHostToDevice, stream), "upload");
-    for (size_t s = 0; s < segments_count; ++s) views[s].params = b.params.as<engine_params_t>() + s;
-    SZS_TRY_HIP(b.segments.reserve(sizeof(segment_view_t) * table_count), "allocation");
-    SZS_TRY_HIP(hipMemcpyAsync(b.segments.data, views.data(), sizeof(segment_view_t) * segments_count,
-                               hipMemcpyHostToDevice, stream), "upload");
-    SZS_TRY_HIP(b.materialize_device.reserve(sizeof(materialize_args_t) * table_count), "allocation");
-    SZS_TRY_HIP(hipMemcpyAsync(b.materialize_device.data, args, sizeof(materialize_args_t) * segments_count,
-                               hipMemcpyHostToDevice, stream), "upload");
     // Every UTF-8 tape is transcoded by one launch.
     std::vector<utf8_tape_t> decode;
     for (auto *tape : b.tapes)
@@ -625,15 +639,35 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
             b.decode_strings += tape->count;
         }
     b.decode_tapes_count = (uint32_t)decode.size();
-    if (!decode.empty()) {
-        SZS_TRY_HIP(b.decode_tapes.reserve(sizeof(utf8_tape_t) * decode.size()), "allocation");
-        SZS_TRY_HIP(hipMemcpyAsync(b.decode_tapes.data, decode.data(), sizeof(utf8_tape_t) * decode.size(),
-                                   hipMemcpyHostToDevice, stream), "upload");
+    // The per-call device tables (engine parameters, segment views, materialization arguments, UTF-8 tapes, the
+    // shape table) go up in one copy from a host image the batch keeps until it is staged again.
+    size_t const table_count = std::max<size_t>(segments_count, 1);
+    auto aligned = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
+    size_t const at_params = 0;
+    size_t const at_views = aligned(at_params + sizeof(engine_params_t) * table_count);
+    size_t const at_args = aligned(at_views + sizeof(segment_view_t) * table_count);
+    size_t const at_decode = aligned(at_args + sizeof(materialize_args_t) * table_count);
+    size_t const at_shapes = aligned(at_decode + sizeof(utf8_tape_t) * std::max<size_t>(decode.size(), 1));
+    size_t const table_bytes = at_shapes + sizeof(shape_table_t);
+    SZS_TRY_HIP(b.tables.reserve(table_bytes), "allocation");
+    char *device_tables = b.tables.as<char>();
+    b.params_table = reinterpret_cast<engine_params_t *>(device_tables + at_params);
+    b.segment_table = reinterpret_cast<segment_view_t *>(device_tables + at_views);
+    b.materialize_table = reinterpret_cast<materialize_args_t *>(device_tables + at_args);
+    b.decode_table = reinterpret_cast<utf8_tape_t *>(device_tables + at_decode);
+    b.shape_table = reinterpret_cast<shape_table_t *>(device_tables + at_shapes);
+    for (size_t s = 0; s < segments_count; ++s) views[s].params = b.params_table + s;
+    b.table_image.assign(table_bytes, 0);
+    char *image = b.table_image.data();
+    std::memcpy(image + at_params, params.data(), sizeof(engine_params_t) * segments_count);
+    std::memcpy(image + at_views, views.data(), sizeof(segment_view_t) * segments_count);
+    std::memcpy(image + at_args, args, sizeof(materialize_args_t) * segments_count);
+    if (!decode.empty()) std::memcpy(image + at_decode, decode.data(), sizeof(utf8_tape_t) * decode.size());
+    {
+        std::lock_guard<std::mutex> lock(szs_amd::tuning_mutex());
+        std::memcpy(image + at_shapes, &active_shape_table(), sizeof(shape_table_t));
     }
-    shape_table_t const shape_table = active_shape_table();
-    SZS_TRY_HIP(b.shape_table.reserve(sizeof(shape_table_t)), "allocation");
-    SZS_TRY_HIP(hipMemcpyAsync(b.shape_table.data, &shape_table, sizeof(shape_table_t), hipMemcpyHostToDevice, stream),
-                "upload");
+    SZS_TRY_HIP(hipMemcpyAsync(device_tables, image, table_bytes, hipMemcpyHostToDevice, stream), "upload");
     if (!b.static_tables_ready) { // constant tables: once per batch object
         std::vector<uint32_t> const &priority = bin_priority();
         static std::vector<uint32_t> const position = [&] {
@@ -659,6 +693,7 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
     SZS_TRY_HIP(b.task_start.reserve(bins_count_k * 4), "allocation");
     SZS_TRY_HIP(b.item_start.reserve((bins_count_k + 1) * 4), "allocation");
     SZS_TRY_HIP(b.queue_head.reserve(64), "allocation");
+    SZS_TRY_HIP(b.scan_blocks.reserve(scan_blocks_k * 2 * 4), "allocation");
     SZS_TRY_HIP(b.stats.reserve(sizeof(prepare_stats_t)), "allocation");
     SZS_TRY_HIP(b.segment_overflow.reserve(table_count * 4), "allocation");
     if (!b.kernel_begin) SZS_TRY_HIP(hipEventCreate(&b.kernel_begin), "event");
@@ -747,10 +782,10 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
     return ok(error_message);
 }
 
-sz_status_t batch_launch(batch_t *batch, char const **error_message) {
+sz_status_t batch_launch(batch_t *batch, char const **error_message, bool ordered) {
     batch_t &b = *batch;
     SZS_TRY_HIP(hipSetDevice(b.device), "hipSetDevice failed");
-    SZS_TRY_HIP(order_after_default_stream(b.stream), "Ordering after the default stream");
+    if (!ordered) SZS_TRY_HIP(order_after_default_stream(b.stream), "Ordering after the default stream");
     if (b.mode == szs_batch_graph_k) { // the events then time the whole replay
         SZS_TRY_HIP(hipEventRecord(b.kernel_begin, b.stream), "event");
         SZS_TRY_HIP(hipGraphLaunch(b.graph_exec, b.stream), "graph launch");
@@ -792,12 +827,6 @@ sz_status_t batch_wait(batch_t *batch, char const **error_message) {
 
 #pragma region Tuning hooks
 
-namespace {
-std::mutex &tuning_mutex() {
-    static std::mutex mutex;
-    return mutex;
-}
-} // namespace
 
 extern "C" {
 
@@ -822,7 +851,7 @@ int szs_tuning_packed_rows(void const *engine) {
 
 sz_status_t szs_tuning_get_ladder(int slot, int *shapes, int *max_rows, int capacity, int *count, int *beyond_shape) {
     if (slot < 0 || slot >= szs_amd::shape_slots_k || !count || !beyond_shape) return sz_unexpected_dimensions_k;
-    std::lock_guard<std::mutex> lock(tuning_mutex());
+    std::lock_guard<std::mutex> lock(szs_amd::tuning_mutex());
     szs_amd::shape_table_t const &table = szs_amd::active_shape_table();
     *count = table.used[slot];
     *beyond_shape = table.multi_pass_shape[slot];
@@ -867,7 +896,7 @@ sz_status_t szs_tuning_set_ladder(int slot, int const *shapes, int const *max_ro
             if (shapes[j] == shapes[i]) return fail(sz_unexpected_dimensions_k, error_message, "Shapes must be distinct");
         previous = max_rows[i];
     }
-    std::lock_guard<std::mutex> lock(tuning_mutex());
+    std::lock_guard<std::mutex> lock(szs_amd::tuning_mutex());
     shape_table_t &table = active_shape_table();
     for (int shape = 0; shape < shapes_count_k; ++shape) table.limits[slot][shape] = -1;
     for (int i = 0; i < count; ++i) table.order[slot][i] = shapes[i], table.limits[slot][shapes[i]] = max_rows[i];

@@ -545,6 +545,13 @@ __global__ void fill_u32_kernel(uint32_t *data, uint32_t count, uint32_t value) 
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) data[i] = value;
 }
 
+/** Several fills in one launch (the counters a batch clears before each run). */
+__global__ void fill_many_kernel(fill_list_t list) {
+    for (uint32_t e = 0; e < list.entries; ++e)
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < list.count[e]; i += gridDim.x * blockDim.x)
+            list.data[e][i] = list.value[e];
+}
+
 /** Byte `k` (0..15) of a 16-byte block held in registers. */
 __device__ __forceinline__ uint32_t block_byte(uint4 block, uint32_t k) {
     uint32_t const word = (k & 8) ? ((k & 4) ? block.w : block.z) : ((k & 4) ? block.y : block.x);
@@ -864,10 +871,14 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
         uint32_t const *c_runes = a.candidate_offsets ? a.candidate_rune_counts : a.query_rune_counts;
         uint64_t const q_begin = a.query_offsets[q], c_begin = c_offsets[c];
         uint64_t const q_end = a.query_offsets[q + 1], c_end = c_offsets[c + 1];
-        uint64_t const c_bytes = a.candidate_offsets ? a.candidate_bytes : a.query_bytes;
+        // A byte count the host did not read back is the tape's last offset, read here.
+        uint64_t const q_bytes = a.query_bytes != unknown_bytes_k ? a.query_bytes : a.query_offsets[a.queries_count];
+        uint64_t const c_bytes = !a.candidate_offsets ? q_bytes
+                                 : a.candidate_bytes != unknown_bytes_k ? a.candidate_bytes
+                                                                        : a.candidate_offsets[a.candidates_count];
         // Offsets that decrease or run past the tape's data would make the sweeps read outside it: such a pair
         // is never scored and its segment fails.
-        bool const bad_tape = q_end < q_begin || q_end > a.query_bytes || c_end < c_begin || c_end > c_bytes;
+        bool const bad_tape = q_end < q_begin || q_end > q_bytes || c_end < c_begin || c_end > c_bytes;
         uint64_t q_length = bad_tape ? 0 : q_end - q_begin;
         uint64_t c_length = bad_tape ? 0 : c_end - c_begin;
         if (a.runes && !bad_tape) q_length = a.query_rune_counts[q], c_length = c_runes[c];
@@ -994,38 +1005,50 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
  *  first work item of each bin (an item is 64 / G pairs of the bin).
  */
 __global__ __launch_bounds__(1024) void scan_bins_kernel(bins_view_t bins) {
-    __shared__ uint32_t task_carry, item_carry;
-    __shared__ uint32_t partial_tasks[1024], partial_items[1024];
-    if (threadIdx.x == 0) task_carry = 0, item_carry = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < bins_count_k; base += 1024) {
-        uint32_t const p = base + threadIdx.x;
-        uint32_t tasks_here = 0, items_here = 0;
-        if (p < bins_count_k) {
-            uint32_t const bin = bins.priority[p];
-            tasks_here = bins.histogram[bin];
-            items_here = items_for(shapes_k[(bin / column_buckets_k) % shapes_count_k], tasks_here);
-        }
-        partial_tasks[threadIdx.x] = tasks_here;
-        partial_items[threadIdx.x] = items_here;
-        __syncthreads();
-        for (uint32_t offset = 1; offset < 1024; offset <<= 1) { // Hillis-Steele inclusive scan
-            uint32_t const add_tasks = threadIdx.x >= offset ? partial_tasks[threadIdx.x - offset] : 0;
-            uint32_t const add_items = threadIdx.x >= offset ? partial_items[threadIdx.x - offset] : 0;
-            __syncthreads();
-            partial_tasks[threadIdx.x] += add_tasks;
-            partial_items[threadIdx.x] += add_items;
-            __syncthreads();
-        }
-        if (p < bins_count_k) {
-            bins.task_start[p] = task_carry + partial_tasks[threadIdx.x] - tasks_here;
-            bins.item_start[p] = item_carry + partial_items[threadIdx.x] - items_here;
-        }
-        __syncthreads();
-        if (threadIdx.x == 1023) task_carry += partial_tasks[1023], item_carry += partial_items[1023];
-        __syncthreads();
+    // Pass 1 of 2: workgroup b scans priority positions [1024 b, 1024 b + 1024) on its own and leaves its totals in
+    // `scan_blocks`; pass 2 adds the totals of the workgroups before it. Every position has its own thread, so
+    // the gathers of the histogram run in parallel across the chip.
+    __shared__ uint32_t wave_tasks[16], wave_items[16];
+    uint32_t const p = blockIdx.x * 1024 + threadIdx.x;
+    uint32_t tasks = 0, items = 0;
+    if (p < bins_count_k) {
+        uint32_t const bin = bins.priority[p];
+        tasks = bins.histogram[bin];
+        items = items_for(shapes_k[(bin / column_buckets_k) % shapes_count_k], tasks);
     }
-    if (threadIdx.x == 0) bins.item_start[bins_count_k] = item_carry;
+    uint32_t const lane = __lane_id(), wave = threadIdx.x / 64;
+    uint32_t tasks_inclusive = tasks, items_inclusive = items;
+    for (int offset = 1; offset < 64; offset <<= 1) {
+        uint32_t const t = (uint32_t)__shfl_up((int)tasks_inclusive, offset, 64);
+        uint32_t const n = (uint32_t)__shfl_up((int)items_inclusive, offset, 64);
+        if ((int)lane >= offset) tasks_inclusive += t, items_inclusive += n;
+    }
+    if (lane == 63) wave_tasks[wave] = tasks_inclusive, wave_items[wave] = items_inclusive;
+    __syncthreads();
+    uint32_t tasks_before = tasks_inclusive - tasks, items_before = items_inclusive - items;
+    for (uint32_t w = 0; w < wave; ++w) tasks_before += wave_tasks[w], items_before += wave_items[w];
+    if (p < bins_count_k) bins.task_start[p] = tasks_before, bins.item_start[p] = items_before;
+    if (threadIdx.x == 1023) {
+        bins.scan_blocks[2 * blockIdx.x] = tasks_before + tasks;
+        bins.scan_blocks[2 * blockIdx.x + 1] = items_before + items;
+    }
+}
+
+/** Pass 2 of the bin scan: each position adds the totals of the workgroups of pass 1 before its own; the last
+ *  workgroup also writes the total item count. */
+__global__ __launch_bounds__(1024) void scan_bins_carry_kernel(bins_view_t bins) {
+    __shared__ uint32_t carry_tasks, carry_items;
+    if (threadIdx.x < 64) {
+        uint32_t t = 0, n = 0;
+        for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 64) t += bins.scan_blocks[2 * b], n += bins.scan_blocks[2 * b + 1];
+        t = wave_sum(t), n = wave_sum(n);
+        if (threadIdx.x == 0) carry_tasks = t, carry_items = n;
+    }
+    __syncthreads();
+    uint32_t const p = blockIdx.x * 1024 + threadIdx.x;
+    if (p < bins_count_k) bins.task_start[p] += carry_tasks, bins.item_start[p] += carry_items;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+        bins.item_start[bins_count_k] = carry_items + bins.scan_blocks[2 * blockIdx.x + 1];
 }
 
 /** Scatters task ids into `order`, grouped by bin (order inside a bin is arbitrary and does not affect results). */
@@ -1053,6 +1076,15 @@ hipError_t szs_amd_launch_fill(uint32_t *data, uint32_t count, uint32_t value, h
     if (!count) return hipSuccess;
     uint32_t const blocks = (count + 255) / 256 < 1024 ? (count + 255) / 256 : 1024;
     hipLaunchKernelGGL(szs_amd::fill_u32_kernel, dim3(blocks), dim3(256), 0, stream, data, count, value);
+    return hipGetLastError();
+}
+
+hipError_t szs_amd_launch_fill_many(szs_amd::fill_list_t const *list, hipStream_t stream) {
+    uint32_t most = 0;
+    for (uint32_t e = 0; e < list->entries; ++e) most = list->count[e] > most ? list->count[e] : most;
+    if (!most) return hipSuccess;
+    uint32_t const blocks = (most + 255) / 256 < 1024 ? (most + 255) / 256 : 1024;
+    hipLaunchKernelGGL(szs_amd::fill_many_kernel, dim3(blocks), dim3(256), 0, stream, *list);
     return hipGetLastError();
 }
 
@@ -1088,7 +1120,10 @@ hipError_t szs_amd_launch_materialize(szs_amd::materialize_args_t const *args, u
 }
 
 hipError_t szs_amd_launch_scan_bins(szs_amd::bins_view_t const *bins, hipStream_t stream) {
-    hipLaunchKernelGGL(szs_amd::scan_bins_kernel, dim3(1), dim3(1024), 0, stream, *bins);
+    constexpr uint32_t blocks = (szs_amd::bins_count_k + 1023) / 1024;
+    static_assert(blocks <= szs_amd::scan_blocks_k, "the scan's block totals fit their buffer");
+    hipLaunchKernelGGL(szs_amd::scan_bins_kernel, dim3(blocks), dim3(1024), 0, stream, *bins);
+    hipLaunchKernelGGL(szs_amd::scan_bins_carry_kernel, dim3(blocks), dim3(1024), 0, stream, *bins);
     return hipGetLastError();
 }
 

@@ -141,7 +141,7 @@ sz_status_t szs_batch_run_noalloc(szs_device_scope_t device, szs_pairs_segment_t
             return fail(sz_bad_alloc_k, error_message,
                         "The input needs more device memory than the allocation-free call's static scratch "
                         "(SZS_NOALLOC_SCRATCH_BYTES per GPU) holds");
-        if (status == sz_success_k) status = batch_launch(&batch, error_message);
+        if (status == sz_success_k) status = batch_launch(&batch, error_message, true);
         if (status == sz_success_k) status = batch_wait(&batch, error_message);
         if (status != sz_success_k) (void)hipStreamSynchronize(batch.stream);
         batch.reset(); // forget the caller's pointers; the buffers stay in the array until the next call

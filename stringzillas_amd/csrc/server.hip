@@ -231,7 +231,7 @@ void server_t::run() {
             sz_status_t const status = guarded(&error, [&]() {
                 sz_status_t prepared = batch_prepare(scope, inputs, szs_batch_persistent_k, &round.batch, &error,
                                                      options);
-                return prepared == sz_success_k ? batch_launch(round.batch, &error) : prepared;
+                return prepared == sz_success_k ? batch_launch(round.batch, &error, true) : prepared;
             });
             double const prepared = monotonic_seconds();
             lock.lock();

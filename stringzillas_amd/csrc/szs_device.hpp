@@ -214,10 +214,14 @@ struct bins_view_t {
     uint32_t *order;         /**< [tasks] task ids grouped by bin */
     uint32_t *queue_head;    /**< persistent-kernel work counter */
     uint32_t *shape_of_task; /**< [tasks] chosen shape id (kept for the per-pair mode) */
+    uint32_t *scan_blocks;   /**< [scan_blocks_k][tasks, items]: totals of the first scan pass's workgroups */
     /** Priority position of the first bin of one-wave items: the bins before it (chained, then cooperative) hold
      *  whole-workgroup items, so `item_start[wave_bins_from]` is where the waves start taking items alone. */
     uint32_t wave_bins_from;
 };
+
+/** Workgroups of 1,024 positions the bin scan uses at most. */
+constexpr uint32_t scan_blocks_k = 128;
 
 /** Shape choice per (family, symbols): single-pass shapes by ascending capacity, then the multi-pass shape. */
 struct shape_table_t {
@@ -235,8 +239,8 @@ struct materialize_args_t {
     uint32_t const *candidate_rune_counts;
     uint64_t queries_count;
     uint64_t candidates_count;
-    uint64_t query_bytes;                /**< bytes of the query tape's data: every string must end within it */
-    uint64_t candidate_bytes;            /**< the same for the candidates */
+    uint64_t query_bytes;                /**< bytes of the query tape's data (`unknown_bytes_k`: its last offset) */
+    uint64_t candidate_bytes;            /**< the same for the candidates; every string must end within them */
     uint64_t results_row_stride;
     uint64_t tasks_count;                /**< pairs, or cells of the (triangular) cross-product */
     uint32_t task_base;
@@ -314,6 +318,17 @@ struct prepare_stats_t {
     uint32_t chained_columns;            /**< most columns of such a pair */
     uint32_t chain_timeout;              /**< nonzero if a chained group gave up waiting for its predecessor */
     uint32_t reserved;
+};
+
+/** A tape byte count the host did not read back from HBM: kernels that check offsets read `offsets[count]`. */
+constexpr uint64_t unknown_bytes_k = ~0ull;
+
+/** Up to 8 fills of u32 counters in one launch. */
+struct fill_list_t {
+    uint32_t *data[8];
+    uint32_t count[8];
+    uint32_t value[8];
+    uint32_t entries;
 };
 
 /** Per-segment flags raised on the device (the batch fails, or only that segment's request when serving). */

@@ -124,11 +124,12 @@ struct staged_tape_t {
     uint8_t const *data = nullptr;
     uint64_t const *offsets = nullptr;
     uint64_t count = 0;
-    uint64_t bytes = 0;             /**< offsets[count] (or the caller's stated byte count) */
+    uint64_t bytes = 0;             /**< offsets[count], the caller's stated count, or `unknown_bytes_k` */
     uint32_t segment = 0;           /**< the batch segment this tape belongs to */
     uint32_t *runes = nullptr;      /**< UTF-8 only: u32[bytes] */
     uint32_t *rune_counts = nullptr;
     device_buffer_t data_copy, offsets_copy, runes_buffer, rune_counts_buffer;
+    std::vector<uint64_t> host_offsets; /**< host offsets widened for upload, kept until the tape is staged again */
     /** Forgets the staged view (keeping the buffers' capacity) so the tape can be staged again. */
     void reset_view() {
         data = nullptr, offsets = nullptr, count = 0, bytes = 0, segment = 0, runes = nullptr, rune_counts = nullptr;
@@ -190,9 +191,17 @@ struct batch_t {
     std::vector<int64_t *> device_results;     /**< per segment: where the kernels write */
     std::vector<device_buffer_t *> result_buffers;
 
-    device_buffer_t params, segments, shape_table, tasks, bin_of_task, order, shape_of_task, identity;
+    /** One device block for the per-call tables, uploaded from `table_image` in one copy. */
+    device_buffer_t tables;
+    std::vector<char> table_image;
+    engine_params_t *params_table = nullptr;       /**< per segment */
+    segment_view_t *segment_table = nullptr;       /**< per segment */
+    materialize_args_t *materialize_table = nullptr; /**< per segment */
+    utf8_tape_t *decode_table = nullptr;           /**< per UTF-8 tape */
+    shape_table_t *shape_table = nullptr;          /**< the tuned shape ladders, as of staging */
+    device_buffer_t tasks, bin_of_task, order, shape_of_task, identity;
     device_buffer_t histogram, cursor, task_start, item_start, priority, position_of_bin, queue_head, scratch, stats;
-    device_buffer_t materialize_device, segment_overflow, results_block, decode_tapes;
+    device_buffer_t segment_overflow, results_block, scan_blocks;
     device_buffer_t chain_rows, chain_progress;  /**< chained pairs' bottom rows between groups, and progress */
     uint32_t chained_pairs = 0;                /**< pairs scored by a chained shape (slots of the chain buffers) */
     uint32_t chain_pitch = 0;
@@ -233,7 +242,9 @@ sz_status_t batch_prepare(device_scope_t *scope, std::vector<segment_input_t> co
 /** The same into an existing batch, reusing its device buffers (grown where the new inputs need more). */
 sz_status_t batch_stage(batch_t &batch, device_scope_t *scope, std::vector<segment_input_t> const &inputs,
                         szs_batch_mode_t mode, char const **error_message, batch_options_t const &options = {});
-sz_status_t batch_launch(batch_t *batch, char const **error_message);
+/** Queues a staged batch. `ordered`: the caller staged it just now, with no device work of its own in between,
+ *  so the staging's ordering after the default stream still holds. */
+sz_status_t batch_launch(batch_t *batch, char const **error_message, bool ordered = false);
 sz_status_t batch_wait(batch_t *batch, char const **error_message);
 
 } // namespace szs_amd
