@@ -457,8 +457,10 @@ void szs_tape_free(szs_device_tape_t tape);
 /**
  *  Tuning (no reference counterpart): the kernel shape chosen for a pair depends on its rows (the shorter side)
  *  through a per-slot ladder of (shape, most rows) steps plus a shape for anything taller. Slots are
- *  `family * 2 + symbols` for the six recurrence families x {bytes, runes}, then the unit-cost Levenshtein
- *  over bytes (12) and over runes (13), the only slots whose ladders may use bit-parallel shapes.
+ *  `family * 2 + symbols` for the six recurrence families x {bytes, runes} (the class-cost families score bytes
+ *  only: their second slot holds engines whose costs minus the gap bias leave int8, which run the plain
+ *  recurrences), then the unit-cost Levenshtein over bytes (12) and over runes (13), the only slots whose ladders
+ *  may use bit-parallel shapes.
  *  An autotuner measures alternatives and installs the fastest ladder; changes apply to batches prepared later.
  */
 int szs_tuning_slots(void);
@@ -471,6 +473,9 @@ int szs_tuning_shapes(void);
  */
 sz_status_t szs_tuning_shape(int index, int *group, int *rows_per_lane, int *cooperative, int *bit_parallel,
                              int *packed, int *chained);
+/** 1 if shape `index` keeps int64 cells (the shape pairs beyond the int32 range bound take; a ladder may also
+ *  list it for any slot), else 0. */
+int szs_tuning_shape_wide(int index);
 /** Most rows per lane of a packed 16-bit shape an engine handle admits: 8, 6, 4, 2, or 0 when scores could leave
  *  the 16-bit window (the engine then runs int32 cells everywhere). */
 int szs_tuning_packed_rows(void const *engine);

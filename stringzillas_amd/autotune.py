@@ -46,6 +46,11 @@ def shapes() -> List[Tuple[int, int, int, int, int, int]]:
     return out
 
 
+def wide_shapes() -> List[int]:
+    """Shapes that keep int64 cells: the range-bound fallback, never a tuning candidate."""
+    return [i for i in range(_native.lib.szs_tuning_shapes()) if _native.lib.szs_tuning_shape_wide(i)]
+
+
 def get_ladder(slot: int):
     shapes_out = (ctypes.c_int * 32)()
     rows_out = (ctypes.c_int * 32)()
@@ -122,8 +127,11 @@ def _time(engine, firsts, seconds, repeats: int = 3) -> float:
 def candidates_for(slot: int, low: int, high: int, table) -> List[int]:
     family = "lev_linear" if slot in UNIT_SLOTS else FAMILIES[slot // 2]
     uniform = family.startswith("lev")
+    wide = set(wide_shapes())
     out = []
     for index, (group, rows_per_lane, cooperative, bit_parallel, packed, chained) in enumerate(table):
+        if index in wide:  # int64 cells are the range bound's fallback, slower than any int32 shape
+            continue
         if bit_parallel and slot not in UNIT_SLOTS:
             continue
         if chained:  # for pairs far taller than the bands measured here

@@ -296,7 +296,7 @@ void batch_t::reset() {
     inputs.clear(), host_results.clear(), results_elements.clear(), result_rows.clear(), result_columns.clear();
     device_results.clear(), materialize_args.clear(), overflowed.clear(), pinned_offsets.clear();
     per_pair_tasks.clear(), per_pair_variants.clear();
-    chained_pairs = 0, chain_pitch = 0, decode_tapes_count = 0, decode_strings = 0;
+    chained_pairs = 0, chain_pitch = 0, decode_tapes_count = 0, decode_strings = 0, wide_pairs = 0, wide_pitch = 0;
     pinned_results = nullptr, pinned_bytes = 0, tasks_count = 0, cells = 0, core_ops = 0;
     scratch_capacity = 0, waves = 0, lds_bytes = 0, last_kernel_ms = 0, materialized = false;
     params_table = nullptr, segment_table = nullptr, materialize_table = nullptr, decode_table = nullptr;
@@ -309,7 +309,7 @@ void batch_t::release_device_memory() {
                                     &cursor, &task_start, &item_start, &priority, &position_of_bin, &queue_head,
                                     &scan_blocks,
                                     &scratch, &stats, &segment_overflow, &results_block, &chain_rows,
-                                    &chain_progress})
+                                    &chain_progress, &wide_rows})
         buffer->release();
     for (staged_tape_t *tape : spare_tapes)
         for (device_buffer_t *buffer : {&tape->data_copy, &tape->offsets_copy, &tape->runes_buffer,
@@ -408,6 +408,8 @@ static chain_view_t chain_view(batch_t &b) {
     chain.progress = b.chain_progress.as<uint32_t>();
     chain.pitch = b.chain_pitch;
     chain.timeout = &b.stats.as<prepare_stats_t>()->chain_timeout;
+    chain.wide_rows = b.wide_rows.as<int64_t>();
+    chain.wide_pitch = b.wide_pitch;
     return chain;
 }
 
@@ -606,7 +608,7 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
         a.extend = engine->params.extend;
         a.unit_costs = engine->params.match == 0 && engine->params.mismatch == 1 && engine->params.open == 1 &&
                        engine->params.extend == 1;
-        a.reach_magnitude = std::max(1, std::max(engine->substitution_magnitude, engine->gap_magnitude));
+        a.cell_growth = std::max(1, engine->substitution_magnitude + 6 * engine->gap_magnitude);
         a.packed_rows = a.symbols == symbols_u8_k ? engine->packed_rows : 0;
         a.latency_stripes = options.latency_stripes;
         tasks_total += tasks;
@@ -722,7 +724,7 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
             return fail(sz_unexpected_dimensions_k, error_message,
                         "A tape's offsets decrease or run past its byte count");
         if (!options.defer_overflow)
-            return fail(sz_overflow_risk_k, error_message, "Scores of this pair would overflow 32-bit cells");
+            return fail(sz_overflow_risk_k, error_message, "A string holds 2^32 or more symbols");
         if (segments_count) {
             SZS_TRY_HIP(hipMemcpyAsync(b.overflowed.data(), b.segment_overflow.data, segments_count * 4,
                                        hipMemcpyDeviceToHost, stream), "readback");
@@ -758,6 +760,11 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
         SZS_TRY_HIP(b.chain_progress.reserve((size_t)b.chained_pairs * chain_groups_k * 4),
                     "Failed to allocate the chain progress words");
     }
+    // Pairs of the int64 shape: each keeps its stripes' bottom row and vertical gaps in its own rows.
+    b.wide_pairs = stats.wide_pairs;
+    b.wide_pitch = ((uint64_t)stats.wide_columns + 1 + 63) / 64 * 64;
+    if (b.wide_pairs)
+        SZS_TRY_HIP(b.wide_rows.reserve((size_t)b.wide_pairs * 2 * b.wide_pitch * 8), "Failed to allocate the int64 rows");
     if (mode == szs_batch_per_pair_k) {
         std::vector<uint32_t> identity(slots);
         std::iota(identity.begin(), identity.end(), 0u);
@@ -845,6 +852,10 @@ sz_status_t szs_tuning_shape(int index, int *group, int *rows_per_lane, int *coo
     return sz_success_k;
 }
 
+int szs_tuning_shape_wide(int index) {
+    return index >= 0 && index < szs_amd::shapes_count_k && szs_amd::shapes_k[index].wide;
+}
+
 int szs_tuning_packed_rows(void const *engine) {
     return engine ? static_cast<szs_amd::engine_t const *>(engine)->packed_rows : 0;
 }
@@ -871,15 +882,17 @@ sz_status_t szs_tuning_set_ladder(int slot, int const *shapes, int const *max_ro
     bool const unit = slot == unit_bytes_slot_k || slot == unit_runes_slot_k;
     int const family = unit ? family_lev_linear_k : slot / 2;
     bool const uniform = family == family_lev_linear_k || family == family_lev_affine_k;
-    bool const runes = unit ? slot == unit_runes_slot_k : slot % 2 == 1;
-    if (!uniform && runes) return fail(sz_unexpected_dimensions_k, error_message, "Class costs never score runes");
+    // Class-cost families never score runes: their second slot holds the engines whose biased costs leave int8,
+    // which run the plain recurrences (no packed 16-bit form).
+    bool const plain = !uniform && slot % 2 == 1;
     auto allowed = [&](int shape) {
         if (shape < 0 || shape >= shapes_count_k) return false;
         shape_t const s = shapes_k[shape];
+        if (s.wide) return true; // int64 cells score every family, bytes or runes
         if (s.bit_parallel) return unit;
         // Chained groups hand over bottom rows only: global families, whose score is the last cell.
         if (s.chained && (unit || family == family_sw_linear_k || family == family_sw_affine_k)) return false;
-        if (s.packed) return family == family_nw_linear_k || family == family_nw_affine_k;
+        if (s.packed) return !plain && (family == family_nw_linear_k || family == family_nw_affine_k);
         if (!uniform && s.rows_per_lane > 8) return false; // LDS profiles are sized for 8 rows per lane
         return true;
     };

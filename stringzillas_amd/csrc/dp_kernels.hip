@@ -20,6 +20,7 @@
 #include "myers_sweep.hpp"
 #include "packed_sweep.hpp"
 #include "szs_device.hpp"
+#include "wide_sweep.hpp"
 
 namespace szs_amd {
 
@@ -310,7 +311,8 @@ template <int family_, typename symbol_type_, int shape_, bool shifted_>
 __device__ __forceinline__ void run_shape(item_args_t const &a) {
     constexpr shape_t shape = shapes_k[shape_];
     constexpr bool uniform = family_traits<family_>::uniform_k;
-    if constexpr (shape.bit_parallel) {
+    if constexpr (shape.wide) run_wide<family_, symbol_type_, shape.rows_per_lane>(a);
+    else if constexpr (shape.bit_parallel) {
         // Myers shapes exist for the unit-cost Levenshtein over bytes and runes only (the binning never picks them
         // otherwise).
         if constexpr (family_ == family_lev_linear_k) {
@@ -378,10 +380,11 @@ __device__ void run_family(int shape, item_args_t const &a) {
     case 21: run_shape<family_, symbol_type_, 21, shifted_>(a); break;
     case 22: run_shape<family_, symbol_type_, 22, shifted_>(a); break;
     case 23: run_shape<family_, symbol_type_, 23, shifted_>(a); break;
+    case 24: run_shape<family_, symbol_type_, 24, shifted_>(a); break;
     default: break;
     }
 }
-static_assert(shapes_count_k == 24, "dispatch switch lists every shape");
+static_assert(shapes_count_k == 25, "dispatch switch lists every shape");
 
 __device__ void run_variant(int variant, item_args_t const &a) {
     int const shape = variant % shapes_count_k;
@@ -464,6 +467,8 @@ __global__ __launch_bounds__(256, 4) void persistent_score_kernel(persistent_arg
     item.chain_progress = args.chain.progress;
     item.chain_pitch = args.chain.pitch;
     item.chain_timeout = args.chain.timeout;
+    item.wide_rows = args.chain.wide_rows;
+    item.wide_pitch = args.chain.wide_pitch;
     uint32_t taken;
     for (;;) { // whole-workgroup items
         if (threadIdx.x == 0) taken_shared = atomicAdd(args.bins.queue_head, 1u);
@@ -537,6 +542,8 @@ __global__ __launch_bounds__(256, 4) void single_task_kernel(task_t const *tasks
     item.chain_progress = chain.progress;
     item.chain_pitch = chain.pitch;
     item.chain_timeout = chain.timeout;
+    item.wide_rows = chain.wide_rows;
+    item.wide_pitch = chain.wide_pitch;
     run_variant(variant, item);
 }
 
@@ -901,11 +908,20 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
         bool const affine = a.family == family_lev_affine_k || a.family == family_nw_affine_k ||
                             a.family == family_sw_affine_k;
         uint64_t const longest = q_length > c_length ? q_length : c_length;
-        uint64_t const span = a.family <= family_lev_affine_k ? longest : q_length + c_length;
-        // The reach bound of serial.hpp:117-140, plus the anti-diagonal offset e*(n+m) that global families
-        // carry in their cells.
-        uint64_t const carried = local ? 0 : q_length + c_length + 2;
-        bool const overflows = (span + (affine ? 3 : 1) + carried) * (uint64_t)a.reach_magnitude >= 0x7FFFFFFFull;
+        uint64_t const shorter = q_length < c_length ? q_length : c_length;
+        // Range of the int32 cells (the reference picks its cell width by the reach bound of serial.hpp:117-140
+        // instead). Every value the int32 sweeps keep for cell (i, j) is a path score over at most min(i, j)
+        // substitutions and 2 min(i, j) + 2 gap runs, taken relative to its anti-diagonal (global families in the
+        // shifted form: H - e(i+j)), or for local families with non-positive gaps an absolute score in
+        // [0, s_max min(i, j)]: either way below (shorter + 1,100) x (max|s| + 6 max|gap|), the rows past n of a last
+        // stripe (at most 1,024) included. The plain recurrences (engines whose biased costs leave int8) and local
+        // families with positive gaps keep absolute scores, which grow with n + m. A pair whose bound reaches 2^31
+        // runs the int64 shape instead; only the final e(n+m) of a global score is added in 64 bits.
+        bool const unshifted = a.family >= family_nw_linear_k && a.symbols == class_unshifted_k;
+        bool const relative = local ? a.open <= 0 && a.extend <= 0 : !unshifted;
+        uint64_t const reach = (relative ? shorter : q_length + c_length) + 1100;
+        bool const wide = reach * (uint64_t)a.cell_growth >= 0x7FFFFFFFull;
+        bool const overflows = longest > 0xFFFFFFFFull; // task lengths are 32-bit
         if (bad_tape) {
             atomicOr(&stats->bad_tape, 1u);
             atomicOr(&segment_overflow[a.segment], (uint32_t)segment_bad_tape_k);
@@ -944,6 +960,7 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
                 uint64_t const stripe_rows = (uint64_t)shapes_k[shape].group * shapes_k[shape].rows_per_lane;
                 if ((rows + stripe_rows - 1) / stripe_rows > (uint64_t)a.latency_stripes) shape = cooperative_twin(shape);
             }
+            if (wide) shape = wide_shape_k;
             shape_t const sh = shapes_k[shape];
             // Latency of the item in lane-row-steps, the key for heaviest-first ordering.
             uint64_t const stripe = (uint64_t)sh.group * sh.rows_per_lane;
@@ -956,6 +973,7 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
             // swept in columns + 2 * group steps.
             uint64_t const per_step = sh.bit_parallel ? (symbols == symbols_u8_k ? 8u : 20u) * sh.rows_per_lane / 64 // ~17 rune planes
                                       : sh.packed     ? (uint64_t)(sh.rows_per_lane * 3 + 3) / 4
+                                      : sh.wide       ? 4ull * sh.rows_per_lane // int64 cells
                                                       : (uint64_t)sh.rows_per_lane;
             uint64_t const ramp = (uint64_t)sh.group * (sh.packed ? 2u : 1u);
             // A tall pair's last stripe may run a shorter sweep (`tail_sweeps_t`): half the Myers word, or 4 or 6
@@ -973,10 +991,15 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
                 atomicMax(&stats->chained_columns, task.columns_length);
                 task.flags |= (chain_slot + 1u) << 8;
             }
+            if (sh.wide) { // a slot in the int64 rows, likewise
+                uint32_t const wide_slot = atomicAdd(&stats->wide_pairs, 1u);
+                atomicMax(&stats->wide_columns, task.columns_length);
+                task.flags |= (wide_slot + 1u) << 8;
+            }
             bin = (uint32_t)variant_of(family, symbols, shape) * column_buckets_k + cost_bucket(work);
             cells += (unsigned long long)rows * task.columns_length;
             core_ops += (unsigned long long)rows * task.columns_length * core_ops_per_64_cells(family, symbols, sh);
-            if (rows > stripe && task.columns_length > widest) widest = task.columns_length;
+            if (rows > stripe && !sh.wide && task.columns_length > widest) widest = task.columns_length;
         }
         (void)wave_counted_add(histogram, bin, bin != ~0u);
         tasks[global] = task;

@@ -85,13 +85,19 @@ __device__ __forceinline__ int32_t opt(int32_t a, int32_t b) {
     else return a > b ? a : b;
 }
 
-/** Score of the matrix border at distance `k` from the origin (serial.hpp:142-151, 716-727, 889-895). */
+/** `a + b` modulo 2^32. Border scores and anti-diagonal offsets grow with i + j and may wrap; their sums, the
+ *  stored values, are small (the range bound guarantees it), so modular arithmetic gives them exactly. */
+__device__ __forceinline__ int32_t wrap_add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+__device__ __forceinline__ int32_t wrap_mul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+
+/** Score of the matrix border at distance `k` from the origin (serial.hpp:142-151, 716-727, 889-895), modulo
+ *  2^32 (see `wrap_add`). */
 template <int family_>
 __device__ __forceinline__ int32_t border_score(int32_t k, int32_t open, int32_t extend) {
     using traits = family_traits<family_>;
     if constexpr (traits::local_k) return 0;
-    else if constexpr (traits::affine_k) return k ? open + extend * (k - 1) : 0;
-    else return open * k;
+    else if constexpr (traits::affine_k) return k ? wrap_add(open, wrap_mul(extend, k - 1)) : 0;
+    else return wrap_mul(open, k);
 }
 
 #pragma endregion
@@ -118,6 +124,8 @@ struct item_args_t {
     uint32_t *chain_progress;
     uint32_t chain_pitch;
     uint32_t *chain_timeout;
+    int64_t *wide_rows;  /**< the int64 shape's per-pair bottom rows ([slot][scores, gaps][wide_pitch]) */
+    uint64_t wide_pitch;
 };
 
 /**
@@ -208,15 +216,22 @@ struct sweep_t {
         running_best = 0; // serial.hpp:781 starts the local best at 0
     }
 
-    /** Offset between a stored score and H at cell (i, j): -e*(i+j), S = open (local families) or 0. */
+    /** Offset between a stored score and H at cell (i, j): -e*(i+j) (modulo 2^32), S = open (local families) or
+     *  0. */
     __device__ __forceinline__ int32_t domain(int32_t i, int32_t j) const {
-        if constexpr (diagonal_k) return -extend * (i + j);
+        if constexpr (diagonal_k) return wrap_mul(-extend, wrap_add(i, j));
+        else return shift;
+    }
+
+    /** The same offset in 64 bits, for the pair's score. */
+    __device__ __forceinline__ int64_t domain64(int64_t i, int64_t j) const {
+        if constexpr (diagonal_k) return -(int64_t)extend * (i + j);
         else return shift;
     }
 
     /** Offset between a stored gap value and its track at cell (i, j). */
     __device__ __forceinline__ int32_t gap_domain(int32_t i, int32_t j) const {
-        if constexpr (diagonal_k) return -extend * (i + j) + (extend - open);
+        if constexpr (diagonal_k) return wrap_add(wrap_mul(-extend, wrap_add(i, j)), extend - open);
         else return 0;
     }
 
@@ -230,9 +245,9 @@ struct sweep_t {
             if (i <= n) symbol = (uint32_t)rows[i - 1];
             row_symbols[r] = symbol;
             int32_t const border = border_score<family_>(i, open, extend);
-            scores[r] = border + domain(i, 0);
+            scores[r] = wrap_add(border, domain(i, 0));
             // The horizontal track starts from its discard value on column 0.
-            horizontal[r] = (open + extend) + border + gap_domain(i, 0);
+            horizontal[r] = wrap_add(wrap_add(open + extend, border), gap_domain(i, 0));
             best[r] = 0;
         }
         if constexpr (!uniform_k) {
@@ -248,7 +263,7 @@ struct sweep_t {
                 }
             }
         }
-        diagonal_top = border_score<family_>(lane_top, open, extend) + domain(lane_top, 0); // the cell above-left
+        diagonal_top = wrap_add(border_score<family_>(lane_top, open, extend), domain(lane_top, 0)); // above-left
         bottom_score = scores[R - 1];
         bottom_gap = 0;
         column_symbol = 0;
@@ -269,8 +284,8 @@ struct sweep_t {
         }
         else {
             int32_t const border = border_score<family_>(j, open, extend);
-            score = border + domain(0, j);
-            gap = (open + extend) + border + gap_domain(0, j); // discard value of the vertical track on row 0
+            score = wrap_add(border, domain(0, j));
+            gap = wrap_add(wrap_add(open + extend, border), gap_domain(0, j)); // the vertical track's discard value
         }
     }
 
@@ -412,9 +427,9 @@ struct sweep_t {
                 int32_t value = 0;
 #pragma unroll
                 for (int r = 0; r < R; ++r) value += scores[r] & -(int32_t)(wanted == r);
-                value -= domain(n, m);
-                results[output] = (int64_t)value;
-                if (mirror != no_mirror_k) results[mirror] = (int64_t)value;
+                int64_t const score = (int64_t)value - domain64(n, m); // may leave int32: H = stored + e(n+m)
+                results[output] = score;
+                if (mirror != no_mirror_k) results[mirror] = score;
             }
         }
     }

@@ -128,10 +128,10 @@ struct packed_sweep_t {
         captured = 0;
     }
 
-    /** H at cell (i, j) minus its stored int32 form: -e*(i+j) (linear: e = g). */
-    __device__ __forceinline__ int32_t domain(int32_t i, int32_t j) const { return -extend * (i + j); }
+    /** The stored int32 form minus H at cell (i, j): -e*(i+j) modulo 2^32 (linear: e = g; see `wrap_add`). */
+    __device__ __forceinline__ int32_t domain(int32_t i, int32_t j) const { return wrap_mul(-extend, wrap_add(i, j)); }
     __device__ __forceinline__ int32_t gap_domain(int32_t i, int32_t j) const {
-        return -extend * (i + j) + (extend - open);
+        return wrap_add(domain(i, j), extend - open);
     }
 
     __device__ __forceinline__ uint32_t to_u16(int32_t value) const {
@@ -143,10 +143,10 @@ struct packed_sweep_t {
 
     /** Column-0 values of any stripe row (constant in the anti-diagonal domain), as stored int32 values. */
     __device__ __forceinline__ int32_t border_cell() const {
-        return border_score<family_>(1, open, extend) + domain(1, 0);
+        return wrap_add(border_score<family_>(1, open, extend), domain(1, 0));
     }
     __device__ __forceinline__ int32_t border_gap() const {
-        return (open + extend) + border_score<family_>(1, open, extend) + gap_domain(1, 0);
+        return wrap_add(wrap_add(open + extend, border_score<family_>(1, open, extend)), gap_domain(1, 0));
     }
 
     __device__ __forceinline__ void read_costs(uint32_t symbol, uint32_t (&out)[RP / 4]) const {
@@ -179,7 +179,7 @@ struct packed_sweep_t {
 #pragma unroll
         for (int p = 0; p < P; ++p) scores[p] = cell, horizontal[p] = gap;
         // Lane 0's first diagonal is the cell above the stripe at column 0 (the origin for the first stripe).
-        int32_t const above = border_score<family_>(lane_top, open, extend) + domain(lane_top, 0);
+        int32_t const above = wrap_add(border_score<family_>(lane_top, open, extend), domain(lane_top, 0));
         diagonal = to_u16(above) | (cell & 0xFFFF0000u);
         vertical = gap;
         symbol_now = symbol_low = 0;
@@ -203,8 +203,8 @@ struct packed_sweep_t {
         }
         else {
             int32_t const border = border_score<family_>(at, open, extend);
-            score = border + domain(0, at);
-            gap = (open + extend) + border + gap_domain(0, at); // discard value of the vertical track on row 0
+            score = wrap_add(border, domain(0, at));
+            gap = wrap_add(wrap_add(open + extend, border), gap_domain(0, at)); // the vertical track's discard value
         }
     }
 
@@ -365,10 +365,10 @@ struct packed_sweep_t {
         if (!last_stripe) return;
         int32_t const last_row = n - 1 - base_row;
         if (last_row / R != lane) return;
-        int32_t value = result_upper ? captured : from_u16(register_at(result_register) >> 16);
-        value -= domain(n, m);
-        results[output] = (int64_t)value;
-        if (mirror != no_mirror_k) results[mirror] = (int64_t)value;
+        int32_t const value = result_upper ? captured : from_u16(register_at(result_register) >> 16);
+        int64_t const score = (int64_t)value + (int64_t)extend * ((int64_t)n + m); // may leave int32
+        results[output] = score;
+        if (mirror != no_mirror_k) results[mirror] = score;
     }
 
     __device__ __forceinline__ int32_t partial() const { return 0; }

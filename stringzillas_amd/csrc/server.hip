@@ -152,7 +152,7 @@ void server_t::retire(round_t &round, sz_status_t status, char const *error) {
             else if (flags & segment_bad_tape_k)
                 ticket.status = sz_unexpected_dimensions_k,
                 ticket.error = "A tape's offsets decrease or run past its byte count";
-            else ticket.status = sz_overflow_risk_k, ticket.error = "Scores of a pair would overflow 32-bit cells";
+            else ticket.status = sz_overflow_risk_k, ticket.error = "A string holds 2^32 or more symbols";
         }
         else ticket.state = ticket_done_k;
     }

@@ -99,6 +99,8 @@ struct shape_t {
     /** Cooperative shapes only: the pair's stripes are split into `chain_groups_k` consecutive work items, each
      *  a workgroup; a group reads the previous group's bottom row from HBM as it is published. */
     int32_t chained;
+    /** int64 cells (one wave per pair, bottom rows in per-pair HBM rows): pairs whose values could leave int32. */
+    int32_t wide = 0;
 };
 
 /** Waves per workgroup of the scoring kernels. */
@@ -109,7 +111,7 @@ constexpr int waves_per_group_k = 4;
  *  16-bit shapes for the global class-cost families of engines whose scores provably fit (else their int32 twin
  *  runs).
  */
-constexpr int shapes_count_k = 24;
+constexpr int shapes_count_k = 25;
 constexpr shape_t shapes_k[shapes_count_k] = {
     {16, 1, 0, 0, 0, 0}, {16, 2, 0, 0, 0, 0}, {16, 4, 0, 0, 0, 0},  // up to 16 / 32 / 64 rows, four pairs per wave
     {64, 2, 0, 0, 0, 0}, {64, 3, 0, 0, 0, 0}, {64, 4, 0, 0, 0, 0},  // up to 128 / 192 / 256 rows
@@ -122,13 +124,17 @@ constexpr shape_t shapes_k[shapes_count_k] = {
     {64, 8, 1, 0, 1, 1}, {64, 8, 1, 0, 0, 1},      // chained cooperative 512-row stripes, packed or int32
     {16, 32, 0, 1, 0, 0}, {64, 32, 0, 1, 0, 0},    // Myers over 32-row words: up to 512 rows, 2048-row stripes
     {16, 8, 0, 0, 0, 0},                           // up to 128 rows, four pairs per wave
+    {64, 8, 0, 0, 0, 0, 1},                        // int64 cells, 512-row stripes on one wave
 };
+/** The int64-cell shape, which pairs beyond the int32 range bound take (any family, bytes or runes). */
+constexpr int wide_shape_k = 24;
 
 /** The int32 shape with the same geometry as a packed one (itself for every other shape). */
 __host__ __device__ constexpr int unpacked_twin(int shape) {
     if (!shapes_k[shape].packed) return shape;
     for (int s = 0; s < shapes_count_k; ++s)
-        if (!shapes_k[s].packed && !shapes_k[s].bit_parallel && shapes_k[s].group == shapes_k[shape].group &&
+        if (!shapes_k[s].packed && !shapes_k[s].bit_parallel && !shapes_k[s].wide &&
+            shapes_k[s].group == shapes_k[shape].group &&
             shapes_k[s].rows_per_lane == shapes_k[shape].rows_per_lane &&
             shapes_k[s].cooperative == shapes_k[shape].cooperative && shapes_k[s].chained == shapes_k[shape].chained)
             return s;
@@ -140,7 +146,7 @@ static_assert(unpacked_twin(15) == 5 && unpacked_twin(16) == 6 && unpacked_twin(
 
 /** The cooperative shape with the same geometry as a one-wave shape (-1 when there is none). */
 __host__ __device__ constexpr int cooperative_twin(int shape) {
-    if (shapes_k[shape].cooperative || shapes_k[shape].group != 64) return -1;
+    if (shapes_k[shape].cooperative || shapes_k[shape].wide || shapes_k[shape].group != 64) return -1;
     for (int s = 0; s < shapes_count_k; ++s)
         if (shapes_k[s].cooperative && !shapes_k[s].chained && shapes_k[s].group == 64 &&
             shapes_k[s].rows_per_lane == shapes_k[shape].rows_per_lane &&
@@ -149,7 +155,8 @@ __host__ __device__ constexpr int cooperative_twin(int shape) {
     return -1;
 }
 static_assert(cooperative_twin(17) == 18 && cooperative_twin(13) == 14 && cooperative_twin(7) == 9 &&
-                  cooperative_twin(8) == 10 && cooperative_twin(15) == -1 && cooperative_twin(22) == -1,
+                  cooperative_twin(8) == 10 && cooperative_twin(15) == -1 && cooperative_twin(22) == -1 &&
+                  cooperative_twin(wide_shape_k) == -1 && unpacked_twin(wide_shape_k) == wide_shape_k,
               "one-wave shapes and their cooperative twins");
 
 /**
@@ -167,6 +174,9 @@ struct chain_view_t {
     uint32_t *progress;
     uint32_t pitch; /**< >= the longest chained pair's columns + 1 */
     uint32_t *timeout;
+    /** Pairs of the int64 shape: per pair, its stripes' bottom row and vertical gaps ([slot][2][wide_pitch]). */
+    int64_t *wide_rows;
+    uint64_t wide_pitch; /**< >= the longest such pair's columns + 1 */
 };
 
 /** Shape-table slots of the unit-cost Levenshtein over bytes and over runes (after the family x symbols slots):
@@ -251,7 +261,7 @@ struct materialize_args_t {
     int32_t open, extend;
     int32_t unit_costs;                  /**< match 0, mismatch 1, gap 1: bit-parallel shapes apply to bytes */
     int32_t runes;                       /**< lengths come from the rune counts (UTF-8) */
-    int32_t reach_magnitude;             /**< max |cost| of the engine, for the int32 reach bound */
+    int32_t cell_growth;                 /**< max |substitution| + 6 max |gap| of the engine, for the range bound */
     int32_t packed_rows;                 /**< most rows per lane of a packed 16-bit shape the engine admits (0: none) */
     int32_t latency_stripes;             /**< > 0: pairs of more stripes than this on one wave go cooperative */
 };
@@ -297,6 +307,7 @@ __host__ __device__ constexpr uint32_t core_ops_per_64_cells(int family, int sym
         return shape.rows_per_lane == 64 ? (runes ? 25 + 3 * 8 : 25) : (runes ? 2 * (15 + 2 * 8) : 2 * 15);
     }
     if (shape.packed) return affine ? 192 : 96;
+    if (shape.wide) return affine ? 20 * 64 : 8 * 64; // int64 adds and selects: two or three lane-ops each
     switch (family) {
     case family_lev_linear_k: return 4 * 64;
     case family_lev_affine_k: return 7 * 64;
@@ -317,6 +328,8 @@ struct prepare_stats_t {
     uint32_t chained_pairs;              /**< pairs scored by a chained shape: slots of the chain buffers */
     uint32_t chained_columns;            /**< most columns of such a pair */
     uint32_t chain_timeout;              /**< nonzero if a chained group gave up waiting for its predecessor */
+    uint32_t wide_pairs;                 /**< pairs scored by the int64 shape: slots of the wide rows */
+    uint32_t wide_columns;               /**< most columns of such a pair */
     uint32_t reserved;
 };
 

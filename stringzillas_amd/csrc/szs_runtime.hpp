@@ -203,6 +203,9 @@ struct batch_t {
     device_buffer_t histogram, cursor, task_start, item_start, priority, position_of_bin, queue_head, scratch, stats;
     device_buffer_t segment_overflow, results_block, scan_blocks;
     device_buffer_t chain_rows, chain_progress;  /**< chained pairs' bottom rows between groups, and progress */
+    device_buffer_t wide_rows;                 /**< the int64 shape's per-pair bottom rows */
+    uint32_t wide_pairs = 0;                   /**< pairs scored by the int64 shape */
+    uint64_t wide_pitch = 0;
     uint32_t chained_pairs = 0;                /**< pairs scored by a chained shape (slots of the chain buffers) */
     uint32_t chain_pitch = 0;
     uint32_t decode_tapes_count = 0;           /**< UTF-8 tapes transcoded by each run */

@@ -213,8 +213,11 @@ def test_tuning_ladders_validate_and_round_trip():
             autotune.set_ladder(nw, [(0, 10_000)], beyond)  # a sub-wave shape holds one stripe
         with pytest.raises(ValueError):
             autotune.set_ladder(nw, [(3, 100), (4, 50)], beyond)  # limits must grow
+        plain = autotune.slot_of("nw_affine", runes=True)  # engines on the plain recurrences: no packed cells
+        packed = next(i for i, s in enumerate(table) if s[4])
         with pytest.raises(ValueError):
-            autotune.set_ladder(autotune.slot_of("nw_affine", runes=True), [], beyond)
+            autotune.set_ladder(plain, [(packed, 512)], beyond)
+        autotune.set_ladder(plain, [], autotune.wide_shapes()[0])  # the int64 shape scores any slot
         assert autotune.get_ladder(nw) == (ladder[:2], beyond)  # rejected ladders change nothing
     finally:
         for slot, (ladder, beyond) in saved.items():
