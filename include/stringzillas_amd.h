@@ -397,6 +397,10 @@ sz_size_t szs_batch_cells(szs_batch_t batch);
  *  the tier that scores it (bit-parallel words, packed 16-bit or int32 DP), data movement excluded; a launch
  *  issues at least this many lane-ops, so this over its duration is a lower bound on the VALU work rate. */
 sz_size_t szs_batch_core_ops(szs_batch_t batch);
+/** Bytes of device memory the prepared batch holds: staged tapes, tasks, bins, stripe-boundary scratch, chain and
+ *  int64 rows. Stripe-boundary scratch is capped (4 GiB or a quarter of the free HBM; a server round, its share of
+ *  the pool's margin) by running fewer workgroups when a batch holds pairs of millions of columns. */
+sz_size_t szs_batch_device_bytes(szs_batch_t batch);
 /** Device time of the last run in milliseconds from HIP events on the batch stream: the scoring kernel alone for the
  *  persistent and per-pair modes, the whole replay for the graph mode. */
 double szs_batch_last_kernel_ms(szs_batch_t batch);

@@ -658,6 +658,11 @@ class Batch:
         return int(_lib.szs_batch_cells(self.handle))
 
     @property
+    def device_bytes(self) -> int:
+        """Device memory the prepared batch holds."""
+        return int(_lib.szs_batch_device_bytes(self.handle))
+
+    @property
     def core_ops(self) -> int:
         """Core recurrence lane-ops per run (each cell weighted by its tier's cell-update op count)."""
         return int(_lib.szs_batch_core_ops(self.handle))

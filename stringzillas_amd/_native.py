@@ -123,7 +123,7 @@ _c = ctypes
 for name, restype in {
     "szs_version_major": _c.c_int, "szs_version_minor": _c.c_int, "szs_version_patch": _c.c_int,
     "szs_capabilities": _c.c_int, "szs_capabilities_comptime": _c.c_int, "szs_capabilities_runtime": _c.c_int,
-    "szs_batch_cells": _c.c_size_t, "szs_batch_core_ops": _c.c_size_t, "szs_batch_last_kernel_ms": _c.c_double, "szs_server_poll": _c.c_int,
+    "szs_batch_cells": _c.c_size_t, "szs_batch_core_ops": _c.c_size_t, "szs_batch_device_bytes": _c.c_size_t, "szs_batch_last_kernel_ms": _c.c_double, "szs_server_poll": _c.c_int,
     "szs_server_bytes_in_use": _c.c_size_t, "szs_unified_alloc": _c.c_void_p, "szs_tape_bytes": _c.c_size_t,
     "szs_fingerprints_dimensions": _c.c_size_t, "szs_fingerprints_last_kernel_ms": _c.c_double,
 }.items():

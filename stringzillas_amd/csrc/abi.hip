@@ -453,6 +453,10 @@ sz_size_t szs_batch_cells(szs_batch_t batch) { return batch ? reinterpret_cast<b
 
 sz_size_t szs_batch_core_ops(szs_batch_t batch) { return batch ? reinterpret_cast<batch_t *>(batch)->core_ops : 0; }
 
+sz_size_t szs_batch_device_bytes(szs_batch_t batch) {
+    return batch ? reinterpret_cast<batch_t *>(batch)->device_bytes() : 0;
+}
+
 double szs_batch_last_kernel_ms(szs_batch_t batch) {
     return batch ? reinterpret_cast<batch_t *>(batch)->last_kernel_ms : 0;
 }

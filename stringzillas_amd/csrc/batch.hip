@@ -303,6 +303,22 @@ void batch_t::reset() {
     shape_table = nullptr;
 }
 
+size_t batch_t::device_bytes() const {
+    size_t total = 0;
+    for (device_buffer_t const *buffer : {&tables, &tasks, &bin_of_task, &order, &shape_of_task, &identity, &histogram,
+                                          &cursor, &task_start, &item_start, &priority, &position_of_bin, &queue_head,
+                                          &scan_blocks, &scratch, &stats, &segment_overflow, &results_block,
+                                          &chain_rows, &chain_progress, &wide_rows})
+        total += buffer->capacity;
+    for (std::vector<staged_tape_t *> const *list : {&tapes, &spare_tapes})
+        for (staged_tape_t const *tape : *list)
+            total += tape->data_copy.capacity + tape->offsets_copy.capacity + tape->runes_buffer.capacity +
+                     tape->rune_counts_buffer.capacity;
+    for (std::vector<device_buffer_t *> const *list : {&result_buffers, &spare_results})
+        for (device_buffer_t const *buffer : *list) total += buffer->capacity;
+    return total;
+}
+
 void batch_t::release_device_memory() {
     reset();
     for (device_buffer_t *buffer : {&tables, &tasks, &bin_of_task, &order, &shape_of_task, &identity, &histogram,
@@ -745,9 +761,20 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
     SZS_TRY_HIP(szs_amd_persistent_occupancy(&blocks_per_cu, b.lds_bytes), "occupancy");
     blocks_per_cu = std::max(1, std::min(blocks_per_cu, 8));
     b.waves = mode == szs_batch_per_pair_k ? 1u : (uint32_t)(compute_units * blocks_per_cu); // workgroups
+    // Every resident workgroup holds stripe-boundary rows as wide as the widest multi-stripe pair: a pair of
+    // millions of columns would ask for tens of GB. Past a limit (the caller's, else 4 GiB or a quarter of the free
+    // HBM) the persistent launch runs fewer workgroups instead; the work queue does not depend on their number.
     uint64_t const group_scratch_bytes = waves_per_group_k * 2ull * b.scratch_capacity * 4;
-    if (widest_multi_pass && options.max_scratch_bytes && b.waves * group_scratch_bytes > options.max_scratch_bytes)
-        b.waves = (uint32_t)std::max<uint64_t>(1, options.max_scratch_bytes / group_scratch_bytes);
+    uint64_t scratch_limit = options.max_scratch_bytes;
+    if (!scratch_limit && widest_multi_pass && b.waves * group_scratch_bytes > (256ull << 20)) {
+        size_t free_bytes = 0, total_bytes = 0;
+        scratch_limit = 4ull << 30;
+        if (hipMemGetInfo(&free_bytes, &total_bytes) == hipSuccess)
+            scratch_limit = std::min<uint64_t>(scratch_limit, std::max<uint64_t>(free_bytes / 4, group_scratch_bytes));
+        else (void)hipGetLastError();
+    }
+    if (widest_multi_pass && scratch_limit && b.waves * group_scratch_bytes > scratch_limit)
+        b.waves = (uint32_t)std::max<uint64_t>(1, scratch_limit / group_scratch_bytes);
     uint64_t const scratch_groups = widest_multi_pass ? b.waves : 1;
     SZS_TRY_HIP(b.scratch.reserve(scratch_groups * waves_per_group_k * 2ull * b.scratch_capacity * 4 + 256),
                 "Failed to allocate scratch");

@@ -103,6 +103,7 @@ struct server_t {
     bool stopping = false;
     std::thread dispatcher;
     arena_t arena; /**< every round's device buffers, so rounds never allocate from the driver */
+    size_t margin = 0; /**< pool bytes beyond the budget: the rounds' stripe-boundary scratch */
     szs_server_stats_t stats {};
 
     void run();
@@ -223,6 +224,8 @@ void server_t::run() {
             // A round's latency is its longest item: pairs taller than 16 stripes of a one-wave shape (8,192 rows of
             // packed NW, 65,536 of Myers words) take the whole workgroup.
             options.latency_stripes = 16;
+            // Stripe-boundary scratch stays within this round's share of the pool's margin over the budget.
+            options.max_scratch_bytes = std::max<size_t>(size_t(16) << 20, margin / streams.size());
             if (pinned[free_stream].reserve(admitted_results) == hipSuccess) {
                 options.pinned_results = pinned[free_stream].data;
                 options.pinned_capacity = pinned[free_stream].capacity;
@@ -271,6 +274,7 @@ sz_status_t szs_server_init(szs_device_scope_t device, sz_size_t hbm_budget_byte
     // The pool holds the admitted footprints plus room for each in-flight round's boundary-row scratch; a round
     // that still does not fit falls back to plain allocations.
     size_t const margin = std::max<size_t>(size_t(256) << 20, hbm_budget_bytes / 8);
+    server->margin = margin;
     hipError_t const pooled = server->arena.init(hbm_budget_bytes + margin);
     if (pooled != hipSuccess) {
         delete server;

@@ -168,8 +168,9 @@ struct batch_options_t {
     /** Latency over throughput: a pair that a one-wave shape would sweep in more than this many stripes takes the
      *  cooperative shape of the same geometry, so no single wave sets the end of a short launch (0: off). */
     int32_t latency_stripes = 0;
-    /** Most bytes of stripe-boundary scratch the launch may use (0: no limit): with wide multi-stripe pairs the
-     *  persistent launch then runs fewer workgroups, each holding its rows. */
+    /** Most bytes of stripe-boundary scratch the launch may use (0: 4 GiB or a quarter of the free HBM, whichever
+     *  is less): with wide multi-stripe pairs the persistent launch then runs fewer workgroups, each holding its
+     *  rows. */
     size_t max_scratch_bytes = 0;
 };
 
@@ -233,6 +234,8 @@ struct batch_t {
     void reset();
     /** `reset`, then returns every device buffer (spares included) to where it came from. */
     void release_device_memory();
+    /** Bytes of device memory the batch holds (every buffer's capacity). */
+    size_t device_bytes() const;
     ~batch_t();
 };
 

@@ -129,3 +129,17 @@ def test_wide_costs_over_millions_of_columns_take_the_int64_shape():
     assert batch.core_ops == 20 * batch.cells  # the int64 shape's accounting (affine)
     got = batch.run()[0].astype(np.int64)
     assert got.tolist() == [O.score(params, short, long_side)]
+
+
+def test_pairs_of_millions_of_columns_keep_stripe_scratch_bounded():
+    """A multi-stripe pair of 1.2 M columns: stripe-boundary rows for every resident workgroup would take ~29 GB, so
+    the launch runs fewer workgroups and the batch stays within a few GB; the score equals the oracle."""
+    rng = np.random.default_rng(1200)
+    b2c, costs = protein_tables(rng)
+    engine = szs.NeedlemanWunschScores(b2c, costs, -11, -1)
+    params = O.Params("needleman_wunsch", open=-11, extend=-1, byte_to_class=b2c, class_costs=costs)
+    short, long_side = random_bytes(rng, PROTEIN, 600, 600), random_bytes(rng, PROTEIN, 1_200_000, 1_200_000)
+    batch = szs.Batch([(engine, szs.Strs([short, b"ACD"]), szs.Strs([long_side, b"ACE"]))])
+    assert batch.device_bytes < (5 << 30)
+    got = batch.run()[0].astype(np.int64)
+    assert got.tolist() == [O.score(params, short, long_side), O.score(params, b"ACD", b"ACE")]
