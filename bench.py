@@ -28,13 +28,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GCUPS on 1 MI355X (batched vs per-call) + p50/p99 latency in serving loop"
 INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # CUs x lanes x clock: 39.3 T lane-ops/s
-VALU_ISSUE_PEAK = 256 * 2.4e9  # wave64 VALU instructions per second: one per cycle per CU
+VALU_ISSUE_PEAK = 256 * 2.4e9  # nominal wave64 VALU issue: one per cycle per CU (VOP3/VOP3P forms issue a little
+# below it on gfx950, simple VOP2 forms such as v_xor_b32 above it)
 MIXED = "mixed_1000000_pairs_log_uniform_16_16384_plus_32x65536"
 # From profiles/persistent_score_kernel_mixed_1M_pairs.md (rocprofv3 --pmc on this workload): HBM bytes per launch
 # (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors a plain 1 GiB device copy reads) and SQ_INSTS_VALU per
 # launch (wave64 instructions), whose rate over the kernel's time is the measured VALU issue fraction.
-MEASURED_TRAFFIC_BYTES = {MIXED: 62.27e9}
-MEASURED_VALU_INSTRUCTIONS = {MIXED: 142.05e9}
+MEASURED_TRAFFIC_BYTES = {MIXED: 62.24e9}
+MEASURED_VALU_INSTRUCTIONS = {MIXED: 142.06e9}
 # The reference's published all-pairs GCUPS on equal-length ACGT (include/stringzillas/similarities/README.md:8-26):
 # Ice Lake AVX-512 on a 16-core Xeon4, and the H100 kernels (no UTF-8 kernel there).
 PUBLISHED = {
@@ -72,6 +73,8 @@ def parse():
                         help="measure only the fingerprint engine and print its report (for profiling)")
     parser.add_argument("--only-headline", action="store_true",
                         help="measure the mixed batch alone, no side measurements (for profiling)")
+    parser.add_argument("--only-kind", choices=["levenshtein", "levenshtein_utf8", "needleman_wunsch"],
+                        help="time one kind of the mixed batch alone in the persistent form (for profiling)")
     return parser.parse_args()
 
 
@@ -201,31 +204,41 @@ def cpu_baseline(host_segments, target_seconds: float, threads: int, seed: int =
     rng = np.random.default_rng(seed)
     total = sum(s.count for s in host_segments)
 
-    def run(sample_pairs):
-        cells = 0
-        start = time.perf_counter()
+    def run(sample_pairs, min_seconds):
+        """Scores a sample drawn once; a sample that is every pair (small workloads) is scored again until it has
+        run `min_seconds`, so the rate is not a timer's rounding."""
+        subs = []
         for segment in host_segments:
             share = max(1, int(round(sample_pairs * segment.count / total)))
-            sub = segment.subset(sample_indices(rng, segment.count, share))
-            tapes = ((sub.firsts_data, sub.firsts_offsets), (sub.seconds_data, sub.seconds_offsets))
-            if variant:
-                O.reference_pairs(oracle_params(sub), *tapes, threads=threads, variant=variant)
-            else:
-                O.pairs(oracle_params(sub), *tapes, threads=threads)
-            cells += sub.cells()
-        return cells, time.perf_counter() - start
+            subs.append(segment.subset(sample_indices(rng, segment.count, share)))
+        cells = passes = 0
+        start = time.perf_counter()
+        while True:
+            for sub in subs:
+                tapes = ((sub.firsts_data, sub.firsts_offsets), (sub.seconds_data, sub.seconds_offsets))
+                if variant:
+                    O.reference_pairs(oracle_params(sub), *tapes, threads=threads, variant=variant)
+                else:
+                    O.pairs(oracle_params(sub), *tapes, threads=threads)
+                cells += sub.cells()
+            passes += 1
+            seconds = time.perf_counter() - start
+            if sample_pairs < total or seconds >= min_seconds or passes >= 10_000:
+                return cells, seconds, passes
 
     sample_pairs = 16 * threads
-    cells, seconds = run(sample_pairs)
-    while seconds < target_seconds / 3 and sample_pairs < 400_000:
+    cells, seconds, passes = run(sample_pairs, target_seconds / 3)
+    while seconds < target_seconds / 3 and sample_pairs < min(total, 400_000):
         sample_pairs = int(min(sample_pairs * max(2.0, 0.9 * target_seconds / max(seconds, 1e-3)), 400_000))
-        cells, seconds = run(sample_pairs)
+        cells, seconds, passes = run(sample_pairs, target_seconds / 3)
+    sample_pairs = min(sample_pairs, total)
     return {
         "value": round(cells / seconds / 1e9, 3),
         "unit": "GCUPS",
         "cores": threads,
         "kind": "reference" if variant else "port",
-        "sample": f"{sample_pairs} pairs drawn uniformly from the same pairs ({cells / 1e9:.1f} G cells, "
+        "sample": f"{sample_pairs} pairs drawn uniformly from the same pairs"
+                  f"{f', scored {passes} times' if passes > 1 else ''} ({cells / 1e9:.1f} G cells, "
                   f"{seconds:.1f}s), scorer={'reference ' + variant if variant else 'oracle'}",
     }
 
@@ -462,6 +475,15 @@ def main():
         tapes.append((TensorTape(segment.firsts_data, segment.firsts_offsets),
                       TensorTape(segment.seconds_data, segment.seconds_offsets)))
         outputs.append(torch.zeros(segment.count, dtype=torch.int64, device=device))
+    if args.only_kind:
+        index = next(i for i, s in enumerate(workload.segments) if s.kind == args.only_kind)
+        alone = szs.Batch([(engines[index], *tapes[index])], mode=szs.BATCH_PERSISTENT, outputs=[outputs[index]])
+        alone.run()
+        _, alone_ms = time_batch(alone, max(1, args.steps), 0)
+        print(json.dumps({"kind": args.only_kind, "pairs": workload.segments[index].count, "cells": alone.cells,
+                          "core_ops": alone.core_ops, "kernel_ms": round(alone_ms, 3),
+                          "gcups": round(alone.cells / (alone_ms / 1e3) / 1e9, 1)}), flush=True)
+        return
     mode = szs.BATCH_GRAPH if args.mode == "graph" else szs.BATCH_PERSISTENT
     t0 = time.time()
     batch = szs.Batch([(e, a, b) for e, (a, b) in zip(engines, tapes)], mode=mode, outputs=outputs)
@@ -483,7 +505,8 @@ def main():
     roofline["note"] = ("frac = core recurrence lane-ops (each cell weighted by the op count of its tier's update: "
                         "Myers words, packed 16-bit or int32 DP) over the scoring kernel's HIP-event time and the "
                         "INT32 VALU peak; valu_issue_frac = all issued VALU instructions (rocprofv3 SQ_INSTS_VALU) "
-                        "over the same time and the issue peak, so frac <= valu_issue_frac <= 1")
+                        "over the same time and the nominal issue rate of one wave64 instruction per cycle per CU (packed and VOP3 forms "
+                        "issue at most ~0.9 of it, simple VOP2 forms more), so frac <= valu_issue_frac")
     del scoring
     log(f"batched: {ms_per_step:.1f} ms/step ({replay_ms:.1f} ms replay), {gcups:.1f} GCUPS; scoring kernel "
         f"{kernel_ms:.1f} ms; roofline {roofline['frac']}")
