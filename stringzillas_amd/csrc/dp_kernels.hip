@@ -614,107 +614,166 @@ __device__ uint64_t decode_walk(uint8_t const *first, uint8_t const *last, uint3
     return written;
 }
 
-/**
- *  The whole wave decodes one string (the same arguments in every lane), 1 KB per pass, 16 bytes per lane. When
- *  the string is well formed (every lead byte followed by exactly its continuation bytes, no stray continuation
- *  byte; a sequence cut by the end of the string is allowed), the walk above visits exactly the bytes that are
- *  not continuation bytes, so a rune's index is the count of such bytes before it: a wave prefix sum. Returns
- *  the rune count, or -1 if some byte breaks the rule (the caller then walks the string in one lane). The output
- *  must have room for one rune per byte. A lane's 16 bytes come in one load when they all belong to the string
- *  (so no load leaves its pages), byte by byte at the string's end; the next pass's load is issued before this
- *  pass decodes, so its latency overlaps the work.
- */
-__device__ int64_t decode_wave(uint8_t const *first, uint8_t const *last, uint32_t *out) {
-    constexpr int32_t lane_bytes_k = 16;
-    constexpr int64_t pass_bytes_k = 64 * lane_bytes_k;
-    // One pass's runes (at most one per byte) per wave of the 256-thread decode workgroup.
-    __shared__ uint32_t pass_runes_lds[4][pass_bytes_k];
-    uint32_t *staging = pass_runes_lds[threadIdx.x / 64];
-    uint32_t const lane = __lane_id();
-    int64_t const length = last - first;
-    int64_t written = 0;
-    uint32_t carry = 0; // the previous pass's last 4 bytes (zero bytes before the string: ASCII, cover nothing)
-    auto whole_chunk = [&](int64_t at) { // 16 bytes at `at`, all inside the string, or zeros
-        uint4 chunk = make_uint4(0, 0, 0, 0);
-        if (at + lane_bytes_k <= length) __builtin_memcpy(&chunk, first + at, lane_bytes_k);
-        return chunk;
-    };
-    uint4 upcoming = whole_chunk(lane_bytes_k * (int64_t)lane);
-    for (int64_t base = 0; base < length; base += pass_bytes_k) {
-        int64_t const mine = base + lane_bytes_k * (int64_t)lane;
-        int32_t const valid = (int32_t)max((int64_t)0, min((int64_t)lane_bytes_k, length - mine));
-        uint4 const chunk = upcoming;
-        upcoming = whole_chunk(mine + pass_bytes_k);
-        // w[0..3]: this lane's bytes; w[4]: the 3 bytes after them (for the last runes' continuation bytes).
-        uint32_t w[5] = {chunk.x, chunk.y, chunk.z, chunk.w, 0};
-        if (valid > 0 && valid < lane_bytes_k) { // the string's last bytes, one at a time
-#pragma unroll
-            for (int32_t i = 0; i < lane_bytes_k; ++i)
-                if (i < valid) w[i / 4] |= (uint32_t)first[mine + i] << (8 * (i % 4));
-        }
-        uint32_t const previous_lane = (uint32_t)__shfl_up((int)w[3], 1, 64);
-        uint32_t const before = lane == 0 ? carry : previous_lane;     // bytes mine-4 .. mine-1
-        uint32_t following = (uint32_t)__shfl_down((int)w[0], 1, 64); // bytes mine+16 .. mine+19
-        if (lane == 63) {
-            following = 0;
-            for (int32_t i = 0; i < 3; ++i)
-                if (mine + lane_bytes_k + i < length) following |= (uint32_t)first[mine + lane_bytes_k + i] << (8 * i);
-        }
-        w[4] = following;
-        // Continuation bytes still owed by the last lead among the 3 bytes before this lane.
-        int32_t need = 0;
-        for (int32_t d = 3; d >= 1; --d) {
-            uint32_t const byte = (before >> (8 * (4 - d))) & 0xFFu;
-            if ((byte & 0xC0u) != 0x80u) need = max(0, (int32_t)utf8_width(byte) - d);
-        }
-        bool broken = false;
-        uint32_t leads = 0;
-#pragma unroll
-        for (int32_t i = 0; i < lane_bytes_k; ++i) {
-            if (i < valid) {
-                uint32_t const byte = (w[i / 4] >> (8 * (i % 4))) & 0xFFu;
-                bool const continuation = (byte & 0xC0u) == 0x80u;
-                if (need > 0) {
-                    broken |= !continuation;
-                    --need;
-                }
-                else if (continuation) broken = true;
-                else {
-                    leads |= 1u << i;
-                    need = (int32_t)utf8_width(byte) - 1;
-                }
-            }
-        }
-        if (__ballot(broken)) return -1;
-        int32_t const count = __builtin_popcount(leads);
-        int32_t inclusive = count;
-        for (int32_t d = 1; d < 64; d <<= 1) {
-            int32_t const other = __shfl_up(inclusive, d, 64);
-            if ((int32_t)lane >= d) inclusive += other;
-        }
-        // Runes go to their ranks in the wave's LDS staging (scattered LDS stores are cheap), then out to HBM in
-        // contiguous dword stores (one scattered global store per byte position would touch ~24 lines each).
-        uint32_t at = (uint32_t)(inclusive - count);
-#pragma unroll
-        for (int32_t i = 0; i < lane_bytes_k; ++i) {
-            if ((leads >> i) & 1u) {
-                auto byte_at = [&](int32_t j) { return (w[j / 4] >> (8 * (j % 4))) & 0xFFu; };
-                auto tail = [&](int32_t d) { // continuation byte d of this lead, 0 past the end of the string
-                    return mine + i + d < length ? byte_at(i + d) : 0u;
-                };
-                uint32_t const lead = byte_at(i);
-                uint32_t const width = utf8_width(lead);
-                staging[at++] = rune_of(lead, width, width > 1 ? tail(1) : 0u, width > 2 ? tail(2) : 0u,
-                                        width > 3 ? tail(3) : 0u);
-            }
-        }
-        int32_t const pass_runes = __shfl(inclusive, 63, 64);
-        // The wave's LDS operations complete in order, so every lane's staging stores precede these loads.
-        for (int32_t t = (int32_t)lane; t < pass_runes; t += 64) out[written + t] = staging[t];
-        written += pass_runes;
-        carry = (uint32_t)__shfl((int)w[3], 63, 64);
+/** Bytes a decode pass stages in LDS: 64 lanes x 16. */
+constexpr int32_t decode_pass_bytes_k = 1024;
+/** Strings a wave draws per ticket of the decode queue (their offsets load in one instruction). */
+constexpr uint32_t decode_strings_per_ticket_k = 8;
+
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) u32x4_t const global_block_t;
+typedef __attribute__((address_space(1))) uint32_t global_rune_t;
+typedef __attribute__((address_space(1))) uint8_t global_byte_t;
+
+/** The 16-byte aligned block at `block` when it holds a byte before `end` (so no load leaves the tape's pages), else
+ *  0x80 bytes; `past_end` replaces its bytes from `end` on when they are staged, not here (a load waited for at once
+ *  would stall the wave for a whole memory round trip). Global-address-space loads, so the kernel's LDS waits never
+ *  wait for them. */
+__device__ __forceinline__ uint4 tape_block(uintptr_t block, uintptr_t end) {
+    constexpr uint32_t past_k = 0x80808080u;
+    uint4 data = make_uint4(past_k, past_k, past_k, past_k);
+    if (block < end) {
+        u32x4_t const loaded = *(global_block_t *)block;
+        data = make_uint4(loaded.x, loaded.y, loaded.z, loaded.w);
     }
-    return written;
+    return data;
+}
+
+/** `data`, the block at `block`, with every byte from `end` on read as 0x80: a continuation byte with no payload
+ *  bits, so a sequence cut by the end of the string decodes as if its missing bytes were 0 and still shows its
+ *  continuation pattern. */
+__device__ __forceinline__ uint4 past_end(uint4 data, uintptr_t block, uintptr_t end) {
+    constexpr uint32_t past_k = 0x80808080u;
+    if (block < end && end - block < 16) {
+        uint32_t const kept = (uint32_t)(end - block); // 1..15 bytes
+        auto keep = [&](uint32_t word, uint32_t value) { // bytes of dword `word` from `kept` on become 0x80
+            uint32_t const bytes = kept > 4 * word ? min(kept - 4 * word, 4u) : 0u;
+            uint32_t const mask = bytes == 4 ? ~0u : (1u << (8 * bytes)) - 1u;
+            return (value & mask) | (past_k & ~mask);
+        };
+        data = make_uint4(keep(0, data.x), keep(1, data.y), keep(2, data.z), keep(3, data.w));
+    }
+    return data;
+}
+
+/** This lane's block of the first pass of `decode_wave` over the string at `first`. */
+__device__ __forceinline__ uint4 decode_head(uint8_t const *first, int64_t length) {
+    uintptr_t const start = (uintptr_t)first & ~(uintptr_t)15;
+    return tape_block(start + 16 * __lane_id(), (uintptr_t)first + (uintptr_t)length);
+}
+
+/**
+ *  The whole wave decodes one string (the same arguments in every lane). When the string is well formed (every lead
+ *  byte followed by exactly its continuation bytes, no stray continuation byte; a sequence cut by the end of the
+ *  string is allowed), the walk above visits exactly the bytes that are not continuation bytes, so a rune's index
+ *  is the count of such bytes before it.
+ *
+ *  The string is read in passes of 1 KB from the 16-byte boundary at or before its first byte, one aligned block
+ *  per lane, through an LDS ring of two slots (the next pass's loads are in flight while this one decodes), and
+ *  read back as rows of 64 byte positions, one per lane: a lane takes the 4 bytes from its position (a lead and the
+ *  continuation bytes it may need), the wave's ballot of continuation bytes gives the leads, and the lead lanes of
+ *  a row store their runes to consecutive indices, one coalesced store.
+ *
+ *  Well-formedness, checked on the way: (a) each lead's next `more` bytes (1-3 by its top bits) are continuation
+ *  bytes, or past the string's end; (b) the continuation bytes the leads claim this way, counted up to the end,
+ *  number exactly the string's continuation bytes (its length minus its leads). Claims cannot overlap (a claimed
+ *  byte is a continuation, never a lead), so (b) means no continuation byte is left unclaimed, and with (a) the
+ *  walk steps from lead to lead. Leads 0xF8-0xFF (a 3-bit payload where 0xF0-0xF7 have 3 as well, but the field
+ *  below would take 4) also send the string to the walk. Returns the rune count, or -1 if the string is not well
+ *  formed (the caller then walks it in one lane). The output must have room for one rune per byte. `head` is this
+ *  lane's block of the first pass (`decode_head`), which the caller loads ahead of time.
+ */
+__device__ int64_t decode_wave(uint8_t const *first, int64_t length, uint32_t *out, uint4 head) {
+    constexpr int32_t slot_words = decode_pass_bytes_k / 4;
+    // Per wave of the 256-thread workgroup: two pass slots, then a copy of slot 0's first 16 bytes, so that the
+    // 4 bytes read from any position of slot 1 continue into slot 0 without wrapping.
+    __shared__ __align__(16) uint32_t ring_lds[4][2 * slot_words + 4];
+    uint32_t *ring = ring_lds[threadIdx.x / 64];
+    uint32_t const lane = __lane_id();
+    uintptr_t const start = (uintptr_t)first & ~(uintptr_t)15, end = (uintptr_t)first + (uintptr_t)length;
+    int64_t const skew = (int64_t)((uintptr_t)first - start), span = skew + length; // positions from `start`
+    auto block_of = [&](int64_t pass) { return start + (uintptr_t)(pass * decode_pass_bytes_k) + 16 * lane; };
+    auto stage = [&](int32_t slot, uint4 block, int64_t pass) {
+        block = past_end(block, block_of(pass), end);
+        *(uint4 *)(ring + slot * slot_words + 4 * lane) = block;
+        if (slot == 0 && lane == 0) *(uint4 *)(ring + 2 * slot_words) = block;
+    };
+    stage(0, head, 0);
+    uint4 next = tape_block(block_of(1), end);
+    global_rune_t *runes = (global_rune_t *)out;
+    int64_t written = 0;
+    uint32_t violations = 0; // nonzero if a lead of this lane broke rule (a)
+    uint64_t high_leads = 0; // lanes that saw a lead 0xF8-0xFF
+    int32_t claimed = 0;     // this lane's share of the continuation bytes the leads claim, rule (b)
+    for (int64_t base = 0, pass = 0; base < span; base += decode_pass_bytes_k, ++pass) {
+        int32_t const slot = (int32_t)(pass & 1);
+        __builtin_amdgcn_wave_barrier(); // the previous pass's reads of the other slot come first
+        stage(slot ^ 1, next, pass + 1); // this pass's last runes may need the next pass's first 3 bytes
+        next = tape_block(block_of(pass + 2), end);
+        __builtin_amdgcn_wave_barrier(); // the wave's LDS operations complete in order: stores, then these loads
+        global_rune_t *pass_out = runes + written; // a pass writes at most 1 KB of runes: 32-bit offsets from here
+        // Positions [from, to) of this pass hold the string's bytes.
+        int32_t const from = (int32_t)max((int64_t)0, skew - base);
+        int32_t const to = (int32_t)min((int64_t)decode_pass_bytes_k, span - base);
+        uint32_t const *words = ring + slot * slot_words + lane / 4;
+        auto row_bytes = [&](int32_t row) { // the 4 bytes from this lane's position of `row`
+            return __builtin_amdgcn_alignbyte(words[row * 16 + 1], words[row * 16], lane & 3u);
+        };
+        // The string's bytes after position 0 of this pass (more than a lead can claim once the string goes on).
+        int32_t const after_row0 = (int32_t)min(span - 1 - base, (int64_t)decode_pass_bytes_k + 3);
+        uint32_t pass_written = 0;
+        auto decode_row = [&](int32_t row, uint32_t bytes, uint64_t in_mask) __attribute__((always_inline)) {
+            uint32_t const lead = bytes & 0xFFu;
+            uint64_t const leads = ~__builtin_amdgcn_ballot_w64((bytes & 0xC0u) == 0x80u) & in_mask;
+            high_leads |= __builtin_amdgcn_ballot_w64(lead >= 0xF8u) & in_mask;
+            if (__builtin_amdgcn_inverse_ballot_w64(leads)) {
+                // Continuation bytes the lead announces: 0 below 0xC0, 1 for 0xC_ and 0xD_, 2 for 0xE_, 3 for 0xF_.
+                uint32_t const more = __builtin_amdgcn_ubfe(0xE5000000u, __builtin_amdgcn_ubfe(bytes, 4, 4) * 2, 2);
+                // (a): bytes 1 .. more are continuation bytes (past the end they read as 0x80).
+                uint32_t const pattern = (bytes ^ 0x80808080u) & 0xC0C0C0C0u; // zero bytes: continuation bytes
+                violations |= __builtin_amdgcn_ubfe(pattern, 8, 8 * more);
+                // (b): the claim, cut at the string's end.
+                claimed += min((int32_t)more, after_row0 - row * 64 - (int32_t)lane);
+                // The lead byte over the continuation bytes' 6 bits each, as if the sequence were 4 bytes long: the
+                // rune is its field from bit 6 * (3 - more), 5 * more + 7 bits wide (the lead's marker bits above
+                // the payload are dropped; the one bit past the payload is zero in every lead below 0xF8).
+                uint32_t const full = lead << 18 | __builtin_amdgcn_ubfe(bytes, 8, 6) << 12 |
+                                      __builtin_amdgcn_ubfe(bytes, 16, 6) << 6 | __builtin_amdgcn_ubfe(bytes, 24, 6);
+                uint32_t const rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(leads >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)leads, 0u));
+                *(global_rune_t *)((global_byte_t *)pass_out + ((pass_written + rank) << 2)) =
+                    __builtin_amdgcn_ubfe(full, 18 - 6 * more, 5 * more + 7);
+            }
+            pass_written += (uint32_t)__builtin_popcountll(leads);
+        };
+        auto partial_row = [&](int32_t row) { // lanes of positions [from, to) only
+            int32_t const low = max(from - row * 64, 0), high = min(to - row * 64, 64);
+            uint64_t const upto = high >= 64 ? ~0ull : (1ull << high) - 1u;
+            decode_row(row, row_bytes(row), upto & ~((1ull << low) - 1u));
+        };
+        // Rows in order: a partial first row, whole rows (four at a time, their LDS reads issued together), a
+        // partial last row.
+        int32_t const whole_begin = (from + 63) / 64, whole_end = to / 64;
+        if (whole_begin > whole_end) partial_row(from / 64); // the pass's bytes lie inside one row
+        else {
+            if (from % 64) partial_row(from / 64);
+            int32_t row = whole_begin;
+            for (; row + 4 <= whole_end; row += 4) {
+                uint32_t const a = row_bytes(row), b = row_bytes(row + 1), c = row_bytes(row + 2);
+                uint32_t const d = row_bytes(row + 3);
+                decode_row(row, a, ~0ull);
+                decode_row(row + 1, b, ~0ull);
+                decode_row(row + 2, c, ~0ull);
+                decode_row(row + 3, d, ~0ull);
+            }
+            for (; row < whole_end; ++row) decode_row(row, row_bytes(row), ~0ull);
+            if (to % 64) partial_row(whole_end);
+        }
+        written += pass_written;
+        if (__builtin_amdgcn_ballot_w64(violations != 0u) | high_leads) return -1;
+    }
+    int64_t claims = claimed;
+    for (int offset = 32; offset; offset >>= 1) claims += __shfl_xor(claims, offset, 64);
+    return claims == length - written ? written : -1;
 }
 
 /**
@@ -723,20 +782,36 @@ __device__ int64_t decode_wave(uint8_t const *first, uint8_t const *last, uint32
  *  validated, and continuation bytes missing at the end of a string read as 0. Runes of string `s` start at
  *  index `offsets[s]` of the tape's `runes`; their count goes to `rune_counts[s]`.
  *
- *  Each wave takes 64 strings: the short ones are walked one per lane, then the whole wave decodes each long one
- *  (`decode_wave`, falling back to a one-lane walk for malformed strings). Runes never go past the buffer's
+ *  Waves draw tickets of 8 strings, T apart (T tickets in all), from a queue (`prepare_stats_t::decode_ticket`,
+ *  zeroed before the launch), so a wave that drew long strings never holds the launch back while others idle; a
+ *  ticket's offsets load at once, then the whole wave decodes its strings one after another (`decode_wave`, with the next
+ *  string's first block and the next ticket requested ahead; a malformed string, or one the buffer has too little
+ *  room for, is walked by one lane). Runes never go past the buffer's
  *  capacity: a tape whose offsets run further than the byte count the host sized it for keeps its strings
  *  truncated there and raises its segment's flag.
  */
-__global__ void utf8_decode_kernel(utf8_tape_t const *tapes, uint32_t tapes_count, uint64_t strings_total,
-                                   prepare_stats_t *stats, uint32_t *segment_flags) {
-    constexpr int64_t long_string_k = 256;
+__global__ __launch_bounds__(256) void utf8_decode_kernel(utf8_tape_t const *tapes, uint32_t tapes_count,
+                                                          uint64_t strings_total, prepare_stats_t *stats,
+                                                          uint32_t *segment_flags) {
     uint32_t const lane = __lane_id();
-    uint64_t const waves = (uint64_t)gridDim.x * (blockDim.x / 64);
-    uint64_t const wave = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    for (uint64_t base = wave * 64; base < strings_total; base += waves * 64) {
-        uint64_t const global = base + lane;
-        bool const mine = global < strings_total;
+    auto uniform = [](uint64_t value, uint32_t from) { // lane `from`'s value, in every lane (a scalar)
+        uint32_t const lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)value, (int)from);
+        uint32_t const hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(value >> 32), (int)from);
+        return (uint64_t)hi << 32 | lo;
+    };
+    auto draw = [&]() { // the first string of a new ticket
+        uint32_t ticket = 0;
+        if (lane == 0) ticket = atomicAdd(&stats->decode_ticket, 1u);
+        return (uint64_t)__builtin_amdgcn_readfirstlane(ticket) * decode_strings_per_ticket_k;
+    };
+    uint64_t const tickets = (strings_total + decode_strings_per_ticket_k - 1) / decode_strings_per_ticket_k;
+    uint64_t first_string = draw();
+    while (first_string < strings_total) {
+        uint64_t const following = draw(); // drawn now, so its latency hides behind this ticket's strings
+        // Ticket t holds strings t, t + T, t + 2T, ... (T tickets in all): runs of long strings, as tapes often
+        // hold, are dealt to different waves instead of one.
+        uint64_t const global = first_string / decode_strings_per_ticket_k + lane * tickets;
+        bool const mine = lane < decode_strings_per_ticket_k && global < strings_total;
         uint32_t low = 0, high = tapes_count - 1;
         while (mine && low < high) {
             uint32_t const middle = (low + high + 1) / 2;
@@ -751,34 +826,44 @@ __global__ void utf8_decode_kernel(utf8_tape_t const *tapes, uint32_t tapes_coun
         bool const out_of_tape = end_read < begin_read || end_read > tape.capacity;
         uint64_t const begin = out_of_tape ? 0 : begin_read, end = out_of_tape ? 0 : end_read;
         uint64_t const room = begin < tape.capacity ? tape.capacity - begin : 0;
-        bool const is_long = mine && (int64_t)(end - begin) >= long_string_k && room >= end - begin;
-        uint64_t written = 0;
-        if (mine && !is_long) written = decode_walk(tape.bytes + begin, tape.bytes + end, tape.runes + begin, room);
-        uint64_t pending = __ballot(is_long);
-        while (pending) {
-            int const source = __builtin_ctzll(pending);
-            pending &= pending - 1;
-            auto broadcast = [&](auto *pointer) { // a pointer of lane `source`, in every lane
-                uint64_t const bits = (uint64_t)pointer;
-                uint32_t const lo = (uint32_t)__shfl((int)(uint32_t)bits, source, 64);
-                uint32_t const hi = (uint32_t)__shfl((int)(uint32_t)(bits >> 32), source, 64);
-                return (decltype(pointer))((uint64_t)hi << 32 | lo);
-            };
-            uint8_t const *first = broadcast(tape.bytes + begin);
-            uint8_t const *last = broadcast(tape.bytes + end);
-            uint32_t *out = broadcast(tape.runes + begin);
-            int64_t const decoded = decode_wave(first, last, out);
-            if ((int)lane == source)
-                written = decoded >= 0 ? (uint64_t)decoded : decode_walk(first, last, out, room);
-        }
-        if (mine) {
-            if (written > room || out_of_tape) { // the offsets disagree with the byte count the buffer was sized for
-                written = out_of_tape ? 0 : room;
-                atomicOr(&stats->bad_tape, 1u);
-                atomicOr(&segment_flags[tape.segment], (uint32_t)segment_bad_tape_k);
+        uint32_t const strings = (uint32_t)__builtin_popcountll(__ballot(mine)); // lanes 0 .. strings - 1
+        // String i's bytes, rune output, length, and whether the whole wave decodes it (room for one rune per byte).
+        auto first_of = [&](uint32_t i) { return (uint8_t const *)uniform((uint64_t)(tape.bytes + begin), i); };
+        auto length_of = [&](uint32_t i) { return (int64_t)uniform(end - begin, i); };
+        auto by_wave = [&](uint32_t i) {
+            int64_t const length = length_of(i);
+            return length > 0 && uniform(room, i) >= (uint64_t)length;
+        };
+        // The first pass's block of the next string is loaded while this one decodes.
+        uint4 head = by_wave(0) ? decode_head(first_of(0), length_of(0)) : make_uint4(0, 0, 0, 0);
+        for (uint32_t i = 0; i < strings; ++i) {
+            uint8_t const *first = first_of(i);
+            uint32_t *out = (uint32_t *)uniform((uint64_t)(tape.runes + begin), i);
+            int64_t const length = length_of(i);
+            uint64_t const space = uniform(room, i);
+            bool const wave_path = by_wave(i);
+            uint4 const current = head;
+            if (i + 1 < strings && by_wave(i + 1)) head = decode_head(first_of(i + 1), length_of(i + 1));
+            uint64_t written = 0;
+            if (length > 0) {
+                int64_t const decoded = wave_path ? decode_wave(first, length, out, current) : -1;
+                if (decoded >= 0) written = (uint64_t)decoded;
+                else {
+                    uint64_t walked = 0;
+                    if (lane == 0) walked = decode_walk(first, first + length, out, space);
+                    written = uniform(walked, 0);
+                }
             }
-            tape.rune_counts[s] = (uint32_t)written;
+            if (lane == i) { // this lane loaded string i's offsets
+                if (written > room || out_of_tape) { // the offsets disagree with the byte count the buffer was sized for
+                    written = out_of_tape ? 0 : room;
+                    atomicOr(&stats->bad_tape, 1u);
+                    atomicOr(&segment_flags[tape.segment], (uint32_t)segment_bad_tape_k);
+                }
+                tape.rune_counts[s] = (uint32_t)written;
+            }
         }
+        first_string = following;
     }
 }
 
@@ -1114,8 +1199,10 @@ hipError_t szs_amd_launch_fill_many(szs_amd::fill_list_t const *list, hipStream_
 hipError_t szs_amd_launch_utf8_decode(szs_amd::utf8_tape_t const *tapes, uint32_t tapes_count, uint64_t strings_total,
                                       szs_amd::prepare_stats_t *stats, uint32_t *segment_flags, hipStream_t stream) {
     if (!strings_total || !tapes_count) return hipSuccess;
-    uint64_t const blocks64 = (strings_total + 255) / 256;
-    uint32_t const blocks = blocks64 < 65536 ? (uint32_t)blocks64 : 65536;
+    // Enough waves to fill every SIMD several times over; they draw tickets of 8 strings until none are left.
+    uint64_t const tickets = (strings_total + szs_amd::decode_strings_per_ticket_k - 1) / szs_amd::decode_strings_per_ticket_k;
+    uint64_t const blocks64 = (tickets + 3) / 4;
+    uint32_t const blocks = blocks64 < 2048 ? (uint32_t)blocks64 : 2048;
     hipLaunchKernelGGL(szs_amd::utf8_decode_kernel, dim3(blocks), dim3(256), 0, stream, tapes, tapes_count,
                        strings_total, stats, segment_flags);
     return hipGetLastError();

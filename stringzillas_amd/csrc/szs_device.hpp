@@ -330,7 +330,7 @@ struct prepare_stats_t {
     uint32_t chain_timeout;              /**< nonzero if a chained group gave up waiting for its predecessor */
     uint32_t wide_pairs;                 /**< pairs scored by the int64 shape: slots of the wide rows */
     uint32_t wide_columns;               /**< most columns of such a pair */
-    uint32_t reserved;
+    uint32_t decode_ticket;              /**< next ticket of the UTF-8 decode queue (8 strings each) */
 };
 
 /** A tape byte count the host did not read back from HBM: kernels that check offsets read `offsets[count]`. */

@@ -159,11 +159,29 @@ def test_myers_tail_stripes_over_runes(alphabet, rune_ladder):
 
 @pytest.mark.parametrize("alphabet", ["cyrillic", "cjk", "mixed"])
 def test_wave_decode_across_lane_and_pass_boundaries(alphabet):
-    """Long strings decode 16 bytes per lane, 1 KB per pass: sequences straddle lanes and passes, and strings cut
-    at byte lengths around those edges end inside a sequence."""
+    """Strings decode in passes of 1 KB from the 16-byte boundary at or before their first byte, as rows of 64 byte
+    positions: sequences straddle rows and passes, strings start at every offset within a 16-byte block, and strings
+    cut at byte lengths around those edges end inside a sequence."""
     rng = np.random.default_rng(zlib.crc32(f"decode:{alphabet}".encode()))
     letters = ALPHABETS[alphabet]
-    sizes = [256, 257, 271, 1008, 1023, 1024, 1025, 1039, 2047, 2048, 2049, 3073, 5000]
+    sizes = [1, 2, 3, 15, 16, 17, 63, 64, 65, 127, 256, 257, 271, 1008, 1023, 1024, 1025, 1039, 2047, 2048, 2049,
+             3073, 5000]
     a = [_text(rng, letters, size)[:size] for size in sizes for _ in range(3)]
     b = [_text(rng, letters, size + 5)[: size + int(rng.integers(0, 40))] for size in sizes for _ in range(3)]
     _check(a, b, f"{alphabet} decode boundaries")
+
+
+@pytest.mark.parametrize("position", [0, 1, 62, 63, 64, 1021, 1022, 1023, 1024, 1025, 2047, 2048])
+def test_wave_decode_rejects_malformed_bytes_at_row_and_pass_edges(position):
+    """A stray continuation byte, a lead cut short by an ASCII byte, or a lead 0xF8-0xFF placed at a row or pass
+    edge of an otherwise well-formed string sends the string to the one-lane walk, whose runes the oracle fixes."""
+    rng = np.random.default_rng(position)
+    letters = ALPHABETS["cjk"]
+    a, b = [], []
+    for flaw in (b"\x80", b"\xe4a", b"\xf0\x9fa", b"\xf9\x80\x80\x80", b"\xbf\xbf"):
+        for skew in (0, 5, 15):
+            clean = _text(rng, letters, position + 64)
+            text = (b"x" * skew + clean)[: skew + position] + flaw + clean[: 40]
+            a.append(text)
+            b.append(_text(rng, letters, 80))
+    _check(a, b, f"malformed at {position}")
