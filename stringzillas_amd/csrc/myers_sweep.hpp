@@ -6,20 +6,19 @@
  *  The reference routes unit costs to Myers as well (include/stringzillas/similarities/serial.hpp:1657-1865,
  *  2126-2140); results equal the unit-cost DP, which is what the tests pin against the oracle.
  *
- *  Layout: lane k of a group owns one 64-row word of the stripe (rows 64k .. 64k+63, bit 0 = top row; 32-row
- *  words for the half-height shapes) as the
- *  vertical-delta vectors Pv/Mv. At step t it processes column j = t + 1 - k and passes the horizontal delta of
- *  its bottom row (+1, 0 or -1, coded in 2 bits) to lane k+1 by DPP; lane 0 receives +1 from the matrix border
- *  or the previous stripe's bottom deltas from HBM scratch. The match vector of a column symbol is built from
- *  bit-planes: plane b holds bit b of every row's symbol code, and Eq = AND_b (plane_b XNOR bit b of the column
- *  code), one `v_bitop3` per plane and half-word. For 64-lane groups the symbols are first renumbered to a dense
- *  code (0 = absent from the rows) so that only ceil(log2(distinct + 1)) planes are needed, e.g. 3 for DNA; every
- *  group renumbers its own pair (4- and 16-lane groups compute codes from a presence set and its prefix counts).
- *  When every group of the wave has at most 15 distinct symbols, each stripe instead writes the match word of
- *  every code to an 8 KB LDS table ([16 codes][lanes] u64 per group), and a step reads its word back (for 64-lane
- *  groups prefetched one step ahead) in place of the plane arithmetic. For 64-lane groups lane 0's inputs of a 64-step chunk (horizontal deltas from
- *  the stripe above, column codes) are staged in the LDS ring and read back as one broadcast entry per step.
- *  The distance is m + the sum of the vertical deltas of the last column over all rows.
+ *  Layout: lane k of a group owns one 64-row word of the stripe (rows 64k .. 64k+63, bit 0 = top row; 32-row words for
+ *  the half-height shapes) as the vertical-delta vectors Pv/Mv. At step t it processes column j = t + 1 - k and passes
+ *  the horizontal delta of its bottom row (+1, 0 or -1, coded in 2 bits) to lane k+1 by DPP; lane 0 receives +1 from
+ *  the matrix border or the previous stripe's bottom deltas from HBM scratch (one byte per column). The match vector of
+ *  a column symbol is built from bit-planes: plane b holds bit b of every row's symbol code, and Eq = AND_b (plane_b
+ *  XNOR bit b of the column code), one `v_bitop3` per plane and half-word. For 64-lane groups the symbols are first
+ *  renumbered to a dense code (0 = absent from the rows) so that only ceil(log2(distinct + 1)) planes are needed, e.g.
+ *  3 for DNA; every group renumbers its own pair (4- and 16-lane groups compute codes from a presence set and its
+ *  prefix counts). When every group of the wave has at most 15 distinct symbols, each stripe instead writes the match
+ *  word of every code to an 8 KB LDS table ([16 codes][lanes] u64 per group), and a step reads its word back (for
+ *  64-lane groups prefetched one step ahead) in place of the plane arithmetic. For 64-lane groups lane 0's inputs of a
+ *  64-step chunk (horizontal deltas from the stripe above, column codes) are staged in the LDS ring and read back as
+ *  one broadcast entry per step. The distance is m + the sum of the vertical deltas of the last column over all rows.
  */
 #pragma once
 #include <hip/hip_runtime.h>
@@ -291,7 +290,8 @@ struct myers_sweep_t {
     }
 
     /**
-     *  Horizontal deltas are coded as bit 0 = +1, bit 1 = -1. `steady_` promises every lane a valid column on
+     *  Horizontal deltas are coded as bit 0 = +1, bit 1 = -1, one byte per column in the boundary rows (`in_codes`
+     *  and `out_codes` point at byte arrays indexed by column). `steady_` promises every lane a valid column on
      *  every step of the chunk; `collect_` (G = 64) stores lane 63's bottom codes for the next stripe.
      */
     template <bool steady_, bool collect_>
@@ -321,7 +321,7 @@ struct myers_sweep_t {
         int32_t const first_step = chunk * 64;
         int32_t const j_lane = first_step + lane + 1;
         uint32_t const symbol = j_lane <= m ? code_of(columns[j_lane - 1]) : 0u;
-        uint32_t const entering = in_codes ? (j_lane <= m ? (uint32_t)in_codes[j_lane] : 0u) : 1u;
+        uint32_t const entering = in_codes ? (j_lane <= m ? (uint32_t)((uint8_t const *)in_codes)[j_lane] : 0u) : 1u;
         *(uint2 *)(ring + lane * 8) = make_uint2(entering, symbol);
         // Column codes do not depend on the scores, so step s + 1's code and match word are fetched during step s.
         uint2 now = *(uint2 const *)ring, after = *(uint2 const *)(ring + 8);
@@ -348,7 +348,7 @@ struct myers_sweep_t {
         if constexpr (collect_) { // entry l holds step l's bottom code: column first_step + l - 62
             int32_t const j_out = first_step + lane - 62;
             uint32_t const bottom = *(uint32_t const *)(ring + lane * 8);
-            if (j_out >= 1 && j_out <= m) out_codes[j_out] = (int32_t)bottom;
+            if (j_out >= 1 && j_out <= m) ((uint8_t *)out_codes)[j_out] = (uint8_t)bottom;
         }
     }
 
@@ -357,7 +357,7 @@ struct myers_sweep_t {
         int32_t const first_step = chunk * G;
         int32_t const j_lane = first_step + k + 1;
         uint32_t chunk_symbols = j_lane <= m ? code_of(columns[j_lane - 1]) : 0u;
-        uint32_t chunk_codes = in_codes ? (j_lane <= m ? (uint32_t)in_codes[j_lane] : 0u) : 1u;
+        uint32_t chunk_codes = in_codes ? (j_lane <= m ? (uint32_t)((uint8_t const *)in_codes)[j_lane] : 0u) : 1u;
         bool const first_in_group = k == 0;
 #pragma unroll 4
         for (int32_t step = 0; step < G; ++step) {

@@ -190,16 +190,21 @@ struct packed_sweep_t {
         result_register = (last_row % R) % P;
     }
 
-    /** Lane-0 inputs of column `j`: the row above the stripe (stored int32 form), its vertical gap, its symbol. */
-    __device__ __forceinline__ void column_inputs(int32_t j, int32_t const *in_scores, int32_t const *in_gaps,
+    /** Lane-0 inputs of column `j`: the row above the stripe (stored int32 form), its vertical gap, its symbol.
+     *  Boundary rows hold, per column, the u16 score (low) and vertical gap (high) in the base of the chunk that
+     *  wrote them (`in_scores`, u32 per column), and that chunk's base (`in_bases`, int32 per chunk): column j comes
+     *  from chunk (j + 126) / 64, whose lane 63 held it. */
+    __device__ __forceinline__ void column_inputs(int32_t j, int32_t const *in_scores, int32_t const *in_bases,
                                                   int32_t &score, int32_t &gap, uint32_t &symbol) const {
         symbol = j <= m ? (uint32_t)params->byte_to_dense[columns[j - 1]] : 0u;
         // Beyond column m the row above repeats column m's values: near every lane's base, and they only feed
         // cells that are discarded.
         int32_t const at = min(j, m);
         if (in_scores) {
-            score = in_scores[at];
-            gap = affine_k ? in_gaps[at] : 0;
+            uint32_t const both = (uint32_t)in_scores[at];
+            int32_t const written_base = in_bases[(at + 126) >> 6] - packed_centre_k;
+            score = (int32_t)(both & 0xFFFFu) + written_base;
+            gap = affine_k ? (int32_t)(both >> 16) + written_base : 0;
         }
         else {
             int32_t const border = border_score<family_>(at, open, extend);
@@ -262,9 +267,10 @@ struct packed_sweep_t {
     }
 
     /**
-     *  Steps `chunk * 64 .. + 64`. The row above the stripe comes from `in_scores`/`in_gaps` (1-based columns, int32
-     *  anti-diagonal form) or the matrix border; with `collect_`, lane 63's bottom row goes to `out_scores` and
-     *  `out_gaps`. `steady_` promises both halves of every lane a column within 1..m - 1 on every step.
+     *  Steps `chunk * 64 .. + 64`. The row above the stripe comes from `in_scores`/`in_gaps` (1-based columns, the
+     *  packed form of `column_inputs`) or the matrix border; with `collect_`, lane 63's bottom row goes to
+     *  `out_scores` (u16 score and gap per column) and its base to `out_gaps[chunk]`. `steady_` promises both halves
+     *  of every lane a column within 1..m - 1 on every step.
      */
     template <bool steady_, bool collect_>
     __device__ __forceinline__ void run_chunk(int32_t chunk, int32_t const *in_scores, int32_t const *in_gaps,
@@ -348,15 +354,13 @@ struct packed_sweep_t {
                 *(uint32_t *)(ring + step * 16 + 12) =
                     affine_k ? __builtin_amdgcn_perm(vertical, scores[P - 1], 0x07060302u) : scores[P - 1] >> 16;
         }
-        // Entry l holds step l's bottom row (lane 63's, in its base): column first_step + l - 126.
+        // Entry l holds step l's bottom row (lane 63's, in its base): column first_step + l - 126. It goes out as it
+        // is, with that base once per chunk (`column_inputs` reads both).
         if constexpr (collect_) {
             int32_t const j_out = first_step + lane - 126;
             uint32_t const bottom = *(uint32_t const *)(ring + lane * 16 + 12);
-            int32_t const bottom_base = __builtin_amdgcn_readlane(base, 63) - packed_centre_k;
-            if (j_out >= 1 && j_out <= m) {
-                out_scores[j_out] = (int32_t)(bottom & 0xFFFFu) + bottom_base;
-                if constexpr (affine_k) out_gaps[j_out] = (int32_t)(bottom >> 16) + bottom_base;
-            }
+            if (j_out >= 1 && j_out <= m) out_scores[j_out] = (int32_t)bottom;
+            if (lane == 0) out_gaps[chunk] = __builtin_amdgcn_readlane(base, 63);
         }
     }
 
