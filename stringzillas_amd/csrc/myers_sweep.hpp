@@ -104,6 +104,7 @@ struct myers_sweep_t {
     word_t planes[planes_k];
     word_t pv, mv;
     uint32_t column_code, bottom_code;
+    uint32_t top_p, top_n; /**< 64-lane groups: Ph's and Mh's top 32 bits before the shift (bit 31: bottom row) */
     int32_t vertical_sum;
 
     __device__ __forceinline__ uint32_t code_of(uint32_t byte) const {
@@ -224,6 +225,7 @@ struct myers_sweep_t {
         pv = ~(word_t)0; // column 0: D[i][0] = i, every vertical delta is +1
         mv = 0;
         bottom_code = 1;
+        top_p = 1u << 31, top_n = 0;
         column_code = 0;
         if (table_mode) { // the match word of every code for this lane's 64 rows
             for (uint32_t code = 0; code < (uint32_t)myers_eq_codes_k; ++code)
@@ -289,6 +291,35 @@ struct myers_sweep_t {
         mv = ph & xv;
     }
 
+    /** `(x << 1) | (from >> 31)`: a word shifted up one row, its top bit from bit 31 of `from`. */
+    __device__ __forceinline__ static word_t shift_in(word_t x, uint32_t from) {
+        uint32_t const low = __builtin_amdgcn_alignbit((uint32_t)x, from, 31);
+        if constexpr (word_bits_ == 64) {
+            uint32_t const high = __builtin_amdgcn_alignbit((uint32_t)((uint64_t)x >> 32), (uint32_t)x, 31);
+            return (word_t)((uint64_t)high << 32 | low);
+        }
+        else return (word_t)low;
+    }
+
+    /**
+     *  `step_word` for 64-lane groups, with the horizontal delta entering the top row given as bit 31 of two words
+     *  (`hp`: +1, `hn`: -1; the lane above's `top_p`/`top_n`, or the chunk's inputs for lane 0), and the one
+     *  leaving the bottom row left the same way in `top_p`/`top_n`: the words move between lanes as they are and
+     *  shift straight into Ph/Mh, with no 2-bit code to pack and unpack on every step.
+     */
+    __device__ __forceinline__ void step_word_bits(word_t eq, uint32_t hp, uint32_t hn) {
+        word_t const xv = eq | mv;
+        eq |= (word_t)(hn >> 31);
+        word_t const xh = bitop3<0xBE>((eq & pv) + pv, pv, eq); // (((eq & pv) + pv) ^ pv) | eq
+        word_t const ph = bitop3<0xF1>(mv, xh, pv);             // mv | ~(xh | pv)
+        word_t const mh = pv & xh;
+        top_p = (uint32_t)((uint64_t)ph >> (word_bits_ - 32));
+        top_n = (uint32_t)((uint64_t)mh >> (word_bits_ - 32));
+        word_t const ph_in = shift_in(ph, hp), mh_in = shift_in(mh, hn);
+        pv = bitop3<0xF1>(mh_in, xv, ph_in); // mh | ~(xv | ph)
+        mv = ph_in & xv;
+    }
+
     /**
      *  Horizontal deltas are coded as bit 0 = +1, bit 1 = -1, one byte per column in the boundary rows (`in_codes`
      *  and `out_codes` point at byte arrays indexed by column). `steady_` promises every lane a valid column on
@@ -322,33 +353,35 @@ struct myers_sweep_t {
         int32_t const j_lane = first_step + lane + 1;
         uint32_t const symbol = j_lane <= m ? code_of(columns[j_lane - 1]) : 0u;
         uint32_t const entering = in_codes ? (j_lane <= m ? (uint32_t)((uint8_t const *)in_codes)[j_lane] : 0u) : 1u;
-        *(uint2 *)(ring + lane * 8) = make_uint2(entering, symbol);
+        // Ring entry l: lane 0's entering delta at step l as the words of `step_word_bits`, and its column code.
+        *(uint4 *)(ring + lane * 16) = make_uint4(entering << 31, (entering >> 1) << 31, symbol, 0u);
         // Column codes do not depend on the scores, so step s + 1's code and match word are fetched during step s.
-        uint2 now = *(uint2 const *)ring, after = *(uint2 const *)(ring + 8);
-        uint32_t code = (uint32_t)shift_down<64>((int32_t)now.y, (int32_t)column_code);
+        uint4 now = *(uint4 const *)ring, after = *(uint4 const *)(ring + 16);
+        uint32_t code = (uint32_t)shift_down<64>((int32_t)now.z, (int32_t)column_code);
         word_t eq_next = 0;
         if constexpr (table_) eq_next = eq_table[code * 64 + lane];
 #pragma unroll 8
         for (int32_t step = 0; step < 64; ++step) {
-            uint2 const later = *(uint2 const *)(ring + ((step + 2) & 63) * 8);
-            uint32_t const hin = (uint32_t)shift_down<64>((int32_t)now.x, (int32_t)bottom_code);
-            uint32_t const code_after = (uint32_t)shift_down<64>((int32_t)after.y, (int32_t)code);
+            uint4 const later = *(uint4 const *)(ring + ((step + 2) & 63) * 16);
+            uint32_t const hp = (uint32_t)shift_down<64>((int32_t)now.x, (int32_t)top_p);
+            uint32_t const hn = (uint32_t)shift_down<64>((int32_t)now.y, (int32_t)top_n);
+            uint32_t const code_after = (uint32_t)shift_down<64>((int32_t)after.z, (int32_t)code);
             word_t const eq_now = eq_next;
             if constexpr (table_) eq_next = eq_table[code_after * 64 + lane];
             column_code = code;
             int32_t const j = first_step + step + 1 - lane;
-            if (steady_ || (uint32_t)(j - 1) < (uint32_t)m) step_word(table_ ? eq_now : planes_eq<PB>(code), hin);
-            // Lane 63's bottom code goes to ring entry `step`, read two steps earlier: an LDS store under a
-            // one-lane mask instead of a shift register through every lane.
-            if (collect_ && lane == 63) *(uint32_t *)(ring + step * 8) = bottom_code;
+            if (steady_ || (uint32_t)(j - 1) < (uint32_t)m) step_word_bits(table_ ? eq_now : planes_eq<PB>(code), hp, hn);
+            // Lane 63's bottom words go to ring entry `step`, read two steps earlier: LDS stores under a one-lane
+            // mask instead of a shift register through every lane.
+            if (collect_ && lane == 63) *(uint2 *)(ring + step * 16) = make_uint2(top_p, top_n);
             code = code_after;
             now = after;
             after = later;
         }
-        if constexpr (collect_) { // entry l holds step l's bottom code: column first_step + l - 62
+        if constexpr (collect_) { // entry l holds step l's bottom words: column first_step + l - 62
             int32_t const j_out = first_step + lane - 62;
-            uint32_t const bottom = *(uint32_t const *)(ring + lane * 8);
-            if (j_out >= 1 && j_out <= m) ((uint8_t *)out_codes)[j_out] = (uint8_t)bottom;
+            uint2 const bottom = *(uint2 const *)(ring + lane * 16);
+            if (j_out >= 1 && j_out <= m) ((uint8_t *)out_codes)[j_out] = (uint8_t)(bottom.x >> 31 | (bottom.y >> 31) << 1);
         }
     }
 
