@@ -61,6 +61,20 @@ __device__ __forceinline__ int32_t rotate_up(int32_t src) {
     else return __builtin_amdgcn_update_dpp(src, src, dpp_row_ror15_k, 0xF, 0xF, false);
 }
 
+/**
+ *  `from` of the lane below plus `step` in lanes 1..63, `lane0` in lane 0: one `v_add_u32_dpp` (wave_shr:1; lane 0
+ *  has no source lane, so it keeps the value its destination register already held). Written out because the
+ *  compiler fuses a DPP move into an add only with a zero for lane 0. The `s_nop 1` gives the two wait states a DPP
+ *  read needs after a vector write of its source, which the compiler cannot see inside the asm.
+ */
+__device__ __forceinline__ uint32_t lane_above_plus(uint32_t from, uint32_t step, uint32_t lane0) {
+    uint32_t result = lane0;
+    asm("s_nop 1\n\tv_add_u32_dpp %0, %1, %2 wave_shr:1 row_mask:0xf bank_mask:0xf"
+        : "+v"(result)
+        : "v"(from), "v"(step));
+    return result;
+}
+
 /** Shift register toward lane 0 in which lane 63 appends its own `fresh` value (G = 64 only). */
 __device__ __forceinline__ int32_t collect_from_last(int32_t fresh, int32_t stage) {
     return __builtin_amdgcn_update_dpp(fresh, stage, dpp_wave_shl1_k, 0xF, 0xF, false);

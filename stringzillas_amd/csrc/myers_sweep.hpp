@@ -105,6 +105,7 @@ struct myers_sweep_t {
     word_t pv, mv;
     uint32_t column_code, bottom_code;
     uint32_t top_p, top_n; /**< 64-lane groups: Ph's and Mh's top 32 bits before the shift (bit 31: bottom row) */
+    uint32_t column_at;    /**< 64-lane groups in table mode: LDS address of this lane's match word of `column_code` */
     int32_t vertical_sum;
 
     __device__ __forceinline__ uint32_t code_of(uint32_t byte) const {
@@ -226,6 +227,7 @@ struct myers_sweep_t {
         mv = 0;
         bottom_code = 1;
         top_p = 1u << 31, top_n = 0;
+        column_at = eq_address(0);
         column_code = 0;
         if (table_mode) { // the match word of every code for this lane's 64 rows
             for (uint32_t code = 0; code < (uint32_t)myers_eq_codes_k; ++code)
@@ -291,6 +293,12 @@ struct myers_sweep_t {
         mv = ph & xv;
     }
 
+    /** The LDS address of this lane's match word of `code` in the table (the low 32 bits of a generic address in
+     *  the LDS aperture are the LDS address). */
+    __device__ __forceinline__ uint32_t eq_address(uint32_t code) const {
+        return (uint32_t)(uintptr_t)(eq_table + code * G + k);
+    }
+
     /** `(x << 1) | (from >> 31)`: a word shifted up one row, its top bit from bit 31 of `from`. */
     __device__ __forceinline__ static word_t shift_in(word_t x, uint32_t from) {
         uint32_t const low = __builtin_amdgcn_alignbit((uint32_t)x, from, 31);
@@ -353,28 +361,43 @@ struct myers_sweep_t {
         int32_t const j_lane = first_step + lane + 1;
         uint32_t const symbol = j_lane <= m ? code_of(columns[j_lane - 1]) : 0u;
         uint32_t const entering = in_codes ? (j_lane <= m ? (uint32_t)((uint8_t const *)in_codes)[j_lane] : 0u) : 1u;
-        // Ring entry l: lane 0's entering delta at step l as the words of `step_word_bits`, and its column code.
-        *(uint4 *)(ring + lane * 16) = make_uint4(entering << 31, (entering >> 1) << 31, symbol, 0u);
+        // Ring entry l: lane 0's entering delta at step l as the words of `step_word_bits`, and its column code (in
+        // table mode the LDS address of lane 0's match word of it: lane k's is lane k - 1's plus one word).
+        constexpr uint32_t word_bytes = sizeof(word_t);
+        *(uint4 *)(ring + lane * 16) = make_uint4(entering << 31, (entering >> 1) << 31,
+                                                  table_ ? eq_address(symbol) - word_bytes * lane : symbol, 0u);
         // Column codes do not depend on the scores, so step s + 1's code and match word are fetched during step s.
         uint4 now = *(uint4 const *)ring, after = *(uint4 const *)(ring + 16);
-        uint32_t code = (uint32_t)shift_down<64>((int32_t)now.z, (int32_t)column_code);
+        typedef __attribute__((address_space(3))) word_t const lds_word_t;
+        uint32_t code = 0, at = 0;
         word_t eq_next = 0;
-        if constexpr (table_) eq_next = eq_table[code * 64 + lane];
+        if constexpr (table_) {
+            at = lane_above_plus(column_at, word_bytes, now.z);
+            eq_next = *(lds_word_t *)(uintptr_t)at;
+        }
+        else code = (uint32_t)shift_down<64>((int32_t)now.z, (int32_t)column_code);
 #pragma unroll 8
         for (int32_t step = 0; step < 64; ++step) {
             uint4 const later = *(uint4 const *)(ring + ((step + 2) & 63) * 16);
             uint32_t const hp = (uint32_t)shift_down<64>((int32_t)now.x, (int32_t)top_p);
             uint32_t const hn = (uint32_t)shift_down<64>((int32_t)now.y, (int32_t)top_n);
-            uint32_t const code_after = (uint32_t)shift_down<64>((int32_t)after.z, (int32_t)code);
+            uint32_t code_after = 0, at_after = 0;
             word_t const eq_now = eq_next;
-            if constexpr (table_) eq_next = eq_table[code_after * 64 + lane];
-            column_code = code;
+            if constexpr (table_) {
+                at_after = lane_above_plus(at, word_bytes, after.z);
+                eq_next = *(lds_word_t *)(uintptr_t)at_after;
+                column_at = at;
+            }
+            else {
+                code_after = (uint32_t)shift_down<64>((int32_t)after.z, (int32_t)code);
+                column_code = code;
+            }
             int32_t const j = first_step + step + 1 - lane;
             if (steady_ || (uint32_t)(j - 1) < (uint32_t)m) step_word_bits(table_ ? eq_now : planes_eq<PB>(code), hp, hn);
             // Lane 63's bottom words go to ring entry `step`, read two steps earlier: LDS stores under a one-lane
             // mask instead of a shift register through every lane.
             if (collect_ && lane == 63) *(uint2 *)(ring + step * 16) = make_uint2(top_p, top_n);
-            code = code_after;
+            code = code_after, at = at_after;
             now = after;
             after = later;
         }
