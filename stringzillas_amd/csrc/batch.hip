@@ -59,9 +59,9 @@ namespace szs_amd {
  *  rows, 1024-row stripes on one wave up to 4096 rows, then the cooperative 1024-row shape, chained 512-row
  *  groups beyond 24,576 rows. Class-cost families keep LDS profiles small: at most 8 rows per lane, 512-row
  *  stripes on one wave up to 2048 rows, then the cooperative 512-row shape (chained beyond 24,576 rows for the
- *  global families). Unit-cost bytes and runes: DP up to 64 rows, then bit-parallel words: 4 lanes up to 256
- *  rows, 16 lanes of 32-row words up to 512, 16 lanes up to 1024, one wave of 32-row words up to 2048, 4096-row
- *  stripes on one wave up to 16384, then cooperative.
+ *  global families). Unit-cost bytes and runes: DP up to 64 rows, then bit-parallel words: 4 lanes of 32-row
+ *  words up to 128 rows (bytes only), 4 lanes up to 256, 16 lanes of 32-row words up to 512, 16 lanes up to 1024,
+ *  one wave of 32-row words up to 2048, 4096-row stripes on one wave up to 16384, then cooperative.
  */
 /** Pairs taller than this go to the chained shape (global families): one workgroup each would set the batch's
  *  critical path, 16 share the pair instead. */
@@ -75,7 +75,11 @@ static shape_table_t default_shape_table() {
         std::vector<std::pair<int, int>> ladder; // (shape, most rows)
         int multi = 10;
         int const family = slot / 2;
-        if (slot == unit_bytes_slot_k || slot == unit_runes_slot_k) {
+        if (slot == unit_bytes_slot_k) {
+            ladder = {{0, 16}, {1, 32}, {2, 64}, {25, 128}, {11, 256}, {21, 512}, {12, 1024}, {22, 2048}, {13, 16384}};
+            multi = 14;
+        }
+        else if (slot == unit_runes_slot_k) { // 32-row words over runes were slower than 64-row ones at 65-128 rows
             ladder = {{0, 16}, {1, 32}, {2, 64}, {11, 256}, {21, 512}, {12, 1024}, {22, 2048}, {13, 16384}};
             multi = 14;
         }

@@ -381,10 +381,11 @@ __device__ void run_family(int shape, item_args_t const &a) {
     case 22: run_shape<family_, symbol_type_, 22, shifted_>(a); break;
     case 23: run_shape<family_, symbol_type_, 23, shifted_>(a); break;
     case 24: run_shape<family_, symbol_type_, 24, shifted_>(a); break;
+    case 25: run_shape<family_, symbol_type_, 25, shifted_>(a); break;
     default: break;
     }
 }
-static_assert(shapes_count_k == 25, "dispatch switch lists every shape");
+static_assert(shapes_count_k == 26, "dispatch switch lists every shape");
 
 __device__ void run_variant(int variant, item_args_t const &a) {
     int const shape = variant % shapes_count_k;

@@ -28,6 +28,9 @@ LADDERS = {
     "cooperative": ([], MYERS[(64, 64, 1)]),
     # 32-row words: 16-lane groups up to 512 rows, then 2048-row stripes on one wave for every taller pair.
     "half_words": ([(MYERS[(16, 32, 0)], 512), (MYERS[(64, 32, 0)], 1 << 20)], MYERS[(64, 64, 1)]),
+    # 4-lane groups of 32-row words up to 128 rows, then 4-lane groups of 64-row words.
+    "small_half_words": ([(MYERS[(4, 32, 0)], 128), (MYERS[(4, 64, 0)], 256), (MYERS[(16, 64, 0)], 1024)],
+                         MYERS[(64, 64, 1)]),
 }
 ALPHABETS = {
     "ascii": [chr(c) for c in range(0x20, 0x7F)],
@@ -83,7 +86,7 @@ def test_rune_myers_matches_oracle(alphabet, ladder, rune_ladder):
     rng = np.random.default_rng(zlib.crc32(f"{alphabet}:{ladder}".encode()))
     rune_ladder(ladder)
     letters = ALPHABETS[alphabet]
-    edges = [e for e in EDGES if ladder != "small_groups" or e <= 1025]
+    edges = [e for e in EDGES if ladder not in ("small_groups", "small_half_words") or e <= 1025]
     if ladder == "half_words":
         edges += [511, 512, 513, 2047, 2048, 2049]
     a = [_text(rng, letters, n) for n in edges for _ in edges]
@@ -114,13 +117,15 @@ def test_rune_myers_wider_columns_and_malformed_bytes(ladder, rune_ladder):
     _check(a, b, f"wide columns and malformed bytes, {ladder}")
 
 
+@pytest.mark.parametrize("ladder", ["half_words", "small_half_words"])
 @pytest.mark.parametrize("alphabet", [b"AC", b"ACGT", bytes(range(32, 127)), bytes(range(256))])
-def test_half_word_myers_over_bytes(alphabet, byte_ladder):
-    """32-row words over bytes: table mode (few symbols) and bit-planes (up to 256), one and several stripes."""
-    rng = np.random.default_rng(zlib.crc32(alphabet))
-    byte_ladder("half_words")
+def test_half_word_myers_over_bytes(alphabet, ladder, byte_ladder):
+    """32-row words over bytes: table mode (few symbols) and bit-planes (up to 256), one and several stripes, in
+    16-lane and 64-lane groups and in 4-lane groups (up to 128 rows, 16 pairs per wave)."""
+    rng = np.random.default_rng(zlib.crc32(alphabet + ladder.encode()))
+    byte_ladder(ladder)
     table = np.frombuffer(alphabet, dtype=np.uint8)
-    lengths = [1, 31, 32, 33, 300, 511, 512, 513, 2047, 2048, 2049, 5000]
+    lengths = [1, 31, 32, 33, 63, 64, 65, 96, 127, 128, 129, 300, 511, 512, 513, 2047, 2048, 2049, 5000]
     a = [bytes(rng.choice(table, size=n)) for n in lengths for _ in lengths]
     b = [bytes(rng.choice(table, size=m)) for _ in lengths for m in lengths]
     engine = szs.LevenshteinDistances()
