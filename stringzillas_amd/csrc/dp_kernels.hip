@@ -886,25 +886,29 @@ __global__ void widen_offsets_kernel(uint32_t const *narrow, uint64_t *wide, uin
 }
 
 /**
- *  Adds one to `counters[key]` for every active lane with `valid`, issuing one atomic per distinct key of the wave
- *  (the pairs of a batch often share a few bins, and one atomic per pair on one counter serializes in L2). Returns
- *  the value the counter had before this lane's increment, as a plain per-lane atomicAdd would (in lane order
- *  within the wave). Any subset of lanes may call it, as long as every active lane does.
+ *  Adds each valid lane's `count` to `counters[key]` with one atomic per distinct key of the wave, and returns the
+ *  counter's value before this lane's share (lanes of one key in lane order). Every lane of the wave must call it.
  */
-__device__ __forceinline__ uint32_t wave_counted_add(uint32_t *counters, uint32_t key, bool valid) {
+__device__ __forceinline__ uint32_t wave_counted_add_n(uint32_t *counters, uint32_t key, uint32_t count, bool valid) {
     uint32_t const lane = __lane_id();
     uint64_t pending = __ballot(valid);
     uint32_t mine = 0;
-    while (pending) { // one pass per distinct key, uniform over the active lanes
+    while (pending) { // one pass per distinct key
         int const leader = __builtin_ctzll(pending);
         uint32_t const leader_key = (uint32_t)__shfl((int)key, leader, 64);
-        uint64_t const same = __ballot(((pending >> lane) & 1ull) && key == leader_key);
+        bool const same = ((pending >> lane) & 1ull) && key == leader_key;
+        uint32_t const share = same ? count : 0u;
+        uint32_t inclusive = share;
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t const other = (uint32_t)__shfl_up((int)inclusive, d, 64);
+            if ((int)lane >= d) inclusive += other;
+        }
+        uint32_t const total = (uint32_t)__shfl((int)inclusive, 63, 64);
         uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&counters[leader_key], (uint32_t)__popcll(same));
+        if ((int)lane == leader) base = atomicAdd(&counters[leader_key], total);
         base = (uint32_t)__shfl((int)base, leader, 64);
-        if ((same >> lane) & 1ull)
-            mine = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0));
-        pending &= ~same;
+        if (same) mine = base + inclusive - share;
+        pending &= ~__ballot(same);
     }
     return mine;
 }
@@ -950,6 +954,10 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
     __syncthreads();
     unsigned long long cells = 0, core_ops = 0;
     uint32_t widest = 0, overflow = 0;
+    // This lane's run of tasks in one bin (consecutive iterations), counted into the histogram when the bin
+    // changes and, wave-aggregated, at the end: a batch of millions of pairs in one bin (a cross-product tile)
+    // would otherwise queue one atomic per wave on one counter.
+    uint32_t run_bin = ~0u, run_tasks = 0;
     for (uint64_t global = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; global < tasks_total;
          global += (uint64_t)gridDim.x * blockDim.x) {
         // The segment holding this task: the last one whose first task is at or before it.
@@ -1098,10 +1106,15 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
             core_ops += (unsigned long long)rows * task.columns_length * core_ops_per_64_cells(family, symbols, sh);
             if (rows > stripe && !sh.wide && task.columns_length > widest) widest = task.columns_length;
         }
-        (void)wave_counted_add(histogram, bin, bin != ~0u);
+        if (bin != run_bin) {
+            if (run_bin != ~0u) atomicAdd(&histogram[run_bin], run_tasks);
+            run_bin = bin, run_tasks = 0;
+        }
+        ++run_tasks;
         tasks[global] = task;
         bin_of_task[global] = bin;
     }
+    (void)wave_counted_add_n(histogram, run_bin, run_tasks, run_bin != ~0u);
     // Every lane is active again here: one LDS atomic per wave.
     cells = wave_sum(cells), core_ops = wave_sum(core_ops);
     widest = wave_max(widest), overflow = wave_max(overflow);
@@ -1171,17 +1184,30 @@ __global__ __launch_bounds__(1024) void scan_bins_carry_kernel(bins_view_t bins)
         bins.item_start[bins_count_k] = carry_items + bins.scan_blocks[2 * blockIdx.x + 1];
 }
 
-/** Scatters task ids into `order`, grouped by bin (order inside a bin is arbitrary and does not affect results). */
+/**
+ *  Scatters task ids into `order`, grouped by bin (order inside a bin is arbitrary and does not affect results).
+ *  Each lane reserves the slots of its run of tasks in one bin (consecutive iterations) at once, when the bin changes
+ *  and, wave-aggregated, at the end, then writes their ids: a bin of millions of tasks takes a few thousand atomics.
+ */
 __global__ void scatter_bins_kernel(bins_view_t bins, uint32_t const *bin_of_task, uint32_t const *position_of_bin,
                                     uint32_t tasks_count) {
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tasks_count; t += gridDim.x * blockDim.x) {
+    uint32_t const stride = gridDim.x * blockDim.x;
+    uint32_t run_bin = ~0u, run_tasks = 0, run_first = 0;
+    auto place = [&](uint32_t rank) { // the run's ids into its slots from `rank` in its bin
+        uint32_t const at = bins.task_start[position_of_bin[run_bin]] + rank;
+        for (uint32_t i = 0; i < run_tasks; ++i) bins.order[at + i] = run_first + i * stride;
+    };
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tasks_count; t += stride) {
         uint32_t const bin = bin_of_task[t];
-        bool const valid = bin != ~0u;
-        uint32_t const rank = wave_counted_add(bins.cursor, bin, valid); // lanes of the wave take adjacent slots
-        if (!valid) continue;
-        bins.order[bins.task_start[position_of_bin[bin]] + rank] = t;
-        bins.shape_of_task[t] = (bin / column_buckets_k) % shapes_count_k;
+        if (bin != run_bin) {
+            if (run_bin != ~0u) place(atomicAdd(&bins.cursor[run_bin], run_tasks));
+            run_bin = bin, run_tasks = 0, run_first = t;
+        }
+        ++run_tasks;
+        if (bin != ~0u) bins.shape_of_task[t] = (bin / column_buckets_k) % shapes_count_k;
     }
+    uint32_t const rank = wave_counted_add_n(bins.cursor, run_bin, run_tasks, run_bin != ~0u);
+    if (run_bin != ~0u) place(rank);
 }
 
 #pragma endregion
@@ -1234,8 +1260,9 @@ hipError_t szs_amd_launch_materialize(szs_amd::materialize_args_t const *args, u
                                       szs_amd::segment_view_t const *segments, szs_amd::prepare_stats_t *stats,
                                       uint32_t *segment_overflow, hipStream_t stream) {
     if (!tasks_total || !segments_count) return hipSuccess;
-    uint64_t const blocks64 = (tasks_total + 255) / 256;
-    uint32_t const blocks = blocks64 < 65536 ? (uint32_t)blocks64 : 65536;
+    // About 8 tasks per lane (more past 2 M tasks): each lane's run of tasks in one bin is counted with one atomic.
+    uint64_t const blocks64 = (tasks_total + 8 * 256 - 1) / (8 * 256);
+    uint32_t const blocks = blocks64 < 1024 ? (uint32_t)blocks64 : 1024;
     hipLaunchKernelGGL(szs_amd::materialize_kernel, dim3(blocks), dim3(256), 0, stream, args, segments_count,
                        tasks_total, table, tasks, bin_of_task, histogram, segments, stats, segment_overflow);
     return hipGetLastError();
@@ -1252,7 +1279,9 @@ hipError_t szs_amd_launch_scan_bins(szs_amd::bins_view_t const *bins, hipStream_
 hipError_t szs_amd_launch_scatter_bins(szs_amd::bins_view_t const *bins, uint32_t const *bin_of_task,
                                        uint32_t const *position_of_bin, uint32_t tasks_count, hipStream_t stream) {
     if (!tasks_count) return hipSuccess;
-    uint32_t const blocks = (tasks_count + 255) / 256 < 65536 ? (tasks_count + 255) / 256 : 65536;
+    // About 8 tasks per lane (more past 2 M tasks), as for materialization.
+    uint32_t const blocks64 = (tasks_count + 8 * 256 - 1) / (8 * 256);
+    uint32_t const blocks = blocks64 < 1024 ? blocks64 : 1024;
     hipLaunchKernelGGL(szs_amd::scatter_bins_kernel, dim3(blocks), dim3(256), 0, stream, *bins, bin_of_task,
                        position_of_bin, tasks_count);
     return hipGetLastError();
