@@ -1260,8 +1260,9 @@ hipError_t szs_amd_launch_materialize(szs_amd::materialize_args_t const *args, u
                                       szs_amd::segment_view_t const *segments, szs_amd::prepare_stats_t *stats,
                                       uint32_t *segment_overflow, hipStream_t stream) {
     if (!tasks_total || !segments_count) return hipSuccess;
-    // About 8 tasks per lane (more past 2 M tasks): each lane's run of tasks in one bin is counted with one atomic.
-    uint64_t const blocks64 = (tasks_total + 8 * 256 - 1) / (8 * 256);
+    // One task per lane up to 256 K tasks, more beyond: each lane's run of tasks in one bin is counted with one
+    // atomic, and a small batch keeps one short dependent chain of loads per lane.
+    uint64_t const blocks64 = (tasks_total + 255) / 256;
     uint32_t const blocks = blocks64 < 1024 ? (uint32_t)blocks64 : 1024;
     hipLaunchKernelGGL(szs_amd::materialize_kernel, dim3(blocks), dim3(256), 0, stream, args, segments_count,
                        tasks_total, table, tasks, bin_of_task, histogram, segments, stats, segment_overflow);
@@ -1279,8 +1280,8 @@ hipError_t szs_amd_launch_scan_bins(szs_amd::bins_view_t const *bins, hipStream_
 hipError_t szs_amd_launch_scatter_bins(szs_amd::bins_view_t const *bins, uint32_t const *bin_of_task,
                                        uint32_t const *position_of_bin, uint32_t tasks_count, hipStream_t stream) {
     if (!tasks_count) return hipSuccess;
-    // About 8 tasks per lane (more past 2 M tasks), as for materialization.
-    uint32_t const blocks64 = (tasks_count + 8 * 256 - 1) / (8 * 256);
+    // One task per lane up to 256 K tasks, more beyond, as for materialization.
+    uint32_t const blocks64 = (tasks_count + 255) / 256;
     uint32_t const blocks = blocks64 < 1024 ? blocks64 : 1024;
     hipLaunchKernelGGL(szs_amd::scatter_bins_kernel, dim3(blocks), dim3(256), 0, stream, *bins, bin_of_task,
                        position_of_bin, tasks_count);
