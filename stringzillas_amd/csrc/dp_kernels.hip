@@ -49,16 +49,19 @@ __device__ __forceinline__ bool is_steady(int32_t chunk, int32_t columns) {
  */
 template <typename sweep_type_>
 struct tail_sweeps_t {
+    using tiny = void;
     using shorter = void;
     using middle = void;
 };
 template <int family_, bool uniform_base_>
-struct tail_sweeps_t<packed_sweep_t<family_, 8, uniform_base_>> { // 512-row stripes: 256- or 384-row tails
+struct tail_sweeps_t<packed_sweep_t<family_, 8, uniform_base_>> { // 512-row stripes: 128-, 256- or 384-row tails
+    using tiny = packed_sweep_t<family_, 2, uniform_base_>;
     using shorter = packed_sweep_t<family_, 4, uniform_base_>;
     using middle = packed_sweep_t<family_, 6, uniform_base_>;
 };
 template <bool runes_>
 struct tail_sweeps_t<myers_sweep_t<64, runes_, 64>> { // 4096-row stripes: 2048-row tails of 32-row words
+    using tiny = void;
     using shorter = myers_sweep_t<64, runes_, 32>;
     using middle = void;
 };
@@ -118,14 +121,16 @@ __device__ void run_independent(item_args_t const &a) {
     int32_t *gaps_row = scores_row + a.scratch_capacity;
     // With one pair per wave, chunks whose 64 columns are valid for every lane skip the per-step column check.
     int32_t const columns = G == 64 ? __builtin_amdgcn_readfirstlane(sweep.m) : 0;
-    // The last stripe's rows pick its sweep (wave-uniform: one pair per wave): 1 = shorter, 2 = middle tail.
+    // The last stripe's rows pick its sweep (wave-uniform: one pair per wave): 1 = shorter, 2 = middle, 3 = tiny tail.
+    using tiny_t = typename tail_sweeps_t<sweep_type_>::tiny;
     using shorter_t = typename tail_sweeps_t<sweep_type_>::shorter;
     using middle_t = typename tail_sweeps_t<sweep_type_>::middle;
     int32_t const tail_base = (passes - 1) * stripe_rows;
     int32_t tail = 0;
     if constexpr (tail_rows<shorter_t>() > 0) {
         int32_t const last_rows = __builtin_amdgcn_readfirstlane(sweep.n) - tail_base;
-        if (passes > 0 && last_rows <= tail_rows<shorter_t>()) tail = 1;
+        if (passes > 0 && last_rows <= tail_rows<tiny_t>()) tail = 3;
+        else if (passes > 0 && last_rows <= tail_rows<shorter_t>()) tail = 1;
         else if (passes > 0 && last_rows <= tail_rows<middle_t>()) tail = 2;
     }
     int32_t const full_passes = passes - (tail ? 1 : 0);
@@ -151,6 +156,9 @@ __device__ void run_independent(item_args_t const &a) {
         int32_t const *in_row = full_passes ? scores_row : nullptr;
         uint32_t const task_index = a.order[a.first + first_slot];
         if (tail == 1) tail_sum = run_tail<shorter_t>(a, task_index, wave, tail_base, in_row, gaps_row);
+        if constexpr (tail_rows<tiny_t>() > 0) {
+            if (tail == 3) tail_sum = run_tail<tiny_t>(a, task_index, wave, tail_base, in_row, gaps_row);
+        }
         if constexpr (tail_rows<middle_t>() > 0) {
             if (tail == 2) tail_sum = run_tail<middle_t>(a, task_index, wave, tail_base, in_row, gaps_row);
         }
@@ -1081,13 +1089,14 @@ __global__ void materialize_kernel(materialize_args_t const *all, uint32_t segme
                                       : sh.wide       ? 4ull * sh.rows_per_lane // int64 cells
                                                       : (uint64_t)sh.rows_per_lane;
             uint64_t const ramp = (uint64_t)sh.group * (sh.packed ? 2u : 1u);
-            // A tall pair's last stripe may run a shorter sweep (`tail_sweeps_t`): half the Myers word, or 4 or 6
-            // packed rows per lane.
+            // A tall pair's last stripe may run a shorter sweep (`tail_sweeps_t`): half the Myers word, or 2, 4 or
+            // 6 packed rows per lane.
             uint64_t last_step = per_step;
             if (!sh.cooperative && sh.group == 64 && passes > 0) {
                 uint64_t const last_rows = rows - (passes - 1) * stripe;
                 if (sh.bit_parallel && sh.rows_per_lane == 64 && last_rows <= 64 * 32) last_step = per_step / 2;
-                else if (sh.packed && sh.rows_per_lane == 8 && last_rows <= 64 * 6) last_step = last_rows <= 64 * 4 ? 3 : 5;
+                else if (sh.packed && sh.rows_per_lane == 8 && last_rows <= 64 * 6)
+                    last_step = last_rows <= 64 * 2 ? 2 : last_rows <= 64 * 4 ? 3 : 5;
             }
             uint64_t const work = (passes ? (passes - 1) * per_step + last_step : 0) *
                                   ((uint64_t)task.columns_length + ramp);

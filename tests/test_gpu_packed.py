@@ -138,12 +138,14 @@ def test_packed_shapes_keep_exact_scores_far_beyond_16_bits(name, ladder, packed
 
 
 # Rows whose last 512-row stripe keeps at most 256 rows, 257-384 or more: it runs 4, 6 or 8 rows per lane.
-TAIL_ROWS = [1, 255, 256, 257, 383, 384, 385, 512, 513, 768, 769, 896, 897, 1024, 1280, 1281, 1408, 1409, 2000]
+TAIL_ROWS = [1, 127, 128, 129, 255, 256, 257, 383, 384, 385, 512, 513, 640, 641, 768, 769, 896, 897, 1024, 1152,
+             1153, 1280, 1281, 1408, 1409, 2000]
 
 
 @pytest.mark.parametrize("name", ["nw_affine", "nw_linear", "nw_asymmetric", "nw_far_gap"])
 def test_packed_tail_stripes(name, packed_ladders):
-    """Every pair on 512-row stripes: the last stripe alone or after full ones, on both sides of each tail size."""
+    """Every pair on 512-row stripes: the last stripe alone or after full ones, on both sides of each tail size
+    (2, 4 and 6 packed rows per lane)."""
     rng = np.random.default_rng(zlib.crc32(f"tails:{name}".encode()))
     engine, params = _engine(name, rng)
     packed_ladders("wave8")
