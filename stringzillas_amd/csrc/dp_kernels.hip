@@ -34,13 +34,20 @@ __device__ __forceinline__ int32_t chunks_for(int32_t columns) {
     return (columns + skew * G - 1 + G - 1) / G;
 }
 
-/** A 64-step chunk in which every lane's columns stay within 1 .. columns - (skew - 1) on every step, so the
+/** A chunk of G steps in which every lane's columns stay within 1 .. columns - (skew - 1) on every step, so the
  *  per-step column check (and the packed sweep's edge fix-ups) can be skipped. */
 template <typename sweep_type_>
 __device__ __forceinline__ bool is_steady(int32_t chunk, int32_t columns) {
-    constexpr int32_t skew = sweep_type_::skew_k;
-    return chunk >= skew && chunk * 64 + 64 + (skew - 1) <= columns;
+    constexpr int32_t G = sweep_type_::group_k, skew = sweep_type_::skew_k;
+    return chunk >= skew && chunk * G + G + (skew - 1) <= columns;
 }
+
+/** Sweeps of several pairs per wave that take steady chunks, bounded by the wave's shortest pair. */
+template <typename sweep_type_, typename = void>
+struct steady_groups : std::false_type {};
+template <typename sweep_type_>
+struct steady_groups<sweep_type_, std::void_t<decltype(sweep_type_::steady_groups_k)>>
+    : std::integral_constant<bool, sweep_type_::steady_groups_k> {};
 
 /**
  *  Shorter sweeps for the last stripe of a tall pair whose remaining rows fit one: the same lanes, skew and
@@ -110,17 +117,23 @@ __device__ void run_independent(item_args_t const &a) {
     bool const active = slot < a.count;
     sweep_type_ sweep;
     sweep.load(a, a.order[a.first + (active ? slot : first_slot)], active, wave);
-    int32_t longest = sweep.m;
+    int32_t longest = sweep.m, shortest = sweep.m;
     if constexpr (G < 64) {
         for (int offset = G; offset < 64; offset <<= 1) longest = max(longest, __shfl_xor(longest, offset, 64));
+        if constexpr (steady_groups<sweep_type_>::value) {
+            for (int offset = G; offset < 64; offset <<= 1) shortest = min(shortest, __shfl_xor(shortest, offset, 64));
+        }
     }
     int32_t const stripe_rows = G * R;
     int32_t const passes = G == 64 ? (sweep.n + stripe_rows - 1) / stripe_rows : 1;
     int32_t const chunks = chunks_for<sweep_type_>(longest);
     int32_t *scores_row = a.scratch + (size_t)wave * 2 * a.scratch_capacity;
     int32_t *gaps_row = scores_row + a.scratch_capacity;
-    // With one pair per wave, chunks whose 64 columns are valid for every lane skip the per-step column check.
-    int32_t const columns = G == 64 ? __builtin_amdgcn_readfirstlane(sweep.m) : 0;
+    // Chunks whose columns are valid for every lane skip the per-step column check: with one pair per wave, or with
+    // several whose sweep supports it (every group's columns valid: the wave's shortest pair bounds them).
+    constexpr bool steady_chunks = G == 64 || steady_groups<sweep_type_>::value;
+    int32_t const columns = G == 64 ? __builtin_amdgcn_readfirstlane(sweep.m)
+                                    : steady_chunks ? __builtin_amdgcn_readfirstlane(shortest) : 0;
     // The last stripe's rows pick its sweep (wave-uniform: one pair per wave): 1 = shorter, 2 = middle, 3 = tiny tail.
     using tiny_t = typename tail_sweeps_t<sweep_type_>::tiny;
     using shorter_t = typename tail_sweeps_t<sweep_type_>::shorter;
@@ -140,7 +153,7 @@ __device__ void run_independent(item_args_t const &a) {
         int32_t const *in_row = pass ? scores_row : nullptr;
         sweep.begin_stripe(base);
         for (int32_t chunk = 0; chunk < chunks; ++chunk) {
-            bool const steady = G == 64 && is_steady<sweep_type_>(chunk, columns);
+            bool const steady = steady_chunks && is_steady<sweep_type_>(chunk, columns);
             if (more) {
                 if (steady) sweep.template run_chunk<true, true>(chunk, in_row, gaps_row, scores_row, gaps_row);
                 else sweep.template run_chunk<false, true>(chunk, in_row, gaps_row, scores_row, gaps_row);
@@ -461,6 +474,23 @@ __device__ __forceinline__ void set_item_priority(uint32_t bin) {
 }
 
 /**
+ *  One-wave items a wave draws with one atomic on the queue counter. Same-address device atomics serialize (about
+ *  10 ns each), so a counter serving thousands of waves one item at a time caps the launch when items are short
+ *  (16 pairs of 100 x 100 cells take ~17 us): items whose bucket is below `draw_bucket_k` are drawn 2, 4, 8 or 16
+ *  at a time, about 100-200 us of work per draw (the bucket counts eighth-octaves of ~30 ns work units). `limit`
+ *  keeps a small batch at one item per draw, so that its few items still spread over every wave.
+ */
+constexpr int draw_bucket_k = 12 * bucket_steps_per_octave_k;
+__device__ __forceinline__ uint32_t items_per_draw(uint32_t bin, uint32_t limit) {
+    int const family = (int)(bin / column_buckets_k) / shapes_count_k / 2;
+    bool const affine = family == family_lev_affine_k || family == family_nw_affine_k || family == family_sw_affine_k;
+    // Affine cells cost ~1.8x linear ones (7 eighth-octaves), as the queue's priority order also weighs them.
+    int const below = draw_bucket_k - (int)(bin % column_buckets_k) - (affine ? 7 : 0);
+    uint32_t const count = below <= 0 ? 1u : 1u << min(below / bucket_steps_per_octave_k, 4);
+    return min(count, limit);
+}
+
+/**
  *  Persistent scoring kernel: work items are popped from one device counter in priority order until the queue
  *  (`item_start[bins]` items) is drained. The queue holds whole-workgroup items first (chained groups, then
  *  cooperative pairs, heaviest first), taken by a workgroup at a time, then one-wave items (heaviest first): once
@@ -520,7 +550,8 @@ __global__ __launch_bounds__(256, 4) void persistent_score_kernel(persistent_arg
         if (__lane_id() == 0) drawn = atomicAdd(args.bins.queue_head, 1u);
         taken = (uint32_t)__builtin_amdgcn_readfirstlane((int)drawn);
     }
-    uint32_t low = 0;
+    uint32_t low = 0, next = 0, end = 0; // items drawn and not yet swept: next .. end - 1
+    uint32_t const draw_limit = max(1u, total_items / (32u * gridDim.x));
     while (taken < total_items) {
         low = position_of_item(args.bins, taken, low);
         uint32_t const bin = args.bins.priority[low];
@@ -532,11 +563,17 @@ __global__ __launch_bounds__(256, 4) void persistent_score_kernel(persistent_arg
         item.first = args.bins.task_start[low] + item_in_bin * per_item;
         item.count = min(per_item, in_bin - item_in_bin * per_item);
         run_variant(variant, item);
+        if (next < end) { // the rest of the last draw (past the queue's end, the loop stops)
+            taken = next++;
+            continue;
+        }
         // Drawn only now: a wave holding the next item while it sweeps a long one would leave that item waiting
-        // for it while other waves run dry at the end of the queue.
+        // for it while other waves run dry at the end of the queue. Later items are no heavier than this one.
+        uint32_t const count = items_per_draw(bin, draw_limit);
         uint32_t drawn = 0;
-        if (__lane_id() == 0) drawn = atomicAdd(args.bins.queue_head, 1u);
+        if (__lane_id() == 0) drawn = atomicAdd(args.bins.queue_head, count);
         taken = (uint32_t)__builtin_amdgcn_readfirstlane((int)drawn);
+        next = taken + 1, end = taken + count;
     }
 }
 

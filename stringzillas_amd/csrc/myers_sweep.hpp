@@ -40,7 +40,9 @@ enum : int {
 template <int G>
 __device__ __forceinline__ uint32_t group_shift_down(uint32_t old, uint32_t src, bool first_in_group) {
     if constexpr (G == 4) {
-        uint32_t const shifted = (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, dpp_quad_shr1_k, 0xF, 0xF, false);
+        // Every lane has a source lane in its quad; the first lane's (itself) is replaced by the select, so the move
+        // can overwrite `src` in place instead of a copy of `old`.
+        uint32_t const shifted = (uint32_t)__builtin_amdgcn_update_dpp((int)src, (int)src, dpp_quad_shr1_k, 0xF, 0xF, false);
         return first_in_group ? old : shifted;
     }
     else return (uint32_t)shift_down<G>((int32_t)old, (int32_t)src);
@@ -52,12 +54,18 @@ __device__ __forceinline__ uint32_t group_rotate_up(uint32_t src) {
     else return (uint32_t)rotate_up<G>((int32_t)src);
 }
 
-/** LDS a smaller group uses to renumber its pair's symbols: codes are computed from these on demand. */
+/** LDS a smaller group uses to renumber its pair's symbols: codes are computed from these on demand. Entry w covers
+ *  byte values 32w .. 32w + 31: x is their presence set in the rows string, y one more than the number of values
+ *  present before 32w (so a value's code is y plus the present values below it in x, read with one 64-bit load). */
 struct myers_group_tables_t {
-    uint32_t presence[8]; /**< 256-bit set of the byte values in the rows string */
-    uint32_t below[8];    /**< values present in the words before each word */
+    uint2 entries[8];
 };
+
 static_assert(sizeof(myers_group_tables_t) * 16 <= myers_tables_bytes_k, "16 four-lane groups per wave");
+
+/** Global-address-space views of the tapes: loads through them never count against the waits for LDS operations. */
+typedef __attribute__((address_space(1))) uint32_t const global_dword_t;
+typedef __attribute__((address_space(1))) uint8_t const global_u8_t;
 
 /** LDS a 64-lane Myers group uses to renumber its pair's symbols. */
 struct myers_tables_t {
@@ -85,6 +93,13 @@ struct myers_sweep_t {
     static constexpr bool renumber_k = !runes_ && G == 64; /**< one pair per wave: a full byte -> code table */
     static constexpr int planes_k = runes_ ? 21 : 8;        /**< bits of a code: a rune has at most 21 */
     static constexpr reduce_kind_t reduce_k = reduce_sum_k;
+    /** Groups of several pairs per wave over bytes: a lane's word of row bytes loads once as dwords. A small group's
+     *  pair fits one stripe (the ladder gives these shapes no taller pairs), so its rows are read once. */
+    static constexpr bool small_bytes_k = !runes_ && G < 64;
+    static constexpr int row_dwords_k = word_bits_ / 4;
+    /** Groups of several pairs per wave skip the per-step column check in chunks where every group's columns are
+     *  all valid (the wave's shortest pair bounds them). */
+    static constexpr bool steady_groups_k = G < 64;
 
     int lane, k;
     bool active;
@@ -107,16 +122,46 @@ struct myers_sweep_t {
     uint32_t top_p, top_n; /**< 64-lane groups: Ph's and Mh's top 32 bits before the shift (bit 31: bottom row) */
     uint32_t column_at;    /**< 64-lane groups in table mode: LDS address of this lane's match word of `column_code` */
     int32_t vertical_sum;
+    uint32_t row_dwords[small_bytes_k ? row_dwords_k : 1]; /**< bytes of this lane's rows, 4 per dword, top row first */
+    symbol_t pending; /**< groups of several pairs: this lane's column symbol of the next chunk, loaded a chunk ahead */
 
     __device__ __forceinline__ uint32_t code_of(uint32_t byte) const {
         if constexpr (runes_) return byte;
         else if constexpr (renumber_k) return tables->codes[byte];
         else {
             if (distinct == 256) return byte; // no absent code left: bytes stay as they are
-            uint32_t const word = group_tables->presence[byte >> 5], bit = byte & 31u;
-            if (!((word >> bit) & 1u)) return 0u;
-            return group_tables->below[byte >> 5] + __builtin_popcount(word & ((1u << bit) - 1u)) + 1u;
+            uint2 const entry = group_tables->entries[byte >> 5];
+            uint32_t const bit = byte & 31u;
+            uint32_t const code = __builtin_popcount(entry.x & ((1u << bit) - 1u)) + entry.y;
+            return (entry.x >> bit) & 1u ? code : 0u;
         }
+    }
+
+    /** Byte `r` of this lane's rows (`row_dwords`). */
+    __device__ __forceinline__ uint32_t row_byte(int r) const {
+        return __builtin_amdgcn_ubfe(row_dwords[r >> 2], 8 * (r & 3), 8);
+    }
+
+    /**
+     *  Loads the bytes of rows k·W .. k·W + `valid` - 1 into `row_dwords` with aligned dword loads issued together
+     *  (a dword holding one of those bytes lies inside the tape's pages); bytes past `valid` are left unspecified.
+     */
+    __device__ __forceinline__ void load_row_bytes(int32_t valid) {
+        uintptr_t const at = (uintptr_t)(rows + k * word_bits_);
+        uint32_t const shift = (uint32_t)at & 3u;
+        global_dword_t *aligned = (global_dword_t *)(at - shift);
+        uint32_t raw[row_dwords_k + 1];
+#pragma unroll
+        for (int j = 0; j <= row_dwords_k; ++j) raw[j] = valid > 0 && 4 * j < valid + (int32_t)shift ? aligned[j] : 0u;
+#pragma unroll
+        for (int i = 0; i < row_dwords_k; ++i) row_dwords[i] = __builtin_amdgcn_alignbyte(raw[i + 1], raw[i], shift);
+    }
+
+    /** This lane's column symbol `j` (0-based), 0 past the columns' end. */
+    __device__ __forceinline__ symbol_t column_symbol(int32_t j) const {
+        if (j >= m) return 0;
+        if constexpr (runes_) return *(global_dword_t *)(uintptr_t)(columns + j);
+        else return *(global_u8_t *)(uintptr_t)(columns + j);
     }
 
     __device__ __forceinline__ void load(item_args_t const &a, uint32_t task_index, bool is_active, uint32_t wave) {
@@ -189,22 +234,30 @@ struct myers_sweep_t {
         }
         else {
             // Several pairs per wave: each group collects its rows' byte values and keeps the per-word prefix
-            // counts; codes 1..D follow byte order, as for the 64-lane groups.
-            for (int w = k; w < 8; w += G) group_tables->presence[w] = 0;
+            // counts; codes 1..D follow byte order, as for the 64-lane groups. Lane k's word holds rows
+            // k·W .. k·W + W - 1, and the group's words cover its pair.
+            int32_t const valid = min(max(n - k * word_bits_, 0), word_bits_);
+            load_row_bytes(valid);
+            for (int w = k; w < 8; w += G) group_tables->entries[w] = make_uint2(0u, 0u);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            for (int32_t i = k; i < n; i += G) {
-                uint32_t const byte = rows[i];
-                atomicOr(&group_tables->presence[byte >> 5], 1u << (byte & 31));
+            if (valid > 0) {
+                uint32_t const fill = row_byte(0); // rows past the pair's end repeat this lane's first row
+#pragma unroll
+                for (int r = 0; r < word_bits_; ++r) {
+                    uint32_t const byte = r < valid ? row_byte(r) : fill;
+                    atomicOr(&group_tables->entries[byte >> 5].x, 1u << (byte & 31u));
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            int32_t count = 0;
+            uint32_t count = 0;
+#pragma unroll
             for (int w = 0; w < 8; ++w) {
-                uint32_t const word = group_tables->presence[w];
-                if ((w % G) == k) group_tables->below[w] = (uint32_t)count;
+                uint32_t const word = group_tables->entries[w].x;
+                if ((w % G) == k) group_tables->entries[w].y = count + 1u;
                 count += __builtin_popcount(word);
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            distinct = count;
+            distinct = (int32_t)count;
             int32_t const largest = distinct == 256 ? 255 : distinct;
             planes_used = largest ? 32 - __builtin_clz((uint32_t)largest) : 1;
             bool const wide = distinct >= myers_eq_codes_k;
@@ -214,6 +267,42 @@ struct myers_sweep_t {
 
     __device__ __forceinline__ void begin_stripe(int32_t base) {
         lane_top = base + k * word_bits_;
+        pv = ~(word_t)0; // column 0: D[i][0] = i, every vertical delta is +1
+        mv = 0;
+        bottom_code = 1;
+        top_p = 1u << 31, top_n = 0;
+        column_at = eq_address(0);
+        column_code = 0;
+        if constexpr (G < 64) pending = column_symbol(k); // the first chunk's, loaded while the words are built
+        if constexpr (small_bytes_k) {
+            // One stripe from row 0 (`base` is 0), rows in `row_dwords`. Rows past the pair's end (bytes left unspecified) only
+            // reach words below the pair's last row, which no result reads. The dwords pass through an empty asm so
+            // that no byte's lookup address is kept live from the presence loop (32-64 of them would spill).
+#pragma unroll
+            for (int i = 0; i < row_dwords_k; ++i) asm volatile("" : "+v"(row_dwords[i]));
+            if (table_mode) { // each row sets its bit in the match word of its code
+#pragma unroll
+                for (int code = 0; code < myers_eq_codes_k; ++code) eq_table[code * G + k] = 0;
+#pragma unroll
+                for (int r = 0; r < word_bits_; ++r) {
+                    word_t *entry = &eq_table[code_of(row_byte(r)) * G + k];
+                    if constexpr (word_bits_ == 64) atomicOr((unsigned long long *)entry, 1ull << r);
+                    else atomicOr((unsigned int *)entry, 1u << r);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
+            else {
+#pragma unroll
+                for (int b = 0; b < planes_k; ++b) planes[b] = 0;
+#pragma unroll
+                for (int r = 0; r < word_bits_; ++r) {
+                    uint32_t const code = code_of(row_byte(r)); // below 2^planes_used: higher planes stay zero
+#pragma unroll
+                    for (int b = 0; b < planes_k; ++b) planes[b] |= (word_t)((code >> b) & 1u) << r;
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int b = 0; b < planes_k; ++b) planes[b] = 0;
         for (int32_t r = 0; r < word_bits_; ++r) {
@@ -223,12 +312,6 @@ struct myers_sweep_t {
             for (int b = 0; b < planes_k; ++b)
                 if (b < planes_used) planes[b] |= (word_t)((code >> b) & 1u) << r;
         }
-        pv = ~(word_t)0; // column 0: D[i][0] = i, every vertical delta is +1
-        mv = 0;
-        bottom_code = 1;
-        top_p = 1u << 31, top_n = 0;
-        column_at = eq_address(0);
-        column_code = 0;
         if (table_mode) { // the match word of every code for this lane's 64 rows
             for (uint32_t code = 0; code < (uint32_t)myers_eq_codes_k; ++code)
                 eq_table[code * G + k] = planes_eq<4>(code); // codes below 16 use planes 0..3
@@ -352,7 +435,7 @@ struct myers_sweep_t {
     template <bool steady_, bool collect_, bool table_, int PB>
     __device__ __forceinline__ void chunk_of(int32_t chunk, int32_t const *in_codes, int32_t *out_codes) {
         if constexpr (G == 64) chunk_64<steady_, collect_, table_, PB>(chunk, in_codes, out_codes);
-        else chunk_small<table_, PB>(chunk, in_codes);
+        else chunk_small<steady_, table_, PB>(chunk);
     }
 
     template <bool steady_, bool collect_, bool table_, int PB>
@@ -408,21 +491,25 @@ struct myers_sweep_t {
         }
     }
 
-    template <bool table_, int PB>
-    __device__ __forceinline__ void chunk_small(int32_t chunk, int32_t const *in_codes) {
+    /**
+     *  A chunk of G steps of a group of several pairs per wave. Such a group sweeps its pair in one stripe, so lane 0
+     *  always enters from the matrix border (+1). Lane k's symbol of the chunk (column first + k + 1) rotates toward
+     *  lane 0, which takes one per step; the next chunk's is loaded now, a chunk before it is needed.
+     */
+    template <bool steady_, bool table_, int PB>
+    __device__ __forceinline__ void chunk_small(int32_t chunk) {
         int32_t const first_step = chunk * G;
-        int32_t const j_lane = first_step + k + 1;
-        uint32_t chunk_symbols = j_lane <= m ? code_of(columns[j_lane - 1]) : 0u;
-        uint32_t chunk_codes = in_codes ? (j_lane <= m ? (uint32_t)((uint8_t const *)in_codes)[j_lane] : 0u) : 1u;
+        // Past the columns' end the symbol is 0; those steps are masked below (and never occur in steady chunks).
+        uint32_t chunk_symbols = code_of(pending);
+        pending = column_symbol(first_step + G + k);
         bool const first_in_group = k == 0;
 #pragma unroll 4
         for (int32_t step = 0; step < G; ++step) {
-            uint32_t const hin = group_shift_down<G>(chunk_codes, bottom_code, first_in_group);
+            uint32_t const hin = group_shift_down<G>(1u, bottom_code, first_in_group);
             column_code = group_shift_down<G>(chunk_symbols, column_code, first_in_group);
-            chunk_codes = group_rotate_up<G>(chunk_codes);
             chunk_symbols = group_rotate_up<G>(chunk_symbols);
             int32_t const j = first_step + step + 1 - k;
-            if ((uint32_t)(j - 1) < (uint32_t)m)
+            if (steady_ || (uint32_t)(j - 1) < (uint32_t)m)
                 step_word(table_ ? eq_table[column_code * G + k] : planes_eq<PB>(column_code), hin);
         }
     }
