@@ -539,6 +539,7 @@ __global__ __launch_bounds__(256, 4) void persistent_score_kernel(persistent_arg
         }
         item.count = 1;
         item.wave_item = false;
+        item.same_variant = false;
         run_variant(variant, item);
     }
     // One-wave items: wave 0 sweeps the item the workgroup drew, every wave then draws its own.
@@ -552,10 +553,13 @@ __global__ __launch_bounds__(256, 4) void persistent_score_kernel(persistent_arg
     }
     uint32_t low = 0, next = 0, end = 0; // items drawn and not yet swept: next .. end - 1
     uint32_t const draw_limit = max(1u, total_items / (32u * gridDim.x));
+    int last_variant = -1;
     while (taken < total_items) {
         low = position_of_item(args.bins, taken, low);
         uint32_t const bin = args.bins.priority[low];
         int const variant = (int)(bin / column_buckets_k);
+        item.same_variant = variant == last_variant;
+        last_variant = variant;
         set_item_priority(bin);
         uint32_t const per_item = pairs_per_item(shapes_k[variant % shapes_count_k]);
         uint32_t const item_in_bin = taken - args.bins.item_start[low];
@@ -590,6 +594,7 @@ __global__ __launch_bounds__(256, 4) void single_task_kernel(task_t const *tasks
     item.first = task_index;
     item.count = 1;
     item.wave_item = false;
+    item.same_variant = false;
     item.scratch = scratch;
     item.scratch_capacity = scratch_capacity;
     item.wave_lds_bytes = wave_lds_bytes;
@@ -1237,13 +1242,19 @@ __global__ __launch_bounds__(1024) void scan_bins_carry_kernel(bins_view_t bins)
  */
 __global__ void scatter_bins_kernel(bins_view_t bins, uint32_t const *bin_of_task, uint32_t const *position_of_bin,
                                     uint32_t tasks_count) {
-    uint32_t const stride = gridDim.x * blockDim.x;
+    // Each wave places one block of consecutive tasks (lane l takes every 64th from the block's l-th), so a bin's
+    // order keeps runs of nearby tasks: the pairs of an all-pairs item share their query string, whose tables a
+    // sweep can keep from one item to the next. With one task per lane the blocks are 64 tasks.
+    uint32_t const waves = gridDim.x * (blockDim.x / 64);
+    uint32_t const per_wave = (uint32_t)(((uint64_t)tasks_count + waves - 1) / waves + 63) & ~63u;
+    uint64_t const block = (uint64_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * per_wave;
+    uint32_t const block_end = (uint32_t)min(block + per_wave, (uint64_t)tasks_count);
     uint32_t run_bin = ~0u, run_tasks = 0, run_first = 0;
     auto place = [&](uint32_t rank) { // the run's ids into its slots from `rank` in its bin
         uint32_t const at = bins.task_start[position_of_bin[run_bin]] + rank;
-        for (uint32_t i = 0; i < run_tasks; ++i) bins.order[at + i] = run_first + i * stride;
+        for (uint32_t i = 0; i < run_tasks; ++i) bins.order[at + i] = run_first + i * 64;
     };
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tasks_count; t += stride) {
+    for (uint64_t t = block + __lane_id(); t < block_end; t += 64) {
         uint32_t const bin = bin_of_task[t];
         if (bin != run_bin) {
             if (run_bin != ~0u) place(atomicAdd(&bins.cursor[run_bin], run_tasks));

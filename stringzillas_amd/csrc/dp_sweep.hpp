@@ -126,6 +126,9 @@ struct item_args_t {
     uint32_t first;
     uint32_t count;
     bool wave_item;        /**< the item is this wave's alone (else the workgroup's: wave w takes its w-th share) */
+    /** The wave's previous item ran the same variant (one-wave items only), so its LDS still holds what that
+     *  sweep left there: a sweep may reuse tables built for the same rows string. */
+    bool same_variant;
     int32_t *scratch;      /**< this workgroup's 4 x 2 x `scratch_capacity` int32 rows */
     uint32_t scratch_capacity;
     uint32_t wave_lds_bytes; /**< dynamic LDS per wave: profile or Myers tables, then the injection ring */

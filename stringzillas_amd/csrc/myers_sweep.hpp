@@ -63,6 +63,15 @@ struct myers_group_tables_t {
 
 static_assert(sizeof(myers_group_tables_t) * 16 <= myers_tables_bytes_k, "16 four-lane groups per wave");
 
+/** What a smaller group's LDS tables were last built for (kept where 64-lane groups stage their chunk inputs): the
+ *  rows string and its distinct byte values, or `myers_eq_codes_k` and above when no match-word table was built. */
+struct myers_group_record_t {
+    uint64_t rows;
+    int32_t n;
+    int32_t distinct;
+};
+static_assert(sizeof(myers_group_record_t) * 16 <= inject_ring_bytes_k, "16 four-lane groups per wave");
+
 /** Global-address-space views of the tapes: loads through them never count against the waits for LDS operations. */
 typedef __attribute__((address_space(1))) uint32_t const global_dword_t;
 typedef __attribute__((address_space(1))) uint8_t const global_u8_t;
@@ -114,6 +123,8 @@ struct myers_sweep_t {
     int8_t *ring;        /**< lane-0 inputs of a 64-step chunk (G = 64) */
     int32_t planes_used;
     bool table_mode;     /**< wave-uniform: at most 15 distinct symbols, match words come from `eq_table` */
+    bool reused;         /**< wave-uniform, smaller groups over bytes: the LDS tables of the wave's previous item
+                              were built for the same rows strings, and stand */
 
     int32_t lane_top;
     word_t planes[planes_k];
@@ -190,6 +201,7 @@ struct myers_sweep_t {
         vertical_sum = 0;
         planes_used = 8;
         table_mode = false;
+        reused = false;
         if constexpr (runes_) {
             // Runes are their own codes: plane b holds bit b, up to the highest bit of the wave's largest row rune.
             uint32_t largest = 0;
@@ -235,7 +247,22 @@ struct myers_sweep_t {
         else {
             // Several pairs per wave: each group collects its rows' byte values and keeps the per-word prefix
             // counts; codes 1..D follow byte order, as for the 64-lane groups. Lane k's word holds rows
-            // k·W .. k·W + W - 1, and the group's words cover its pair.
+            // k·W .. k·W + W - 1, and the group's words cover its pair. When the wave's previous item ran this
+            // sweep on the same rows strings in table mode (the pairs of an all-pairs product share their query),
+            // the tables it left in LDS stand and nothing is rebuilt.
+            myers_group_record_t *record = reinterpret_cast<myers_group_record_t *>(ring) + group;
+            if (a.same_variant) {
+                myers_group_record_t const kept = *record;
+                bool const same = kept.rows == (uint64_t)(uintptr_t)rows && kept.n == n &&
+                                  kept.distinct < myers_eq_codes_k;
+                reused = __ballot(!same) == 0 && a.wave_lds_bytes >= myers_lds_bytes_k;
+                if (reused) {
+                    distinct = kept.distinct;
+                    planes_used = distinct ? 32 - __builtin_clz((uint32_t)distinct) : 1;
+                    table_mode = true;
+                    return;
+                }
+            }
             int32_t const valid = min(max(n - k * word_bits_, 0), word_bits_);
             load_row_bytes(valid);
             for (int w = k; w < 8; w += G) group_tables->entries[w] = make_uint2(0u, 0u);
@@ -262,6 +289,9 @@ struct myers_sweep_t {
             planes_used = largest ? 32 - __builtin_clz((uint32_t)largest) : 1;
             bool const wide = distinct >= myers_eq_codes_k;
             table_mode = __ballot(wide) == 0 && a.wave_lds_bytes >= myers_lds_bytes_k;
+            // The tables now stand for these rows (the match words only in table mode).
+            if (k == 0 && a.wave_lds_bytes >= myers_lds_bytes_k)
+                *record = myers_group_record_t{(uint64_t)(uintptr_t)rows, n, table_mode ? distinct : myers_eq_codes_k};
         }
     }
 
@@ -280,7 +310,7 @@ struct myers_sweep_t {
             // that no byte's lookup address is kept live from the presence loop (32-64 of them would spill).
 #pragma unroll
             for (int i = 0; i < row_dwords_k; ++i) asm volatile("" : "+v"(row_dwords[i]));
-            if (table_mode) { // each row sets its bit in the match word of its code
+            if (table_mode && !reused) { // each row sets its bit in the match word of its code
 #pragma unroll
                 for (int code = 0; code < myers_eq_codes_k; ++code) eq_table[code * G + k] = 0;
 #pragma unroll
@@ -291,7 +321,7 @@ struct myers_sweep_t {
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
-            else {
+            else if (!table_mode) {
 #pragma unroll
                 for (int b = 0; b < planes_k; ++b) planes[b] = 0;
 #pragma unroll
