@@ -6,19 +6,21 @@
  *  The reference routes unit costs to Myers as well (include/stringzillas/similarities/serial.hpp:1657-1865,
  *  2126-2140); results equal the unit-cost DP, which is what the tests pin against the oracle.
  *
- *  Layout: lane k of a group owns one 64-row word of the stripe (rows 64k .. 64k+63, bit 0 = top row; 32-row words for
- *  the half-height shapes) as the vertical-delta vectors Pv/Mv. At step t it processes column j = t + 1 - k and passes
- *  the horizontal delta of its bottom row (+1, 0 or -1, coded in 2 bits) to lane k+1 by DPP; lane 0 receives +1 from
- *  the matrix border or the previous stripe's bottom deltas from HBM scratch (one byte per column). The match vector of
- *  a column symbol is built from bit-planes: plane b holds bit b of every row's symbol code, and Eq = AND_b (plane_b
- *  XNOR bit b of the column code), one `v_bitop3` per plane and half-word. For 64-lane groups the symbols are first
- *  renumbered to a dense code (0 = absent from the rows) so that only ceil(log2(distinct + 1)) planes are needed, e.g.
- *  3 for DNA; every group renumbers its own pair (4- and 16-lane groups compute codes from a presence set and its
- *  prefix counts). When every group of the wave has at most 15 distinct symbols, each stripe instead writes the match
- *  word of every code to an 8 KB LDS table ([16 codes][lanes] u64 per group), and a step reads its word back (for
- *  64-lane groups prefetched one step ahead) in place of the plane arithmetic. For 64-lane groups lane 0's inputs of a
- *  64-step chunk (horizontal deltas from the stripe above, column codes) are staged in the LDS ring and read back as
- *  one broadcast entry per step. The distance is m + the sum of the vertical deltas of the last column over all rows.
+ *  Layout: lane k of a group owns one 64-row word of the stripe (rows 64k .. 64k+63, bit 0 = top row; 32-row words
+ *  for the half-height shapes) as the vertical-delta vectors Pv/Mv. At step t it processes column j = t + 1 - k and
+ *  passes the horizontal delta of its bottom row (+1, 0 or -1: the top bits of its Ph and Mh words) to lane k+1 by DPP;
+ *  lane 0 receives +1 from the matrix border or the previous stripe's bottom deltas, coded in 2 bits, from HBM scratch
+ *  (one byte per column). The match vector of a column symbol is built from bit-planes: plane b holds bit b of every
+ *  row's symbol code, and Eq = AND_b (plane_b XNOR bit b of the column code), one `v_bitop3` per plane and half-word.
+ *  For 64-lane groups the symbols are first renumbered to a dense code (0 = absent from the rows) so that only
+ *  ceil(log2(distinct + 1)) planes are needed, e.g. 3 for DNA; every group renumbers its own pair (4- and 16-lane
+ *  groups compute codes from a presence set and its prefix counts). When every group of the wave has at most 15
+ *  distinct symbols, each stripe instead writes the match word of every code to an 8 KB LDS table ([16 codes][lanes]
+ *  u64 per group), and a step reads its word back (for 64-lane groups prefetched one step ahead) in place of the plane
+ *  arithmetic; 4- and 16-lane groups keep that table from one item to the next while their rows strings stay the same
+ *  (the queries of an all-pairs product). For 64-lane groups lane 0's inputs of a 64-step chunk (horizontal deltas
+ *  from the stripe above, column codes) are staged in the LDS ring and read back as one broadcast entry per step. The
+ *  distance is m + the sum of the vertical deltas of the last column over all rows.
  */
 #pragma once
 #include <hip/hip_runtime.h>
@@ -129,8 +131,8 @@ struct myers_sweep_t {
     int32_t lane_top;
     word_t planes[planes_k];
     word_t pv, mv;
-    uint32_t column_code, bottom_code;
-    uint32_t top_p, top_n; /**< 64-lane groups: Ph's and Mh's top 32 bits before the shift (bit 31: bottom row) */
+    uint32_t column_code;
+    uint32_t top_p, top_n; /**< Ph's and Mh's top 32 bits before the shift (bit 31: the bottom row's delta) */
     uint32_t column_at;    /**< 64-lane groups in table mode: LDS address of this lane's match word of `column_code` */
     int32_t vertical_sum;
     uint32_t row_dwords[small_bytes_k ? row_dwords_k : 1]; /**< bytes of this lane's rows, 4 per dword, top row first */
@@ -299,7 +301,6 @@ struct myers_sweep_t {
         lane_top = base + k * word_bits_;
         pv = ~(word_t)0; // column 0: D[i][0] = i, every vertical delta is +1
         mv = 0;
-        bottom_code = 1;
         top_p = 1u << 31, top_n = 0;
         column_at = eq_address(0);
         column_code = 0;
@@ -387,25 +388,6 @@ struct myers_sweep_t {
         else return (word_t)low;
     }
 
-    /**
-     *  One column of this lane's word (Hyyro's step); `hin` is the horizontal delta entering its top row. The
-     *  three-input terms are single `v_bitop3` per half: (x ^ y) | z is table 0xBE, x | ~(y | z) is table 0xF1.
-     */
-    __device__ __forceinline__ void step_word(word_t eq, uint32_t hin) {
-        constexpr int top_bit = word_bits_ - 1;
-        word_t const hin_positive = hin & 1u, hin_negative = (hin >> 1) & 1u;
-        word_t const xv = eq | mv;
-        eq |= hin_negative;
-        word_t const xh = bitop3<0xBE>((eq & pv) + pv, pv, eq); // (((eq & pv) + pv) ^ pv) | eq
-        word_t ph = bitop3<0xF1>(mv, xh, pv);                   // mv | ~(xh | pv)
-        word_t mh = pv & xh;
-        bottom_code = (uint32_t)(ph >> top_bit) | (uint32_t)(mh >> top_bit) << 1;
-        ph = (ph << 1) | hin_positive;
-        mh = (mh << 1) | hin_negative;
-        pv = bitop3<0xF1>(mh, xv, ph); // mh | ~(xv | ph)
-        mv = ph & xv;
-    }
-
     /** The LDS address of this lane's match word of `code` in the table (the low 32 bits of a generic address in
      *  the LDS aperture are the LDS address). */
     __device__ __forceinline__ uint32_t eq_address(uint32_t code) const {
@@ -423,10 +405,11 @@ struct myers_sweep_t {
     }
 
     /**
-     *  `step_word` for 64-lane groups, with the horizontal delta entering the top row given as bit 31 of two words
-     *  (`hp`: +1, `hn`: -1; the lane above's `top_p`/`top_n`, or the chunk's inputs for lane 0), and the one
-     *  leaving the bottom row left the same way in `top_p`/`top_n`: the words move between lanes as they are and
-     *  shift straight into Ph/Mh, with no 2-bit code to pack and unpack on every step.
+     *  One column of this lane's word (Hyyro's step), with the horizontal delta entering the top row given as bit 31
+     *  of two words (`hp`: +1, `hn`: -1; the lane above's `top_p`/`top_n`, or the chunk's inputs or the border for
+     *  a group's first lane), and the one leaving the bottom row left the same way in `top_p`/`top_n`: the words
+     *  move between lanes as they are and shift straight into Ph/Mh, with no 2-bit code to pack and unpack on every
+     *  step. The three-input terms are single `v_bitop3` per half: (x ^ y) | z is table 0xBE, x | ~(y | z) is 0xF1.
      */
     __device__ __forceinline__ void step_word_bits(word_t eq, uint32_t hp, uint32_t hn) {
         word_t const xv = eq | mv;
@@ -535,12 +518,14 @@ struct myers_sweep_t {
         bool const first_in_group = k == 0;
 #pragma unroll 4
         for (int32_t step = 0; step < G; ++step) {
-            uint32_t const hin = group_shift_down<G>(1u, bottom_code, first_in_group);
+            // The delta entering the top row: the lane above's bottom-row Ph/Mh bits, or +1 from the border.
+            uint32_t const hp = group_shift_down<G>(1u << 31, top_p, first_in_group);
+            uint32_t const hn = group_shift_down<G>(0u, top_n, first_in_group);
             column_code = group_shift_down<G>(chunk_symbols, column_code, first_in_group);
             chunk_symbols = group_rotate_up<G>(chunk_symbols);
             int32_t const j = first_step + step + 1 - k;
             if (steady_ || (uint32_t)(j - 1) < (uint32_t)m)
-                step_word(table_ ? eq_table[column_code * G + k] : planes_eq<PB>(column_code), hin);
+                step_word_bits(table_ ? eq_table[column_code * G + k] : planes_eq<PB>(column_code), hp, hn);
         }
     }
 
