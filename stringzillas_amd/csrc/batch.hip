@@ -56,12 +56,14 @@ namespace szs_amd {
 
 /**
  *  Default shape choice. Uniform-cost (Levenshtein) families: the smallest single-stripe shape that holds all
- *  rows, 1024-row stripes on one wave up to 4096 rows, then the cooperative 1024-row shape, chained 512-row
- *  groups beyond 24,576 rows. Class-cost families keep LDS profiles small: at most 8 rows per lane, 512-row
- *  stripes on one wave up to 2048 rows, then the cooperative 512-row shape (chained beyond 24,576 rows for the
- *  global families). Unit-cost bytes and runes: DP up to 64 rows, then bit-parallel words: 4 lanes of 32-row
- *  words up to 128 rows (bytes only), 4 lanes up to 256, 16 lanes of 32-row words up to 512, 16 lanes up to 1024,
- *  one wave of 32-row words up to 2048, 4096-row stripes on one wave up to 16384, then cooperative.
+ *  rows (65-128 rows on 16 lanes of 8 rows, four pairs per wave, rather than one pair on 64 lanes of 2: a quarter
+ *  of the ramp steps, and each step's overhead spread over 8 rows), 1024-row stripes on one wave up to 4096 rows,
+ *  then the cooperative 1024-row shape, chained 512-row groups beyond 24,576 rows. Class-cost families likewise,
+ *  keeping LDS profiles small: at most 8 rows per lane, 512-row stripes on one wave up to 2048 rows, then the
+ *  cooperative 512-row shape (chained beyond 24,576 rows for the global families). Unit-cost bytes and runes: DP up
+ *  to 64 rows, then bit-parallel words: 4 lanes of 32-row words up to 128 rows (bytes only), 4 lanes up to 256, 16
+ *  lanes of 32-row words up to 512, 16 lanes up to 1024, one wave of 32-row words up to 2048, 4096-row stripes on
+ *  one wave up to 16384, then cooperative.
  */
 /** Pairs taller than this go to the chained shape (global families): one workgroup each would set the batch's
  *  critical path, 16 share the pair instead. */
@@ -85,14 +87,14 @@ static shape_table_t default_shape_table() {
         }
         else if (family == family_lev_linear_k || family == family_lev_affine_k) {
             for (int shape = 0; shape <= 8; ++shape)
-                ladder.push_back({shape, shapes_k[shape].group * shapes_k[shape].rows_per_lane});
+                ladder.push_back({shape == 3 ? 23 : shape, shapes_k[shape].group * shapes_k[shape].rows_per_lane});
             ladder.back().second = 4096; // up to 4 stripes on one wave
             ladder.push_back({10, chained_rows_k});
             multi = 20;
         }
         else {
             for (int shape = 0; shape <= 7; ++shape)
-                ladder.push_back({shape, shapes_k[shape].group * shapes_k[shape].rows_per_lane});
+                ladder.push_back({shape == 3 ? 23 : shape, shapes_k[shape].group * shapes_k[shape].rows_per_lane});
             ladder.back().second = 2048;
             bool const local = family == family_sw_linear_k || family == family_sw_affine_k;
             if (local) multi = 9; // local scores need every stripe's maximum: no chained groups

@@ -214,7 +214,7 @@ def test_utf8_malformed_and_truncated_input(low, high):
         np.testing.assert_array_equal(engine.pairs(szs.to_device(a), szs.to_device(b)), expected)
 
 
-@pytest.mark.parametrize("family", ["nw_affine", "lev_affine"])
+@pytest.mark.parametrize("family", ["nw_affine", "nw_linear", "sw_affine", "sw_linear", "lev_affine"])
 def test_sixteen_lane_eight_row_groups(family):
     """Four pairs per wave on 16 lanes of 8 rows (up to 128 rows), forced through the tuning ladder."""
     from stringzillas_amd import autotune
@@ -224,10 +224,13 @@ def test_sixteen_lane_eight_row_groups(family):
     slot = autotune.slot_of(family)
     saved = autotune.get_ladder(slot)
     rng = np.random.default_rng(zlib.crc32(f"g16r8:{family}".encode()))
-    if family == "nw_affine":
+    if family != "lev_affine":
         b2c, costs = protein_tables(rng)
-        engine = szs.NeedlemanWunschScores(b2c, costs, -11, -1)
-        params = O.Params("needleman_wunsch", open=-11, extend=-1, byte_to_class=b2c, class_costs=costs)
+        kind, engine_type = {"nw": ("needleman_wunsch", szs.NeedlemanWunschScores),
+                             "sw": ("smith_waterman", szs.SmithWatermanScores)}[family[:2]]
+        gaps = (-11, -1) if family.endswith("affine") else (-4, -4)
+        engine = engine_type(b2c, costs, *gaps)
+        params = O.Params(kind, open=gaps[0], extend=gaps[1], byte_to_class=b2c, class_costs=costs)
         alphabet = PROTEIN
     else:
         engine = szs.LevenshteinDistances(0, 4, 3, 2)
