@@ -767,6 +767,12 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
     SZS_TRY_HIP(szs_amd_persistent_occupancy(&blocks_per_cu, b.lds_bytes), "occupancy");
     blocks_per_cu = std::max(1, std::min(blocks_per_cu, 8));
     b.waves = mode == szs_batch_per_pair_k ? 1u : (uint32_t)(compute_units * blocks_per_cu); // workgroups
+    // A batch of few pairs needs few waves: a one-wave item holds at least one pair, and the queue hands out every
+    // item in order whatever the grid (a chained group only waits for groups drawn before it), so such a batch
+    // launches one workgroup per 4 pairs instead of filling the chip: an idle workgroup still starts, draws once
+    // from the queue's single counter and exits, and those draws serialize.
+    b.waves = (uint32_t)std::min<uint64_t>(b.waves, std::max<uint64_t>(1, (tasks_total + waves_per_group_k - 1) /
+                                                                            waves_per_group_k));
     // Every resident workgroup holds stripe-boundary rows as wide as the widest multi-stripe pair: a pair of
     // millions of columns would ask for tens of GB. Past a limit (the caller's, else 4 GiB or a quarter of the free
     // HBM) the persistent launch runs fewer workgroups instead; the work queue does not depend on their number.

@@ -446,19 +446,24 @@ struct persistent_args_t {
 
 /**
  *  Priority position p of work item `taken`: item_start[p] <= taken < item_start[p + 1], searched upward from
- *  `from`, a position at or before it (a wave's items only move forward through the queue): one load when the
- *  item is in the same bin as the wave's last one, a gallop and a short binary search otherwise, instead of 17
- *  dependent loads over all the bins each time.
+ *  `from`, a position at or before it (a wave's items only move forward through the queue). One load when the item
+ *  is in the same bin as the wave's last one; otherwise the whole wave searches: each round its 64 lanes probe
+ *  evenly spaced positions and a ballot keeps the span between the last probe at or below `taken` and the next, so
+ *  the 70,656 bins take 3 rounds of loads instead of a chain of 17-34 dependent ones (a wave's first item of a
+ *  small batch waited tens of microseconds on that chain). Called by whole waves with a wave-uniform `taken`.
  */
 __device__ __forceinline__ uint32_t position_of_item(bins_view_t const &bins, uint32_t taken, uint32_t from = 0) {
     if (bins.item_start[from + 1] > taken) return from; // from + 1 <= bins_count_k: item_start has one more entry
-    uint32_t low = from + 1, span = 1; // item_start[low] <= taken
-    while (low + span < bins_count_k && bins.item_start[low + span] <= taken) low += span, span *= 2;
-    uint32_t high = min(low + span, (uint32_t)bins_count_k); // item_start[high] > taken, or high = the end
+    uint32_t low = from + 1, high = bins_count_k; // item_start[low] <= taken < item_start[high]
+    uint32_t const lane = __lane_id();
     while (high - low > 1) {
-        uint32_t const mid = (low + high) / 2;
-        if (bins.item_start[mid] <= taken) low = mid;
-        else high = mid;
+        uint32_t const stride = (high - low + 63) / 64;
+        uint32_t const probe = low + lane * stride; // lane 0 probes `low`, which qualifies
+        bool const below = probe < high && bins.item_start[probe] <= taken;
+        uint64_t const qualifying = __ballot(below); // lanes 0 .. L - 1: item_start is nondecreasing
+        uint32_t const last = 63 - (uint32_t)__builtin_clzll(qualifying);
+        low += last * stride;
+        high = min(high, low + stride);
     }
     return low;
 }
