@@ -170,11 +170,15 @@ struct myers_sweep_t {
         for (int i = 0; i < row_dwords_k; ++i) row_dwords[i] = __builtin_amdgcn_alignbyte(raw[i + 1], raw[i], shift);
     }
 
+    /** Symbol `i` of a tape string, loaded through the global address space. */
+    __device__ __forceinline__ static symbol_t column_symbol_at(symbol_t const *string, int32_t i) {
+        if constexpr (runes_) return *(global_dword_t *)(uintptr_t)(string + i);
+        else return *(global_u8_t *)(uintptr_t)(string + i);
+    }
+
     /** This lane's column symbol `j` (0-based), 0 past the columns' end. */
     __device__ __forceinline__ symbol_t column_symbol(int32_t j) const {
-        if (j >= m) return 0;
-        if constexpr (runes_) return *(global_dword_t *)(uintptr_t)(columns + j);
-        else return *(global_u8_t *)(uintptr_t)(columns + j);
+        return j < m ? column_symbol_at(columns, j) : (symbol_t)0;
     }
 
     __device__ __forceinline__ void load(item_args_t const &a, uint32_t task_index, bool is_active, uint32_t wave) {
@@ -206,8 +210,15 @@ struct myers_sweep_t {
         reused = false;
         if constexpr (runes_) {
             // Runes are their own codes: plane b holds bit b, up to the highest bit of the wave's largest row rune.
+            // Loads go 16 at a time, so a pair's rows cost one memory round trip per 16 G rows.
             uint32_t largest = 0;
-            for (int32_t i = k; i < n; i += G) largest = max(largest, (uint32_t)rows[i]);
+            for (int32_t first = k; first < n; first += 16 * G) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    int32_t const i = first + q * G;
+                    largest = max(largest, i < n ? (uint32_t)*(global_dword_t *)(uintptr_t)(rows + i) : 0u);
+                }
+            }
             for (int offset = 1; offset < 64; offset <<= 1)
                 largest = max(largest, (uint32_t)__shfl_xor((int)largest, offset, 64));
             planes_used = __builtin_amdgcn_readfirstlane(largest ? 32 - __builtin_clz(largest) : 1);
@@ -336,12 +347,35 @@ struct myers_sweep_t {
         }
 #pragma unroll
         for (int b = 0; b < planes_k; ++b) planes[b] = 0;
-        for (int32_t r = 0; r < word_bits_; ++r) {
-            int32_t const i = lane_top + r; // 0-based row
-            uint32_t const code = i < n ? code_of(rows[i]) : 0u;
+        if constexpr (G == 64) { // one pair per wave: a stripe's setup is spread over all its columns
+            for (int32_t r = 0; r < word_bits_; ++r) {
+                int32_t const i = lane_top + r; // 0-based row
+                uint32_t const code = i < n ? code_of(rows[i]) : 0u;
 #pragma unroll
-            for (int b = 0; b < planes_k; ++b)
-                if (b < planes_used) planes[b] |= (word_t)((code >> b) & 1u) << r;
+                for (int b = 0; b < planes_k; ++b)
+                    if (b < planes_used) planes[b] |= (word_t)((code >> b) & 1u) << r;
+            }
+        }
+        // Groups of several pairs (runes) sweep few columns: 16 rows at a time, so that their loads issue together
+        // (one memory round trip per 16 rows), then each plane takes their bits (`planes_used` is wave-uniform).
+        else for (int32_t first = 0; first < word_bits_; first += 16) {
+            uint32_t codes[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                int32_t const i = lane_top + first + q; // 0-based row
+                uint32_t symbol = 0;
+                if (i < n) symbol = column_symbol_at(rows, i);
+                codes[q] = i < n ? code_of(symbol) : 0u;
+            }
+#pragma unroll
+            for (int b = 0; b < planes_k; ++b) {
+                if (b < planes_used) {
+                    uint32_t bits = 0;
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) bits |= ((codes[q] >> b) & 1u) << q;
+                    planes[b] |= (word_t)bits << first;
+                }
+            }
         }
         if (table_mode) { // the match word of every code for this lane's 64 rows
             for (uint32_t code = 0; code < (uint32_t)myers_eq_codes_k; ++code)
