@@ -14,8 +14,8 @@
  *  row's symbol code, and Eq = AND_b (plane_b XNOR bit b of the column code), one `v_bitop3` per plane and half-word.
  *  For 64-lane groups the symbols are first renumbered to a dense code (0 = absent from the rows) so that only
  *  ceil(log2(distinct + 1)) planes are needed, e.g. 3 for DNA; every group renumbers its own pair (4- and 16-lane
- *  groups compute codes from a presence set and its prefix counts). When every group of the wave has at most 15
- *  distinct symbols, each stripe instead writes the match word of every code to an 8 KB LDS table ([16 codes][lanes]
+ *  groups compute codes from a presence set and its prefix counts). When every group of the wave has at most 13
+ *  distinct symbols, each stripe instead writes the match word of every code to a 7 KB LDS table ([14 codes][lanes]
  *  u64 per group), and a step reads its word back (for 64-lane groups prefetched one step ahead) in place of the plane
  *  arithmetic; 4- and 16-lane groups keep that table from one item to the next while their rows strings stay the same
  *  (the queries of an all-pairs product). For 64-lane groups lane 0's inputs of a 64-step chunk (horizontal deltas
@@ -121,10 +121,10 @@ struct myers_sweep_t {
     myers_tables_t *tables;
     myers_group_tables_t *group_tables; /**< this lane's group (G < 64) */
     int32_t distinct;    /**< distinct byte values in the rows */
-    word_t *eq_table;    /**< [16 codes][G lanes] match words of this group's current stripe (table mode) */
+    word_t *eq_table;    /**< [14 codes][G lanes] match words of this group's current stripe (table mode) */
     int8_t *ring;        /**< lane-0 inputs of a 64-step chunk (G = 64) */
     int32_t planes_used;
-    bool table_mode;     /**< wave-uniform: at most 15 distinct symbols, match words come from `eq_table` */
+    bool table_mode;     /**< wave-uniform: at most 13 distinct symbols, match words come from `eq_table` */
     bool reused;         /**< wave-uniform, smaller groups over bytes: the LDS tables of the wave's previous item
                               were built for the same rows strings, and stand */
 

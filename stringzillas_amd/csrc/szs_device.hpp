@@ -271,9 +271,11 @@ struct materialize_args_t {
  *  follows the wave's profile or Myers tables in dynamic LDS. */
 constexpr uint32_t inject_ring_bytes_k = 64 * 16;
 /** Myers LDS per wave: renumbering tables (one set for a 64-lane group, a 64-byte set per smaller group), then a
- *  [16 codes][64 lanes] u64 match table (split among the groups), then the ring. */
+ *  [14 codes][64 lanes] u64 match table (split among the groups; code 0 = absent from the rows, so at most 13
+ *  distinct symbols), then the ring: 9 KB, so that four workgroups of four waves fit a CU's 160 KB (with 16 codes
+ *  the 10 KB per wave plus the kernel's static LDS left room for three). */
 constexpr uint32_t myers_tables_bytes_k = 1024;
-constexpr int32_t myers_eq_codes_k = 16;
+constexpr int32_t myers_eq_codes_k = 14;
 constexpr uint32_t myers_lds_bytes_k = myers_tables_bytes_k + myers_eq_codes_k * 64 * 8 + inject_ring_bytes_k;
 
 /** One UTF-8 tape to transcode: its strings are numbered from `string_base` across the batch. */

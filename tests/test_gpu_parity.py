@@ -161,10 +161,12 @@ def test_batch_modes_agree_and_graph_replays():
         assert batch.cells > 0
 
 
-@pytest.mark.parametrize("alphabet_name", ["binary", "dna", "printable", "all_bytes"])
+@pytest.mark.parametrize("alphabet_name", ["binary", "dna", "thirteen", "fourteen", "printable", "all_bytes"])
 def test_unit_levenshtein_bit_parallel_alphabets(alphabet_name):
-    """Unit costs take the bit-parallel tier above 64 rows; its symbol renumbering must not depend on the alphabet."""
-    alphabets = {"binary": b"ab", "dna": DNA, "printable": bytes(range(32, 127)), "all_bytes": bytes(range(256))}
+    """Unit costs take the bit-parallel tier above 64 rows; its symbol renumbering must not depend on the alphabet.
+    Rows of at most 13 distinct bytes take the match-word table, 14 and more the bit-planes."""
+    alphabets = {"binary": b"ab", "dna": DNA, "thirteen": b"abcdefghijklm", "fourteen": b"abcdefghijklmn",
+                 "printable": bytes(range(32, 127)), "all_bytes": bytes(range(256))}
     rng = np.random.default_rng(zlib.crc32(alphabet_name.encode()))
     alphabet = alphabets[alphabet_name]
     lengths = [(50, 300), (250, 1100), (1000, 4200), (4000, 9000)]
