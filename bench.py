@@ -73,6 +73,9 @@ def parse():
                         help="measure only the fingerprint engine and print its report (for profiling)")
     parser.add_argument("--only-headline", action="store_true",
                         help="measure the mixed batch alone, no side measurements (for profiling)")
+    parser.add_argument("--only-allpairs", type=int, nargs=2, metavar=("BYTES", "TILE"),
+                        help="time one all-pairs Levenshtein call on TILE x TILE random ACGT strings of BYTES bytes "
+                             "(the reference-shaped call, tapes and results in HBM) and print it (for profiling)")
     parser.add_argument("--only-kind", choices=["levenshtein", "levenshtein_utf8", "needleman_wunsch"],
                         help="time one kind of the mixed batch alone in the persistent form (for profiling)")
     return parser.parse_args()
@@ -347,6 +350,40 @@ def config_line(szs, workload, args):
     return line
 
 
+def allpairs_alone(szs, length: int, tile: int, calls: int) -> dict:
+    """One all-pairs Levenshtein call (the reference-shaped `szs_levenshtein_distances_u64tape` on a scope) over
+    `tile` random ACGT strings of `length` bytes against a second tape of the same strings, timed over `calls`
+    calls after one warm-up; for profiling the published all-pairs shape on its own."""
+    import torch
+    from stringzillas_amd import _native
+
+    rng = np.random.default_rng(0x535A0011)
+    letters = np.frombuffer(b"acgt", dtype=np.uint8)
+    data = letters[rng.integers(0, 4, size=tile * length)]
+    offsets = np.arange(tile + 1, dtype=np.uint64) * length
+    queries, candidates = to_device_tape(data, offsets), to_device_tape(data, offsets)
+    results = torch.empty((tile, tile), dtype=torch.int64, device="cuda")
+    engine, scope = szs.LevenshteinDistances(), szs._scope(None)
+    call = getattr(_native.lib, engine._prefix + "_u64tape")
+    error = ctypes.c_char_p()
+    first, second = queries.tape(), candidates.tape()
+
+    def once():
+        status = call(engine.handle, scope.handle, ctypes.byref(first), ctypes.byref(second),
+                      ctypes.c_void_p(results.data_ptr()), ctypes.c_size_t(tile), ctypes.byref(error))
+        _native.check(status, error, "all-pairs")
+
+    once()
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for _ in range(calls):
+        once()
+    torch.cuda.synchronize()
+    seconds = (time.perf_counter() - start) / calls
+    return {"allpairs": "levenshtein", "bytes": length, "tile": f"{tile}x{tile}", "calls": calls,
+            "call_ms": round(seconds * 1e3, 3), "gcups": round(tile * tile * length * length / seconds / 1e9, 1)}
+
+
 def cross_shapes(szs, args):
     """The reference's published shape: all-pairs cross-products of equal-length ACGT strings through the
     reference-shaped call (`szs_*_u64tape` with tapes and the result matrix in HBM), under unit costs
@@ -460,6 +497,9 @@ def main():
     if args.only_fingerprints:
         print(json.dumps({"fingerprints": fingerprints_run(args.fingerprint_mb * 1_000_000, args.steps,
                                                            not args.skip_cpu)}), flush=True)
+        return
+    if args.only_allpairs:
+        print(json.dumps(allpairs_alone(szs, *args.only_allpairs, max(1, args.steps))), flush=True)
         return
     device = torch.device("cuda", 0)
     t0 = time.time()
