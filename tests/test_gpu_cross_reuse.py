@@ -1,5 +1,7 @@
 """GPU parity of large all-pairs products, where a wave sweeps many consecutive items of one query.
 
+The queue hands out up to 16 cheap items per draw at these sizes (8 per draw for the 100-byte 4,096^2 product).
+
 At these sizes the queue hands out several cheap items per draw and the pairs of an item share their query, so the
 bit-parallel groups keep the query's LDS tables from one item to the next instead of rebuilding them. Queries of
 small alphabets (match-word tables) alternate with wide ones (bit-planes, no table to keep), so kept and rebuilt
@@ -49,3 +51,22 @@ def test_all_pairs_products_keep_query_tables(length):
     b = O.tape([strings[c] for _, c in picks])
     expected = O.pairs(params, a, b, threads=16)
     np.testing.assert_array_equal(got[picks[:, 0], picks[:, 1]], expected)
+
+
+@pytest.mark.parametrize("length", [60, 100])
+def test_all_pairs_products_with_many_items_per_draw(length):
+    """4,096^2 products: the queue hands out 16 (60-byte strings, DP groups) or 8 (100 bytes, Myers groups with
+    kept query tables) cheap items per draw. Every entry must be written and equal its transposed twin, and whole
+    query rows equal the oracle."""
+    rng = np.random.default_rng(1000 + length)
+    count = 4096
+    strings = _strings(rng, count, length)
+    engine = szs.LevenshteinDistances()
+    got = engine(szs.Strs(strings), szs.Strs(list(strings))).astype(np.int64)
+    assert got.shape == (count, count)
+    np.testing.assert_array_equal(np.diagonal(got), 0)
+    assert (got >= 0).all() and (got <= length).all()
+    assert np.array_equal(got, got.T)
+    rows = [0, 2049, 4095]
+    expected = O.cross(O.Params("levenshtein"), O.tape([strings[q] for q in rows]), O.tape(strings), threads=16)
+    np.testing.assert_array_equal(got[rows], expected)
