@@ -9,7 +9,7 @@ Beside it, in the same JSON line: the roofline of the scoring kernel (core recur
 duration), each kind alone, the per-call form on a shared sample, the reference's CPU scorers on all cores of the
 share and on one core, the Levenshtein (1,024 pairs), Needleman-Wunsch (16,384) and Smith-Waterman (100,000)
 configurations, the reference's published all-pairs shapes (equal-length ACGT, 100 and 1,000 bytes), the serving
-loop (60 s of Poisson arrivals per rate, every result verified) and the fingerprint engine.
+loop (20 s of Poisson arrivals per rate, every result verified) and the fingerprint engine.
 """
 
 from __future__ import annotations
@@ -64,7 +64,7 @@ def parse():
     parser.add_argument("--skip-cross", action="store_true", help="skip the published all-pairs shapes")
     parser.add_argument("--serve-qps", type=float, nargs="*", default=[200.0, 1000.0, 5000.0],
                         help="offered rates of the serving run (none to skip)")
-    parser.add_argument("--serve-seconds", type=float, default=60.0, help="arrival window per serving rate")
+    parser.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window per serving rate")
     parser.add_argument("--baseline-seconds", type=float, default=20.0,
                         help="arrival window of the one-launch-per-request baseline")
     parser.add_argument("--skip-fingerprints", action="store_true", help="skip the fingerprint engine's measurement")
