@@ -60,11 +60,11 @@ struct tail_sweeps_t {
     using shorter = void;
     using middle = void;
 };
-template <int family_, bool uniform_base_>
-struct tail_sweeps_t<packed_sweep_t<family_, 8, uniform_base_>> { // 512-row stripes: 128-, 256- or 384-row tails
-    using tiny = packed_sweep_t<family_, 2, uniform_base_>;
-    using shorter = packed_sweep_t<family_, 4, uniform_base_>;
-    using middle = packed_sweep_t<family_, 6, uniform_base_>;
+template <int family_, bool uniform_base_, bool biased_>
+struct tail_sweeps_t<packed_sweep_t<family_, 8, uniform_base_, biased_>> { // 512-row stripes: 128/256/384-row tails
+    using tiny = packed_sweep_t<family_, 2, uniform_base_, biased_>;
+    using shorter = packed_sweep_t<family_, 4, uniform_base_, biased_>;
+    using middle = packed_sweep_t<family_, 6, uniform_base_, biased_>;
 };
 template <bool runes_>
 struct tail_sweeps_t<myers_sweep_t<64, runes_, 64>> { // 4096-row stripes: 2048-row tails of 32-row words
@@ -176,7 +176,9 @@ __device__ void run_independent(item_args_t const &a) {
             if (tail == 2) tail_sum = run_tail<middle_t>(a, task_index, wave, tail_base, in_row, gaps_row);
         }
     }
-    if constexpr (sweep_type_::reduce_k != reduce_none_k) sweep.write_result(sweep.partial() + tail_sum);
+    // A tail stripe's share: its best (local alignment, at least 0 like every best) or its sum.
+    if constexpr (sweep_type_::reduce_k == reduce_max_k) sweep.write_result(max(sweep.partial(), tail_sum));
+    else if constexpr (sweep_type_::reduce_k != reduce_none_k) sweep.write_result(sweep.partial() + tail_sum);
 }
 
 /** Polls of a predecessor's progress (each a device-scope load plus a ~128-cycle sleep) a group may make on top of
@@ -328,6 +330,16 @@ __device__ void run_cooperative(item_args_t const &a) {
 
 #pragma region Dispatch
 
+/** One packed shape with the sweep's compile-time options. */
+template <int family_, int shape_, bool uniform_base_, bool biased_>
+__device__ __forceinline__ void run_packed(item_args_t const &a) {
+    constexpr shape_t shape = shapes_k[shape_];
+    using sweep = packed_sweep_t<family_, shape.rows_per_lane, uniform_base_, biased_>;
+    if constexpr (shape.chained) run_cooperative<sweep, true>(a);
+    else if constexpr (shape.cooperative) run_cooperative<sweep>(a);
+    else run_independent<sweep>(a);
+}
+
 template <int family_, typename symbol_type_, int shape_, bool shifted_>
 __device__ __forceinline__ void run_shape(item_args_t const &a) {
     constexpr shape_t shape = shapes_k[shape_];
@@ -343,23 +355,27 @@ __device__ __forceinline__ void run_shape(item_args_t const &a) {
         }
     }
     else if constexpr (shape.packed) {
-        // Packed 16-bit shapes exist for the global class-cost families in their anti-diagonal form only (the
-        // binning never picks them otherwise).
-        if constexpr ((family_ == family_nw_linear_k || family_ == family_nw_affine_k) && sizeof(symbol_type_) == 1 &&
-                      shifted_) {
+        // Packed 16-bit shapes exist for the global class-cost families in their anti-diagonal form and for local
+        // linear gaps in their `H + g` form (the binning never picks them otherwise); chained groups for the global
+        // ones only (a local score needs every stripe's best).
+        constexpr bool global = family_ == family_nw_linear_k || family_ == family_nw_affine_k;
+        constexpr bool local = family_ == family_sw_linear_k;
+        if constexpr ((global || (local && !shape.chained)) && sizeof(symbol_type_) == 1 && shifted_) {
             // One base for the whole wave when the pair's engine admits it at these rows (no per-step moves of
             // values between lane bases), else one base per lane. Every pair of a packed item has one wave to
             // itself or the whole workgroup, so the item's first pair decides.
+            // Local alignment: a profile biased below g costs one more add per two cells (`biased_`).
             engine_params_t const *params = a.segments[a.tasks[a.order[a.first]].segment].params;
-            constexpr int rows = shape.rows_per_lane;
-            if (params->packed_uniform_rows >= rows) {
-                if constexpr (shape.chained) run_cooperative<packed_sweep_t<family_, rows, true>, true>(a);
-                else if constexpr (shape.cooperative) run_cooperative<packed_sweep_t<family_, rows, true>>(a);
-                else run_independent<packed_sweep_t<family_, rows, true>>(a);
+            bool const uniform_base = params->packed_uniform_rows >= shape.rows_per_lane;
+            if constexpr (local) {
+                if (params->packed_bias != params->open) {
+                    if (uniform_base) run_packed<family_, shape_, true, true>(a);
+                    else run_packed<family_, shape_, false, true>(a);
+                    return;
+                }
             }
-            else if constexpr (shape.chained) run_cooperative<packed_sweep_t<family_, rows>, true>(a);
-            else if constexpr (shape.cooperative) run_cooperative<packed_sweep_t<family_, rows>>(a);
-            else run_independent<packed_sweep_t<family_, rows>>(a);
+            if (uniform_base) run_packed<family_, shape_, true, false>(a);
+            else run_packed<family_, shape_, false, false>(a);
         }
     }
     // Class-cost families keep their LDS profile at <= 8 rows per lane; larger shapes are never binned for them.

@@ -1,12 +1,22 @@
 /**
  *  @file packed_sweep.hpp
- *  @brief Needleman-Wunsch sweeps with two 16-bit cells per 32-bit register (packed integer VALU).
+ *  @brief Needleman-Wunsch and linear-gap Smith-Waterman sweeps with two 16-bit cells per 32-bit register (packed
+ *  integer VALU).
  *
- *  The recurrences are the global ones of include/stringzillas/similarities/serial.hpp:669-749 (linear) and
- *  854-964 (affine, Gotoh), in the anti-diagonal domain of dp_sweep.hpp: Ĉ = H - e*(i+j), ĥ and v̂ likewise plus
- *  (e - o), so that
+ *  The global recurrences are those of include/stringzillas/similarities/serial.hpp:669-749 (linear) and 854-964
+ *  (affine, Gotoh), in the anti-diagonal domain of dp_sweep.hpp: Ĉ = H - e*(i+j), ĥ and v̂ likewise plus (e - o),
+ *  so that
  *      ĥ = max(Ĉ_left, ĥ_left),  v̂ = max(Ĉ_up, v̂_up),  Ĉ = max(ĥ, v̂, Ĉ_diag + s - o - e) + (o - e)
  *      linear: Ĉ = max(Ĉ_up, Ĉ_left, Ĉ_diag + s - 2g).
+ *  Local linear alignment (serial.hpp:762-841, best over all cells from 0) keeps the int32 sweep's `H + g` form
+ *  instead, whose zero floor is one constant per lane and chunk: X = H + g and
+ *      X = max(X_up, X_left, max(X_diag + s - g, 0)) + g,
+ *  the gap terms read the stored values as they are. Per two cells: the cost `v_perm_b32`, two `v_pk_add_u16` (the
+ *  profile bytes hold `cost - min(g, s_min)` so that they read as unsigned; the second add takes g - min(g, s_min)
+ *  off again), one `v_pk_max_u16` (substitution against the floor; all three off the row-to-row chain), the
+ *  three-way maximum, the `+ g` add and one `v_pk_max_u16` into the lane's running best, which is folded into an
+ *  int32 best at the end of every 64-step chunk (the base changes between chunks) over the halves whose rows and
+ *  columns lie in the matrix.
  *
  *  Packing. The rows of one lane form a serial chain (each row needs the row above at the same column), so two
  *  cells of one register must come from independent chains: lane k holds R rows, and its register p carries row
@@ -33,7 +43,8 @@
  *  `packed_rows_limit`). Otherwise pairs run the int32 sweep. A lane whose upper half has passed column m holds no
  *  cell that is kept and keeps its base; the row above the stripe repeats column m's values beyond it; cells beyond
  *  column m or row n only feed cells that are discarded, and the two halves of a register never mix. Profile
- *  bytes hold `cost - (o + e)`, which the engine requires in [0, 127], so they read as unsigned bytes.
+ *  bytes hold `cost - (o + e)` (global) or `cost - min(g, s_min)` (local), which the engine requires in [0, 127],
+ *  so they read as unsigned bytes.
  */
 #pragma once
 #include <hip/hip_runtime.h>
@@ -67,22 +78,26 @@ __device__ __forceinline__ uint32_t pk_max3_ordered(uint32_t a, uint32_t b, uint
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_maximum(__builtin_elementwise_maximum(x, y), z));
 }
 
-template <int family_, int R, bool uniform_base_ = false>
+/** `biased_` (local linear only): the profile bias differs from g, so each substitution term takes g - bias off. */
+template <int family_, int R, bool uniform_base_ = false, bool biased_ = false>
 struct packed_sweep_t {
-    static_assert(family_ == family_nw_linear_k || family_ == family_nw_affine_k, "global class-cost families only");
+    static_assert(family_ == family_nw_linear_k || family_ == family_nw_affine_k || family_ == family_sw_linear_k,
+                  "global class-cost families and local linear gaps only");
     static_assert(R % 2 == 0 && R >= 2 && R <= 8, "two half-stripes of R/2 rows; profiles hold at most 8 rows");
     static constexpr int P = R / 2;               // packed registers per lane
+    static constexpr int B = (P + 1) / 2;         // local best registers per lane: one per two score registers
     static constexpr int RP = (R + 3) / 4 * 4;    // profile bytes per lane and class (the int32 sweep's layout)
     static constexpr bool affine_k = family_ == family_nw_affine_k;
+    static constexpr bool local_k = family_ == family_sw_linear_k;
     static constexpr int group_k = 64;
     static constexpr int rows_per_lane = R;
     static constexpr int skew_k = 2;              // lane k + 1 runs two columns behind lane k
-    static constexpr reduce_kind_t reduce_k = reduce_none_k;
+    static constexpr reduce_kind_t reduce_k = local_k ? reduce_max_k : reduce_none_k;
 
     // The pair.
     int lane;
     int32_t n, m;
-    int32_t open, extend, dense_count, shift;
+    int32_t open, extend, dense_count, shift; // shift: the profile bias (`packed_bias`)
     bool transposed;
     engine_params_t const *params;
     uint8_t const *rows;
@@ -103,6 +118,9 @@ struct packed_sweep_t {
     bool result_upper;                /**< the pair's last row is in this lane's upper half */
     int32_t result_register;
     int32_t captured;                 /**< the last row's score at column m, when it sits in the upper half */
+    // Local alignment.
+    uint32_t best[B];                 /**< per half: the chunk's largest cell (u16 of H in the chunk's base), 0 = none */
+    int32_t running_best;             /**< the lane's best H over the chunks folded so far (0 from the start) */
 
     __device__ __forceinline__ void load(item_args_t const &a, uint32_t task_index, bool, uint32_t wave) {
         int8_t *wave_profile = wave_lds(wave, a.wave_lds_bytes);
@@ -114,7 +132,7 @@ struct packed_sweep_t {
         m = (int32_t)task.columns_length;
         open = params->open, extend = params->extend;
         dense_count = params->dense_count;
-        shift = params->shift;
+        shift = params->packed_bias;
         transposed = task_transposed(task);
         uint8_t const *query = (uint8_t const *)segment.query_symbols;
         uint8_t const *candidate = (uint8_t const *)segment.candidate_symbols;
@@ -126,10 +144,15 @@ struct packed_sweep_t {
         profile = wave_profile;
         ring = wave_profile + (a.wave_lds_bytes - inject_ring_bytes_k);
         captured = 0;
+        running_best = 0; // serial.hpp:781 starts the local best at 0
     }
 
-    /** The stored int32 form minus H at cell (i, j): -e*(i+j) modulo 2^32 (linear: e = g; see `wrap_add`). */
-    __device__ __forceinline__ int32_t domain(int32_t i, int32_t j) const { return wrap_mul(-extend, wrap_add(i, j)); }
+    /** The stored int32 form minus H at cell (i, j): -e*(i+j) modulo 2^32 (linear: e = g; see `wrap_add`), or the
+     *  constant g of the local form. */
+    __device__ __forceinline__ int32_t domain(int32_t i, int32_t j) const {
+        if constexpr (local_k) return open;
+        else return wrap_mul(-extend, wrap_add(i, j));
+    }
     __device__ __forceinline__ int32_t gap_domain(int32_t i, int32_t j) const {
         return wrap_add(domain(i, j), extend - open);
     }
@@ -170,7 +193,11 @@ struct packed_sweep_t {
             for (int r = 0; r < R; ++r) {
                 int8_t const cost = transposed ? params->dense_costs[c * max_dense_classes_k + row_classes[r]]
                                                : params->dense_costs[row_classes[r] * max_dense_classes_k + c];
-                mine[c * 64 * RP + r] = (int8_t)(cost - shift); // in [0, 127]: checked when the engine was made
+                // In [0, 127]: checked when the engine was made. Local alignment gives rows past n the lowest cost,
+                // min(g, s_min) <= 0: every value such a row takes is then the floor or a cell of the pair plus
+                // steps <= 0, never above the pair's best, so the running best needs no row mask.
+                bool const past = local_k && lane_top + r + 1 > n;
+                mine[c * 64 * RP + r] = past ? (int8_t)0 : (int8_t)(cost - shift);
             }
         }
         base = border_cell();
@@ -188,6 +215,10 @@ struct packed_sweep_t {
         int32_t const last_row = n - 1 - base_row; // beyond this stripe unless it is the last one
         result_upper = last_row >= 0 && last_row / R == lane && last_row % R < P;
         result_register = (last_row % R) % P;
+        if constexpr (local_k) {
+#pragma unroll
+            for (int b = 0; b < B; ++b) best[b] = 0;
+        }
     }
 
     /** Lane-0 inputs of column `j`: the row above the stripe (stored int32 form), its vertical gap, its symbol.
@@ -226,10 +257,13 @@ struct packed_sweep_t {
         base = new_base;
     }
 
-    /** One step of both half-stripes: `up` and `vertical_in` are the packed top inputs. */
-    __device__ __forceinline__ void update_rows(uint32_t up, uint32_t vertical_in, uint32_t kappa) {
+    /** One step of both half-stripes: `up` and `vertical_in` are the packed top inputs. Local alignment: `zero_floor`
+     *  is the packed H = 0, `keep` the halves whose columns lie within the pair (their cells count for the best). */
+    __device__ __forceinline__ void update_rows(uint32_t up, uint32_t vertical_in, uint32_t kappa, uint32_t zero_floor,
+                                                uint32_t unbias, uint32_t keep) {
         uint32_t diag = diagonal;
         uint32_t vert = vertical_in;
+        uint32_t held = 0; // local alignment: the previous register's cell, paired with this one's into the best
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             // Upper row p reads this column's profile byte p, lower row P + p the previous column's byte P + p;
@@ -249,6 +283,16 @@ struct packed_sweep_t {
                 horizontal[p] = h;
                 vert = v;
                 stored = pk_add_u16(pk_max3_ordered(h, if_substitution, v), kappa);
+            }
+            else if constexpr (local_k) {
+                // - (g - packed_bias) when the profile is biased below g
+                uint32_t const substitution = biased_ ? pk_add_u16(if_substitution, unbias) : if_substitution;
+                uint32_t const cell = pk_max3_ordered(up, left, pk_max_u16(substitution, zero_floor));
+                uint32_t const counted = cell & keep; // all ones on steady steps
+                if (p % 2 == 1) best[p / 2] = pk_max3_ordered(best[p / 2], held, counted);
+                else if (p == P - 1) best[p / 2] = pk_max_u16(best[p / 2], counted);
+                held = counted;
+                stored = pk_add_u16(cell, kappa);
             }
             else stored = pk_max3_ordered(up, left, if_substitution);
             diag = left;
@@ -305,7 +349,13 @@ struct packed_sweep_t {
         // Lane 0's column symbol comes from the ring; the other lanes fetched theirs during the previous step.
         if (lane == 0) symbol_now = (uint32_t)inject.z;
         read_costs(symbol_now, costs_now);
-        uint32_t const kappa = ((uint32_t)(open - extend) & 0xFFFFu) * 0x10001u;
+        // The add after the maximum: o - e (affine global), g (local: stores H + g).
+        uint32_t const kappa = ((uint32_t)(local_k ? open : open - extend) & 0xFFFFu) * 0x10001u;
+        // The packed H = 0 of the local form in this chunk's base; below every value the lane holds (and so clamped
+        // to 0) once the base is far above zero.
+        uint32_t zero_floor = 0;
+        if constexpr (local_k) zero_floor = (uint32_t)max(0, packed_centre_k - base) * 0x10001u;
+        uint32_t const unbias = ((uint32_t)(shift - open) & 0xFFFFu) * 0x10001u;
         uint32_t const border_high = to_u16(border_cell()) << 16, border_gap_high = to_u16(border_gap()) << 16;
 
 #pragma unroll 8
@@ -328,7 +378,9 @@ struct packed_sweep_t {
             read_costs(symbol_next, costs_next);
             int32_t const c = first_step + step + 1 - 2 * lane; // upper column; the lower half is at c - 1
             if (steady_ || (uint32_t)(c - 1) <= (uint32_t)m) {
-                update_rows(up, vertical_in, kappa);
+                // Upper half at column c, lower half at c - 1: both within 1..m on steady steps.
+                uint32_t const keep = steady_ ? ~0u : (c <= m ? 0xFFFFu : 0u) | (c >= 2 ? 0xFFFF0000u : 0u);
+                update_rows(up, vertical_in, kappa, zero_floor, unbias, keep);
                 if (!steady_) {
                     if (c == 1) { // the lower half stays on the column-0 border
 #pragma unroll
@@ -338,7 +390,7 @@ struct packed_sweep_t {
                         }
                     }
                     // The upper half moves on to column m + 1 next step: keep the last row's score at column m.
-                    if (result_upper && c == m) captured = from_u16(register_at(result_register));
+                    if (!local_k && result_upper && c == m) captured = from_u16(register_at(result_register));
                 }
             }
             diagonal = up;
@@ -362,11 +414,25 @@ struct packed_sweep_t {
             if (j_out >= 1 && j_out <= m) out_scores[j_out] = (int32_t)bottom;
             if (lane == 0) out_gaps[chunk] = __builtin_amdgcn_readlane(base, 63);
         }
+        if constexpr (local_k) fold_best();
     }
 
-    /** Writes the pair's score from the lane holding its last row, on the last stripe. */
+    /** Local alignment: moves the chunk's best cells (u16 of H in this chunk's base) into the int32 best, and clears
+     *  them for the next chunk. A half that took no cell in the chunk holds 0. */
+    __device__ __forceinline__ void fold_best() {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            uint32_t const low = best[b] & 0xFFFFu, high = best[b] >> 16;
+            if (low) running_best = max(running_best, (int32_t)low - packed_centre_k + base);
+            if (high) running_best = max(running_best, (int32_t)high - packed_centre_k + base);
+            best[b] = 0;
+        }
+    }
+
+    /** Writes the pair's score from the lane holding its last row, on the last stripe (global families; the local
+     *  best is folded chunk by chunk and reduced by `partial`). */
     __device__ __forceinline__ void end_stripe(int32_t base_row, bool last_stripe) {
-        if (!last_stripe) return;
+        if (local_k || !last_stripe) return;
         int32_t const last_row = n - 1 - base_row;
         if (last_row / R != lane) return;
         int32_t const value = result_upper ? captured : from_u16(register_at(result_register) >> 16);
@@ -375,8 +441,22 @@ struct packed_sweep_t {
         if (mirror != no_mirror_k) results[mirror] = score;
     }
 
-    __device__ __forceinline__ int32_t partial() const { return 0; }
-    __device__ __forceinline__ void write_result(int32_t) const {}
+    /** Local alignment: the largest `running_best` of the wave's lanes. */
+    __device__ __forceinline__ int32_t partial() const {
+        if constexpr (!local_k) return 0;
+        int32_t value = running_best;
+        for (int offset = 1; offset < 64; offset <<= 1) value = max(value, __shfl_xor(value, offset, 64));
+        return value;
+    }
+
+    __device__ __forceinline__ void write_result(int32_t value) const {
+        if constexpr (local_k) {
+            if (lane == 0) {
+                results[output] = (int64_t)value;
+                if (mirror != no_mirror_k) results[mirror] = (int64_t)value;
+            }
+        }
+    }
 };
 
 } // namespace szs_amd

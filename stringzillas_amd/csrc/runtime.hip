@@ -257,16 +257,23 @@ static sz_status_t densify(engine_t *engine, uint8_t const *byte_to_class, int8_
  *  the chunk (its R rows and the row above, from its lower half's column to 64 columns on) lie within a Manhattan
  *  distance of R + 63 of it; R + 70 is used. Profile bytes `cost - (o + e)` must also lie in [0, 127] to read as
  *  unsigned bytes.
+ *  Local linear gaps (serial.hpp:762-841, g <= 0) keep X = H + g instead. There H(i,j) - H(i-1,j) >= g (the gap
+ *  term, or the zero floor when H(i-1,j) + g < 0), and H(i,j) - H(i-1,j) <= max(0, s_max - g) by the same
+ *  induction along the row (the floor only lowers the bound, the border row is 0): adjacent cells differ by at most
+ *  max(|g|, s_max - g), below the global delta with o = e = g, so the same limits hold. Its profile bytes hold
+ *  `cost - packed_bias` with packed_bias = min(g, s_min), and the substitution term subtracts g - packed_bias
+ *  again (modulo 2^16, so only the corrected sum must be in range). The zero floor is the lowest value of every cell.
  */
 int packed_rows_limit(engine_params_t const &params) {
-    if (params.family != family_nw_linear_k && params.family != family_nw_affine_k) return 0;
+    bool const local = params.family == family_sw_linear_k;
+    if (params.family != family_nw_linear_k && params.family != family_nw_affine_k && !local) return 0;
     int64_t const o = params.open, e = params.extend;
-    if (o > 0 || e > 0 || params.shift != o + e || params.dense_count == 0) return 0;
+    if (o > 0 || e > 0 || params.shift != (local ? o : o + e) || params.dense_count == 0) return 0;
     int64_t s_max = INT64_MIN, s_min = INT64_MAX;
     for (int x = 0; x < params.dense_count; ++x)
         for (int y = 0; y < params.dense_count; ++y) {
             int64_t const cost = params.dense_costs[x * max_dense_classes_k + y];
-            if (cost - params.shift < 0 || cost - params.shift > 127) return 0;
+            if (cost - params.packed_bias < 0 || cost - params.packed_bias > 127) return 0;
             s_max = std::max(s_max, cost), s_min = std::min(s_min, cost);
         }
     int64_t const rise = std::max<int64_t>(0, s_max - std::min(o, e)) + std::max<int64_t>(0, o - e);
@@ -348,6 +355,12 @@ sz_status_t make_engine(int kind, int match, int mismatch, int open, int extend,
             int const biased = engine->params.dense_costs[x * max_dense_classes_k + y] - engine->params.shift;
             if (biased < -128 || biased > 127) engine->params.shift = 0;
         }
+    engine->params.packed_bias = engine->params.shift;
+    if (family == family_sw_linear_k && engine->params.shift == open)
+        for (int x = 0; x < engine->params.dense_count; ++x)
+            for (int y = 0; y < engine->params.dense_count; ++y)
+                engine->params.packed_bias = std::min<int32_t>(engine->params.packed_bias,
+                                                               engine->params.dense_costs[x * max_dense_classes_k + y]);
     engine->packed_rows = packed_rows_limit(engine->params);
     engine->params.packed_uniform_rows = packed_uniform_rows_limit(engine->params, engine->packed_rows);
     *handle = engine;
