@@ -74,8 +74,10 @@ def parse():
     parser.add_argument("--only-headline", action="store_true",
                         help="measure the mixed batch alone, no side measurements (for profiling)")
     parser.add_argument("--only-allpairs", type=int, nargs=2, metavar=("BYTES", "TILE"),
-                        help="time one all-pairs Levenshtein call on TILE x TILE random ACGT strings of BYTES bytes "
-                             "(the reference-shaped call, tapes and results in HBM) and print it (for profiling)")
+                        help="time one all-pairs call on TILE x TILE random ACGT strings of BYTES bytes (the "
+                             "reference-shaped call, tapes and results in HBM) and print it (for profiling)")
+    parser.add_argument("--allpairs-kind", choices=["levenshtein", "needleman_wunsch", "smith_waterman"],
+                        default="levenshtein", help="engine of --only-allpairs (unit costs: 0/-1/-1 for NW and SW)")
     parser.add_argument("--only-kind", choices=["levenshtein", "levenshtein_utf8", "needleman_wunsch"],
                         help="time one kind of the mixed batch alone in the persistent form (for profiling)")
     return parser.parse_args()
@@ -350,10 +352,11 @@ def config_line(szs, workload, args):
     return line
 
 
-def allpairs_alone(szs, length: int, tile: int, calls: int) -> dict:
-    """One all-pairs Levenshtein call (the reference-shaped `szs_levenshtein_distances_u64tape` on a scope) over
-    `tile` random ACGT strings of `length` bytes against a second tape of the same strings, timed over `calls`
-    calls after one warm-up; for profiling the published all-pairs shape on its own."""
+def allpairs_alone(szs, length: int, tile: int, calls: int, kind: str = "levenshtein") -> dict:
+    """One all-pairs call (the reference-shaped `szs_*_u64tape` on a scope) over `tile` random ACGT strings of
+    `length` bytes against a second tape of the same strings, timed over `calls` calls after one warm-up; for
+    profiling the published all-pairs shape on its own. NW and SW score with the unit costs of the published shape
+    (match 0, mismatch and gap -1)."""
     import torch
     from stringzillas_amd import _native
 
@@ -363,7 +366,16 @@ def allpairs_alone(szs, length: int, tile: int, calls: int) -> dict:
     offsets = np.arange(tile + 1, dtype=np.uint64) * length
     queries, candidates = to_device_tape(data, offsets), to_device_tape(data, offsets)
     results = torch.empty((tile, tile), dtype=torch.int64, device="cuda")
-    engine, scope = szs.LevenshteinDistances(), szs._scope(None)
+    if kind == "levenshtein":
+        engine = szs.LevenshteinDistances()
+    else:
+        class_costs = np.full((32, 32), -1, dtype=np.int8)
+        np.fill_diagonal(class_costs, 0)
+        byte_to_class = np.zeros(256, dtype=np.uint8)
+        byte_to_class[letters] = np.arange(1, 5, dtype=np.uint8)
+        maker = szs.NeedlemanWunschScores if kind == "needleman_wunsch" else szs.SmithWatermanScores
+        engine = maker(byte_to_class, class_costs, -1, -1)
+    scope = szs._scope(None)
     call = getattr(_native.lib, engine._prefix + "_u64tape")
     error = ctypes.c_char_p()
     first, second = queries.tape(), candidates.tape()
@@ -380,7 +392,7 @@ def allpairs_alone(szs, length: int, tile: int, calls: int) -> dict:
         once()
     torch.cuda.synchronize()
     seconds = (time.perf_counter() - start) / calls
-    return {"allpairs": "levenshtein", "bytes": length, "tile": f"{tile}x{tile}", "calls": calls,
+    return {"allpairs": kind, "bytes": length, "tile": f"{tile}x{tile}", "calls": calls,
             "call_ms": round(seconds * 1e3, 3), "gcups": round(tile * tile * length * length / seconds / 1e9, 1)}
 
 
@@ -499,7 +511,7 @@ def main():
                                                            not args.skip_cpu)}), flush=True)
         return
     if args.only_allpairs:
-        print(json.dumps(allpairs_alone(szs, *args.only_allpairs, max(1, args.steps))), flush=True)
+        print(json.dumps(allpairs_alone(szs, *args.only_allpairs, max(1, args.steps), args.allpairs_kind)), flush=True)
         return
     device = torch.device("cuda", 0)
     t0 = time.time()
