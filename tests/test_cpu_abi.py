@@ -200,7 +200,9 @@ def test_tuning_ladders_validate_and_round_trip():
     assert len(table) == _native.lib.szs_tuning_shapes()
     unit = autotune.slot_of("lev_linear", unit=True)
     nw = autotune.slot_of("nw_affine")
-    saved = {slot: autotune.get_ladder(slot) for slot in (unit, nw)}
+    plain = autotune.slot_of("nw_affine", runes=True)  # engines on the plain recurrences: no packed cells
+    sw_linear, sw_affine = autotune.slot_of("sw_linear"), autotune.slot_of("sw_affine")
+    saved = {slot: autotune.get_ladder(slot) for slot in (unit, nw, plain, sw_linear, sw_affine)}
     try:
         ladder, beyond = saved[nw]
         assert ladder and all(b[1] > a[1] for a, b in zip(ladder, ladder[1:]))
@@ -213,12 +215,21 @@ def test_tuning_ladders_validate_and_round_trip():
             autotune.set_ladder(nw, [(0, 10_000)], beyond)  # a sub-wave shape holds one stripe
         with pytest.raises(ValueError):
             autotune.set_ladder(nw, [(3, 100), (4, 50)], beyond)  # limits must grow
-        plain = autotune.slot_of("nw_affine", runes=True)  # engines on the plain recurrences: no packed cells
         packed = next(i for i, s in enumerate(table) if s[4])
         with pytest.raises(ValueError):
             autotune.set_ladder(plain, [(packed, 512)], beyond)
         autotune.set_ladder(plain, [], autotune.wide_shapes()[0])  # the int64 shape scores any slot
         assert autotune.get_ladder(nw) == (ladder[:2], beyond)  # rejected ladders change nothing
+        # Local alignment takes packed shapes with linear gaps only, and never chained groups (its score needs
+        # every stripe's best).
+        cooperative_packed = next(i for i, s in enumerate(table) if s[4] and s[2] and not s[5])
+        chained_packed = next(i for i, s in enumerate(table) if s[4] and s[5])
+        autotune.set_ladder(sw_linear, [(packed, 512)], cooperative_packed)
+        assert autotune.get_ladder(sw_linear) == ([(packed, 512)], cooperative_packed)
+        with pytest.raises(ValueError):
+            autotune.set_ladder(sw_linear, [(packed, 512)], chained_packed)
+        with pytest.raises(ValueError):
+            autotune.set_ladder(sw_affine, [(packed, 512)], saved[sw_affine][1])
     finally:
         for slot, (ladder, beyond) in saved.items():
             autotune.set_ladder(slot, ladder, beyond)
@@ -323,11 +334,21 @@ def test_packed_rows_follow_the_range_bound():
     # 809; 4 rows: 200 * 74 + 809 = 15,609 fits, 6 rows: 200 * 76 + 809 = 16,009 does not.
     assert rows(szs.NeedlemanWunschScores(b2c, (costs.astype(np.int64) - 85).astype(np.int8), -128, -72)) == 4
     assert rows(szs.NeedlemanWunschScores(b2c, costs, 3, 1)) == 0  # odd signs: the bound does not apply
-    assert rows(szs.SmithWatermanScores(b2c, costs, -11, -1)) == 0  # local families stay int32
+    assert rows(szs.SmithWatermanScores(b2c, costs, -11, -1)) == 0  # local affine gaps stay int32
+    # Local linear gaps: profile bytes `cost - min(g, s_min)`, the same delta with o = e = g.
+    assert rows(szs.SmithWatermanScores(b2c, costs, -4, -4)) == 8  # bytes cost + 4 in [0, 15], delta = 19
+    assert rows(szs.SmithWatermanScores(b2c, costs, -2, -2)) == 8  # bias -4 below g: the correction add runs
+    assert rows(szs.SmithWatermanScores(b2c, costs, -95, -95)) == 4  # delta = 201: 201 * 74 + 660 = 15,534
+    assert rows(szs.SmithWatermanScores(b2c, costs, 1, 1)) == 0  # a positive gap: the bound does not apply
+    wide = costs.copy()
+    np.fill_diagonal(wide, 100)
+    wide[1, 2] = wide[2, 1] = -40
+    assert rows(szs.SmithWatermanScores(b2c, wide, -20, -20)) == 0  # 100 - (-40) = 140: not an unsigned byte
     assert rows(szs.LevenshteinDistances()) == 0
     dna_b2c, dna_costs = dna_tables()
     assert rows(szs.NeedlemanWunschScores(dna_b2c, dna_costs, -1, -1)) == 0  # -3 - (-2) < 0: no unsigned profile
     assert rows(szs.NeedlemanWunschScores(dna_b2c, dna_costs, -2, -2)) == 8  # -3 - (-4) = 1: fits
+    assert rows(szs.SmithWatermanScores(dna_b2c, dna_costs, -2, -2)) == 8  # bias -3: cost + 3 in [0, 5]
 
 
 REFERENCE_INCLUDE = "/root/reference/include"
