@@ -217,3 +217,22 @@ def test_packed_local_batches_and_graph_replays(packed_ladders):
                 out.fill(777)
             for got, want in zip(batch.run(), expected):
                 np.testing.assert_array_equal(got.astype(np.int64), want, err_msg=f"mode {mode} replay {replay}")
+
+
+@pytest.mark.parametrize("ladder", ["wave8", "mixed"])
+def test_packed_local_rows_past_the_pair_never_raise_the_best(ladder, packed_ladders):
+    """Columns with long runs of 'N' (class 0, which scores +2 against itself under the DNA matrix) against ACGT rows
+    that end inside their stripe: a row past the pair given a real class-0 profile would collect those matches, so
+    these pairs pin that such rows never raise the best."""
+    rng = np.random.default_rng(zlib.crc32(f"past:{ladder}".encode()))
+    engine, params, _ = _engine("sw_dna", rng)
+    packed_ladders(ladder)
+    a, b = [], []
+    for n in (1, 40, 130, 300, 450, 513, 700, 1100):
+        rows = random_bytes(rng, DNA, n, n)
+        columns = b"N" * 400 + random_bytes(rng, DNA, 300, 900) + b"N" * 600
+        a.append(rows if n % 2 else columns)
+        b.append(columns if n % 2 else rows)
+    expected = _check(engine, params, a, b, f"{ladder} rows past the pair")
+    # A best of at most 2 per row: for the short rows far below what the 600-byte 'N' run would give such a row.
+    assert all(score <= 2 * min(len(x), len(y)) for x, y, score in zip(a, b, expected))
