@@ -136,7 +136,7 @@ def candidates_for(slot: int, low: int, high: int, table) -> List[int]:
             continue
         if chained:  # for pairs far taller than the bands measured here
             continue
-        if packed and (slot in UNIT_SLOTS or family not in ("nw_linear", "nw_affine", "sw_linear") or slot % 2 == 1):
+        if packed and (slot in UNIT_SLOTS or family.startswith("lev") or slot % 2 == 1):
             continue
         if not uniform and rows_per_lane > 8:
             continue

@@ -932,7 +932,7 @@ sz_status_t szs_tuning_set_ladder(int slot, int const *shapes, int const *max_ro
         // Chained groups hand over bottom rows only: global families, whose score is the last cell.
         if (s.chained && (unit || family == family_sw_linear_k || family == family_sw_affine_k)) return false;
         if (s.packed)
-            return !plain && (family == family_nw_linear_k || family == family_nw_affine_k || family == family_sw_linear_k);
+            return !uniform && !plain; // class-cost families, when the engine's range bound admits the rows
         if (!uniform && s.rows_per_lane > 8) return false; // LDS profiles are sized for 8 rows per lane
         return true;
     };

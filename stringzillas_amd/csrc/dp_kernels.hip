@@ -355,16 +355,16 @@ __device__ __forceinline__ void run_shape(item_args_t const &a) {
         }
     }
     else if constexpr (shape.packed) {
-        // Packed 16-bit shapes exist for the global class-cost families in their anti-diagonal form and for local
-        // linear gaps in their `H + g` form (the binning never picks them otherwise); chained groups for the global
+        // Packed 16-bit shapes exist for the global class-cost families in their anti-diagonal form and for the
+        // local ones in their `H + o` form (the binning never picks them otherwise); chained groups for the global
         // ones only (a local score needs every stripe's best).
         constexpr bool global = family_ == family_nw_linear_k || family_ == family_nw_affine_k;
-        constexpr bool local = family_ == family_sw_linear_k;
+        constexpr bool local = family_ == family_sw_linear_k || family_ == family_sw_affine_k;
         if constexpr ((global || (local && !shape.chained)) && sizeof(symbol_type_) == 1 && shifted_) {
             // One base for the whole wave when the pair's engine admits it at these rows (no per-step moves of
             // values between lane bases), else one base per lane. Every pair of a packed item has one wave to
             // itself or the whole workgroup, so the item's first pair decides.
-            // Local alignment: a profile biased below g costs one more add per two cells (`biased_`).
+            // Local alignment: a profile biased below o costs one more add per two cells (`biased_`).
             engine_params_t const *params = a.segments[a.tasks[a.order[a.first]].segment].params;
             bool const uniform_base = params->packed_uniform_rows >= shape.rows_per_lane;
             if constexpr (local) {

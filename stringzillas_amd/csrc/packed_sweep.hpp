@@ -1,22 +1,23 @@
 /**
  *  @file packed_sweep.hpp
- *  @brief Needleman-Wunsch and linear-gap Smith-Waterman sweeps with two 16-bit cells per 32-bit register (packed
- *  integer VALU).
+ *  @brief Needleman-Wunsch and Smith-Waterman sweeps with two 16-bit cells per 32-bit register (packed integer
+ *  VALU).
  *
  *  The global recurrences are those of include/stringzillas/similarities/serial.hpp:669-749 (linear) and 854-964
  *  (affine, Gotoh), in the anti-diagonal domain of dp_sweep.hpp: Ĉ = H - e*(i+j), ĥ and v̂ likewise plus (e - o),
  *  so that
  *      ĥ = max(Ĉ_left, ĥ_left),  v̂ = max(Ĉ_up, v̂_up),  Ĉ = max(ĥ, v̂, Ĉ_diag + s - o - e) + (o - e)
  *      linear: Ĉ = max(Ĉ_up, Ĉ_left, Ĉ_diag + s - 2g).
- *  Local linear alignment (serial.hpp:762-841, best over all cells from 0) keeps the int32 sweep's `H + g` form
- *  instead, whose zero floor is one constant per lane and chunk: X = H + g and
- *      X = max(X_up, X_left, max(X_diag + s - g, 0)) + g,
- *  the gap terms read the stored values as they are. Per two cells: the cost `v_perm_b32`, two `v_pk_add_u16` (the
- *  profile bytes hold `cost - min(g, s_min)` so that they read as unsigned; the second add takes g - min(g, s_min)
- *  off again), one `v_pk_max_u16` (substitution against the floor; all three off the row-to-row chain), the
- *  three-way maximum, the `+ g` add and one `v_pk_max_u16` into the lane's running best, which is folded into an
- *  int32 best at the end of every 64-step chunk (the base changes between chunks) over the halves whose rows and
- *  columns lie in the matrix.
+ *  Local alignment (serial.hpp:762-841 linear, 977-1079 affine; best over all cells from 0) keeps the int32
+ *  sweep's `H + o` form instead, whose zero floor is one constant per lane and chunk: X = H + o and
+ *      linear: X = max(X_up, X_left, max(X_diag + s - o, 0)) + o,
+ *      affine: E = max(X_left, E_left + e), F = max(X_up, F_up + e), X = max(E, F, max(X_diag + s - o, 0)) + o,
+ *  the gap terms read the stored values as they are. Per two cells: the cost `v_perm_b32`, one `v_pk_add_u16`
+ *  (a second one when the profile bytes hold `cost - min(o, s_min)` below o, so that they read as unsigned), one
+ *  `v_pk_max_u16` (substitution against the floor; all off the row-to-row chain), the three-way maximum, the `+ o`
+ *  add and half a `v_pk_max3` into the lane's running best (two registers per maximum), which is folded into an
+ *  int32 best at the end of every 64-step chunk (the base changes between chunks); affine gaps add an add and a
+ *  maximum per track.
  *
  *  Packing. The rows of one lane form a serial chain (each row needs the row above at the same column), so two
  *  cells of one register must come from independent chains: lane k holds R rows, and its register p carries row
@@ -81,14 +82,15 @@ __device__ __forceinline__ uint32_t pk_max3_ordered(uint32_t a, uint32_t b, uint
 /** `biased_` (local linear only): the profile bias differs from g, so each substitution term takes g - bias off. */
 template <int family_, int R, bool uniform_base_ = false, bool biased_ = false>
 struct packed_sweep_t {
-    static_assert(family_ == family_nw_linear_k || family_ == family_nw_affine_k || family_ == family_sw_linear_k,
-                  "global class-cost families and local linear gaps only");
+    static_assert(family_ == family_nw_linear_k || family_ == family_nw_affine_k || family_ == family_sw_linear_k ||
+                      family_ == family_sw_affine_k,
+                  "class-cost families only");
     static_assert(R % 2 == 0 && R >= 2 && R <= 8, "two half-stripes of R/2 rows; profiles hold at most 8 rows");
     static constexpr int P = R / 2;               // packed registers per lane
     static constexpr int B = (P + 1) / 2;         // local best registers per lane: one per two score registers
     static constexpr int RP = (R + 3) / 4 * 4;    // profile bytes per lane and class (the int32 sweep's layout)
-    static constexpr bool affine_k = family_ == family_nw_affine_k;
-    static constexpr bool local_k = family_ == family_sw_linear_k;
+    static constexpr bool affine_k = family_ == family_nw_affine_k || family_ == family_sw_affine_k;
+    static constexpr bool local_k = family_ == family_sw_linear_k || family_ == family_sw_affine_k;
     static constexpr int group_k = 64;
     static constexpr int rows_per_lane = R;
     static constexpr int skew_k = 2;              // lane k + 1 runs two columns behind lane k
@@ -148,13 +150,15 @@ struct packed_sweep_t {
     }
 
     /** The stored int32 form minus H at cell (i, j): -e*(i+j) modulo 2^32 (linear: e = g; see `wrap_add`), or the
-     *  constant g of the local form. */
+     *  constant o of the local form. */
     __device__ __forceinline__ int32_t domain(int32_t i, int32_t j) const {
         if constexpr (local_k) return open;
         else return wrap_mul(-extend, wrap_add(i, j));
     }
+    /** The same for the gap tracks: the local form keeps them as they are. */
     __device__ __forceinline__ int32_t gap_domain(int32_t i, int32_t j) const {
-        return wrap_add(domain(i, j), extend - open);
+        if constexpr (local_k) return 0;
+        else return wrap_add(domain(i, j), extend - open);
     }
 
     __device__ __forceinline__ uint32_t to_u16(int32_t value) const {
@@ -260,7 +264,7 @@ struct packed_sweep_t {
     /** One step of both half-stripes: `up` and `vertical_in` are the packed top inputs. Local alignment: `zero_floor`
      *  is the packed H = 0, `keep` the halves whose columns lie within the pair (their cells count for the best). */
     __device__ __forceinline__ void update_rows(uint32_t up, uint32_t vertical_in, uint32_t kappa, uint32_t zero_floor,
-                                                uint32_t unbias, uint32_t keep) {
+                                                uint32_t unbias, uint32_t extension, uint32_t keep) {
         uint32_t diag = diagonal;
         uint32_t vert = vertical_in;
         uint32_t held = 0; // local alignment: the previous register's cell, paired with this one's into the best
@@ -277,22 +281,31 @@ struct packed_sweep_t {
             // alone (linear), so the scheduler interleaves the rows instead of padding back-to-back dependent
             // packed ops with wait states.
             uint32_t stored;
-            if constexpr (affine_k) {
-                uint32_t const h = pk_max_u16(left, horizontal[p]);
-                uint32_t const v = pk_max_u16(up, vert);
-                horizontal[p] = h;
-                vert = v;
-                stored = pk_add_u16(pk_max3_ordered(h, if_substitution, v), kappa);
-            }
-            else if constexpr (local_k) {
-                // - (g - packed_bias) when the profile is biased below g
+            if constexpr (local_k) {
+                // - (o - packed_bias) when the profile is biased below o
                 uint32_t const substitution = biased_ ? pk_add_u16(if_substitution, unbias) : if_substitution;
-                uint32_t const cell = pk_max3_ordered(up, left, pk_max_u16(substitution, zero_floor));
+                uint32_t const floored = pk_max_u16(substitution, zero_floor);
+                uint32_t cell;
+                if constexpr (affine_k) { // E = max(X_left, E_left + e), F = max(X_up, F_up + e)
+                    uint32_t const h = pk_max_u16(left, pk_add_u16(horizontal[p], extension));
+                    uint32_t const v = pk_max_u16(up, pk_add_u16(vert, extension));
+                    horizontal[p] = h;
+                    vert = v;
+                    cell = pk_max3_ordered(h, v, floored);
+                }
+                else cell = pk_max3_ordered(up, left, floored);
                 uint32_t const counted = cell & keep; // all ones on steady steps
                 if (p % 2 == 1) best[p / 2] = pk_max3_ordered(best[p / 2], held, counted);
                 else if (p == P - 1) best[p / 2] = pk_max_u16(best[p / 2], counted);
                 held = counted;
                 stored = pk_add_u16(cell, kappa);
+            }
+            else if constexpr (affine_k) {
+                uint32_t const h = pk_max_u16(left, horizontal[p]);
+                uint32_t const v = pk_max_u16(up, vert);
+                horizontal[p] = h;
+                vert = v;
+                stored = pk_add_u16(pk_max3_ordered(h, if_substitution, v), kappa);
             }
             else stored = pk_max3_ordered(up, left, if_substitution);
             diag = left;
@@ -349,8 +362,9 @@ struct packed_sweep_t {
         // Lane 0's column symbol comes from the ring; the other lanes fetched theirs during the previous step.
         if (lane == 0) symbol_now = (uint32_t)inject.z;
         read_costs(symbol_now, costs_now);
-        // The add after the maximum: o - e (affine global), g (local: stores H + g).
+        // The add after the maximum: o - e (affine global), o (local: stores H + o).
         uint32_t const kappa = ((uint32_t)(local_k ? open : open - extend) & 0xFFFFu) * 0x10001u;
+        uint32_t const extension = ((uint32_t)extend & 0xFFFFu) * 0x10001u; // local affine gap tracks
         // The packed H = 0 of the local form in this chunk's base; below every value the lane holds (and so clamped
         // to 0) once the base is far above zero.
         uint32_t zero_floor = 0;
@@ -380,7 +394,7 @@ struct packed_sweep_t {
             if (steady_ || (uint32_t)(c - 1) <= (uint32_t)m) {
                 // Upper half at column c, lower half at c - 1: both within 1..m on steady steps.
                 uint32_t const keep = steady_ ? ~0u : (c <= m ? 0xFFFFu : 0u) | (c >= 2 ? 0xFFFF0000u : 0u);
-                update_rows(up, vertical_in, kappa, zero_floor, unbias, keep);
+                update_rows(up, vertical_in, kappa, zero_floor, unbias, extension, keep);
                 if (!steady_) {
                     if (c == 1) { // the lower half stays on the column-0 border
 #pragma unroll

@@ -257,15 +257,16 @@ static sz_status_t densify(engine_t *engine, uint8_t const *byte_to_class, int8_
  *  the chunk (its R rows and the row above, from its lower half's column to 64 columns on) lie within a Manhattan
  *  distance of R + 63 of it; R + 70 is used. Profile bytes `cost - (o + e)` must also lie in [0, 127] to read as
  *  unsigned bytes.
- *  Local linear gaps (serial.hpp:762-841, g <= 0) keep X = H + g instead. There H(i,j) - H(i-1,j) >= g (the gap
- *  term, or the zero floor when H(i-1,j) + g < 0), and H(i,j) - H(i-1,j) <= max(0, s_max - g) by the same
- *  induction along the row (the floor only lowers the bound, the border row is 0): adjacent cells differ by at most
- *  max(|g|, s_max - g), below the global delta with o = e = g, so the same limits hold. Its profile bytes hold
- *  `cost - packed_bias` with packed_bias = min(g, s_min), and the substitution term subtracts g - packed_bias
- *  again (modulo 2^16, so only the corrected sum must be in range). The zero floor is the lowest value of every cell.
+ *  Local gaps (serial.hpp:762-841 linear, 977-1079 affine; o, e <= 0) keep X = H + o instead. There
+ *  H(i,j) - H(i-1,j) >= o (the gap opened from above, or the zero floor), and the upper bound of the global
+ *  induction holds as well, since the floor only lowers it and the border row is 0: the same limits apply (their
+ *  delta adds |e| for the anti-diagonal domain, which the local form does not need). The profile bytes hold
+ *  `cost - packed_bias` with packed_bias = min(o, s_min), and the substitution term subtracts o - packed_bias
+ *  again (modulo 2^16, so only the corrected sum must be in range). The zero floor is the lowest value of every
+ *  cell, and the gap tracks lie between the cell beside them plus o and the cell they end in.
  */
 int packed_rows_limit(engine_params_t const &params) {
-    bool const local = params.family == family_sw_linear_k;
+    bool const local = params.family == family_sw_linear_k || params.family == family_sw_affine_k;
     if (params.family != family_nw_linear_k && params.family != family_nw_affine_k && !local) return 0;
     int64_t const o = params.open, e = params.extend;
     if (o > 0 || e > 0 || params.shift != (local ? o : o + e) || params.dense_count == 0) return 0;
@@ -356,7 +357,7 @@ sz_status_t make_engine(int kind, int match, int mismatch, int open, int extend,
             if (biased < -128 || biased > 127) engine->params.shift = 0;
         }
     engine->params.packed_bias = engine->params.shift;
-    if (family == family_sw_linear_k && engine->params.shift == open)
+    if (local && engine->params.shift == open)
         for (int x = 0; x < engine->params.dense_count; ++x)
             for (int y = 0; y < engine->params.dense_count; ++y)
                 engine->params.packed_bias = std::min<int32_t>(engine->params.packed_bias,

@@ -99,7 +99,7 @@ struct shape_t {
     int32_t rows_per_lane;
     int32_t cooperative;
     int32_t bit_parallel; /**< unit-cost bytes: one 64-row Myers word per lane instead of R DP rows */
-    int32_t packed;       /**< class costs, global or local linear: two 16-bit cells per register (two half-stripes) */
+    int32_t packed;       /**< class costs: two 16-bit cells per register (two half-stripes per lane) */
     /** Cooperative shapes only: the pair's stripes are split into `chain_groups_k` consecutive work items, each
      *  a workgroup; a group reads the previous group's bottom row from HBM as it is published. */
     int32_t chained;
@@ -112,8 +112,7 @@ constexpr int waves_per_group_k = 4;
 
 /**
  *  The shapes compiled; index = shape id. Bit-parallel shapes exist for the unit-cost byte Levenshtein only, packed
- *  16-bit shapes for the global class-cost families and local linear gaps, for engines whose values provably fit
- *  (else their int32 twin runs).
+ *  16-bit shapes for the class-cost families, for engines whose values provably fit (else their int32 twin runs).
  */
 constexpr int shapes_count_k = 26;
 constexpr shape_t shapes_k[shapes_count_k] = {
@@ -303,8 +302,9 @@ struct utf8_tape_t {
  *    step on 32-bit halves, 15 per 32-row word; over runes the match word adds one `v_bfe` and one `v_bitop3` per
  *    half for each of at least 8 bit-planes (the smallest plane count compiled);
  *  - packed 16-bit NW: 6 packed ops per two cells (affine: perm, add, two max, three-way maximum, add), 3 for
- *    linear gaps (perm, add, three-way maximum); packed local linear 7 (perm, two adds, max against the zero
- *    floor, three-way maximum, add, max into the best);
+ *    linear gaps (perm, add, three-way maximum); packed local 7 with linear gaps (perm, two adds, max against the
+ *    zero floor, three-way maximum, add, max into the best) and 11 with affine ones (two adds and two maxima for
+ *    the gap tracks);
  *  - int32 DP in the anti-diagonal domain (global families): linear 2 (add, max3), affine 5 (two max, add, max3,
  *    add); uniform costs add a compare and a select (Levenshtein 4 and 7); local families 4 (linear) and 8 (affine).
  */
@@ -314,7 +314,7 @@ __host__ __device__ constexpr uint32_t core_ops_per_64_cells(int family, int sym
         bool const runes = symbols == symbols_u32_k;
         return shape.rows_per_lane == 64 ? (runes ? 25 + 3 * 8 : 25) : (runes ? 2 * (15 + 2 * 8) : 2 * 15);
     }
-    if (shape.packed) return family == family_sw_linear_k ? 224 : affine ? 192 : 96;
+    if (shape.packed) return family == family_sw_linear_k ? 224 : family == family_sw_affine_k ? 352 : affine ? 192 : 96;
     if (shape.wide) return affine ? 20 * 64 : 8 * 64; // int64 adds and selects: two or three lane-ops each
     switch (family) {
     case family_lev_linear_k: return 4 * 64;

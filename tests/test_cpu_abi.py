@@ -202,7 +202,8 @@ def test_tuning_ladders_validate_and_round_trip():
     nw = autotune.slot_of("nw_affine")
     plain = autotune.slot_of("nw_affine", runes=True)  # engines on the plain recurrences: no packed cells
     sw_linear, sw_affine = autotune.slot_of("sw_linear"), autotune.slot_of("sw_affine")
-    saved = {slot: autotune.get_ladder(slot) for slot in (unit, nw, plain, sw_linear, sw_affine)}
+    lev_affine = autotune.slot_of("lev_affine")
+    saved = {slot: autotune.get_ladder(slot) for slot in (unit, nw, plain, sw_linear, sw_affine, lev_affine)}
     try:
         ladder, beyond = saved[nw]
         assert ladder and all(b[1] > a[1] for a, b in zip(ladder, ladder[1:]))
@@ -220,16 +221,17 @@ def test_tuning_ladders_validate_and_round_trip():
             autotune.set_ladder(plain, [(packed, 512)], beyond)
         autotune.set_ladder(plain, [], autotune.wide_shapes()[0])  # the int64 shape scores any slot
         assert autotune.get_ladder(nw) == (ladder[:2], beyond)  # rejected ladders change nothing
-        # Local alignment takes packed shapes with linear gaps only, and never chained groups (its score needs
-        # every stripe's best).
+        # Local alignment takes packed shapes, never chained groups (its score needs every stripe's best);
+        # uniform costs never take packed shapes.
         cooperative_packed = next(i for i, s in enumerate(table) if s[4] and s[2] and not s[5])
         chained_packed = next(i for i, s in enumerate(table) if s[4] and s[5])
         autotune.set_ladder(sw_linear, [(packed, 512)], cooperative_packed)
         assert autotune.get_ladder(sw_linear) == ([(packed, 512)], cooperative_packed)
         with pytest.raises(ValueError):
             autotune.set_ladder(sw_linear, [(packed, 512)], chained_packed)
+        autotune.set_ladder(sw_affine, [(packed, 512)], cooperative_packed)
         with pytest.raises(ValueError):
-            autotune.set_ladder(sw_affine, [(packed, 512)], saved[sw_affine][1])
+            autotune.set_ladder(lev_affine, [(packed, 512)], saved[lev_affine][1])
     finally:
         for slot, (ladder, beyond) in saved.items():
             autotune.set_ladder(slot, ladder, beyond)
@@ -334,8 +336,9 @@ def test_packed_rows_follow_the_range_bound():
     # 809; 4 rows: 200 * 74 + 809 = 15,609 fits, 6 rows: 200 * 76 + 809 = 16,009 does not.
     assert rows(szs.NeedlemanWunschScores(b2c, (costs.astype(np.int64) - 85).astype(np.int8), -128, -72)) == 4
     assert rows(szs.NeedlemanWunschScores(b2c, costs, 3, 1)) == 0  # odd signs: the bound does not apply
-    assert rows(szs.SmithWatermanScores(b2c, costs, -11, -1)) == 0  # local affine gaps stay int32
-    # Local linear gaps: profile bytes `cost - min(g, s_min)`, the same delta with o = e = g.
+    # Local alignment: profile bytes `cost - min(o, s_min)`, the same delta as the global families.
+    assert rows(szs.SmithWatermanScores(b2c, costs, -11, -1)) == 8  # bytes cost + 11 in [7, 22], delta = 23
+    assert rows(szs.SmithWatermanScores(b2c, costs, -110, -80)) == 4  # delta = 201: 201 * 74 + 690 = 15,564
     assert rows(szs.SmithWatermanScores(b2c, costs, -4, -4)) == 8  # bytes cost + 4 in [0, 15], delta = 19
     assert rows(szs.SmithWatermanScores(b2c, costs, -2, -2)) == 8  # bias -4 below g: the correction add runs
     assert rows(szs.SmithWatermanScores(b2c, costs, -95, -95)) == 4  # delta = 201: 201 * 74 + 660 = 15,534

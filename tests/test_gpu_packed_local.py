@@ -1,12 +1,12 @@
-"""GPU parity of the packed 16-bit Smith-Waterman sweep with linear gaps (two cells per register, values around a
-moving base, the zero floor one constant per lane and chunk, the best folded chunk by chunk).
+"""GPU parity of the packed 16-bit Smith-Waterman sweeps, linear and affine gaps (two cells per register, values
+around a moving base, the zero floor one constant per lane and chunk, the best folded chunk by chunk).
 
-The slot's ladder is replaced by ladders that send every pair to the packed shapes (one stripe, several stripes on
-one wave with 2-, 4- or 6-row tails, the cooperative workgroup), and every score is compared bit-exactly with the
-CPU oracle. Engines on both sides of the range bound run: profile bytes that need no correction (gap <= every
-cost) and ones that do (a mismatch below the gap, as in the usual DNA scoring), one admitting only 4 rows per lane
-(its 6- and 8-row shapes run the int32 twin), and ones admitting none. Long similar strings drive the best score
-far beyond 16 bits, and local matches embedded in random flanks put it in the middle of the matrix.
+The local slots' ladders are replaced by ladders that send every pair to the packed shapes (one stripe, several
+stripes on one wave with 2-, 4- or 6-row tails, the cooperative workgroup), and every score is compared bit-exactly
+with the CPU oracle. Engines on both sides of the range bound run: profile bytes that need no correction (the open
+cost <= every cost) and ones that do (a mismatch below it, as in the usual DNA scoring), ones admitting only 4 rows
+per lane (their 6- and 8-row shapes run the int32 twin), and ones admitting none. Long similar strings drive the
+best score far beyond 16 bits, and local matches embedded in random flanks put it in the middle of the matrix.
 """
 
 import zlib
@@ -32,7 +32,22 @@ LADDERS = {
 EDGES = [1, 2, 3, 5, 63, 64, 65, 127, 128, 129, 191, 255, 256, 257, 383, 384, 385, 511, 512, 513, 1023, 1025]
 
 
+AFFINE = {  # name -> (alphabet, open, extend)
+    "sw_affine_protein": ("protein", -11, -1),  # costs + 11 >= 0: no correction
+    "sw_affine_small_open": ("protein", -3, -1),  # costs below the open cost: biased profile
+    "sw_affine_dna": ("dna", -5, -2),
+    "sw_affine_dna_biased": ("dna", -2, -1),  # mismatch -3 below the open cost
+    "sw_affine_far": ("protein", -110, -80),  # delta = 201: admits 4 rows per lane only
+}
+
+
 def _engine(name, rng):
+    if name in AFFINE:
+        kind, open_, extend = AFFINE[name]
+        b2c, costs = dna_tables(2, -3) if kind == "dna" else protein_tables(rng)
+        engine = szs.SmithWatermanScores(b2c, costs, open_, extend)
+        params = O.Params("smith_waterman", open=open_, extend=extend, byte_to_class=b2c, class_costs=costs)
+        return engine, params, DNA if kind == "dna" else PROTEIN
     if name.startswith("sw_dna"):
         b2c, costs = dna_tables(2, -3)
         gap = {"sw_dna": -2, "sw_dna_wide_gap": -5}[name]
@@ -73,21 +88,25 @@ def _engine(name, rng):
 
 
 EXPECTED_ROWS = {"sw_protein": 8, "sw_protein_small_gap": 8, "sw_dna": 8, "sw_dna_wide_gap": 8, "sw_zero_match": 8,
-                 "sw_big_diagonal": 8, "sw_far_gap": 4, "sw_wide_range": 0, "sw_over_byte": 0}
+                 "sw_big_diagonal": 8, "sw_far_gap": 4, "sw_wide_range": 0, "sw_over_byte": 0,
+                 "sw_affine_protein": 8, "sw_affine_small_open": 8, "sw_affine_dna": 8, "sw_affine_dna_biased": 8,
+                 "sw_affine_far": 4}
 
 
 @pytest.fixture
 def packed_ladders():
-    """Installs a ladder on the local linear byte slot, restoring the tuned one afterwards."""
-    slot = autotune.slot_of("sw_linear")
-    saved = autotune.get_ladder(slot)
+    """Installs a ladder on both local byte slots, restoring the tuned ones afterwards."""
+    slots = [autotune.slot_of("sw_linear"), autotune.slot_of("sw_affine")]
+    saved = {slot: autotune.get_ladder(slot) for slot in slots}
 
     def install(name):
         ladder, beyond = LADDERS[name]
-        autotune.set_ladder(slot, ladder, beyond)
+        for slot in slots:
+            autotune.set_ladder(slot, ladder, beyond)
 
     yield install
-    autotune.set_ladder(slot, *saved)
+    for slot, (ladder, beyond) in saved.items():
+        autotune.set_ladder(slot, ladder, beyond)
 
 
 def _mutated(rng, alphabet, text: bytes, substitutions: float, indels: float) -> bytes:
@@ -134,7 +153,8 @@ def test_packed_local_shapes_match_oracle(name, ladder, packed_ladders):
 
 
 @pytest.mark.parametrize("ladder", ["wave8", "cooperative"])
-@pytest.mark.parametrize("name", ["sw_protein", "sw_protein_small_gap", "sw_dna", "sw_big_diagonal", "sw_far_gap"])
+@pytest.mark.parametrize("name", ["sw_protein", "sw_protein_small_gap", "sw_dna", "sw_big_diagonal", "sw_far_gap",
+                                  "sw_affine_protein", "sw_affine_small_open", "sw_affine_dna_biased", "sw_affine_far"])
 def test_packed_local_exact_scores_far_beyond_16_bits(name, ladder, packed_ladders):
     """Identical and mutated long strings (the best grows along the whole diagonal, far past a 16-bit window), and
     local matches embedded at random offsets in random flanks (the best sits in the middle of the matrix)."""
@@ -160,7 +180,8 @@ TAIL_ROWS = [1, 127, 128, 129, 255, 256, 257, 383, 384, 385, 512, 513, 640, 641,
              1153, 1280, 1281, 1408, 1409, 2000]
 
 
-@pytest.mark.parametrize("name", ["sw_protein_small_gap", "sw_dna", "sw_far_gap"])
+@pytest.mark.parametrize("name", ["sw_protein_small_gap", "sw_dna", "sw_far_gap", "sw_affine_protein",
+                                  "sw_affine_dna_biased", "sw_affine_far"])
 def test_packed_local_tail_stripes(name, packed_ladders):
     """Every pair on 512-row stripes: the last stripe alone or after full ones, on both sides of each tail size
     (2, 4 and 6 packed rows per lane); the best of the full stripes and the tail's are combined."""
@@ -181,7 +202,7 @@ def test_packed_local_tail_stripes(name, packed_ladders):
     _check(engine, params, a, b, f"{name} tail stripes")
 
 
-@pytest.mark.parametrize("name", ["sw_dna", "sw_protein"])
+@pytest.mark.parametrize("name", ["sw_dna", "sw_protein", "sw_affine_protein", "sw_affine_dna_biased"])
 def test_packed_local_cross_products(name, packed_ladders):
     """A symmetric all-pairs product (each score also written to its mirrored slot) and a rectangular one."""
     rng = np.random.default_rng(zlib.crc32(f"cross:{name}".encode()))
@@ -198,18 +219,24 @@ def test_packed_local_cross_products(name, packed_ladders):
 
 
 def test_packed_local_batches_and_graph_replays(packed_ladders):
-    """The packed local sweep inside a mixed batch, as a persistent launch, one launch per pair and graph replays."""
+    """The packed local sweeps inside a mixed batch (linear and affine gaps, the profile corrected or not, beside
+    Levenshtein pairs), as a persistent launch, one launch per pair and graph replays."""
     rng = np.random.default_rng(31)
     packed_ladders("mixed")
     engine, params, _ = _engine("sw_dna", rng)
+    affine, affine_params, _ = _engine("sw_affine_protein", rng)
     lev = szs.LevenshteinDistances()
     a = [random_bytes(rng, DNA, 1, 3000) for _ in range(60)]
     b = [random_bytes(rng, DNA, 1, 3000) for _ in range(60)]
+    c = [random_bytes(rng, PROTEIN, 1, 3000) for _ in range(60)]
+    d = [random_bytes(rng, PROTEIN, 1, 3000) for _ in range(60)]
     dna_a = [random_bytes(rng, b"ACGT", 1, 2000) for _ in range(40)]
     dna_b = [random_bytes(rng, b"ACGT", 1, 2000) for _ in range(40)]
     expected = [O.pairs(params, O.tape(a), O.tape(b), threads=16),
+                O.pairs(affine_params, O.tape(c), O.tape(d), threads=16),
                 O.pairs(O.Params("levenshtein"), O.tape(dna_a), O.tape(dna_b), threads=16)]
-    segments = [(engine, szs.Strs(a), szs.Strs(b)), (lev, szs.Strs(dna_a), szs.Strs(dna_b))]
+    segments = [(engine, szs.Strs(a), szs.Strs(b)), (affine, szs.Strs(c), szs.Strs(d)),
+                (lev, szs.Strs(dna_a), szs.Strs(dna_b))]
     for mode in (szs.BATCH_PERSISTENT, szs.BATCH_PER_PAIR, szs.BATCH_GRAPH):
         batch = szs.Batch(segments, mode=mode)
         for replay in range(3 if mode == szs.BATCH_GRAPH else 1):
@@ -219,13 +246,14 @@ def test_packed_local_batches_and_graph_replays(packed_ladders):
                 np.testing.assert_array_equal(got.astype(np.int64), want, err_msg=f"mode {mode} replay {replay}")
 
 
+@pytest.mark.parametrize("name", ["sw_dna", "sw_affine_dna_biased"])
 @pytest.mark.parametrize("ladder", ["wave8", "mixed"])
-def test_packed_local_rows_past_the_pair_never_raise_the_best(ladder, packed_ladders):
+def test_packed_local_rows_past_the_pair_never_raise_the_best(ladder, name, packed_ladders):
     """Columns with long runs of 'N' (class 0, which scores +2 against itself under the DNA matrix) against ACGT rows
     that end inside their stripe: a row past the pair given a real class-0 profile would collect those matches, so
     these pairs pin that such rows never raise the best."""
-    rng = np.random.default_rng(zlib.crc32(f"past:{ladder}".encode()))
-    engine, params, _ = _engine("sw_dna", rng)
+    rng = np.random.default_rng(zlib.crc32(f"past:{ladder}:{name}".encode()))
+    engine, params, _ = _engine(name, rng)
     packed_ladders(ladder)
     a, b = [], []
     for n in (1, 40, 130, 300, 450, 513, 700, 1100):
