@@ -471,9 +471,8 @@ int szs_tuning_slots(void);
 int szs_tuning_shapes(void);
 /**
  *  Shape `index` as (lanes per pair, rows per lane, cooperative, bit-parallel, packed, chained). Packed shapes keep
- *  two 16-bit cells per register; they score the global class-cost families and local alignment with linear gaps
- *  (byte slots of `nw_linear`/`nw_affine`/`sw_linear`) when the engine's range bound admits their rows per lane,
- *  and their int32 twin otherwise. Chained shapes
+ *  two 16-bit cells per register; they score the class-cost families (byte slots of `nw_*` and `sw_*`) when the
+ *  engine's range bound admits their rows per lane, and their int32 twin otherwise. Chained shapes
  *  split a tall pair's stripes over 16 workgroups that pass bottom rows through HBM (global families only).
  */
 sz_status_t szs_tuning_shape(int index, int *group, int *rows_per_lane, int *cooperative, int *bit_parallel,
