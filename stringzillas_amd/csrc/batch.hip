@@ -60,7 +60,10 @@ namespace szs_amd {
  *  of the ramp steps, and each step's overhead spread over 8 rows), 1024-row stripes on one wave up to 4096 rows,
  *  then the cooperative 1024-row shape, chained 512-row groups beyond 24,576 rows. Class-cost families likewise,
  *  keeping LDS profiles small: at most 8 rows per lane, 512-row stripes on one wave up to 2048 rows, then the
- *  cooperative 512-row shape (chained beyond 24,576 rows for the global families). Unit-cost bytes and runes: DP up
+ *  cooperative 512-row shape (chained beyond 24,576 rows for the global families); the byte slots of affine NW
+ *  and of SW take the packed 16-bit shapes where they measured faster than int32 (affine NW from 193 rows, up to
+ *  16,384 on one wave; affine SW from 257 rows and linear SW from 513, up to 6,144 on one wave), and an engine
+ *  outside the packed range bound runs their int32 twins. Unit-cost bytes and runes: DP up
  *  to 64 rows, then bit-parallel words: 4 lanes of 32-row words up to 128 rows (bytes only), 4 lanes up to 256, 16
  *  lanes of 32-row words up to 512, 16 lanes up to 1024, one wave of 32-row words up to 2048, 4096-row stripes on
  *  one wave up to 16384, then cooperative.
@@ -91,6 +94,19 @@ static shape_table_t default_shape_table() {
             ladder.back().second = 4096; // up to 4 stripes on one wave
             ladder.push_back({10, chained_rows_k});
             multi = 20;
+        }
+        else if (slot == family_nw_affine_k * 2 + symbols_u8_k) {
+            ladder = {{0, 16}, {1, 32}, {2, 64}, {23, 128}, {4, 192}, {15, 256}, {16, 384}, {17, 16384},
+                      {18, chained_rows_k}};
+            multi = 19;
+        }
+        else if (slot == family_sw_linear_k * 2 + symbols_u8_k) { // local scores: no chained groups
+            ladder = {{0, 16}, {1, 32}, {2, 64}, {23, 128}, {4, 192}, {5, 256}, {6, 384}, {7, 512}, {17, 6144}};
+            multi = 18;
+        }
+        else if (slot == family_sw_affine_k * 2 + symbols_u8_k) {
+            ladder = {{0, 16}, {1, 32}, {2, 64}, {23, 128}, {4, 192}, {5, 256}, {16, 384}, {17, 6144}};
+            multi = 18;
         }
         else {
             for (int shape = 0; shape <= 7; ++shape)

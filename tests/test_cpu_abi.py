@@ -237,6 +237,36 @@ def test_tuning_ladders_validate_and_round_trip():
             autotune.set_ladder(slot, ladder, beyond)
 
 
+def test_built_in_class_cost_ladders_match_the_tuned_ones():
+    """A C caller that never imports the package gets the library's built-in ladders: for the byte slots of affine
+    NW and of SW (packed shapes where they measured faster) they equal the measured ladders of `tuned_shapes.json`.
+    A fresh process loads the library alone, since importing the package installs the tuned file."""
+    import json
+
+    from stringzillas_amd import autotune
+
+    script = (
+        "import ctypes, json, sys\n"
+        "lib = ctypes.CDLL(sys.argv[1])\n"
+        "out = {}\n"
+        "for slot in map(int, sys.argv[2:]):\n"
+        "    shapes, rows = (ctypes.c_int * 64)(), (ctypes.c_int * 64)()\n"
+        "    count, beyond = ctypes.c_int(), ctypes.c_int()\n"
+        "    assert lib.szs_tuning_get_ladder(slot, shapes, rows, 64, ctypes.byref(count), ctypes.byref(beyond)) == 0\n"
+        "    out[slot] = [[[shapes[i], rows[i]] for i in range(count.value)], beyond.value]\n"
+        "print(json.dumps(out))\n"
+    )
+    slots = [autotune.slot_of(family) for family in ("nw_affine", "sw_linear", "sw_affine")]
+    done = subprocess.run(["python3", "-c", script, _native.LIBRARY_PATH, *map(str, slots)], capture_output=True,
+                          text=True, timeout=120)
+    assert done.returncode == 0, done.stderr
+    built_in = json.loads(done.stdout)
+    with open(autotune.CACHE) as f:
+        tuned = json.load(f)["slots"]
+    for slot in slots:
+        assert built_in[str(slot)] == [tuned[str(slot)]["ladder"], tuned[str(slot)]["beyond"]], slot
+
+
 def test_refinement_moves_are_admissible_neighbours():
     """Every ladder the workload-level search would try is one the library accepts, and differs from the start."""
     from stringzillas_amd import autotune
