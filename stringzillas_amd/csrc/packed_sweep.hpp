@@ -44,7 +44,7 @@
  *  `packed_rows_limit`). Otherwise pairs run the int32 sweep. A lane whose upper half has passed column m holds no
  *  cell that is kept and keeps its base; the row above the stripe repeats column m's values beyond it; cells beyond
  *  column m or row n only feed cells that are discarded, and the two halves of a register never mix. Profile
- *  bytes hold `cost - (o + e)` (global) or `cost - min(g, s_min)` (local), which the engine requires in [0, 127],
+ *  bytes hold `cost - (o + e)` (global) or `cost - min(o, s_min)` (local), which the engine requires in [0, 127],
  *  so they read as unsigned bytes.
  */
 #pragma once
@@ -79,7 +79,7 @@ __device__ __forceinline__ uint32_t pk_max3_ordered(uint32_t a, uint32_t b, uint
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_maximum(__builtin_elementwise_maximum(x, y), z));
 }
 
-/** `biased_` (local linear only): the profile bias differs from g, so each substitution term takes g - bias off. */
+/** `biased_` (local families only): the profile bias lies below o, so each substitution term takes o - bias off. */
 template <int family_, int R, bool uniform_base_ = false, bool biased_ = false>
 struct packed_sweep_t {
     static_assert(family_ == family_nw_linear_k || family_ == family_nw_affine_k || family_ == family_sw_linear_k ||
@@ -121,7 +121,7 @@ struct packed_sweep_t {
     int32_t result_register;
     int32_t captured;                 /**< the last row's score at column m, when it sits in the upper half */
     // Local alignment.
-    uint32_t best[B];                 /**< per half: the chunk's largest cell (u16 of H in the chunk's base), 0 = none */
+    uint32_t best[B];                 /**< per half of two registers: the chunk's largest cell (u16 of H in its base) */
     int32_t running_best;             /**< the lane's best H over the chunks folded so far (0 from the start) */
 
     __device__ __forceinline__ void load(item_args_t const &a, uint32_t task_index, bool, uint32_t wave) {
@@ -198,8 +198,8 @@ struct packed_sweep_t {
                 int8_t const cost = transposed ? params->dense_costs[c * max_dense_classes_k + row_classes[r]]
                                                : params->dense_costs[row_classes[r] * max_dense_classes_k + c];
                 // In [0, 127]: checked when the engine was made. Local alignment gives rows past n the lowest cost,
-                // min(g, s_min) <= 0: every value such a row takes is then the floor or a cell of the pair plus
-                // steps <= 0, never above the pair's best, so the running best needs no row mask.
+                // min(o, s_min) <= 0: every value such a row takes is then the floor or a cell of the pair plus
+                // steps <= 0 (gap costs are <= 0 too), never above the pair's best, so the best needs no row mask.
                 bool const past = local_k && lane_top + r + 1 > n;
                 mine[c * 64 * RP + r] = past ? (int8_t)0 : (int8_t)(cost - shift);
             }
