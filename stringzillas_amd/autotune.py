@@ -138,7 +138,7 @@ def candidates_for(slot: int, low: int, high: int, table) -> List[int]:
             continue
         if packed and (slot in UNIT_SLOTS or family.startswith("lev") or slot % 2 == 1):
             continue
-        if not uniform and rows_per_lane > 8:
+        if not uniform and rows_per_lane > 12:
             continue
         capacity = group * rows_per_lane
         if cooperative:

@@ -63,7 +63,7 @@ namespace szs_amd {
  *  cooperative 512-row shape (chained beyond 24,576 rows for the global families); the byte slots of affine NW
  *  and of SW take the packed 16-bit shapes where they measured faster than int32 (affine NW from 193 rows, up to
  *  16,384 on one wave; affine SW from 257 rows and linear SW from 513, up to 6,144 on one wave), and an engine
- *  outside the packed range bound runs their int32 twins. Unit-cost bytes and runes: DP up
+ *  outside the packed range bound runs their int32 twins; linear SW takes 16 lanes of 10 rows for 129-160 rows. Unit-cost bytes and runes: DP up
  *  to 64 rows, then bit-parallel words: 4 lanes of 32-row words up to 128 rows (bytes only), 4 lanes up to 256, 16
  *  lanes of 32-row words up to 512, 16 lanes up to 1024, one wave of 32-row words up to 2048, 4096-row stripes on
  *  one wave up to 16384, then cooperative.
@@ -101,7 +101,8 @@ static shape_table_t default_shape_table() {
             multi = 19;
         }
         else if (slot == family_sw_linear_k * 2 + symbols_u8_k) { // local scores: no chained groups
-            ladder = {{0, 16}, {1, 32}, {2, 64}, {23, 128}, {4, 192}, {5, 256}, {6, 384}, {7, 512}, {17, 6144}};
+            ladder = {{0, 16}, {1, 32}, {2, 64}, {23, 128}, {26, 160}, {4, 192}, {5, 256}, {6, 384}, {7, 512},
+                      {17, 6144}};
             multi = 18;
         }
         else if (slot == family_sw_affine_k * 2 + symbols_u8_k) {
@@ -561,6 +562,22 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
     std::vector<segment_view_t> views(segments_count);
     uint64_t tasks_total = 0;
     uint32_t lds_bytes = 0;
+    // The ladders this batch is binned with: one copy, so that the LDS profile sizes below and the device's shape
+    // choice agree even if a caller installs another ladder meanwhile.
+    shape_table_t const table = [] {
+        std::lock_guard<std::mutex> lock(szs_amd::tuning_mutex());
+        return active_shape_table();
+    }();
+    // Profile bytes per lane and class a slot's ladder needs: its widest DP rows per lane in whole dwords.
+    auto profile_row_bytes = [&table](int slot) {
+        int rows = 8;
+        auto consider = [&](int shape) {
+            if (!shapes_k[shape].wide && !shapes_k[shape].bit_parallel) rows = std::max(rows, shapes_k[shape].rows_per_lane);
+        };
+        for (int i = 0; i < table.used[slot]; ++i) consider(table.order[slot][i]);
+        consider(table.multi_pass_shape[slot]);
+        return (uint32_t)((rows + 3) / 4 * 4);
+    };
     for (size_t s = 0; s < segments_count; ++s) {
         segment_input_t const &input = inputs[s];
         engine_t const *engine = input.engine;
@@ -652,8 +669,10 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
         tasks_total += tasks;
         if (tasks_total >= 0xFFFFFFF0ull)
             return fail(sz_overflow_risk_k, error_message, "A batch holds at most 2^32 - 16 scored cells");
-        if (engine->params.family >= family_nw_linear_k) // per-wave profile: classes x 64 lanes x 8 rows
-            lds_bytes = std::max<uint32_t>(lds_bytes, (uint32_t)engine->params.dense_count * 64u * 8u);
+        if (engine->params.family >= family_nw_linear_k) { // per-wave profile: classes x 64 lanes x rows per lane
+            int const slot = engine->params.family * 2 + (a.symbols == class_unshifted_k ? 1 : 0);
+            lds_bytes = std::max<uint32_t>(lds_bytes, (uint32_t)engine->params.dense_count * 64u * profile_row_bytes(slot));
+        }
         if (a.unit_costs && !runes && engine->params.family == family_lev_linear_k) // Myers tables + match table
             lds_bytes = std::max<uint32_t>(lds_bytes, myers_lds_bytes_k - inject_ring_bytes_k);
     }
@@ -703,10 +722,7 @@ sz_status_t batch_stage(batch_t &b, device_scope_t *scope, std::vector<segment_i
     std::memcpy(image + at_views, views.data(), sizeof(segment_view_t) * segments_count);
     std::memcpy(image + at_args, args, sizeof(materialize_args_t) * segments_count);
     if (!decode.empty()) std::memcpy(image + at_decode, decode.data(), sizeof(utf8_tape_t) * decode.size());
-    {
-        std::lock_guard<std::mutex> lock(szs_amd::tuning_mutex());
-        std::memcpy(image + at_shapes, &active_shape_table(), sizeof(shape_table_t));
-    }
+    std::memcpy(image + at_shapes, &table, sizeof(shape_table_t));
     SZS_TRY_HIP(hipMemcpyAsync(device_tables, image, table_bytes, hipMemcpyHostToDevice, stream), "upload");
     if (!b.static_tables_ready) { // constant tables: once per batch object
         std::vector<uint32_t> const &priority = bin_priority();
@@ -949,7 +965,7 @@ sz_status_t szs_tuning_set_ladder(int slot, int const *shapes, int const *max_ro
         if (s.chained && (unit || family == family_sw_linear_k || family == family_sw_affine_k)) return false;
         if (s.packed)
             return !uniform && !plain; // class-cost families, when the engine's range bound admits the rows
-        if (!uniform && s.rows_per_lane > 8) return false; // LDS profiles are sized for 8 rows per lane
+        if (!uniform && s.rows_per_lane > 12) return false; // LDS profiles hold at most 12 rows per lane
         return true;
     };
     if (!allowed(beyond_shape) || shapes_k[beyond_shape].group != 64)

@@ -378,8 +378,8 @@ __device__ __forceinline__ void run_shape(item_args_t const &a) {
             else run_packed<family_, shape_, false, false>(a);
         }
     }
-    // Class-cost families keep their LDS profile at <= 8 rows per lane; larger shapes are never binned for them.
-    else if constexpr (!uniform && shape.rows_per_lane > 8) return;
+    // Class-cost families keep their LDS profile at <= 12 rows per lane; larger shapes are never binned for them.
+    else if constexpr (!uniform && shape.rows_per_lane > 12) return;
     else if constexpr (shape.chained) {
         // Chained groups pass only bottom rows along: global families, whose result is the last cell.
         if constexpr (!family_traits<family_>::local_k)
@@ -419,10 +419,11 @@ __device__ void run_family(int shape, item_args_t const &a) {
     case 23: run_shape<family_, symbol_type_, 23, shifted_>(a); break;
     case 24: run_shape<family_, symbol_type_, 24, shifted_>(a); break;
     case 25: run_shape<family_, symbol_type_, 25, shifted_>(a); break;
+    case 26: run_shape<family_, symbol_type_, 26, shifted_>(a); break;
     default: break;
     }
 }
-static_assert(shapes_count_k == 26, "dispatch switch lists every shape");
+static_assert(shapes_count_k == 27, "dispatch switch lists every shape");
 
 __device__ void run_variant(int variant, item_args_t const &a) {
     int const shape = variant % shapes_count_k;

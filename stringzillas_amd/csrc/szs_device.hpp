@@ -114,7 +114,7 @@ constexpr int waves_per_group_k = 4;
  *  The shapes compiled; index = shape id. Bit-parallel shapes exist for the unit-cost byte Levenshtein only, packed
  *  16-bit shapes for the class-cost families, for engines whose values provably fit (else their int32 twin runs).
  */
-constexpr int shapes_count_k = 26;
+constexpr int shapes_count_k = 27;
 constexpr shape_t shapes_k[shapes_count_k] = {
     {16, 1, 0, 0, 0, 0}, {16, 2, 0, 0, 0, 0}, {16, 4, 0, 0, 0, 0},  // up to 16 / 32 / 64 rows, four pairs per wave
     {64, 2, 0, 0, 0, 0}, {64, 3, 0, 0, 0, 0}, {64, 4, 0, 0, 0, 0},  // up to 128 / 192 / 256 rows
@@ -129,6 +129,7 @@ constexpr shape_t shapes_k[shapes_count_k] = {
     {16, 8, 0, 0, 0, 0},                           // up to 128 rows, four pairs per wave
     {64, 8, 0, 0, 0, 0, 1},                        // int64 cells, 512-row stripes on one wave
     {4, 32, 0, 1, 0, 0},                           // Myers over 32-row words: up to 128 rows (16 pairs/wave)
+    {16, 10, 0, 0, 0, 0},                          // up to 160 rows, four pairs per wave
 };
 /** The int64-cell shape, which pairs beyond the int32 range bound take (any family, bytes or runes). */
 constexpr int wide_shape_k = 24;

@@ -216,16 +216,19 @@ def test_utf8_malformed_and_truncated_input(low, high):
         np.testing.assert_array_equal(engine.pairs(szs.to_device(a), szs.to_device(b)), expected)
 
 
+@pytest.mark.parametrize("rows_per_lane", [8, 10])
 @pytest.mark.parametrize("family", ["nw_affine", "nw_linear", "sw_affine", "sw_linear", "lev_affine"])
-def test_sixteen_lane_eight_row_groups(family):
-    """Four pairs per wave on 16 lanes of 8 rows (up to 128 rows), forced through the tuning ladder."""
+def test_sixteen_lane_groups(family, rows_per_lane):
+    """Four pairs per wave on 16 lanes of 8 or 10 rows (up to 128 or 160 rows; class-cost profiles of 10 rows per
+    lane take 12 bytes per lane and class), forced through the tuning ladder."""
     from stringzillas_amd import autotune
 
     table = autotune.shapes()
-    shape = next(i for i, s in enumerate(table) if tuple(s) == (16, 8, 0, 0, 0, 0))
+    shape = next(i for i, s in enumerate(table) if tuple(s) == (16, rows_per_lane, 0, 0, 0, 0))
+    capacity = 16 * rows_per_lane
     slot = autotune.slot_of(family)
     saved = autotune.get_ladder(slot)
-    rng = np.random.default_rng(zlib.crc32(f"g16r8:{family}".encode()))
+    rng = np.random.default_rng(zlib.crc32(f"g16r{rows_per_lane}:{family}".encode()))
     if family != "lev_affine":
         b2c, costs = protein_tables(rng)
         kind, engine_type = {"nw": ("needleman_wunsch", szs.NeedlemanWunschScores),
@@ -238,13 +241,13 @@ def test_sixteen_lane_eight_row_groups(family):
         engine = szs.LevenshteinDistances(0, 4, 3, 2)
         params = O.Params("levenshtein", 0, 4, 3, 2)
         alphabet = DNA
-    edges = [1, 2, 15, 16, 17, 63, 64, 65, 127, 128]
+    edges = [n for n in (1, 2, 15, 16, 17, 63, 64, 65, 127, 128, 129, 150, 159, 160) if n <= capacity]
     a = [random_bytes(rng, alphabet, n, n) for n in edges for _ in range(3)]
     b = [random_bytes(rng, alphabet, 1, 3000) for _ in edges for _ in range(3)]
-    a += [random_bytes(rng, alphabet, 1, 128) for _ in range(60)]
+    a += [random_bytes(rng, alphabet, 1, capacity) for _ in range(60)]
     b += [random_bytes(rng, alphabet, 1, 2000) for _ in range(60)]
     try:
-        autotune.set_ladder(slot, [(shape, 128)], saved[1])
+        autotune.set_ladder(slot, [(shape, capacity)], saved[1])
         expected = O.pairs(params, O.tape(a), O.tape(b), threads=16)
         got = engine.pairs(szs.Strs(a), szs.Strs(b))
         np.testing.assert_array_equal(got.astype(np.int64), expected)
